@@ -1,0 +1,1678 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+// Hydro spectral-difference update: transforms, Euler fluxes, SD trace
+// boundaries, LLF Riemann, ADER predictor/corrector, SSP-RK, dt, and the FV
+// fallback (face integrals, FV update, NAD/SED/relax/PAD, theta blending,
+// MUSCL-Hancock + HLLC), restated in the reference's loop and summation
+// order.
+//
+// Pinning:
+//  - SD ADER path (transforms, fluxes, periodic traces, LLF, predictor,
+//    corrector, dt): the reference golden tests/golden/hydro_sd (rtol 1e-6,
+//    whole file), the analytic L1 limits of tests/run_tests.py:52,63,72,109,
+//    and exact invariance of a uniform state.
+//  - RK path: analytic L1 limit (run_tests.py:109) only.
+//  - FV fallback, NAD/SED/relax/PAD, blending, MUSCL/HLLC: exact mass
+//    conservation (< 1e-12, run_tests.py:260) plus hand-built closed-form
+//    cases in tests/test_oracle_fallback.py (NAD band, SED alpha, PAD,
+//    blending weights, HLLC on a contact, minmod). Value-level parity of the
+//    blended fallback flux against the reference is "parity unpinned"
+//    (no reference fixture covers it).
+//  - gradfree / reflective traces: mass conservation with reflective walls
+//    (run_tests.py:123-132) and mirror-symmetry tests; values are otherwise
+//    parity unpinned.
+#include <algorithm>
+#include <chrono>
+#include <fstream>
+#include <sys/stat.h>
+#include "oracle_core.hpp"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// loop helpers: src/define.hpp:233-340 (sd_for_cells, sd_for_active_cells,
+// fv_for_cells, fv_for_cells_ngh, fv_for_cells_2ngh, fv_for_faces)
+template <class F>
+static void for6(int Mz, int My, int Mx, int nz, int ny, int nx, int oz, int oy, int ox, F f) {
+    if (Mz <= 0 || My <= 0 || Mx <= 0) return;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int k = 0; k < Mz; k++)
+        for (int j = 0; j < My; j++)
+            for (int i = 0; i < Mx; i++)
+                for (int kk = 0; kk < nz; kk++)
+                    for (int jj = 0; jj < ny; jj++)
+                        for (int ii = 0; ii < nx; ii++) f(k + oz, j + oy, i + ox, kk, jj, ii);
+}
+template <class F>
+static void for3(int Mz, int My, int Mx, int oz, int oy, int ox, F f) {
+    if (Mz <= 0 || My <= 0 || Mx <= 0) return;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int k = 0; k < Mz; k++)
+        for (int j = 0; j < My; j++)
+            for (int i = 0; i < Mx; i++) f(k + oz, j + oy, i + ox);
+}
+
+static inline int choose(int dim, int i, int j, int k) { return dim == 0 ? i : (dim == 1 ? j : k); }
+
+// ---------------------------------------------------------------------------
+// src/transforms.cpp:120-147 (transform_a_to_b_1d) and :151-177 (_slice):
+// U_b[...,id] = sum_{ll<q} U_a[...,ll] * M(id, ll), ascending ll.
+static void transform_1d(const SDA& A, SDA& B, const Mat& M, int dim, int t_src) {
+    int q = choose(dim, A.nx, A.ny, A.nz);
+    int nader = (t_src < 0) ? A.nt : 1;
+    int nvar = A.nv;
+    for6(B.Nz, B.Ny, B.Nx, B.nz, B.ny, B.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        int id = choose(dim, ii, jj, kk);
+        for (int tt = 0; tt < nader; tt++) {
+            int ts = (t_src < 0) ? tt : t_src;
+            for (int var = 0; var < nvar; var++) {
+                double u = 0;
+                for (int ll = 0; ll < q; ll++) {
+                    int a = dim == 0 ? ll : ii, b = dim == 1 ? ll : jj, c = dim == 2 ? ll : kk;
+                    u += A(ts, var, k, j, i, c, b, a) * M(id, ll);
+                }
+                B(tt, var, k, j, i, kk, jj, ii) = u;
+            }
+        }
+    });
+}
+
+// src/transforms.cpp:99-117: one sweep per active axis, x then y then z.
+static void transform_full(const Config& cfg, const SDA& A, SDA& B, SDA& T, const Mat& M) {
+    int dims[3], nd = 0;
+    for (int d = 0; d < 3; d++)
+        if (cfg.act[d]) dims[nd++] = d;
+    const SDA* src = &A;
+    for (int s = 0; s < nd; s++) {
+        SDA* dst = ((nd - 1 - s) % 2 == 0) ? &B : &T;
+        transform_1d(*src, *dst, M, dims[s], -1);
+        src = dst;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// src/hydro.cpp:27-45, :95-106, :156-162
+static inline double sound_speed(double rho, double p, double gm) {
+    double c2 = gm * p / rho;
+    c2 = smax(c2, MIN_C2);
+    return std::sqrt(c2);
+}
+static inline void conservatives(const double* w, double* u, double gm) {
+    double E_kin = 0;
+    u[0] = w[0];
+    u[VX] = w[0] * w[VX];
+    E_kin += w[VX] * u[VX];
+    u[VY] = w[0] * w[VY];
+    E_kin += w[VY] * u[VY];
+    u[VZ] = w[0] * w[VZ];
+    E_kin += w[VZ] * u[VZ];
+    u[E_] = w[P_] / (gm - 1.) + 0.5 * E_kin;
+}
+static inline void primitives(const double* u, double* w, double gm) {
+    double E_kin = 0;
+    w[0] = u[0];
+    w[VX] = u[VX] / u[0];
+    E_kin += w[VX] * u[VX];
+    w[VY] = u[VY] / u[0];
+    E_kin += w[VY] * u[VY];
+    w[VZ] = u[VZ] / u[0];
+    E_kin += w[VZ] * u[VZ];
+    w[P_] = (u[E_] - 0.5 * E_kin) * (gm - 1.);
+}
+static inline void fluxes(const double* u, const double* w, double* f, int v1, int v2, int v3) {
+    f[0] = u[0] * w[v1];
+    f[v1] = u[v1] * w[v1] + w[P_];
+    f[v2] = u[v2] * w[v1];
+    f[v3] = u[v3] * w[v1];
+    f[E_] = (u[E_] + w[P_]) * w[v1];
+}
+static inline void vel_idx(int D, int& v1, int& v2, int& v3) {
+    v1 = D == 0 ? VX : (D == 1 ? VY : VZ);
+    v2 = D == 0 ? VY : (D == 1 ? VZ : VX);
+    v3 = D == 0 ? VZ : (D == 1 ? VX : VY);
+}
+
+// src/hydro.cpp:47-72 / :74-93
+static void compute_conservatives(const Config& cfg, const SDA& W, SDA& U) {
+    double gm = cfg.gamma;
+    for6(W.Nz, W.Ny, W.Nx, W.nz, W.ny, W.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < W.nt; t++) {
+            double u[NVAR] = {0}, w[NVAR];
+            for (int v = 0; v < W.nv; v++) w[v] = W(t, v, k, j, i, kk, jj, ii);
+            conservatives(w, u, gm);
+            for (int v = 0; v < W.nv; v++) U(t, v, k, j, i, kk, jj, ii) = u[v];
+        }
+    });
+}
+// src/hydro.cpp:108-132
+static void compute_primitives(const Config& cfg, const SDA& U, SDA& W) {
+    double gm = cfg.gamma;
+    for6(W.Nz, W.Ny, W.Nx, W.nz, W.ny, W.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < W.nt; t++) {
+            double u[NVAR], w[NVAR] = {0};
+            for (int v = 0; v < NVAR; v++) u[v] = U(t, v, k, j, i, kk, jj, ii);
+            primitives(u, w, gm);
+            for (int v = 0; v < NVAR; v++) W(t, v, k, j, i, kk, jj, ii) = w[v];
+        }
+    });
+}
+// src/hydro.cpp:134-153
+static void compute_primitives_fv(const Config& cfg, const FVA& U, FVA& W) {
+    double gm = cfg.gamma;
+    for3(U.Nz, U.Ny, U.Nx, 0, 0, 0, [&](int k, int j, int i) {
+        double u[10] = {0}, w[10] = {0};
+        for (int v = 0; v < U.nv; v++) u[v] = U(v, k, j, i);
+        primitives(u, w, gm);
+        for (int v = 0; v < U.nv; v++) W(v, k, j, i) = w[v];
+    });
+}
+
+// src/hydro.cpp:167-206 (compute_fluxes_t<V1,V2,V3>)
+static void compute_fluxes(const Config& cfg, const SDA& U, SDA& F, int D) {
+    double gm = cfg.gamma;
+    int v1, v2, v3;
+    vel_idx(D, v1, v2, v3);
+    for6(U.Nz, U.Ny, U.Nx, U.nz, U.ny, U.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < U.nt; t++) {
+            double u[NVAR], w[NVAR] = {0}, f[NVAR] = {0};
+            for (int v = 0; v < NVAR; v++) u[v] = U(t, v, k, j, i, kk, jj, ii);
+            primitives(u, w, gm);
+            fluxes(u, w, f, v1, v2, v3);
+            for (int v = 0; v < NVAR; v++) F(t, v, k, j, i, kk, jj, ii) = f[v];
+        }
+    });
+}
+
+// src/hydro.cpp:208-262 (inviscid + viscous clause)
+static double compute_dt(const Config& cfg, const SDA& W, double dx, double dy, double dz, double nu) {
+    double gm = cfg.gamma, cfl = cfg.cfl;
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    int Mz = W.Nz - 2 * cfg.ngh[2], My = W.Ny - 2 * cfg.ngh[1], Mx = W.Nx - 2 * cfg.ngh[0];
+    int px = W.nx;
+    double result = 1e308 * 10;  // +inf: identity of the min reduction
+    std::vector<double> part((size_t)Mz * My * Mx, result);
+    for3(Mz, My, Mx, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i) {
+        double r = 1e308 * 10;
+        for (int kk = 0; kk < W.nz; kk++)
+            for (int jj = 0; jj < W.ny; jj++)
+                for (int ii = 0; ii < W.nx; ii++) {
+                    double c_s, c_max = 0, dx_min = 1, dt_min;
+                    c_s = sound_speed(W(0, 0, k, j, i, kk, jj, ii), W(0, P_, k, j, i, kk, jj, ii), gm);
+                    if (ax) { c_max += (std::fabs(W(0, VX, k, j, i, kk, jj, ii)) + c_s); dx_min = smin(dx_min, dx); }
+                    if (ay) { c_max += (std::fabs(W(0, VY, k, j, i, kk, jj, ii)) + c_s); dx_min = smin(dx_min, dy); }
+                    if (az) { c_max += (std::fabs(W(0, VZ, k, j, i, kk, jj, ii)) + c_s); dx_min = smin(dx_min, dz); }
+                    dt_min = cfl * dx_min / c_max / px;
+                    if (nu > 0.0) {
+                        double dx_sub = dx_min / px;
+                        double dt_visc = 0.25 * cfl * dx_sub * dx_sub / nu;
+                        dt_min = dt_min < dt_visc ? dt_min : dt_visc;
+                    }
+                    r = r < dt_min ? r : dt_min;
+                }
+        part[((size_t)(k - cfg.ngh[2]) * My + (j - cfg.ngh[1])) * Mx + (i - cfg.ngh[0])] = r;
+    });
+    for (double r : part) result = result < r ? result : r;
+    return result;
+}
+
+// src/hydro.cpp:270-289
+static inline void riemann_llf(double* F, const double* U_L, const double* U_R, int v1, int v2, int v3, double gm) {
+    double W_L[NVAR] = {0}, W_R[NVAR] = {0}, F_L[NVAR] = {0}, F_R[NVAR] = {0};
+    primitives(U_L, W_L, gm);
+    primitives(U_R, W_R, gm);
+    fluxes(U_L, W_L, F_L, v1, v2, v3);
+    fluxes(U_R, W_R, F_R, v1, v2, v3);
+    double c_L = sound_speed(W_L[0], W_L[P_], gm) + std::fabs(W_L[v1]);
+    double c_R = sound_speed(W_R[0], W_R[P_], gm) + std::fabs(W_R[v1]);
+    double c_max = smax(c_L, c_R);
+    for (int v = 0; v < NVAR; v++) F[v] = 0.5 * (F_R[v] + F_L[v]) - 0.5 * c_max * (U_R[v] - U_L[v]);
+}
+
+// src/hydro.cpp:368-412: one thread per element face, writes both sides.
+static void sd_riemann(const Config& cfg, const SDA& U, SDA& F, int D) {
+    double gm = cfg.gamma;
+    int v1, v2, v3;
+    vel_idx(D, v1, v2, v3);
+    int Nx = U.Nx - (D == 0), Ny = U.Ny - (D == 1), Nz = U.Nz - (D == 2);
+    int px = D == 0 ? 1 : U.nx, py = D == 1 ? 1 : U.ny, pz = D == 2 ? 1 : U.nz;
+    int n = choose(D, U.nx, U.ny, U.nz);
+    for6(Nz, Ny, Nx, pz, py, px, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        int kL = k, jL = j, iL = i, kkL = kk, jjL = jj, iiL = ii;
+        int kR = k, jR = j, iR = i, kkR = kk, jjR = jj, iiR = ii;
+        if (D == 0) { iR = i + 1; iiL = n - 1; iiR = 0; }
+        if (D == 1) { jR = j + 1; jjL = n - 1; jjR = 0; }
+        if (D == 2) { kR = k + 1; kkL = n - 1; kkR = 0; }
+        for (int t = 0; t < U.nt; t++) {
+            double uL[NVAR], uR[NVAR], f[NVAR] = {0};
+            for (int v = 0; v < NVAR; v++) {
+                uL[v] = U(t, v, kL, jL, iL, kkL, jjL, iiL);
+                uR[v] = U(t, v, kR, jR, iR, kkR, jjR, iiR);
+            }
+            riemann_llf(f, uL, uR, v1, v2, v3, gm);
+            for (int v = 0; v < NVAR; v++) {
+                F(t, v, kL, jL, iL, kkL, jjL, iiL) = f[v];
+                F(t, v, kR, jR, iR, kkR, jjR, iiR) = f[v];
+            }
+        }
+    });
+}
+
+// src/boundary.cpp:48-113: fill the interface points of the two ghost
+// elements along `dim` of an fp array staggered along `dim`.
+static void sd_boundaries(const Config& cfg, SDA& U, int dim) {
+    int type = cfg.bc[dim];
+    int N = choose(dim, U.Nx, U.Ny, U.Nz);  // N_total along dim
+    int n = choose(dim, U.nx, U.ny, U.nz);  // n_fp along dim
+    int Mx = dim == 0 ? 1 : U.Nx, My = dim == 1 ? 1 : U.Ny, Mz = dim == 2 ? 1 : U.Nz;
+    int px = dim == 0 ? 1 : U.nx, py = dim == 1 ? 1 : U.ny, pz = dim == 2 ? 1 : U.nz;
+    auto at = [&](int t, int v, int k, int j, int i, int kk, int jj, int ii, int l, int ll) -> double& {
+        if (dim == 0) return U(t, v, k, j, l, kk, jj, ll);
+        if (dim == 1) return U(t, v, k, l, i, kk, ll, ii);
+        return U(t, v, l, j, i, ll, jj, ii);
+    };
+    for6(Mz, My, Mx, pz, py, px, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < U.nt; t++)
+            for (int v = 0; v < U.nv; v++) {
+                double BL, BR;
+                if (type == BC_PERIODIC) {
+                    BL = at(t, v, k, j, i, kk, jj, ii, N - 2, n - 1);
+                    BR = at(t, v, k, j, i, kk, jj, ii, 1, 0);
+                } else if (type == BC_GRADFREE) {
+                    BL = at(t, v, k, j, i, kk, jj, ii, 1, 0);
+                    BR = at(t, v, k, j, i, kk, jj, ii, N - 2, n - 1);
+                } else {
+                    double sgn = (v == 1 + dim) ? -1.0 : 1.0;
+                    BL = sgn * at(t, v, k, j, i, kk, jj, ii, 1, 0);
+                    BR = sgn * at(t, v, k, j, i, kk, jj, ii, N - 2, n - 1);
+                }
+                at(t, v, k, j, i, kk, jj, ii, 0, n - 1) = BL;
+                at(t, v, k, j, i, kk, jj, ii, N - 1, 0) = BR;
+            }
+    });
+}
+
+// src/transforms.cpp:17-27
+static inline double gravity_source(const SDA& U, int var, int t, int k, int j, int i, int kk, int jj, int ii,
+                                    double gx, double gy, double gz) {
+    if (var == VX) return U(t, D_, k, j, i, kk, jj, ii) * gx;
+    if (var == VY) return U(t, D_, k, j, i, kk, jj, ii) * gy;
+    if (var == VZ) return U(t, D_, k, j, i, kk, jj, ii) * gz;
+    if (var == P_)
+        return U(t, VX, k, j, i, kk, jj, ii) * gx + U(t, VY, k, j, i, kk, jj, ii) * gy + U(t, VZ, k, j, i, kk, jj, ii) * gz;
+    return 0.0;
+}
+
+// src/transforms.cpp:263-325: Picard update of the space-time predictor.
+static void update_prediction(const Config& cfg, const SDA& U, SDA& U_ader, const SDA& Fx, const SDA& Fy, const SDA& Fz,
+                              const Mat& Dm, const Mat& invader, const std::vector<double>& w, double dx, double dy,
+                              double dz, double dt) {
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    int q = Fx.nx, nader = Fx.nt, nvar = U.nv;
+    double gx = cfg.g[0], gy = cfg.g[1], gz = cfg.g[2];
+    bool grav = (gx != 0.0 || gy != 0.0 || gz != 0.0);
+    for6(U.Nz, U.Ny, U.Nx, U.nz, U.ny, U.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int var = 0; var < nvar; var++) {
+            double dudt[10];
+            double u_old = U(0, var, k, j, i, kk, jj, ii);
+            for (int t = 0; t < nader; t++) {
+                dudt[t] = 0;
+                for (int ll = 0; ll < q; ll++) {
+                    if (ax) dudt[t] += Fx(t, var, k, j, i, kk, jj, ll) * Dm(ii, ll) / dx;
+                    if (ay) dudt[t] += Fy(t, var, k, j, i, kk, ll, ii) * Dm(jj, ll) / dy;
+                    if (az) dudt[t] += Fz(t, var, k, j, i, ll, jj, ii) * Dm(kk, ll) / dz;
+                }
+                if (grav) dudt[t] -= gravity_source(U_ader, var, t, k, j, i, kk, jj, ii, gx, gy, gz);
+            }
+            for (int t = 0; t < nader; t++) {
+                double du = 0;
+                for (int ll = 0; ll < nader; ll++) du += dudt[ll] * invader(t, ll) * w[ll] * dt;
+                U_ader(t, var, k, j, i, kk, jj, ii) = u_old - du;
+            }
+        }
+    });
+}
+
+// src/transforms.cpp:327-374: conservative corrector over all elements.
+static void update_solution(const Config& cfg, SDA& U, const SDA& U_ader, const SDA& Fx, const SDA& Fy, const SDA& Fz,
+                            const Mat& Dm, const std::vector<double>& w, double dx, double dy, double dz, double dt) {
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    int q = Fx.nx, nader = Fx.nt, nvar = U.nv;
+    double gx = cfg.g[0], gy = cfg.g[1], gz = cfg.g[2];
+    bool grav = (gx != 0.0 || gy != 0.0 || gz != 0.0);
+    for6(U.Nz, U.Ny, U.Nx, U.nz, U.ny, U.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int var = 0; var < nvar; var++) {
+            double du = 0;
+            for (int t = 0; t < nader; t++) {
+                double dudt = 0;
+                for (int ll = 0; ll < q; ll++) {
+                    if (ax) dudt += Fx(t, var, k, j, i, kk, jj, ll) * Dm(ii, ll) / dx;
+                    if (ay) dudt += Fy(t, var, k, j, i, kk, ll, ii) * Dm(jj, ll) / dy;
+                    if (az) dudt += Fz(t, var, k, j, i, ll, jj, ii) * Dm(kk, ll) / dz;
+                }
+                if (grav) dudt -= gravity_source(U_ader, var, t, k, j, i, kk, jj, ii, gx, gy, gz);
+                du += dudt * w[t] * dt;
+            }
+            U(0, var, k, j, i, kk, jj, ii) -= du;
+        }
+    });
+}
+
+// src/hydro_ader.hpp:428-444
+static void copy_ader(const SDA& U, SDA& U_ader) {
+    for6(U.Nz, U.Ny, U.Nx, U.nz, U.ny, U.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < U_ader.nt; t++)
+            for (int v = 0; v < U.nv; v++) U_ader(t, v, k, j, i, kk, jj, ii) = U(0, v, k, j, i, kk, jj, ii);
+    });
+}
+
+// src/transforms.cpp:252-261
+static void combine_solution(SDA& U, const SDA& U0, double a) {
+    for6(U.Nz, U.Ny, U.Nx, U.nz, U.ny, U.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int v = 0; v < U.nv; v++)
+            U(0, v, k, j, i, kk, jj, ii) = a * U0(0, v, k, j, i, kk, jj, ii) + (1.0 - a) * U(0, v, k, j, i, kk, jj, ii);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// FV fallback
+// Global FV cell index of (element, point): src/define.hpp:87-89
+#define ORC_IJK                                                    \
+    int I = ii + cfg.fgh[0] + (i - cfg.ngh[0]) * qx;               \
+    int J = jj + cfg.fgh[1] + (j - cfg.ngh[1]) * qy;               \
+    int K = kk + cfg.fgh[2] + (k - cfg.ngh[2]) * qz;
+
+// src/transforms.cpp:378-427 (face_integral_ref) and :435-481 (face_integral)
+static void face_integral(const Config& cfg, const SDA& Ffp, FVA& F, SDA& T, const Mat& sp_to_cv, int t_id, int dim) {
+    int dim1 = choose(dim, 1, 2, 0), dim2 = choose(dim1, 1, 2, 0);
+    int Nx = Ffp.Nx + (dim == 0), Ny = Ffp.Ny + (dim == 1), Nz = Ffp.Nz + (dim == 2);
+    int px = Ffp.nx - (dim == 0), py = Ffp.ny - (dim == 1), pz = Ffp.nz - (dim == 2);
+    int qx = px, qy = py, qz = pz;
+    int Mz = Nz - 2 * cfg.ngh[2], My = Ny - 2 * cfg.ngh[1], Mx = Nx - 2 * cfg.ngh[0];
+    int Ni = F.Nx, Nj = F.Ny, Nk = F.Nz;
+    auto node = [&](int d, int l, int kk, int jj, int ii, int& c, int& b, int& a) {
+        a = d == 0 ? l : ii; b = d == 1 ? l : jj; c = d == 2 ? l : kk;
+    };
+    if (!cfg.act[dim1] || !cfg.act[dim2]) {
+        int q1 = choose(dim1, px, py, pz), q2 = choose(dim2, px, py, pz);
+        bool a1 = cfg.act[dim1], a2 = cfg.act[dim2];
+        for6(Mz, My, Mx, pz, py, px, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+            int id1 = choose(dim1, ii, jj, kk), id2 = choose(dim2, ii, jj, kk);
+            ORC_IJK
+            for (int v = 0; v < Ffp.nv; v++) {
+                double f = 0;
+                for (int nn = 0; nn < q2; nn++) {
+                    int c0, b0, a0;
+                    node(dim2, nn, kk, jj, ii, c0, b0, a0);
+                    for (int ll = 0; ll < q1; ll++) {
+                        int c, b, a;
+                        node(dim1, ll, c0, b0, a0, c, b, a);
+                        double s = Ffp(t_id, v, k, j, i, c, b, a);
+                        if (a1) s *= sp_to_cv(id1, ll);
+                        if (a2) s *= sp_to_cv(id2, nn);
+                        f += s;
+                    }
+                }
+                if (K < Nk && J < Nj && I < Ni) F(v, K, J, I) = f;
+            }
+        });
+        return;
+    }
+    transform_1d(Ffp, T, sp_to_cv, dim1, t_id);
+    int q2 = choose(dim2, px, py, pz);
+    for6(Mz, My, Mx, pz, py, px, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+        int id2 = choose(dim2, ii, jj, kk);
+        ORC_IJK
+        for (int v = 0; v < Ffp.nv; v++) {
+            double f = 0;
+            for (int nn = 0; nn < q2; nn++) {
+                int c, b, a;
+                node(dim2, nn, kk, jj, ii, c, b, a);
+                f += T(0, v, k, j, i, c, b, a) * sp_to_cv(id2, nn);
+            }
+            if (K < Nk && J < Nj && I < Ni) F(v, K, J, I) = f;
+        }
+    });
+}
+
+// src/transforms.cpp:483-543
+static void fv_update_solution(const Config& cfg, FVA& U_new, FVA& U_old, SDA& U_cv, const FVA& Fx, const std::vector<double>& fx,
+                               const FVA& Fy, const std::vector<double>& fy, const FVA& Fz, const std::vector<double>& fz,
+                               const std::vector<double>& w, int t_id, double dt, bool update) {
+    int px = U_cv.nx, py = U_cv.ny, pz = U_cv.nz;
+    int qx = px, qy = py, qz = pz;
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    double gx = cfg.g[0], gy = cfg.g[1], gz = cfg.g[2];
+    bool grav = (gx != 0.0 || gy != 0.0 || gz != 0.0);
+    int Mz = U_cv.Nz - 2 * cfg.ngh[2], My = U_cv.Ny - 2 * cfg.ngh[1], Mx = U_cv.Nx - 2 * cfg.ngh[0];
+    for6(Mz, My, Mx, pz, py, px, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+        ORC_IJK
+        for (int v = 0; v < U_cv.nv; v++) {
+            double h, F = 0;
+            U_old(v, K, J, I) = U_cv(0, v, k, j, i, kk, jj, ii);
+            if (ax) { h = fx[I + 1] - fx[I]; F += (Fx(v, K, J, I + 1) - Fx(v, K, J, I)) / h; }
+            if (ay) { h = fy[J + 1] - fy[J]; F += (Fy(v, K, J + 1, I) - Fy(v, K, J, I)) / h; }
+            if (az) { h = fz[K + 1] - fz[K]; F += (Fz(v, K + 1, J, I) - Fz(v, K, J, I)) / h; }
+            if (grav) F -= gravity_source(U_cv, v, 0, k, j, i, kk, jj, ii, gx, gy, gz);
+            double u_new = U_old(v, K, J, I) - w[t_id] * dt * F;
+            U_new(v, K, J, I) = u_new;
+            if (update) U_cv(0, v, k, j, i, kk, jj, ii) = u_new;
+        }
+    });
+}
+
+// src/boundary.cpp:123-190 for cell-centred FV fields (alignment = centre).
+static void fv_boundaries(const Config& cfg, FVA& U, int dim) {
+    int type = cfg.bc[dim];
+    int N = choose(dim, U.Nx, U.Ny, U.Nz);
+    int Nx = dim == 0 ? nGH : U.Nx, Ny = dim == 1 ? nGH : U.Ny, Nz = dim == 2 ? nGH : U.Nz;
+    auto at = [&](int v, int k, int j, int i, int l) -> double& {
+        if (dim == 0) return U(v, k, j, l);
+        if (dim == 1) return U(v, k, l, i);
+        return U(v, l, j, i);
+    };
+    std::vector<double> BL((size_t)U.nv * Nz * Ny * Nx), BR(BL.size());
+    auto bidx = [&](int v, int k, int j, int i) { return (((size_t)v * Nz + k) * Ny + j) * Nx + i; };
+    for3(Nz, Ny, Nx, 0, 0, 0, [&](int k, int j, int i) {
+        int l = choose(dim, i, j, k);
+        for (int v = 0; v < U.nv; v++) {
+            if (type == BC_PERIODIC) {
+                BL[bidx(v, k, j, i)] = at(v, k, j, i, N - 2 * nGH + l);
+                BR[bidx(v, k, j, i)] = at(v, k, j, i, nGH + l);
+            } else if (type == BC_GRADFREE) {
+                BL[bidx(v, k, j, i)] = at(v, k, j, i, nGH + l);
+                BR[bidx(v, k, j, i)] = at(v, k, j, i, N - 2 * nGH + l);
+            } else {
+                double sgn = (v == 1 + dim) ? -1.0 : 1.0;
+                BL[bidx(v, k, j, i)] = sgn * at(v, k, j, i, 2 * nGH - 1 - l);
+                BR[bidx(v, k, j, i)] = sgn * at(v, k, j, i, N - nGH - 1 - l);
+            }
+        }
+    });
+    for3(Nz, Ny, Nx, 0, 0, 0, [&](int k, int j, int i) {
+        int l = choose(dim, i, j, k);
+        for (int v = 0; v < U.nv; v++) {
+            at(v, k, j, i, l) = BL[bidx(v, k, j, i)];
+            at(v, k, j, i, N - nGH + l) = BR[bidx(v, k, j, i)];
+        }
+    });
+}
+static void apply_fv_boundaries(const Config& cfg, FVA& U) {
+    for (int d = 0; d < 3; d++)
+        if (cfg.act[d]) fv_boundaries(cfg, U, d);
+}
+
+// src/trouble_detection.cpp:13-31
+static inline double Alpha(double dv, double dUm, double dU, double dUp) {
+    double vL = dU - dUm;
+    double alpha_p = smin(1.0, smax(vL, 0.0) / dv);
+    double alpha_m = smin(1.0, smin(vL, 0.0) / dv);
+    double alphaL = (dv < 0.0 ? alpha_m : alpha_p);
+    alphaL = (dv == 0.0 ? 1.0 : alphaL);
+    double vR = dUp - dU;
+    alpha_p = smin(1.0, smax(vR, 0.0) / dv);
+    alpha_m = smin(1.0, smin(vR, 0.0) / dv);
+    double alphaR = (dv < 0.0 ? alpha_m : alpha_p);
+    alphaR = (dv == 0.0 ? 1.0 : alphaR);
+    return smin(alphaL, alphaR);
+}
+
+// src/trouble_detection.cpp:35-104 (cells 1..N-2 of every active axis)
+static void NAD(const Config& cfg, const FVA& U_new, const FVA& U, FVA& tr, int mask) {
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    double tol = cfg.nad_tol;
+    int nvar = U.nv - 1;
+    int Mz = U.Nz - 2 * cfg.ngh[2], My = U.Ny - 2 * cfg.ngh[1], Mx = U.Nx - 2 * cfg.ngh[0];
+    for3(Mz, My, Mx, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i) {
+        for (int v = 0; v < nvar; v++) {
+            if (!((mask >> v) & 1)) continue;
+            double u_new = U_new(v, k, j, i);
+            double mx = U(v, k, j, i), mn = U(v, k, j, i);
+            if (cfg.nad_moore) {
+                for (int dk = -(int)az; dk <= (int)az; dk++)
+                    for (int dj = -(int)ay; dj <= (int)ay; dj++)
+                        for (int di = -(int)ax; di <= (int)ax; di++) {
+                            double u = U(v, k + dk, j + dj, i + di);
+                            mx = smax(mx, u);
+                            mn = smin(mn, u);
+                        }
+            } else {
+                double uL, uR;
+                if (ax) { uL = U(v, k, j, i - 1); uR = U(v, k, j, i + 1); mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (ay) { uL = U(v, k, j - 1, i); uR = U(v, k, j + 1, i); mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (az) { uL = U(v, k - 1, j, i); uR = U(v, k + 1, j, i); mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+            }
+            if (cfg.nad_delta) {
+                double eps = tol * (mx - mn);
+                mn -= eps;
+                mx += eps;
+            } else {
+                mn -= std::fabs(mn) * tol;
+                mx += std::fabs(mx) * tol;
+            }
+            tr(v, k, j, i) = (u_new > mx || u_new < mn) ? 1 : 0;
+        }
+    });
+}
+
+// src/trouble_detection.cpp:107-148 (cells 2..N-3 of every active axis)
+static void smooth_extrema(const Config& cfg, const FVA& U, FVA& alpha, const std::vector<double>& c,
+                           const std::vector<double>& f, int dim, int mask) {
+    int nvar = U.nv - 1;
+    int Mz = U.Nz - 4 * cfg.ngh[2], My = U.Ny - 4 * cfg.ngh[1], Mx = U.Nx - 4 * cfg.ngh[0];
+    for3(Mz, My, Mx, 2 * cfg.ngh[2], 2 * cfg.ngh[1], 2 * cfg.ngh[0], [&](int k, int j, int i) {
+        for (int v = 0; v < nvar; v++) {
+            if (!((mask >> v) & 1)) continue;
+            double u = U(v, k, j, i);
+            int l = choose(dim, i, j, k);
+            double h = (c[l + 1] - c[l - 1]);
+            double hp = (c[l + 2] - c[l]);
+            double hm = (c[l] - c[l - 2]);
+            double up1, um1, up2, um2;
+            if (dim == 0) { up1 = U(v, k, j, i + 1); um1 = U(v, k, j, i - 1); up2 = U(v, k, j, i + 2); um2 = U(v, k, j, i - 2); }
+            else if (dim == 1) { up1 = U(v, k, j + 1, i); um1 = U(v, k, j - 1, i); up2 = U(v, k, j + 2, i); um2 = U(v, k, j - 2, i); }
+            else { up1 = U(v, k + 1, j, i); um1 = U(v, k - 1, j, i); up2 = U(v, k + 2, j, i); um2 = U(v, k - 2, j, i); }
+            double du = (up1 - um1) / h;
+            double dup = (up2 - u) / hp;
+            double dum = (u - um2) / hm;
+            double d2u = (dup - dum) / (c[l + 1] - c[l - 1]);
+            double dv = 0.5 * d2u * (f[l + 1] - f[l]);
+            alpha(v, k, j, i) = Alpha(dv, dum, du, dup);
+        }
+    });
+}
+
+// src/trouble_detection.cpp:152-183
+static void relax_NAD(const Config& cfg, FVA& tr, const FVA& ax_, const FVA& ay_, const FVA& az_, int mask, bool use_sed) {
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    int nvar = tr.nv - 1;
+    int Mz = tr.Nz - 2 * cfg.ngh[2], My = tr.Ny - 2 * cfg.ngh[1], Mx = tr.Nx - 2 * cfg.ngh[0];
+    for3(Mz, My, Mx, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i) {
+        for (int v = 0; v < nvar; v++) {
+            if (!((mask >> v) & 1)) continue;
+            double alpha = 1;
+            if (ax) alpha = smin(smin(ax_(v, k, j, i - 1), ax_(v, k, j, i)), ax_(v, k, j, i + 1));
+            if (ay) alpha = smin(alpha, smin(smin(ay_(v, k, j - 1, i), ay_(v, k, j, i)), ay_(v, k, j + 1, i)));
+            if (az) alpha = smin(alpha, smin(smin(az_(v, k - 1, j, i), az_(v, k, j, i)), az_(v, k + 1, j, i)));
+            double trouble = tr(v, k, j, i);
+            if (use_sed) tr(v, k, j, i) = alpha < 1 ? trouble : 0;
+        }
+        double trouble = 0;
+        for (int v = 0; v < nvar; v++)
+            if ((mask >> v) & 1) trouble = smax(trouble, tr(v, k, j, i));
+        tr(nvar, k, j, i) = trouble;
+    });
+}
+
+// src/trouble_detection.cpp:244-259
+static void PAD_criteria(const Config& cfg, const FVA& W, FVA& tr) {
+    int nvar = W.nv - 1;
+    int Mz = W.Nz - 2 * cfg.ngh[2], My = W.Ny - 2 * cfg.ngh[1], Mx = W.Nx - 2 * cfg.ngh[0];
+    for3(Mz, My, Mx, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i) {
+        double d = W(D_, k, j, i), p = W(P_, k, j, i);
+        if (d < RHO_MIN || d > RHO_MAX) tr(nvar, k, j, i) = 1;
+        if (p < P_MIN || p > P_MAX) tr(nvar, k, j, i) = 1;
+    });
+}
+
+// src/trouble_detection.cpp:261-290
+static void detect_troubles(const Config& cfg, const FVA& Wn, const FVA& Wo, FVA& tr, FVA& alx, FVA& aly, FVA& alz,
+                            const Dim& X, const Dim& Y, const Dim& Z, bool PAD, int mask) {
+    bool use_sed = cfg.sed && X.p > 1;
+    NAD(cfg, Wn, Wo, tr, mask);
+    if (use_sed) {
+        if (cfg.act[0]) smooth_extrema(cfg, Wn, alx, X.fvc, X.fvf, 0, mask);
+        if (cfg.act[1]) smooth_extrema(cfg, Wn, aly, Y.fvc, Y.fvf, 1, mask);
+        if (cfg.act[2]) smooth_extrema(cfg, Wn, alz, Z.fvc, Z.fvf, 2, mask);
+    }
+    relax_NAD(cfg, tr, alx, aly, alz, mask, use_sed);
+    if (PAD) PAD_criteria(cfg, Wn, tr);
+}
+
+// src/trouble_detection.cpp:191-210
+static void apply_blending(const Config& cfg, const FVA& tr, FVA& th) {
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    int nvar = tr.nv - 1;
+    int Mz = tr.Nz - 2 * cfg.ngh[2], My = tr.Ny - 2 * cfg.ngh[1], Mx = tr.Nx - 2 * cfg.ngh[0];
+    for3(Mz, My, Mx, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i) {
+        const double w[4] = {1.0, 0.75, 0.5, 0.375};
+        double t = 0;
+        for (int dk = -(int)az; dk <= (int)az; dk++)
+            for (int dj = -(int)ay; dj <= (int)ay; dj++)
+                for (int di = -(int)ax; di <= (int)ax; di++) {
+                    int n0 = (di != 0) + (dj != 0) + (dk != 0);
+                    t = smax(t, w[n0] * tr(nvar, k + dk, j + dj, i + di));
+                }
+        th(0, k, j, i) = t;
+    });
+}
+// src/trouble_detection.cpp:213-221
+static void theta_from_troubles(const FVA& tr, FVA& th) {
+    int nvar = tr.nv - 1;
+    for3(tr.Nz, tr.Ny, tr.Nx, 0, 0, 0, [&](int k, int j, int i) { th(0, k, j, i) = tr(nvar, k, j, i); });
+}
+// src/trouble_detection.cpp:226-242
+static void blending_ring(const Config& cfg, const FVA& ti, FVA& to) {
+    bool ax = cfg.act[0], ay = cfg.act[1], az = cfg.act[2];
+    int Mz = ti.Nz - 2 * cfg.ngh[2], My = ti.Ny - 2 * cfg.ngh[1], Mx = ti.Nx - 2 * cfg.ngh[0];
+    for3(Mz, My, Mx, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i) {
+        double t = ti(0, k, j, i), ring = 0;
+        for (int dk = -(int)az; dk <= (int)az; dk++)
+            for (int dj = -(int)ay; dj <= (int)ay; dj++)
+                for (int di = -(int)ax; di <= (int)ax; di++)
+                    if (ti(0, k + dk, j + dj, i + di) > 0) ring = 0.25;
+        to(0, k, j, i) = smax(t, ring);
+    });
+}
+
+// src/muscl.hpp:2-13
+static inline double minmod(double SL, double SR, double xL, double xR) {
+    double ratio = SR / SL;
+    ratio = smax(0.0, smin(ratio, 1.0));
+    double slope = ratio * SL;
+    return 0.5 * slope * (xR - xL);
+}
+
+// src/hydro.cpp:591-615
+static inline void corrector(const double* W, double* dWt, const double* dWx, const double* dWy, const double* dWz, double gm) {
+    dWt[D_] = 0;
+    dWt[P_] = 0;
+    dWt[D_] -= W[VX] * dWx[D_] + W[D_] * dWx[VX];
+    dWt[VX] = -W[VX] * dWx[VX] - dWx[P_] / W[D_];
+    dWt[P_] -= W[VX] * dWx[P_] + dWx[VX] * gm * W[P_];
+    dWt[VX] -= W[VY] * dWy[VX];
+    dWt[VX] -= W[VZ] * dWz[VX];
+    dWt[D_] -= W[VY] * dWy[D_] + W[D_] * dWy[VY];
+    dWt[VY] = -W[VY] * dWy[VY] - dWy[P_] / W[D_];
+    dWt[P_] -= W[VY] * dWy[P_] + dWy[VY] * gm * W[P_];
+    dWt[VY] -= W[VX] * dWx[VY];
+    dWt[VY] -= W[VZ] * dWz[VY];
+    dWt[D_] -= W[VZ] * dWz[D_] + W[D_] * dWz[VZ];
+    dWt[VZ] = -W[VZ] * dWz[VZ] - dWz[P_] / W[D_];
+    dWt[P_] -= W[VZ] * dWz[P_] + dWz[VZ] * gm * W[P_];
+    dWt[VZ] -= W[VX] * dWx[VZ];
+    dWt[VZ] -= W[VY] * dWy[VZ];
+}
+
+// src/hydro.cpp:291-363. The slot (index 5) of F is left unset by the
+// reference; it is zeroed here (it only ever feeds the unused slot).
+static inline void riemann_hllc(double* F, const double* U_L, const double* U_R, int v1, int v2, int v3, double gm) {
+    double W_L[NVAR] = {0}, W_R[NVAR] = {0};
+    primitives(U_L, W_L, gm);
+    primitives(U_R, W_R, gm);
+    double c_L = sound_speed(W_L[D_], W_L[P_], gm) + std::fabs(W_L[v1]);
+    double c_R = sound_speed(W_R[D_], W_R[P_], gm) + std::fabs(W_R[v1]);
+    double c_max = smax(c_L, c_R);
+    double v_L = W_L[v1], v_R = W_R[v1];
+    double s_L = smin(v_L, v_R) - c_max;
+    double s_R = smax(v_L, v_R) + c_max;
+    double rc_L = W_L[0] * (v_L - s_L);
+    double rc_R = W_R[0] * (s_R - v_R);
+    double v_star = (rc_R * v_R + rc_L * v_L + (W_L[P_] - W_R[P_])) / (rc_R + rc_L);
+    double p_star = (rc_R * W_L[P_] + rc_L * W_R[P_] + rc_L * rc_R * (v_L - v_R)) / (rc_R + rc_L);
+    double r_starL = W_L[D_] * (s_L - v_L) / (s_L - v_star);
+    double e_starL = ((s_L - v_L) * U_L[E_] - W_L[P_] * v_L + p_star * v_star) / (s_L - v_star);
+    double r_starR = W_R[0] * (s_R - v_R) / (s_R - v_star);
+    double e_starR = ((s_R - v_R) * U_R[E_] - W_R[P_] * v_R + p_star * v_star) / (s_R - v_star);
+    double r_g, v_g, p_g, e_g;
+    if (s_L > 0) { r_g = W_L[D_]; v_g = W_L[v1]; p_g = W_L[P_]; e_g = U_L[E_]; }
+    else if (v_star > 0) { r_g = r_starL; v_g = v_star; p_g = p_star; e_g = e_starL; }
+    else if (s_R > 0) { r_g = r_starR; v_g = v_star; p_g = p_star; e_g = e_starR; }
+    else { r_g = W_R[D_]; v_g = W_R[v1]; p_g = W_R[P_]; e_g = U_R[E_]; }
+    F[D_] = r_g * v_g;
+    F[v1] = r_g * v_g * v_g + p_g;
+    F[v2] = r_g * v_g * (v_star > 0 ? W_L[v2] : W_R[v2]);
+    F[v3] = r_g * v_g * (v_star > 0 ? W_L[v3] : W_R[v3]);
+    F[P_] = v_g * (e_g + p_g);
+    F[5] = 0.0;
+}
+
+struct FVGeom { const std::vector<double> *xc, *xf, *yc, *yf, *zc, *zf; };
+
+// src/hydro.cpp:622-681 (slopes_d<D>). The slot carries no slope: its
+// Hancock term is zero (the reference leaves it unset; it only feeds the slot).
+static void slopes_d(int D, const FVA& W, const FVGeom& g, int k, int j, int i, double* WL, double* WR, double dt,
+                     bool ay, bool az, double gm) {
+    double w[NVAR], dwx[NVAR], dwy[NVAR], dwz[NVAR], dwt[NVAR];
+    const std::vector<double>&xc = *g.xc, &xf = *g.xf, &yc = *g.yc, &yf = *g.yf, &zc = *g.zc, &zf = *g.zf;
+    for (int v = 0; v < NVAR; v++) {
+        w[v] = W(v, k, j, i);
+        double wL = W(v, k, j, i - 1), wR = W(v, k, j, i + 1);
+        dwx[v] = minmod((wR - w[v]) / (xc[i + 1] - xc[i]), (w[v] - wL) / (xc[i] - xc[i - 1]), xf[i], xf[i + 1]);
+        dwy[v] = 0;
+        dwz[v] = 0;
+        if (ay) {
+            wL = W(v, k, j - 1, i);
+            wR = W(v, k, j + 1, i);
+            dwy[v] = minmod((wR - w[v]) / (yc[j + 1] - yc[j]), (w[v] - wL) / (yc[j] - yc[j - 1]), yf[j], yf[j + 1]);
+        }
+        if (az) {
+            wL = W(v, k - 1, j, i);
+            wR = W(v, k + 1, j, i);
+            dwz[v] = minmod((wR - w[v]) / (zc[k + 1] - zc[k]), (w[v] - wL) / (zc[k] - zc[k - 1]), zf[k], zf[k + 1]);
+        }
+    }
+    dwt[5] = 0.0;
+    corrector(w, dwt, dwx, dwy, dwz, gm);
+    for (int v = 0; v < NVAR; v++) {
+        double h = (D == 0 ? xf[i + 1] - xf[i] : (D == 1 ? yf[j + 1] - yf[j] : zf[k + 1] - zf[k]));
+        double dw = (D == 0 ? dwx[v] : (D == 1 ? dwy[v] : dwz[v]));
+        WR[v] = w[v] - dw + dwt[v] * dt / h;
+        WL[v] = w[v] + dw + dwt[v] * dt / h;
+    }
+}
+
+// src/hydro.cpp:683-751 (compute_fluxes<D>)
+static void fb_face_flux(int D, const FVA& W, FVA& F, const FVA& theta, const FVGeom& g, int k, int j, int i, double dt,
+                         bool ay, bool az, double gm) {
+    double wL[2][NVAR], wR[2][NVAR], uL[NVAR] = {0}, uR[NVAR] = {0}, fL[NVAR];
+    int v1, v2, v3;
+    vel_idx(D, v1, v2, v3);
+    for (int l = -1; l < 1; l++)
+        slopes_d(D, W, g, k + (D == 2 ? l : 0), j + (D == 1 ? l : 0), i + (D == 0 ? l : 0), wL[l + 1], wR[l + 1], dt, ay,
+                 az, gm);
+    conservatives(wL[0], uL, gm);
+    conservatives(wR[1], uR, gm);
+    riemann_hllc(fL, uL, uR, v1, v2, v3, gm);
+    double th = smax(theta(0, k, j, i), theta(0, k - (D == 2), j - (D == 1), i - (D == 0)));
+    for (int v = 0; v < NVAR; v++) {
+        double f = F(v, k, j, i);
+        f = f + th * (fL[v] - f);
+        F(v, k, j, i) = f;
+    }
+}
+
+// src/hydro.cpp:753-788 (faces 2..N-2 of every active axis)
+static void fallback_fluxes(const Config& cfg, const FVA& W, const FVA& theta, const FVGeom& g, FVA& Fx, FVA& Fy, FVA& Fz,
+                            double dt) {
+    double gm = cfg.gamma;
+    bool ay = cfg.act[1], az = cfg.act[2];
+    int Mz = W.Nz - 2 * cfg.fgh[2] + (cfg.fgh[2] > 0), My = W.Ny - 2 * cfg.fgh[1] + (cfg.fgh[1] > 0),
+        Mx = W.Nx - 2 * cfg.fgh[0] + (cfg.fgh[0] > 0);
+    for3(Mz, My, Mx, cfg.fgh[2], cfg.fgh[1], cfg.fgh[0], [&](int k, int j, int i) {
+        fb_face_flux(0, W, Fx, theta, g, k, j, i, dt, ay, az, gm);
+        if (ay) fb_face_flux(1, W, Fy, theta, g, k, j, i, dt, ay, az, gm);
+        if (az) fb_face_flux(2, W, Fz, theta, g, k, j, i, dt, ay, az, gm);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions: src/initial_conditions.cpp:9-228 and src/user_ic.hpp:28-39
+
+static inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+// One N(0,1) sample per FV sub-cell (global index `cell`) by Box-Muller
+// from two splitmix64 draws of (seed, cell). Not in the reference: the
+// seeded Kelvin-Helmholtz perturbation of the 3-D benchmark configuration.
+double normal_sample(uint64_t seed, uint64_t cell) {
+    uint64_t a = splitmix64(seed ^ splitmix64(2 * cell));
+    uint64_t b = splitmix64(seed ^ splitmix64(2 * cell + 1));
+    double u1 = ((double)(a >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+    double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * PI * u2);
+}
+
+static double ic_value(const Config& cfg, int var, double x, double y, double z, double noise) {
+    const Problem& pp = cfg.pp;
+    double gm = cfg.gamma, gy = cfg.g[1];
+    bool ay = cfg.act[1], az = cfg.act[2];
+    switch (cfg.problem) {
+        case IC_SINE:  // :27-35
+            if (var == 0) return pp.d0 + pp.amp * (std::sin(2 * PI * (x + y)));
+            if (var == VX) return pp.v1;
+            if (var == VY) return pp.v2;
+            if (var == VZ) return pp.v3;
+            if (var == P_) return pp.p0;
+            return 0;
+        case IC_SEDOV: {  // :37-53
+            double R = pp.radius;
+            double r = std::sqrt(std::pow(x - 0.5, 2.0) + std::pow(y - 0.5, 2.0) + (az ? std::pow(z - 0.5, 2.0) : 0.0));
+            if (var == 0) return pp.d0;
+            if (var == P_) return r < R ? 1E-6 + (gm - 1) : 1E-6;
+            return 0;
+        }
+        case IC_SPHERICAL_BLAST: {  // :55-75
+            double R = pp.radius;
+            double r = std::sqrt(std::pow(x - pp.cx, 2) + std::pow(y - pp.cy, 2) + (az ? std::pow(z - pp.cz, 2) : 0.0));
+            if (var == 0) return pp.d0;
+            if (var == P_) return r < R ? pp.p0 : pp.p1;
+            return 0;
+        }
+        case IC_SQUARE: {  // :9-25
+            if (var == 0) {
+                bool in = std::fabs(x - 0.5) < pp.radius;
+                if (ay) in = in && std::fabs(y - 0.5) < pp.radius;
+                if (az) in = in && std::fabs(z - 0.5) < pp.radius;
+                return in ? pp.d1 : pp.d0;
+            }
+            if (var == VX) return pp.v1;
+            if (var == VY) return pp.v2;
+            if (var == VZ) return pp.v3;
+            if (var == P_) return pp.p0;
+            return 0;
+        }
+        case IC_SOD: {  // :77-87
+            double s = (pp.dir == 0 ? x : (pp.dir == 1 ? y : z));
+            bool left = s < pp.radius;
+            if (var == 0) return left ? pp.d0 : pp.d1;
+            if (var == P_) return left ? pp.p0 : pp.p1;
+            return 0;
+        }
+        case IC_SHU_OSHER: {  // :89-101
+            double xs = x - 1.0;
+            bool left = xs < -0.8;
+            if (var == 0) return left ? 3.857143 : 1.0 + pp.amp * std::sin(5 * PI * xs);
+            if (var == VX) return left ? 2.629369 : 0.0;
+            if (var == P_) return left ? 10.33333 : 1.0;
+            return 0;
+        }
+        case IC_KH: {  // :103-117; perturb=1 is the seeded single-interface variant
+            bool inner = (y > 0.25 && y < 0.75);
+            if (var == 0) return inner ? pp.d1 : pp.d0;
+            if (var == VX) return inner ? pp.v1 : -pp.v1;
+            if (var == VY) {
+                if (pp.perturb == 1) return pp.amp * noise * std::exp(-std::pow(y - 0.25, 2) / (2 * pp.sigma * pp.sigma));
+                return pp.amp * std::sin(4 * PI * x) *
+                       (std::exp(-std::pow(y - 0.25, 2) / (2 * pp.sigma * pp.sigma)) +
+                        std::exp(-std::pow(y - 0.75, 2) / (2 * pp.sigma * pp.sigma)));
+            }
+            if (var == P_) return pp.p0;
+            return 0;
+        }
+        case IC_IMPLOSION: {  // :119-129
+            bool inside = (x + y) < pp.radius;
+            if (var == 0) return inside ? pp.d1 : pp.d0;
+            if (var == P_) return inside ? pp.p1 : pp.p0;
+            return 0;
+        }
+        case IC_RTI: {  // :131-159
+            double yc = pp.radius, rho_hi = pp.d0, rho_lo = pp.d1, P0 = pp.p0;
+            bool top = y > yc;
+            if (var == D_) return top ? rho_lo : rho_hi;
+            if (var == VY) {
+                double dv = std::sqrt(gm * (rho_hi * yc + 1.0) / rho_hi);
+                return -pp.amp * dv * std::cos(8.0 * PI * x);
+            }
+            if (var == P_) return top ? (P0 + rho_lo * gy * y + (rho_hi - rho_lo) * gy * yc) : (P0 + rho_hi * gy * y);
+            return 0;
+        }
+        case IC_USER: {  // src/user_ic.hpp:28-39
+            double dx = x - pp.radius;
+            double pulse = pp.amp * std::exp(-dx * dx / (2 * pp.sigma * pp.sigma));
+            if (var == 0) return pp.d0;
+            if (var == VX) return pp.v1;
+            if (var == VY) return pulse;
+            if (var == VZ) return 0.0;
+            if (var == P_) return pp.p0;
+            return 0;
+        }
+        case IC_SEDOV_CELLS: {
+            // Not in the reference: energy pp.amp deposited uniformly in the
+            // central 2^d elements (half-width one element), background p1.
+            if (var == 0) return pp.d0;
+            if (var == P_) {
+                bool in = std::fabs(x - pp.cx) < pp.hx;
+                double vol = 2 * pp.hx;
+                if (ay) { in = in && std::fabs(y - pp.cy) < pp.hy; vol *= 2 * pp.hy; }
+                if (az) { in = in && std::fabs(z - pp.cz) < pp.hz; vol *= 2 * pp.hz; }
+                return in ? (gm - 1) * pp.amp / vol : pp.p1;
+            }
+            return 0;
+        }
+    }
+    return 0;
+}
+
+// src/initial_conditions.cpp:179-228: GL(p+1) quadrature average over each
+// control volume [sd_faces(i,ii), sd_faces(i,ii+1)] per axis.
+static void Initialize(const Config& cfg, SDA& W, const Dim& X, const Dim& Y, const Dim& Z, const std::vector<double>& xq,
+                       const std::vector<double>& wq) {
+    bool ay = cfg.act[1], az = cfg.act[2];
+    int px = W.nx, py = W.ny, pz = W.nz;
+    int NXc = X.N * X.nsp, NYc = Y.N * Y.nsp;
+    for6(W.Nz, W.Ny, W.Nx, pz, py, px, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        double noise = 0;
+        if (cfg.problem == IC_KH && cfg.pp.perturb == 1) {
+            auto wrap = [](int e, int ng, int N, int n, int p) { int g = ((e - ng) % N + N) % N; return (long long)g * n + p; };
+            long long gx = wrap(i, cfg.ngh[0], X.N, X.nsp, ii);
+            long long gy = wrap(j, cfg.ngh[1], Y.N, Y.nsp, jj);
+            long long gz = wrap(k, cfg.ngh[2], Z.N, Z.nsp, kk);
+            noise = normal_sample(cfg.pp.seed, (uint64_t)((gz * NYc + gy) * NXc + gx));
+        }
+        for (int t = 0; t < W.nt; t++)
+            for (int var = 0; var < W.nv; var++) {
+                double value = 0, x, y = 0, z = 0;
+                for (int nn = 0; nn < pz; nn++) {
+                    if (az) z = Z.sdf(k, kk) + xq[nn] * (Z.sdf(k, kk + 1) - Z.sdf(k, kk));
+                    for (int mm = 0; mm < py; mm++) {
+                        if (ay) y = Y.sdf(j, jj) + xq[mm] * (Y.sdf(j, jj + 1) - Y.sdf(j, jj));
+                        for (int ll = 0; ll < px; ll++) {
+                            x = X.sdf(i, ii) + xq[ll] * (X.sdf(i, ii + 1) - X.sdf(i, ii));
+                            double s = ic_value(cfg, var, x, y, z, noise);
+                            s *= wq[ll];
+                            if (ay) s *= wq[mm];
+                            if (az) s *= wq[nn];
+                            value += s;
+                        }
+                    }
+                }
+                W(t, var, k, j, i, kk, jj, ii) = value;
+            }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Output: src/output.cpp:4-50, src/structs.hpp:10-16
+std::string output_folder(const char* outdir) {
+    std::string f = outdir ? std::string(outdir) : std::string("output");
+    std::string cmd = f;
+    // mkdir -p
+    for (size_t pos = 1; pos <= cmd.size(); pos++)
+        if (pos == cmd.size() || cmd[pos] == '/') mkdir(cmd.substr(0, pos).c_str(), 0755);
+    return f + "/";
+}
+static void write_raw(const std::string& name, const double* a, size_t n) {
+    std::ofstream o(name, std::ios::binary);
+    o.write((const char*)a, n * sizeof(double));
+}
+void write_sd(const char* outdir, const SDA& U, int n) {
+    int p = U.nx - 1, N = U.Nx - 2 * NGH;
+    write_raw(output_folder(outdir) + U.label + "_N" + std::to_string(N) + "p" + std::to_string(p) + "_" + std::to_string(n) + "_0.dat",
+              U.a.data(), U.a.size());
+}
+void write_fv(const char* outdir, const FVA& U, int n) {
+    write_raw(output_folder(outdir) + U.label + "_N" + std::to_string(U.Nx) + "_" + std::to_string(n) + "_0.dat", U.a.data(),
+              U.a.size());
+}
+void write_dims(const char* outdir, const Dim& X, const Dim& Y, const Dim& Z) {
+    const Dim* d[3] = {&X, &Y, &Z};
+    const char* nm[3] = {"X", "Y", "Z"};
+    for (int a = 0; a < 3; a++)
+        write_raw(output_folder(outdir) + nm[a] + "_N" + std::to_string(d[a]->N) + "p" + std::to_string(d[a]->p) + "_0.dat",
+                  d[a]->fvf.data(), d[a]->fvf.size());
+}
+
+// ---------------------------------------------------------------------------
+// Hydro integrator: src/hydro_ader.hpp:76-586
+struct Hydro {
+    Config& cfg;
+    const char* outdir;
+    Dim X, Y, Z;
+    Ops op;
+    int nvar = NVAR, n_ader = 1, n_stages = 1, n_output = 0, n_step = 0;
+    double rk_a[3] = {0, 0, 0};
+    double t = 0, dt = 0, Dt = 0;
+    SDA W_sp, U_sp, W_cv, U_cv, T_sweep, U0_sp, U_ader_sp, U_fp[3], F_fp[3];
+    FVA U_new, W_new, U_old, W_old, troubles, theta, theta_tmp, Fv[3], alpha[3];
+    SDA T_fp[3];
+
+    Hydro(Config& c, const char* od) : cfg(c), outdir(od) {}
+
+    void setup() {
+        int p = cfg.p;
+        op.build(p, cfg.integrator == INT_ADER);
+        X.init(0, cfg.Nglob[0], cfg.act[0] ? cfg.Nglob[0] : 1, cfg.act[0] ? p : 0, 0, cfg.L[0], op.x_fp.data(), cfg.act[0]);
+        Y.init(1, cfg.Nglob[1], cfg.act[1] ? cfg.Nglob[1] : 1, cfg.act[1] ? p : 0, 0, cfg.L[1], op.x_fp.data(), cfg.act[1]);
+        Z.init(2, cfg.Nglob[2], cfg.act[2] ? cfg.Nglob[2] : 1, cfg.act[2] ? p : 0, 0, cfg.L[2], op.x_fp.data(), cfg.act[2]);
+        cfg.pp.hx = X.h; cfg.pp.hy = Y.h; cfg.pp.hz = Z.h;
+        if (cfg.outputs) write_dims(outdir, X, Y, Z);
+        if (cfg.integrator == INT_RK) {
+            n_ader = 1;
+            n_stages = cfg.rk_order;
+            rk_a[0] = rk_a[1] = rk_a[2] = 0;
+            if (cfg.rk_order == 2) rk_a[1] = 0.5;
+            if (cfg.rk_order == 3) { rk_a[1] = 0.75; rk_a[2] = 1.0 / 3.0; }
+        } else {
+            n_ader = p + 1;
+            n_stages = 1;
+        }
+        W_sp.init("W_sp", 1, nvar, Z, Y, X, 0, 0, 0);
+        U_sp.init("U_sp", 1, nvar, Z, Y, X, 0, 0, 0);
+        W_cv.init("W_cv", 1, nvar, Z, Y, X, 0, 0, 0);
+        U_cv.init("U_cv", 1, nvar, Z, Y, X, 0, 0, 0);
+        T_sweep.init("T_sweep", 1, nvar, Z, Y, X, 0, 0, 0);
+        U_ader_sp.init("U_ader_sp", n_ader, nvar, Z, Y, X, 0, 0, 0);
+        if (cfg.integrator == INT_RK) U0_sp.init("U0_sp", 1, nvar, Z, Y, X, 0, 0, 0);
+        const char* un[3] = {"U_ader_fp_x", "U_ader_fp_y", "U_ader_fp_z"};
+        const char* fn[3] = {"F_ader_fp_x", "F_ader_fp_y", "F_ader_fp_z"};
+        for (int d = 0; d < 3; d++) {
+            bool a = cfg.act[d];
+            U_fp[d].init(un[d], n_ader, nvar, Z, Y, X, d == 2 && a, d == 1 && a, d == 0 && a);
+            F_fp[d].init(fn[d], n_ader, nvar, Z, Y, X, d == 2 && a, d == 1 && a, d == 0 && a);
+        }
+        if (cfg.fallback) {
+            U_new.init("U_new", nvar, Z, Y, X, 0, 0, 0);
+            W_new.init("W_new", nvar, Z, Y, X, 0, 0, 0);
+            U_old.init("U_old", nvar, Z, Y, X, 0, 0, 0);
+            W_old.init("W_old", nvar, Z, Y, X, 0, 0, 0);
+            troubles.init("troubles", nvar, Z, Y, X, 0, 0, 0);
+            theta.init("theta", 1, Z, Y, X, 0, 0, 0);
+            theta_tmp.init("theta_tmp", 1, Z, Y, X, 0, 0, 0);
+            const char* Fn[3] = {"F_x", "F_y", "F_z"};
+            const char* an[3] = {"alpha_x", "alpha_y", "alpha_z"};
+            const char* tn[3] = {"T_fp_x", "T_fp_y", "T_fp_z"};
+            for (int d = 0; d < 3; d++) {
+                bool a = cfg.act[d];
+                Fv[d].init(Fn[d], nvar, Z, Y, X, d == 2 && a, d == 1 && a, d == 0 && a);
+                alpha[d].init(an[d], nvar, Z, Y, X, 0, 0, 0);
+                T_fp[d].init(tn[d], 1, nvar, Z, Y, X, d == 2 && a, d == 1 && a, d == 0 && a);
+            }
+        }
+        Initialize(cfg, W_cv, X, Y, Z, op.xx, op.wx);
+        transform_full(cfg, W_cv, W_sp, T_sweep, op.cv_to_sp);
+        compute_conservatives(cfg, W_sp, U_sp);
+        Dt = compute_dt(cfg, W_cv, X.h, Y.h, Z.h, cfg.nu);
+        if (cfg.outputs) write_outputs();
+    }
+
+    void write_outputs() {
+        if (cfg.fallback) write_fv(outdir, troubles, n_output);
+        write_sd(outdir, F_fp[0], n_output);
+        write_sd(outdir, W_cv, n_output++);
+    }
+
+    void solve_fluxes() {  // hydro_ader.hpp:278-289 (inviscid)
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) transform_1d(U_ader_sp, U_fp[d], op.sp_to_fp, d, -1);
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) compute_fluxes(cfg, U_fp[d], F_fp[d], d);
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) sd_boundaries(cfg, U_fp[d], d);
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) sd_riemann(cfg, U_fp[d], F_fp[d], d);
+    }
+
+    void ader_step() {  // hydro_ader.hpp:291-305
+        copy_ader(U_sp, U_ader_sp);
+        for (int a = 0; a < n_ader; a++) {
+            solve_fluxes();
+            if (a < n_ader - 1)
+                update_prediction(cfg, U_sp, U_ader_sp, F_fp[0], F_fp[1], F_fp[2], op.dfp_to_sp, op.invader, op.wt, X.h, Y.h, Z.h, dt);
+        }
+        if (cfg.fallback) fv_update();
+        else update_solution(cfg, U_sp, U_ader_sp, F_fp[0], F_fp[1], F_fp[2], op.dfp_to_sp, op.wt, X.h, Y.h, Z.h, dt);
+    }
+
+    void rk_step() {  // hydro_ader.hpp:313-325
+        U0_sp.a = U_sp.a;
+        for (int s = 0; s < n_stages; s++) {
+            copy_ader(U_sp, U_ader_sp);
+            solve_fluxes();
+            if (cfg.fallback) fv_update();
+            else update_solution(cfg, U_sp, U_ader_sp, F_fp[0], F_fp[1], F_fp[2], op.dfp_to_sp, op.wt, X.h, Y.h, Z.h, dt);
+            if (rk_a[s] > 0) combine_solution(U_sp, U0_sp, rk_a[s]);
+        }
+    }
+
+    void fv_update() {  // hydro_ader.hpp:500-575
+        transform_full(cfg, U_sp, U_cv, T_sweep, op.sp_to_cv);
+        FVGeom g{&X.fvc, &X.fvf, &Y.fvc, &Y.fvf, &Z.fvc, &Z.fvf};
+        for (int a = 0; a < n_ader; a++) {
+            for (int d = 0; d < 3; d++)
+                if (cfg.act[d]) face_integral(cfg, F_fp[d], Fv[d], T_fp[d], op.sp_to_cv, a, d);
+            fv_update_solution(cfg, U_new, U_old, U_cv, Fv[0], X.fvf, Fv[1], Y.fvf, Fv[2], Z.fvf, op.wt, a, dt, false);
+            apply_fv_boundaries(cfg, U_old);
+            apply_fv_boundaries(cfg, U_new);
+            compute_primitives_fv(cfg, U_new, W_new);
+            compute_primitives_fv(cfg, U_old, W_old);
+            detect_troubles(cfg, W_new, W_old, troubles, alpha[0], alpha[1], alpha[2], X, Y, Z, true, (1 << D_) | (1 << P_));
+            apply_fv_boundaries(cfg, troubles);
+            if (cfg.blending) {
+                apply_blending(cfg, troubles, theta_tmp);
+                blending_ring(cfg, theta_tmp, theta);
+            } else
+                theta_from_troubles(troubles, theta);
+            apply_fv_boundaries(cfg, theta);
+            fallback_fluxes(cfg, W_old, theta, g, Fv[0], Fv[1], Fv[2], op.wt[a] * dt);
+            fv_update_solution(cfg, U_new, U_old, U_cv, Fv[0], X.fvf, Fv[1], Y.fvf, Fv[2], Z.fvf, op.wt, a, dt, true);
+        }
+        transform_full(cfg, U_cv, U_sp, T_sweep, op.cv_to_sp);
+    }
+
+    double evolve(int* steps) {  // hydro_ader.hpp:215-273
+        dt = Dt;
+        double t_output = cfg.dt_output;
+        auto t0 = std::chrono::steady_clock::now();
+        double t_io = 0;
+        int step0 = n_step;
+        while (t < cfg.tlim) {
+            if (cfg.max_steps >= 0 && n_step - step0 >= cfg.max_steps) break;
+            if (cfg.integrator == INT_RK) rk_step();
+            else ader_step();
+            compute_primitives(cfg, U_sp, W_sp);
+            transform_full(cfg, W_sp, W_cv, T_sweep, op.sp_to_cv);
+            t += dt;
+            n_step++;
+            dt = compute_dt(cfg, W_cv, X.h, Y.h, Z.h, cfg.nu);
+            if (cfg.outputs) {
+                if (t >= t_output) {
+                    t_output = t + cfg.dt_output;
+                    auto a = std::chrono::steady_clock::now();
+                    write_outputs();
+                    t_io += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                }
+                if (t + dt > t_output) dt = t_output - t;
+            }
+        }
+        *steps = n_step - step0;
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+    }
+};
+
+int run_hydro(Config& cfg, const char* outdir, double* evol_seconds, int* steps, long long* dofs) {
+    Hydro h(cfg, outdir);
+    h.setup();
+    std::printf("dx = %.17g dt = %.17g\n", h.X.h, h.Dt);
+    *evol_seconds = h.evolve(steps);
+    *dofs = (long long)(h.X.N * h.X.nsp) * (h.Y.N * h.Y.nsp) * (h.Z.N * h.Z.nsp);
+    return 0;
+}
+
+
+// ===========================================================================
+// Induction (kinematic, staggered B, edge E): src/induction.cpp,
+// src/induction_ader.hpp. 3-D only, as in the reference.
+//
+// Pinning: the reference golden tests/golden/induction_fv (rtol 1e-8, whole
+// file, fallback on) and div(B) = 0 to round-off. The trouble flags of the
+// induction path are computed but never feed the solution; their values are
+// parity unpinned.
+
+// src/induction.cpp:94-137 (B = curl A at face points)
+static void rotational_a_to_b(const SDA& A1, const SDA& A2, SDA& B, const Mat& Dm, double d1, double d2, int dim) {
+    int q = choose(dim, B.nx, B.ny, B.nz);
+    for6(B.Nz, B.Ny, B.Nx, B.nz, B.ny, B.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < B.nt; t++) {
+            double d1a2 = 0, d2a1 = 0;
+            for (int ll = 0; ll < q; ll++) {
+                if (dim == 0) {
+                    d1a2 += A2(t, 0, k, j, i, kk, ll, ii) * Dm(jj, ll);
+                    d2a1 += A1(t, 0, k, j, i, ll, jj, ii) * Dm(kk, ll);
+                } else if (dim == 1) {
+                    d1a2 += A2(t, 0, k, j, i, ll, jj, ii) * Dm(kk, ll);
+                    d2a1 += A1(t, 0, k, j, i, kk, jj, ll) * Dm(ii, ll);
+                } else {
+                    d1a2 += A2(t, 0, k, j, i, kk, jj, ll) * Dm(ii, ll);
+                    d2a1 += A1(t, 0, k, j, i, kk, ll, ii) * Dm(jj, ll);
+                }
+            }
+            B(t, 0, k, j, i, kk, jj, ii) = d1a2 / d1 - d2a1 / d2;
+        }
+    });
+}
+
+// src/induction.cpp:139-208: edge E = v1*b2 - v2*b1 with B interpolated
+// sp->fp transversely; stores [E, b1, b2, v1, v2, 0, b1, b2].
+static void compute_E(const Config& cfg, SDA& E, const SDA& B1, const SDA& B2, const Mat& s2f, int dim) {
+    int q = choose(dim, E.nx, E.ny, E.nz);
+    int dim1 = choose(dim, 1, 2, 0), dim2 = choose(dim1, 1, 2, 0);
+    double v1 = cfg.vel[dim1], v2 = cfg.vel[dim2];
+    for6(E.Nz, E.Ny, E.Nx, E.nz, E.ny, E.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < E.nt; t++) {
+            double b1 = 0, b2 = 0;
+            for (int ll = 0; ll < q; ll++) {
+                if (dim == 2) {
+                    b1 += B1(t, 0, k, j, i, kk, ll, ii) * s2f(jj, ll);
+                    b2 += B2(t, 0, k, j, i, kk, jj, ll) * s2f(ii, ll);
+                } else if (dim == 1) {
+                    b1 += B1(t, 0, k, j, i, kk, jj, ll) * s2f(ii, ll);
+                    b2 += B2(t, 0, k, j, i, ll, jj, ii) * s2f(kk, ll);
+                } else {
+                    b1 += B1(t, 0, k, j, i, ll, jj, ii) * s2f(kk, ll);
+                    b2 += B2(t, 0, k, j, i, kk, ll, ii) * s2f(jj, ll);
+                }
+            }
+            E(t, 0, k, j, i, kk, jj, ii) = v1 * b2 - v2 * b1;
+            E(t, 1, k, j, i, kk, jj, ii) = b1;
+            E(t, 2, k, j, i, kk, jj, ii) = b2;
+            E(t, 3, k, j, i, kk, jj, ii) = v1;
+            E(t, 4, k, j, i, kk, jj, ii) = v2;
+            E(t, 5, k, j, i, kk, jj, ii) = 0;
+            E(t, 6, k, j, i, kk, jj, ii) = b1;
+            E(t, 7, k, j, i, kk, jj, ii) = b2;
+        }
+    });
+}
+
+// src/induction.cpp:502-506
+static inline double upwind(double l, double r, double v) {
+    if (v == 0) return 0.5 * (l + r);
+    return v > 0 ? l : r;
+}
+// src/induction.cpp:509-560 (in place, both sides of each face)
+static void E_riemann(SDA& E, int dim, int v1) {
+    int Nx = E.Nx - (dim == 0), Ny = E.Ny - (dim == 1), Nz = E.Nz - (dim == 2);
+    int px = dim == 0 ? 1 : E.nx, py = dim == 1 ? 1 : E.ny, pz = dim == 2 ? 1 : E.nz;
+    int n = choose(dim, E.nx, E.ny, E.nz);
+    int nvar = E.nv;
+    for6(Nz, Ny, Nx, pz, py, px, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        int kL = k, jL = j, iL = i, kkL = kk, jjL = jj, iiL = ii, kR = k, jR = j, iR = i, kkR = kk, jjR = jj, iiR = ii;
+        if (dim == 0) { iR = i + 1; iiL = n - 1; iiR = 0; }
+        if (dim == 1) { jR = j + 1; jjL = n - 1; jjR = 0; }
+        if (dim == 2) { kR = k + 1; kkL = n - 1; kkR = 0; }
+        for (int t = 0; t < E.nt; t++) {
+            double vL = E(t, v1, kL, jL, iL, kkL, jjL, iiL), vR = E(t, v1, kR, jR, iR, kkR, jjR, iiR);
+            double v = smax(vL, vR);
+            for (int var = 0; var < nvar - 3; var++) {
+                double e = upwind(E(t, var, kL, jL, iL, kkL, jjL, iiL), E(t, var, kR, jR, iR, kkR, jjR, iiR), v);
+                E(t, var, kL, jL, iL, kkL, jjL, iiL) = e;
+                E(t, var, kR, jR, iR, kkR, jjR, iiR) = e;
+            }
+            for (int var = nvar - 2; var < nvar; var++) {
+                double e = upwind(E(t, var, kL, jL, iL, kkL, jjL, iiL), E(t, var, kR, jR, iR, kkR, jjR, iiR), 0);
+                E(t, var, kL, jL, iL, kkL, jjL, iiL) = e;
+                E(t, var, kR, jR, iR, kkR, jjR, iiR) = e;
+            }
+        }
+    });
+}
+
+// dB/dt term of one face-B component: src/induction.cpp:303-365 (dBdt of
+// update_B_prediction) and :367-420 (update_B_solution).
+static inline double dBdt_at(const SDA& E1, const SDA& E2, const Mat& Dm, double d1, double d2, int dim, int t, int k, int j,
+                             int i, int kk, int jj, int ii, int q) {
+    double dBdt = 0, E;
+    for (int ll = 0; ll < q; ll++) {
+        if (dim == 0) {
+            E = E1(t, 0, k, j, i, ll, jj, ii);
+            dBdt += E * Dm(kk, ll) / d2;
+            E = E2(t, 0, k, j, i, kk, ll, ii);
+            dBdt -= E * Dm(jj, ll) / d1;
+        } else if (dim == 1) {
+            E = E1(t, 0, k, j, i, kk, jj, ll);
+            dBdt += E * Dm(ii, ll) / d2;
+            E = E2(t, 0, k, j, i, ll, jj, ii);
+            dBdt -= E * Dm(kk, ll) / d1;
+        } else {
+            E = E1(t, 0, k, j, i, kk, ll, ii);
+            dBdt += E * Dm(jj, ll) / d2;
+            E = E2(t, 0, k, j, i, kk, jj, ll);
+            dBdt -= E * Dm(ii, ll) / d1;
+        }
+    }
+    return dBdt;
+}
+static void update_B_prediction(const Config& cfg, const SDA& B, SDA& Ba, const SDA& E1, const SDA& E2, const Mat& Dm,
+                                const Mat& inv, const std::vector<double>& w, double d1, double d2, double dt, int dim) {
+    int q = choose(dim, B.nx, B.ny, B.nz), nader = Ba.nt;
+    for6(B.Nz - 2 * cfg.ngh[2], B.Ny - 2 * cfg.ngh[1], B.Nx - 2 * cfg.ngh[0], B.nz, B.ny, B.nx, cfg.ngh[2], cfg.ngh[1],
+         cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+             double dBdt[10];
+             double B_old = B(0, 0, k, j, i, kk, jj, ii);
+             for (int t = 0; t < nader; t++) dBdt[t] = dBdt_at(E1, E2, Dm, d1, d2, dim, t, k, j, i, kk, jj, ii, q);
+             for (int t = 0; t < nader; t++) {
+                 double dB = 0;
+                 for (int ll = 0; ll < nader; ll++) dB += dBdt[ll] * inv(t, ll) * w[ll] * dt;
+                 Ba(t, 0, k, j, i, kk, jj, ii) = B_old - dB;
+             }
+         });
+}
+static void update_B_solution(const Config& cfg, SDA& B, const SDA& E1, const SDA& E2, const Mat& Dm, const std::vector<double>& w,
+                              double d1, double d2, double dt, int dim) {
+    int q = choose(dim, B.nx, B.ny, B.nz), nader = E1.nt;
+    for6(B.Nz - 2 * cfg.ngh[2], B.Ny - 2 * cfg.ngh[1], B.Nx - 2 * cfg.ngh[0], B.nz, B.ny, B.nx, cfg.ngh[2], cfg.ngh[1],
+         cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+             double dB = 0;
+             for (int t = 0; t < nader; t++) dB += dBdt_at(E1, E2, Dm, d1, d2, dim, t, k, j, i, kk, jj, ii, q) * w[t] * dt;
+             B(0, 0, k, j, i, kk, jj, ii) -= dB;
+         });
+}
+
+// src/induction.cpp:422-465
+static void compute_B2_cv(SDA& B2, const SDA& Bx, const SDA& By, const SDA& Bz, const Mat& f2c, const Mat& s2c) {
+    int q = Bx.nx, px = B2.nx, py = B2.ny, pz = B2.nz;
+    for6(B2.Nz, B2.Ny, B2.Nx, B2.nz, B2.ny, B2.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        double bx = 0, by = 0, bz = 0;
+        for (int nn = 0; nn < pz; nn++)
+            for (int mm = 0; mm < py; mm++)
+                for (int ll = 0; ll < q; ll++) bx += Bx(0, 0, k, j, i, nn, mm, ll) * s2c(kk, nn) * s2c(jj, mm) * f2c(ii, ll);
+        for (int nn = 0; nn < pz; nn++)
+            for (int mm = 0; mm < q; mm++)
+                for (int ll = 0; ll < px; ll++) by += By(0, 0, k, j, i, nn, mm, ll) * s2c(kk, nn) * f2c(jj, mm) * s2c(ii, ll);
+        for (int nn = 0; nn < q; nn++)
+            for (int mm = 0; mm < py; mm++)
+                for (int ll = 0; ll < px; ll++) bz += Bz(0, 0, k, j, i, nn, mm, ll) * f2c(kk, nn) * s2c(jj, mm) * s2c(ii, ll);
+        B2(0, 0, k, j, i, kk, jj, ii) = bx;
+        B2(0, 1, k, j, i, kk, jj, ii) = by;
+        B2(0, 2, k, j, i, kk, jj, ii) = bz;
+        B2(0, 3, k, j, i, kk, jj, ii) = bx * bx + by * by + bz * bz;
+    });
+}
+
+// src/induction.cpp:467-500 (active elements, cv points)
+static void compute_B_cv_from_cf(const Config& cfg, FVA& B, const SDA& Bx, const SDA& By, const SDA& Bz, const Mat& f2c) {
+    int px = Bx.nx - 1, py = Bx.ny, pz = Bx.nz, q = px + 1;
+    int qx = px, qy = py, qz = pz;
+    for6(Bx.Nz - 2 * cfg.ngh[2], Bx.Ny - 2 * cfg.ngh[1], Bx.Nx - 2 * cfg.ngh[0], pz, py, px, cfg.ngh[2], cfg.ngh[1],
+         cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+             ORC_IJK
+             double bx = 0, by = 0, bz = 0;
+             for (int ll = 0; ll < q; ll++) {
+                 bx += Bx(0, 0, k, j, i, kk, jj, ll) * f2c(ii, ll);
+                 by += By(0, 0, k, j, i, kk, ll, ii) * f2c(jj, ll);
+                 bz += Bz(0, 0, k, j, i, ll, jj, ii) * f2c(kk, ll);
+             }
+             B(0, K, J, I) = bx;
+             B(1, K, J, I) = by;
+             B(2, K, J, I) = bz;
+             B(3, K, J, I) = bx * bx + by * by + bz * bz;
+         });
+}
+
+// src/induction.cpp:602-654 (edge-averaged E on the FV sub-grid)
+static void edge_integral(const Config& cfg, const SDA& Eep, FVA& E, const Mat& s2c, int t_id, int dim) {
+    int Nx = Eep.Nx + (dim == 0 ? 0 : 1), Ny = Eep.Ny + (dim == 1 ? 0 : 1), Nz = Eep.Nz + (dim == 2 ? 0 : 1);
+    int px = Eep.nx - (dim == 0 ? 0 : 1), py = Eep.ny - (dim == 1 ? 0 : 1), pz = Eep.nz - (dim == 2 ? 0 : 1);
+    int qx = px, qy = py, qz = pz;
+    int q = choose(dim, px, py, pz);
+    int Ni = E.Nx, Nj = E.Ny, Nk = E.Nz;
+    for6(Nz - 2 * cfg.ngh[2], Ny - 2 * cfg.ngh[1], Nx - 2 * cfg.ngh[0], pz, py, px, cfg.ngh[2], cfg.ngh[1], cfg.ngh[0],
+         [&](int k, int j, int i, int kk, int jj, int ii) {
+             int id = choose(dim, ii, jj, kk);
+             double e = 0;
+             for (int ll = 0; ll < q; ll++) {
+                 int a = dim == 0 ? ll : ii, b = dim == 1 ? ll : jj, c = dim == 2 ? ll : kk;
+                 double s = Eep(t_id, 0, k, j, i, c, b, a);
+                 s *= s2c(id, ll);
+                 e += s;
+             }
+             ORC_IJK
+             if (K < Nk && J < Nj && I < Ni) E(0, K, J, I) = e;
+         });
+}
+
+// src/induction.cpp:656-720. The last face point of element i and the first
+// of element i+1 map to the same FV face; both carry identical values.
+static void fv_update_B(const Config& cfg, FVA& Bn, FVA& Bo, SDA& B, const FVA& E1, const FVA& E2, const std::vector<double>& f1,
+                        const std::vector<double>& f2, const std::vector<double>& w, double dt, int t_id, int dim, bool update) {
+    int px = B.nx, py = B.ny, pz = B.nz;
+    int qx = px - (dim == 0), qy = py - (dim == 1), qz = pz - (dim == 2);
+    int Ni = Bn.Nx, Nj = Bn.Ny, Nk = Bn.Nz;
+    int dim1 = choose(dim, 1, 2, 0), dim2 = choose(dim1, 1, 2, 0);
+    // serial: the duplicate writes at element interfaces must not race
+    int Mz = B.Nz - 2 * cfg.ngh[2], My = B.Ny - 2 * cfg.ngh[1], Mx = B.Nx - 2 * cfg.ngh[0];
+    for (int k = cfg.ngh[2]; k < Mz + cfg.ngh[2]; k++)
+        for (int j = cfg.ngh[1]; j < My + cfg.ngh[1]; j++)
+            for (int i = cfg.ngh[0]; i < Mx + cfg.ngh[0]; i++)
+                for (int kk = 0; kk < pz; kk++)
+                    for (int jj = 0; jj < py; jj++)
+                        for (int ii = 0; ii < px; ii++) {
+                            ORC_IJK
+                            double b_old = B(0, 0, k, j, i, kk, jj, ii);
+                            if (K < Nk && J < Nj && I < Ni) {
+                                Bo(0, K, J, I) = b_old;
+                                double ep = E1(0, K + (dim2 == 2), J + (dim2 == 1), I + (dim2 == 0));
+                                int id = choose(dim2, I, J, K);
+                                double h = f2[id + 1] - f2[id];
+                                double dBdt = (ep - E1(0, K, J, I)) / h;
+                                ep = E2(0, K + (dim1 == 2), J + (dim1 == 1), I + (dim1 == 0));
+                                id = choose(dim1, I, J, K);
+                                h = f1[id + 1] - f1[id];
+                                dBdt -= (ep - E2(0, K, J, I)) / h;
+                                double b_new = b_old - dBdt * w[t_id] * dt;
+                                Bn(0, K, J, I) = b_new;
+                                if (update) B(0, 0, k, j, i, kk, jj, ii) = b_new;
+                            }
+                        }
+}
+
+// src/boundary.cpp:123-190 with face alignment. The reference iterates the
+// transverse face count of the FV_Boundaries buffer, which runs one past the
+// extent of cell-centred axes of these arrays; those extra iterations only
+// repeat writes of in-range ones (or fall past the allocation) and are
+// omitted here (induction FV fields feed trouble detection only).
+static void fv_boundaries_aligned(const Config& cfg, FVA& U, int dim, int alignment, int a_dim) {
+    int type = cfg.bc[dim];
+    int shift = alignment * (a_dim == dim);
+    int N = choose(dim, U.Nx, U.Ny, U.Nz) - shift;  // BC.N = fv_ncells of `dim`
+    int Nx = dim == 0 ? nGH : U.Nx, Ny = dim == 1 ? nGH : U.Ny, Nz = dim == 2 ? nGH : U.Nz;
+    auto at = [&](int v, int k, int j, int i, int l) -> double& {
+        if (dim == 0) return U(v, k, j, l);
+        if (dim == 1) return U(v, k, l, i);
+        return U(v, l, j, i);
+    };
+    std::vector<double> BL((size_t)U.nv * Nz * Ny * Nx), BR(BL.size());
+    auto bidx = [&](int v, int k, int j, int i) { return (((size_t)v * Nz + k) * Ny + j) * Nx + i; };
+    for3(Nz, Ny, Nx, 0, 0, 0, [&](int k, int j, int i) {
+        int l = choose(dim, i, j, k);
+        for (int v = 0; v < U.nv; v++) {
+            if (type == BC_PERIODIC) {
+                BL[bidx(v, k, j, i)] = at(v, k, j, i, N - 2 * nGH + l - shift);
+                BR[bidx(v, k, j, i)] = at(v, k, j, i, nGH + l + shift);
+            } else if (type == BC_GRADFREE) {
+                BL[bidx(v, k, j, i)] = at(v, k, j, i, nGH + l + shift);
+                BR[bidx(v, k, j, i)] = at(v, k, j, i, N - 2 * nGH + l - shift);
+            } else {
+                double sgn = (v == 1 + dim) ? -1.0 : 1.0;
+                BL[bidx(v, k, j, i)] = sgn * at(v, k, j, i, 2 * nGH - 1 - l);
+                BR[bidx(v, k, j, i)] = sgn * at(v, k, j, i, N - nGH - 1 - l);
+            }
+        }
+    });
+    for3(Nz, Ny, Nx, 0, 0, 0, [&](int k, int j, int i) {
+        int l = choose(dim, i, j, k);
+        for (int v = 0; v < U.nv; v++) {
+            at(v, k, j, i, l) = BL[bidx(v, k, j, i)];
+            at(v, k, j, i, N - nGH + l) = BR[bidx(v, k, j, i)];
+        }
+    });
+}
+
+// src/transforms.cpp:226-249: sweeps along the active transverse axes.
+static void transform_2d(const Config& cfg, const SDA& A, SDA& B, SDA& T, const Mat& M, int dim) {
+    int dim1 = choose(dim, 1, 2, 0), dim2 = choose(dim1, 1, 2, 0);
+    int dims[2], nd = 0;
+    if (cfg.act[dim1]) dims[nd++] = dim1;
+    if (cfg.act[dim2]) dims[nd++] = dim2;
+    if (nd == 0) { B.a = A.a; return; }
+    const SDA* src = &A;
+    for (int s = 0; s < nd; s++) {
+        SDA* dst = ((nd - 1 - s) % 2 == 0) ? &B : &T;
+        transform_1d(*src, *dst, M, dims[s], -1);
+        src = dst;
+    }
+}
+
+// src/initial_conditions.cpp:234-249 (loop centre and radius parametrised;
+// defaults reproduce the reference) and :348-370
+static void Initialize_ep(const Config& cfg, SDA& A, const Mat& Xs, const Mat& Ys, const Mat& Zs, int dim) {
+    for6(A.Nz, A.Ny, A.Nx, A.nz, A.ny, A.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        double x = Xs(i, ii), y = Ys(j, jj);
+        double val = 0;
+        if (dim == 2) {
+            x = x - cfg.loop_c[0];
+            y = y - cfg.loop_c[1];
+            double r = std::sqrt(x * x + y * y);
+            double A0 = cfg.loop_A0, R = cfg.loop_R;
+            val = (r <= R) ? A0 * (R - r) : 0;
+        }
+        A(0, 0, k, j, i, kk, jj, ii) = val;
+    });
+}
+
+struct Induction {
+    Config& cfg;
+    const char* outdir;
+    Dim X, Y, Z;
+    Ops op;
+    int n_ader = 1, n_stages = 1, n_output = 0, n_step = 0;
+    double rk_a[3] = {0, 0, 0};
+    double t = 0, dt = 0, Dt = 0;
+    SDA Bf[3], B2_cv, Aep[3], Ba[3], B0[3], Eep[3], Bcf[3], Tfp[3];
+    FVA Bold[3], B_old, Bnew[3], B_new, Efv[3], troubles, alpha[3];
+    Induction(Config& c, const char* od) : cfg(c), outdir(od) {}
+
+    void setup() {
+        int p = cfg.p;
+        op.build(p, cfg.integrator == INT_ADER);
+        X.init(0, cfg.Nglob[0], cfg.Nglob[0], p, 0, cfg.L[0], op.x_fp.data(), true);
+        Y.init(1, cfg.Nglob[1], cfg.Nglob[1], p, 0, cfg.L[1], op.x_fp.data(), true);
+        Z.init(2, cfg.Nglob[2], cfg.Nglob[2], p, 0, cfg.L[2], op.x_fp.data(), true);
+        if (cfg.outputs) write_dims(outdir, X, Y, Z);
+        if (cfg.integrator == INT_RK) {
+            n_ader = 1;
+            n_stages = cfg.rk_order;
+            if (cfg.rk_order == 2) rk_a[1] = 0.5;
+            if (cfg.rk_order == 3) { rk_a[1] = 0.75; rk_a[2] = 1.0 / 3.0; }
+        } else
+            n_ader = p + 1;
+        const char* bn[3] = {"Bx_fp_x", "By_fp_y", "Bz_fp_z"};
+        const char* an[3] = {"Ax_ep_yz", "Ay_ep_zx", "Az_ep_xy"};
+        const char* ban[3] = {"Bx_ader_fp_x", "By_ader_fp_y", "Bz_ader_fp_z"};
+        const char* en[3] = {"Ex_ader_ep_yz", "Ey_ader_ep_zx", "Ez_ader_ep_xy"};
+        for (int d = 0; d < 3; d++) {
+            Bf[d].init(bn[d], 1, 1, Z, Y, X, d == 2, d == 1, d == 0);
+            Aep[d].init(an[d], 1, 1, Z, Y, X, d != 2, d != 1, d != 0);
+            Ba[d].init(ban[d], n_ader, 1, Z, Y, X, d == 2, d == 1, d == 0);
+            Eep[d].init(en[d], n_ader, 8, Z, Y, X, d != 2, d != 1, d != 0);
+            if (cfg.integrator == INT_RK) B0[d].init("B0", 1, 1, Z, Y, X, d == 2, d == 1, d == 0);
+        }
+        B2_cv.init("B2_cv", 1, 4, Z, Y, X, 0, 0, 0);
+        Initialize_ep(cfg, Aep[0], X.sdc, Y.sdf, Z.sdf, 0);
+        Initialize_ep(cfg, Aep[1], X.sdf, Y.sdc, Z.sdf, 1);
+        Initialize_ep(cfg, Aep[2], X.sdf, Y.sdf, Z.sdc, 2);
+        rotational_a_to_b(Aep[1], Aep[2], Bf[0], op.dfp_to_sp, Y.h, Z.h, 0);
+        rotational_a_to_b(Aep[2], Aep[0], Bf[1], op.dfp_to_sp, Z.h, X.h, 1);
+        rotational_a_to_b(Aep[0], Aep[1], Bf[2], op.dfp_to_sp, X.h, Y.h, 2);
+        if (cfg.fallback) {
+            const char* cn[3] = {"Bx", "By", "Bz"};
+            for (int d = 0; d < 3; d++) {
+                Bcf[d].init(cn[d], 1, 1, Z, Y, X, d == 2, d == 1, d == 0);
+                Tfp[d].init("T_fp", 1, 1, Z, Y, X, d == 2, d == 1, d == 0);
+                Bold[d].init("B_old_d", 1, Z, Y, X, d == 2, d == 1, d == 0);
+                Bnew[d].init("B_new_d", 1, Z, Y, X, d == 2, d == 1, d == 0);
+                Efv[d].init("E_fv", 1, Z, Y, X, d != 2, d != 1, d != 0);
+                alpha[d].init("alpha", 4, Z, Y, X, 0, 0, 0);
+            }
+            B_old.init("B_old", 4, Z, Y, X, 0, 0, 0);
+            B_new.init("B_new", 4, Z, Y, X, 0, 0, 0);
+            troubles.init("troubles", 4, Z, Y, X, 0, 0, 0);
+        }
+        // src/induction.cpp:52-92 and induction_ader.hpp:254-268
+        double vmax = std::fabs(cfg.vel[0]) + std::fabs(cfg.vel[1]) + std::fabs(cfg.vel[2]);
+        double dx_min = smin(smin(smin(1.0, X.h), Y.h), Z.h);
+        dt = cfg.cfl * dx_min / vmax / B2_cv.nx;
+        if (cfg.eta > 0.0) Dt = 0.5 * cfg.cfl * (X.h * X.h / cfg.eta) / (cfg.ndim) / (cfg.ndim) / (p + 1) / (p + 1);
+        else Dt = 1E6;
+        Dt = smin(dt, Dt);
+        if (cfg.outputs) write_outputs();
+    }
+
+    void write_outputs() {
+        compute_B2_cv(B2_cv, Bf[0], Bf[1], Bf[2], op.fp_to_cv, op.sp_to_cv);
+        if (cfg.fallback) write_fv(outdir, troubles, n_output);
+        write_sd(outdir, B2_cv, n_output++);
+    }
+
+    void solve_E() {  // induction_ader.hpp:332-341, :386-448
+        compute_E(cfg, Eep[2], Ba[0], Ba[1], op.sp_to_fp, 2);
+        compute_E(cfg, Eep[1], Ba[2], Ba[0], op.sp_to_fp, 1);
+        compute_E(cfg, Eep[0], Ba[1], Ba[2], op.sp_to_fp, 0);
+        sd_boundaries(cfg, Eep[1], 0);
+        sd_boundaries(cfg, Eep[2], 0);
+        sd_boundaries(cfg, Eep[0], 1);
+        sd_boundaries(cfg, Eep[2], 1);
+        sd_boundaries(cfg, Eep[0], 2);
+        sd_boundaries(cfg, Eep[1], 2);
+        E_riemann(Eep[1], 0, 4);
+        E_riemann(Eep[2], 0, 3);
+        E_riemann(Eep[2], 1, 4);
+        E_riemann(Eep[0], 1, 3);
+        E_riemann(Eep[0], 2, 4);
+        E_riemann(Eep[1], 2, 3);
+    }
+    void update_prediction() {
+        update_B_prediction(cfg, Bf[0], Ba[0], Eep[1], Eep[2], op.dfp_to_sp, op.invader, op.wt, Y.h, Z.h, dt, 0);
+        update_B_prediction(cfg, Bf[1], Ba[1], Eep[2], Eep[0], op.dfp_to_sp, op.invader, op.wt, Z.h, X.h, dt, 1);
+        update_B_prediction(cfg, Bf[2], Ba[2], Eep[0], Eep[1], op.dfp_to_sp, op.invader, op.wt, X.h, Y.h, dt, 2);
+    }
+    void update_solution() {
+        update_B_solution(cfg, Bf[0], Eep[1], Eep[2], op.dfp_to_sp, op.wt, Y.h, Z.h, dt, 0);
+        update_B_solution(cfg, Bf[1], Eep[2], Eep[0], op.dfp_to_sp, op.wt, Z.h, X.h, dt, 1);
+        update_B_solution(cfg, Bf[2], Eep[0], Eep[1], op.dfp_to_sp, op.wt, X.h, Y.h, dt, 2);
+    }
+    void fv_update() {  // induction_ader.hpp:534-603
+        for (int d = 0; d < 3; d++) transform_2d(cfg, Bf[d], Bcf[d], Tfp[d], op.sp_to_cv, d);
+        compute_B_cv_from_cf(cfg, B_old, Bcf[0], Bcf[1], Bcf[2], op.fp_to_cv);
+        for (int a = 0; a < n_ader; a++) {
+            for (int d = 0; d < 3; d++) edge_integral(cfg, Eep[d], Efv[d], op.sp_to_cv, a, d);
+            fv_update_B(cfg, Bnew[0], Bold[0], Bcf[0], Efv[1], Efv[2], Y.fvf, Z.fvf, op.wt, dt, a, 0, true);
+            fv_update_B(cfg, Bnew[1], Bold[1], Bcf[1], Efv[2], Efv[0], Z.fvf, X.fvf, op.wt, dt, a, 1, true);
+            fv_update_B(cfg, Bnew[2], Bold[2], Bcf[2], Efv[0], Efv[1], X.fvf, Y.fvf, op.wt, dt, a, 2, true);
+            compute_B_cv_from_cf(cfg, B_new, Bcf[0], Bcf[1], Bcf[2], op.fp_to_cv);
+            for (int d = 0; d < 3; d++) {
+                fv_boundaries_aligned(cfg, B_old, d, 0, 0);
+                fv_boundaries_aligned(cfg, Bold[0], d, 1, 0);
+                fv_boundaries_aligned(cfg, Bold[1], d, 1, 1);
+                fv_boundaries_aligned(cfg, Bold[2], d, 1, 2);
+            }
+            detect_troubles(cfg, B_new, B_old, troubles, alpha[0], alpha[1], alpha[2], X, Y, Z, false, 0b0111);
+        }
+        for (int d = 0; d < 3; d++) transform_2d(cfg, Bcf[d], Bf[d], Tfp[d], op.cv_to_sp, d);
+    }
+    void ader_step() {
+        for (int d = 0; d < 3; d++) copy_ader(Bf[d], Ba[d]);
+        for (int a = 0; a < n_ader; a++) {
+            solve_E();
+            if (a < n_ader - 1) update_prediction();
+        }
+        if (cfg.fallback) fv_update();
+        else update_solution();
+    }
+    void rk_step() {
+        for (int d = 0; d < 3; d++) B0[d].a = Bf[d].a;
+        for (int s = 0; s < n_stages; s++) {
+            for (int d = 0; d < 3; d++) copy_ader(Bf[d], Ba[d]);
+            solve_E();
+            if (cfg.fallback) fv_update();
+            else update_solution();
+            if (rk_a[s] > 0)
+                for (int d = 0; d < 3; d++) combine_solution(Bf[d], B0[d], rk_a[s]);
+        }
+    }
+    double evolve(int* steps) {  // induction_ader.hpp:274-327
+        dt = Dt;
+        double t_output = cfg.dt_output;
+        auto t0 = std::chrono::steady_clock::now();
+        double t_io = 0;
+        int step0 = n_step;
+        while (t < cfg.tlim) {
+            if (cfg.max_steps >= 0 && n_step - step0 >= cfg.max_steps) break;
+            if (cfg.integrator == INT_RK) rk_step();
+            else ader_step();
+            t += dt;
+            n_step++;
+            dt = Dt;
+            if (cfg.outputs) {
+                if (t >= t_output) {
+                    t_output = t + cfg.dt_output;
+                    auto a = std::chrono::steady_clock::now();
+                    write_outputs();
+                    t_io += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                }
+                if (t + dt > t_output) dt = t_output - t;
+            }
+        }
+        *steps = n_step - step0;
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+    }
+};
+
+int run_induction(Config& cfg, const char* outdir, double* evol_seconds, int* steps, long long* dofs) {
+    if (cfg.ndim != 3) {
+        std::fprintf(stderr, "oracle: the induction solver requires 3 active dimensions\n");
+        return 1;
+    }
+    Induction s(cfg, outdir);
+    s.setup();
+    *evol_seconds = s.evolve(steps);
+    *dofs = (long long)(s.X.N * s.X.nsp) * (s.Y.N * s.Y.nsp) * (s.Z.N * s.Z.nsp);
+    return 0;
+}
+
+}  // namespace orc

@@ -1,0 +1,212 @@
+/*
+ * spdk.h — C ABI of the MI355X spectral-difference kernel library (libspdk.so).
+ *
+ * This is the drop-in boundary for the per-time-step spectral-difference
+ * update of spd_K. Each stage entry point replaces one free function of the
+ * reference kernel layer declared in src/prototypes.hpp:13-105 (cited per
+ * entry), with the same argument meaning; the fused entry points replace a
+ * whole integrator step of src/hydro_ader.hpp:291-325.
+ *
+ * Conventions
+ *  - All array arguments are device pointers to FP64 data in the reference's
+ *    C-order layouts: SD arrays (t, var, k, j, i, kk, jj, ii) and FV arrays
+ *    (var, k, j, i) (src/define.hpp:124-125). Shapes travel as plain ints.
+ *  - Every call returns 0 on success or a negative SPDK_E_* code; calls are
+ *    asynchronous on the context's stream unless stated otherwise.
+ *  - One context per host thread; a context is not reentrant.
+ */
+#ifndef SPDK_H_
+#define SPDK_H_
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define SPDK_OK 0
+#define SPDK_E_SHAPE (-1)   /* array shapes inconsistent with the call       */
+#define SPDK_E_HIP (-2)     /* a HIP runtime call or launch failed           */
+#define SPDK_E_ARG (-3)     /* bad argument (null pointer, unknown enum ...) */
+#define SPDK_E_NOGPU (-4)   /* no usable gfx950 device                       */
+#define SPDK_E_CONFIG (-5)  /* configuration rejected (maps to exit(1))      */
+
+/* boundary types (src/define.hpp:44) */
+#define SPDK_BC_PERIODIC 0
+#define SPDK_BC_GRADFREE 1
+#define SPDK_BC_REFLECTIVE 2
+
+/* operator matrices uploaded at context creation (src/hydro_ader.hpp:122-141) */
+#define SPDK_M_SP_TO_FP 0  /* (p+2) x (p+1) */
+#define SPDK_M_FP_TO_SP 1  /* (p+1) x (p+2) */
+#define SPDK_M_DFP_TO_SP 2 /* (p+1) x (p+2) */
+#define SPDK_M_SP_TO_CV 3  /* (p+1) x (p+1) */
+#define SPDK_M_CV_TO_SP 4  /* (p+1) x (p+1) */
+#define SPDK_M_FP_TO_CV 5  /* (p+1) x (p+2) */
+#define SPDK_M_INVADER 6   /*  T x T        */
+#define SPDK_M_COUNT 7
+
+/* Shape of an SD array: (nt, nv, Nz, Ny, Nx, nz, ny, nx); N* count elements
+ * including ghosts, n* points per element (structs.hpp:104-129). */
+typedef struct spdk_sd_shape {
+    int nt, nv, Nz, Ny, Nx, nz, ny, nx;
+} spdk_sd_shape;
+
+/* Shape of an FV array: (nv, Nz, Ny, Nx) (fv_structs.hpp:68-93). */
+typedef struct spdk_fv_shape {
+    int nv, Nz, Ny, Nx;
+} spdk_fv_shape;
+
+/* Run parameters the kernels need (src/global.hpp:41-58). */
+typedef struct spdk_params {
+    int p;              /* polynomial degree                                   */
+    int active[3];      /* axis x,y,z active                                   */
+    int bc[3];          /* SPDK_BC_* per axis                                  */
+    int n_ader;         /* time slices T: p+1 (ADER) or 1 (SSP-RK)             */
+    double gamma, cfl;
+    double g[3];        /* constant gravity (source term)                      */
+    double nu;          /* kinematic viscosity (dt clause only)               */
+    double h[3];        /* element widths dx, dy, dz                           */
+    double nad_tol;     /* NAD band width                                      */
+    int nad_delta;      /* band scaled by local range                          */
+    int nad_moore;      /* Moore (box) neighbourhood, else face neighbours     */
+    int sed;            /* smooth-extrema relaxation (applied when p>1)        */
+    int blending;       /* fractional theta blending                           */
+    double vel[3];      /* induction advection velocity                        */
+} spdk_params;
+
+/* Host-side operator data copied to the device at creation. Matrices are
+ * row-major, sizes as listed by SPDK_M_*; wt has n_ader entries; fv_faces[d]
+ * has fv_nfaces[d] entries and fv_centers[d] fv_ncells[d] entries (the FV
+ * sub-grid of axis d, structs.hpp:40-75). */
+typedef struct spdk_operators {
+    const double* mat[SPDK_M_COUNT];
+    const double* wt;
+    const double* fv_faces[3];
+    const double* fv_centers[3];
+    int fv_nfaces[3];
+    int fv_ncells[3];
+} spdk_operators;
+
+typedef struct spdk_ctx spdk_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on the current HIP device. `stream` is a hipStream_t (NULL
+ * = a stream created and owned by the context). Replaces the operator set-up
+ * of src/hydro_ader.hpp:76-186 (matrix upload). */
+int spdk_ctx_create(const spdk_params* prm, const spdk_operators* ops, void* stream, spdk_ctx** out);
+int spdk_ctx_destroy(spdk_ctx* ctx);
+int spdk_ctx_sync(spdk_ctx* ctx);          /* blocks until the stream drains   */
+void* spdk_ctx_stream(spdk_ctx* ctx);      /* the hipStream_t kernels run on   */
+const char* spdk_last_error(void);         /* message of the last failure      */
+int spdk_device_info(char* name, size_t len, int* cus);
+
+/* HIP-event timing of the dominant kernel (the Picard sweep of the fused path,
+ * the flux-point interpolation of the stage path): enable, reset, read the
+ * accumulated device milliseconds and launch count. */
+int spdk_timing_enable(spdk_ctx* ctx, int on);
+int spdk_timing_reset(spdk_ctx* ctx);
+int spdk_timing_read(spdk_ctx* ctx, double* ms_total, int* launches, char* name, size_t len);
+
+/* ---- device memory helpers (plain bytes) --------------------------------- */
+int spdk_malloc(void** ptr, size_t bytes);
+int spdk_free(void* ptr);
+int spdk_memset0(spdk_ctx* ctx, void* ptr, size_t bytes);
+int spdk_memcpy_h2d(spdk_ctx* ctx, void* dst, const void* src, size_t bytes);  /* blocking */
+int spdk_memcpy_d2h(spdk_ctx* ctx, void* dst, const void* src, size_t bytes);  /* blocking */
+int spdk_memcpy_d2d(spdk_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- SD stage kernels (hydro) -------------------------------------------- */
+/* transform_a_to_b_1d / _slice (src/transforms.cpp:120-177): one-axis
+ * contraction with matrix `mat` (SPDK_M_*) along `dim`; t_src < 0 maps every
+ * slice, else slice t_src of A into slice 0 of B. */
+int spdk_transform_1d(spdk_ctx* ctx, const double* A, spdk_sd_shape sa, double* B, spdk_sd_shape sb, int mat, int dim,
+                      int t_src);
+/* transform_a_to_b (src/transforms.cpp:99-117): x, y, z sweeps over the active
+ * axes through scratch T (shape of B). */
+int spdk_transform(spdk_ctx* ctx, const double* A, double* B, double* T, spdk_sd_shape s, int mat);
+/* compute_conservatives / compute_primitives, SD (src/hydro.cpp:47-72, :108-132) */
+int spdk_conservatives(spdk_ctx* ctx, const double* W, double* U, spdk_sd_shape s);
+int spdk_primitives(spdk_ctx* ctx, const double* U, double* W, spdk_sd_shape s);
+/* compute_fluxes (src/hydro.cpp:167-206): Euler flux normal to `dim` */
+int spdk_hydro_flux(spdk_ctx* ctx, const double* U, double* F, spdk_sd_shape s, int dim);
+/* boundaries, SD traces (src/boundary.cpp:48-113): ghost interface points of
+ * an array staggered along `dim` */
+int spdk_fill_sd_traces(spdk_ctx* ctx, double* U, spdk_sd_shape s, int dim);
+/* sd_riemann_solver (src/hydro.cpp:368-418): LLF at element faces along `dim`,
+ * written in place over F on both sides */
+int spdk_riemann_llf(spdk_ctx* ctx, const double* U, double* F, spdk_sd_shape s, int dim);
+/* copy_ader (src/hydro_ader.hpp:428-444): zero-order Picard guess */
+int spdk_copy_ader(spdk_ctx* ctx, const double* U, double* U_ader, spdk_sd_shape s_ader);
+/* update_prediction (src/transforms.cpp:263-325); Fy/Fz may be NULL on
+ * inactive axes. s = shape of U (slice 0); sf[d] = shapes of the F arrays. */
+int spdk_ader_predict(spdk_ctx* ctx, const double* U, double* U_ader, const double* Fx, const double* Fy,
+                      const double* Fz, spdk_sd_shape s, int q, double dt);
+/* update_solution (src/transforms.cpp:327-374) */
+int spdk_ader_correct(spdk_ctx* ctx, double* U, const double* U_ader, const double* Fx, const double* Fy,
+                      const double* Fz, spdk_sd_shape s, int q, double dt);
+/* combine_solution (src/transforms.cpp:252-261): U <- a*U0 + (1-a)*U */
+int spdk_rk_combine(spdk_ctx* ctx, double* U, const double* U0, spdk_sd_shape s, double a);
+/* compute_dt (src/hydro.cpp:208-262): min over active elements; writes the
+ * result to *dt_host (blocking). */
+int spdk_min_dt(spdk_ctx* ctx, const double* W_cv, spdk_sd_shape s, double* dt_host);
+
+/* ---- FV fallback stage kernels -------------------------------------------- */
+/* face_integral (src/transforms.cpp:435-481); T is an SD scratch of the F_fp
+ * shape with nt = 1 */
+int spdk_fv_face_integral(spdk_ctx* ctx, const double* F_fp, spdk_sd_shape s_fp, double* F, spdk_fv_shape sf,
+                          double* T, int t_id, int dim);
+/* fv_update_solution (src/transforms.cpp:483-543) */
+int spdk_fv_update(spdk_ctx* ctx, double* U_new, double* U_old, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
+                   const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id, double dt, int update);
+/* boundaries, FV cell-centred fields (src/boundary.cpp:123-190), all active axes x->y->z */
+int spdk_fill_fv_ghosts(spdk_ctx* ctx, double* U, spdk_fv_shape s);
+/* compute_primitives, FV (src/hydro.cpp:134-153) */
+int spdk_fv_primitives(spdk_ctx* ctx, const double* U, double* W, spdk_fv_shape s);
+/* detect_troubles (src/trouble_detection.cpp:261-290): NAD, SED per axis,
+ * relax, PAD. alpha_* are persistent per-axis scratch (zero-initialised). */
+int spdk_detect_troubles(spdk_ctx* ctx, const double* W_new, const double* W_old, double* troubles, double* alpha_x,
+                         double* alpha_y, double* alpha_z, spdk_fv_shape s, int pad, int limit_mask);
+/* apply_blending + blending_ring, or theta_from_troubles when blending is off
+ * (src/trouble_detection.cpp:191-242) */
+int spdk_blend_theta(spdk_ctx* ctx, const double* troubles, double* theta_tmp, double* theta, spdk_fv_shape s_tr);
+/* fallback_fluxes (src/hydro.cpp:753-788): MUSCL-Hancock + HLLC blended into
+ * the FV face fluxes; dt_sub = w_t * dt */
+int spdk_muscl_hllc_blend(spdk_ctx* ctx, const double* W, const double* theta, double* Fx, double* Fy, double* Fz,
+                          spdk_fv_shape s, double dt_sub);
+
+/* ---- fused element-tile path (hydro) -------------------------------------- */
+/* Device workspace for the fused path of a mesh with ghost-inclusive element
+ * counts N[3]; all scratch lives in caller memory. */
+size_t spdk_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
+/* One full SD step on U_sp (1,6,Nz,Ny,Nx,n,n,n) with the plain FP64 arithmetic
+ * of ADER_step/RK_step + update_solution (src/hydro_ader.hpp:291-325),
+ * followed by compute_primitives, sp->cv and compute_dt. Writes the next dt
+ * to *dt_next (blocking). When W_cv / F_x are non-NULL the step also writes
+ * the cv primitives and the x flux-point fluxes of its last sweep (outputs).
+ * Bit-identical to the stage sequence. */
+int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
+                          double* F_x, double* dt_next);
+
+/* ---- whole-run driver ------------------------------------------------------ */
+typedef struct spdk_run_result {
+    int steps;                 /* completed time steps                           */
+    long long dofs;            /* active solution points                         */
+    double evolution_seconds;  /* evolution loop, IO excluded                    */
+    double dof_updates_per_s;  /* dofs * steps / evolution_seconds               */
+    double t_final;            /* simulated time reached                         */
+    double kernel_ms_total;    /* device time of the dominant kernel (HIP events)*/
+    int kernel_launches;       /* launches of that kernel                        */
+    char kernel_name[64];
+} spdk_run_result;
+
+/* Runs the host driver (src/main.cpp:86-257) on an argv-style command line:
+ * "-i file.athinput" plus "block/name=value" overrides; outputs go to outdir
+ * (SPD_OUTPUT_DIR semantics). Extra keys: job/path = fused|stages. */
+int spdk_run_argv(int argc, const char* const* argv, const char* outdir, spdk_run_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,555 @@
+// FV fallback stage kernels: SD flux integration over sub-cell faces, the FV
+// update, FV ghost fills, trouble detection (NAD / SED / relax / PAD), theta
+// blending and the blended MUSCL-Hancock + HLLC face fluxes. Each kernel
+// restates the reference function cited at its launcher.
+#include <hip/hip_runtime.h>
+#include "ctx.hpp"
+#include "physics.hpp"
+
+using namespace spdk;
+static constexpr int BS = 256;
+
+struct Geo {  // FV sub-grid coordinates per axis (device pointers)
+    const double* f[3];
+    const double* c[3];
+};
+
+// ---------------------------------------------------------------------------
+// face_integral_ref: src/transforms.cpp:378-427 (at most one transverse axis active)
+__global__ void __launch_bounds__(BS) k_face_int_ref(const double* __restrict__ Ffp, SD s, double* __restrict__ F, FV sf,
+                                                     const double* __restrict__ S, int lds, int t_id, int dim, int dim1,
+                                                     int dim2, int a1, int a2, int gx, int gy, int gz, int fx, int fy,
+                                                     int fz, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Nx = s.Nx + (dim == 0), Ny = s.Ny + (dim == 1), Nz = s.Nz + (dim == 2);
+    int px = s.nx - (dim == 0), py = s.ny - (dim == 1), pz = s.nz - (dim == 2);
+    int k, j, i, kk, jj, ii;
+    split6(idx, Nz - 2 * gz, Ny - 2 * gy, Nx - 2 * gx, pz, py, px, k, j, i, kk, jj, ii);
+    k += gz; j += gy; i += gx;
+    int I = ii + fx + (i - gx) * px, J = jj + fy + (j - gy) * py, K = kk + fz + (k - gz) * pz;
+    if (!(K < sf.Nz && J < sf.Ny && I < sf.Nx)) return;
+    int q1 = dim1 == 0 ? px : (dim1 == 1 ? py : pz), q2 = dim2 == 0 ? px : (dim2 == 1 ? py : pz);
+    int id1 = dim1 == 0 ? ii : (dim1 == 1 ? jj : kk), id2 = dim2 == 0 ? ii : (dim2 == 1 ? jj : kk);
+    for (int v = 0; v < s.nv; v++) {
+        double f = 0;
+        for (int nn = 0; nn < q2; nn++) {
+            int a0 = dim2 == 0 ? nn : ii, b0 = dim2 == 1 ? nn : jj, c0 = dim2 == 2 ? nn : kk;
+            for (int ll = 0; ll < q1; ll++) {
+                int a = dim1 == 0 ? ll : a0, b = dim1 == 1 ? ll : b0, c = dim1 == 2 ? ll : c0;
+                double x = Ffp[s.off(t_id, v, k, j, i, c, b, a)];
+                if (a1) x *= S[id1 * lds + ll];
+                if (a2) x *= S[id2 * lds + nn];
+                f += x;
+            }
+        }
+        F[sf.off(v, K, J, I)] = f;
+    }
+}
+
+// face_integral, second sweep: src/transforms.cpp:466-480 (T holds the dim1 sweep)
+__global__ void __launch_bounds__(BS) k_face_int_scatter(const double* __restrict__ Tm, SD s, double* __restrict__ F,
+                                                         FV sf, const double* __restrict__ S, int lds, int dim, int dim2,
+                                                         int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Nx = s.Nx + (dim == 0), Ny = s.Ny + (dim == 1), Nz = s.Nz + (dim == 2);
+    int px = s.nx - (dim == 0), py = s.ny - (dim == 1), pz = s.nz - (dim == 2);
+    int k, j, i, kk, jj, ii;
+    split6(idx, Nz - 2 * gz, Ny - 2 * gy, Nx - 2 * gx, pz, py, px, k, j, i, kk, jj, ii);
+    k += gz; j += gy; i += gx;
+    int I = ii + fx + (i - gx) * px, J = jj + fy + (j - gy) * py, K = kk + fz + (k - gz) * pz;
+    if (!(K < sf.Nz && J < sf.Ny && I < sf.Nx)) return;
+    int q2 = dim2 == 0 ? px : (dim2 == 1 ? py : pz);
+    int id2 = dim2 == 0 ? ii : (dim2 == 1 ? jj : kk);
+    for (int v = 0; v < s.nv; v++) {
+        double f = 0;
+        for (int nn = 0; nn < q2; nn++) {
+            int a = dim2 == 0 ? nn : ii, b = dim2 == 1 ? nn : jj, c = dim2 == 2 ? nn : kk;
+            f += Tm[s.off(0, v, k, j, i, c, b, a)] * S[id2 * lds + nn];
+        }
+        F[sf.off(v, K, J, I)] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fv_update_solution: src/transforms.cpp:483-543
+struct FvUpd {
+    int act[3], grav;
+    double g[3];
+    double wdt;  // w(t_id)*dt
+    int update;
+};
+__global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, double* __restrict__ Uo, double* __restrict__ Ucv,
+                                                  SD s, const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                  const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz, Geo geo,
+                                                  FvUpd a, int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i, kk, jj, ii;
+    split6(idx, s.Nz - 2 * gz, s.Ny - 2 * gy, s.Nx - 2 * gx, s.nz, s.ny, s.nx, k, j, i, kk, jj, ii);
+    k += gz; j += gy; i += gx;
+    int I = ii + fx + (i - gx) * s.nx, J = jj + fy + (j - gy) * s.ny, K = kk + fz + (k - gz) * s.nz;
+    size_t slab = s.slab();
+    size_t o = s.off(0, 0, k, j, i, kk, jj, ii);
+    for (int v = 0; v < s.nv; v++) {
+        double h, F = 0;
+        double uo = Ucv[v * slab + o];
+        Uo[sc.off(v, K, J, I)] = uo;
+        if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
+        if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
+        if (a.act[2]) { h = geo.f[2][K + 1] - geo.f[2][K]; F += (Fz[sz.off(v, K + 1, J, I)] - Fz[sz.off(v, K, J, I)]) / h; }
+        if (a.grav) {
+            double S = 0.0;
+            if (v == 1) S = Ucv[o] * a.g[0];
+            else if (v == 2) S = Ucv[o] * a.g[1];
+            else if (v == 3) S = Ucv[o] * a.g[2];
+            else if (v == 4) S = Ucv[slab + o] * a.g[0] + Ucv[2 * slab + o] * a.g[1] + Ucv[3 * slab + o] * a.g[2];
+            F -= S;
+        }
+        double un = uo - a.wdt * F;
+        Un[sc.off(v, K, J, I)] = un;
+        if (a.update) Ucv[v * slab + o] = un;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundaries, FV cell-centred (src/boundary.cpp:123-190) along one axis.
+// Reads come from active cells and writes go to ghost cells, so one pass is race-free.
+__global__ void __launch_bounds__(BS) k_fv_bc(double* __restrict__ U, FV s, int dim, int type, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Nx = dim == 0 ? 2 : s.Nx, Ny = dim == 1 ? 2 : s.Ny;
+    int k, j, i;
+    split3(idx, Ny, Nx, k, j, i);
+    int l = dim == 0 ? i : (dim == 1 ? j : k);
+    int N = dim == 0 ? s.Nx : (dim == 1 ? s.Ny : s.Nz);
+    auto at = [&](int v, int ll) -> size_t {
+        if (dim == 0) return s.off(v, k, j, ll);
+        if (dim == 1) return s.off(v, k, ll, i);
+        return s.off(v, ll, j, i);
+    };
+    for (int v = 0; v < s.nv; v++) {
+        double BL, BR;
+        if (type == SPDK_BC_PERIODIC) { BL = U[at(v, N - 4 + l)]; BR = U[at(v, 2 + l)]; }
+        else if (type == SPDK_BC_GRADFREE) { BL = U[at(v, 2 + l)]; BR = U[at(v, N - 4 + l)]; }
+        else {
+            double sgn = (v == 1 + dim) ? -1.0 : 1.0;
+            BL = sgn * U[at(v, 3 - l)];
+            BR = sgn * U[at(v, N - 3 - l)];
+        }
+        U[at(v, l)] = BL;
+        U[at(v, N - 2 + l)] = BR;
+    }
+}
+
+// compute_primitives, FV: src/hydro.cpp:134-153
+__global__ void __launch_bounds__(BS) k_fv_prim(const double* __restrict__ U, double* __restrict__ W, FV s, double gm,
+                                                uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    size_t pl = s.plane();
+    double u[NVP], w[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) u[v] = U[v * pl + idx];
+    prim(u, w, gm);
+#pragma unroll
+    for (int v = 0; v < NVP; v++) W[v * pl + idx] = w[v];
+    for (int v = NVP; v < s.nv; v++) W[v * pl + idx] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// NAD (src/trouble_detection.cpp:35-104) on cells 1..N-2 and SED
+// (:107-148) on cells 2..N-3 of every active axis, one launch.
+__device__ __forceinline__ double alpha_fn(double dv, double dUm, double dU, double dUp) {
+    double vL = dU - dUm;
+    double ap = smin(1.0, smax(vL, 0.0) / dv);
+    double am = smin(1.0, smin(vL, 0.0) / dv);
+    double aL = (dv < 0.0 ? am : ap);
+    aL = (dv == 0.0 ? 1.0 : aL);
+    double vR = dUp - dU;
+    ap = smin(1.0, smax(vR, 0.0) / dv);
+    am = smin(1.0, smin(vR, 0.0) / dv);
+    double aR = (dv < 0.0 ? am : ap);
+    aR = (dv == 0.0 ? 1.0 : aR);
+    return smin(aL, aR);
+}
+struct Det {
+    int act[3];
+    int moore, delta, use_sed, pad, mask;
+    double tol;
+};
+__global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, const double* __restrict__ Wo,
+                                                double* __restrict__ tr, double* __restrict__ alx, double* __restrict__ aly,
+                                                double* __restrict__ alz, FV s, Det a, Geo geo, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i;
+    split3(idx, s.Ny, s.Nx, k, j, i);
+    int ax = a.act[0], ay = a.act[1], az = a.act[2];
+    int nvar = s.nv - 1;
+    bool in1 = (!ax || (i >= 1 && i <= s.Nx - 2)) && (!ay || (j >= 1 && j <= s.Ny - 2)) && (!az || (k >= 1 && k <= s.Nz - 2));
+    bool in2 = (!ax || (i >= 2 && i <= s.Nx - 3)) && (!ay || (j >= 2 && j <= s.Ny - 3)) && (!az || (k >= 2 && k <= s.Nz - 3));
+    for (int v = 0; v < nvar; v++) {
+        if (!((a.mask >> v) & 1)) continue;
+        if (in1) {
+            double un = Wn[s.off(v, k, j, i)];
+            double mx = Wo[s.off(v, k, j, i)], mn = mx;
+            if (a.moore) {
+                for (int dk = -az; dk <= az; dk++)
+                    for (int dj = -ay; dj <= ay; dj++)
+                        for (int di = -ax; di <= ax; di++) {
+                            double u = Wo[s.off(v, k + dk, j + dj, i + di)];
+                            mx = smax(mx, u);
+                            mn = smin(mn, u);
+                        }
+            } else {
+                double uL, uR;
+                if (ax) { uL = Wo[s.off(v, k, j, i - 1)]; uR = Wo[s.off(v, k, j, i + 1)]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (ay) { uL = Wo[s.off(v, k, j - 1, i)]; uR = Wo[s.off(v, k, j + 1, i)]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (az) { uL = Wo[s.off(v, k - 1, j, i)]; uR = Wo[s.off(v, k + 1, j, i)]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+            }
+            if (a.delta) {
+                double eps = a.tol * (mx - mn);
+                mn -= eps;
+                mx += eps;
+            } else {
+                mn -= fabs(mn) * a.tol;
+                mx += fabs(mx) * a.tol;
+            }
+            tr[s.off(v, k, j, i)] = (un > mx || un < mn) ? 1.0 : 0.0;
+        }
+        if (a.use_sed && in2) {
+            double u = Wn[s.off(v, k, j, i)];
+            for (int d = 0; d < 3; d++) {
+                if (!a.act[d]) continue;
+                const double* c = geo.c[d];
+                const double* f = geo.f[d];
+                int l = d == 0 ? i : (d == 1 ? j : k);
+                int di = d == 0, dj = d == 1, dk = d == 2;
+                double h = (c[l + 1] - c[l - 1]);
+                double hp = (c[l + 2] - c[l]);
+                double hm = (c[l] - c[l - 2]);
+                double du = (Wn[s.off(v, k + dk, j + dj, i + di)] - Wn[s.off(v, k - dk, j - dj, i - di)]) / h;
+                double dup = (Wn[s.off(v, k + 2 * dk, j + 2 * dj, i + 2 * di)] - u) / hp;
+                double dum = (u - Wn[s.off(v, k - 2 * dk, j - 2 * dj, i - 2 * di)]) / hm;
+                double d2u = (dup - dum) / (c[l + 1] - c[l - 1]);
+                double dv = 0.5 * d2u * (f[l + 1] - f[l]);
+                double* al = d == 0 ? alx : (d == 1 ? aly : alz);
+                al[s.off(v, k, j, i)] = alpha_fn(dv, dum, du, dup);
+            }
+        }
+    }
+}
+
+// relax_NAD (src/trouble_detection.cpp:152-183) then PAD_criteria (:244-259), cells 1..N-2
+__global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn, double* __restrict__ tr,
+                                                  const double* __restrict__ alx, const double* __restrict__ aly,
+                                                  const double* __restrict__ alz, FV s, Det a, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int ax = a.act[0], ay = a.act[1], az = a.act[2];
+    int Mx = s.Nx - 2 * ax, My = s.Ny - 2 * ay;
+    int k, j, i;
+    split3(idx, My, Mx, k, j, i);
+    k += az; j += ay; i += ax;
+    int nvar = s.nv - 1;
+    for (int v = 0; v < nvar; v++) {
+        if (!((a.mask >> v) & 1)) continue;
+        double alpha = 1;
+        if (ax) alpha = smin(smin(alx[s.off(v, k, j, i - 1)], alx[s.off(v, k, j, i)]), alx[s.off(v, k, j, i + 1)]);
+        if (ay) alpha = smin(alpha, smin(smin(aly[s.off(v, k, j - 1, i)], aly[s.off(v, k, j, i)]), aly[s.off(v, k, j + 1, i)]));
+        if (az) alpha = smin(alpha, smin(smin(alz[s.off(v, k - 1, j, i)], alz[s.off(v, k, j, i)]), alz[s.off(v, k + 1, j, i)]));
+        double t = tr[s.off(v, k, j, i)];
+        if (a.use_sed) tr[s.off(v, k, j, i)] = alpha < 1 ? t : 0;
+    }
+    double t = 0;
+    for (int v = 0; v < nvar; v++)
+        if ((a.mask >> v) & 1) t = smax(t, tr[s.off(v, k, j, i)]);
+    tr[s.off(nvar, k, j, i)] = t;
+    if (a.pad) {
+        double d = Wn[s.off(0, k, j, i)], p = Wn[s.off(4, k, j, i)];
+        if (d < kRhoMin || d > kRhoMax) tr[s.off(nvar, k, j, i)] = 1;
+        if (p < kPMin || p > kPMax) tr[s.off(nvar, k, j, i)] = 1;
+    }
+}
+
+// apply_blending (src/trouble_detection.cpp:191-210), cells 1..N-2
+__global__ void __launch_bounds__(BS) k_blend(const double* __restrict__ tr, double* __restrict__ th, FV s, int ax, int ay,
+                                              int az, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i;
+    split3(idx, s.Ny - 2 * ay, s.Nx - 2 * ax, k, j, i);
+    k += az; j += ay; i += ax;
+    const double w[4] = {1.0, 0.75, 0.5, 0.375};
+    int nvar = s.nv - 1;
+    double t = 0;
+    for (int dk = -az; dk <= az; dk++)
+        for (int dj = -ay; dj <= ay; dj++)
+            for (int di = -ax; di <= ax; di++) {
+                int n0 = (di != 0) + (dj != 0) + (dk != 0);
+                t = smax(t, w[n0] * tr[s.off(nvar, k + dk, j + dj, i + di)]);
+            }
+    th[((size_t)k * s.Ny + j) * s.Nx + i] = t;
+}
+// blending_ring (src/trouble_detection.cpp:226-242), cells 1..N-2
+__global__ void __launch_bounds__(BS) k_ring(const double* __restrict__ ti, double* __restrict__ to, FV s, int ax, int ay,
+                                             int az, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i;
+    split3(idx, s.Ny - 2 * ay, s.Nx - 2 * ax, k, j, i);
+    k += az; j += ay; i += ax;
+    auto at = [&](int kk, int jj, int ii) { return ((size_t)kk * s.Ny + jj) * s.Nx + ii; };
+    double t = ti[at(k, j, i)], ring = 0;
+    for (int dk = -az; dk <= az; dk++)
+        for (int dj = -ay; dj <= ay; dj++)
+            for (int di = -ax; di <= ax; di++)
+                if (ti[at(k + dk, j + dj, i + di)] > 0) ring = 0.25;
+    to[at(k, j, i)] = smax(t, ring);
+}
+// theta_from_troubles (src/trouble_detection.cpp:213-221)
+__global__ void __launch_bounds__(BS) k_theta_raw(const double* __restrict__ tr, double* __restrict__ th, FV s, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    th[idx] = tr[(size_t)(s.nv - 1) * s.plane() + idx];
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL-Hancock + HLLC fallback flux: src/muscl.hpp:2-13,
+// src/hydro.cpp:591-788
+__device__ __forceinline__ double minmod(double SL, double SR, double xL, double xR) {
+    double ratio = SR / SL;
+    ratio = smax(0.0, smin(ratio, 1.0));
+    double slope = ratio * SL;
+    return 0.5 * slope * (xR - xL);
+}
+// src/hydro.cpp:591-615
+__device__ __forceinline__ void corrector(const double* W, double* dWt, const double* dWx, const double* dWy,
+                                          const double* dWz, double gm) {
+    dWt[0] = 0;
+    dWt[4] = 0;
+    dWt[0] -= W[1] * dWx[0] + W[0] * dWx[1];
+    dWt[1] = -W[1] * dWx[1] - dWx[4] / W[0];
+    dWt[4] -= W[1] * dWx[4] + dWx[1] * gm * W[4];
+    dWt[1] -= W[2] * dWy[1];
+    dWt[1] -= W[3] * dWz[1];
+    dWt[0] -= W[2] * dWy[0] + W[0] * dWy[2];
+    dWt[2] = -W[2] * dWy[2] - dWy[4] / W[0];
+    dWt[4] -= W[2] * dWy[4] + dWy[2] * gm * W[4];
+    dWt[2] -= W[1] * dWx[2];
+    dWt[2] -= W[3] * dWz[2];
+    dWt[0] -= W[3] * dWz[0] + W[0] * dWz[3];
+    dWt[3] = -W[3] * dWz[3] - dWz[4] / W[0];
+    dWt[4] -= W[3] * dWz[4] + dWz[3] * gm * W[4];
+    dWt[3] -= W[1] * dWx[3];
+    dWt[3] -= W[2] * dWy[3];
+}
+// slopes_d<D> (src/hydro.cpp:622-681): reconstructed primitive states of
+// cell (k,j,i) at its two faces along D.
+template <int D>
+__device__ __forceinline__ void slopes_d(const double* __restrict__ W, FV s, const Geo& g, int k, int j, int i, double* WL,
+                                         double* WR, double dt, int ay, int az, double gm) {
+    double w[NVP], dwx[NVP], dwy[NVP], dwz[NVP], dwt[NVP];
+    const double *xc = g.c[0], *xf = g.f[0], *yc = g.c[1], *yf = g.f[1], *zc = g.c[2], *zf = g.f[2];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        w[v] = W[s.off(v, k, j, i)];
+        double wL = W[s.off(v, k, j, i - 1)], wR = W[s.off(v, k, j, i + 1)];
+        dwx[v] = minmod((wR - w[v]) / (xc[i + 1] - xc[i]), (w[v] - wL) / (xc[i] - xc[i - 1]), xf[i], xf[i + 1]);
+        dwy[v] = 0;
+        dwz[v] = 0;
+        if (ay) {
+            wL = W[s.off(v, k, j - 1, i)];
+            wR = W[s.off(v, k, j + 1, i)];
+            dwy[v] = minmod((wR - w[v]) / (yc[j + 1] - yc[j]), (w[v] - wL) / (yc[j] - yc[j - 1]), yf[j], yf[j + 1]);
+        }
+        if (az) {
+            wL = W[s.off(v, k - 1, j, i)];
+            wR = W[s.off(v, k + 1, j, i)];
+            dwz[v] = minmod((wR - w[v]) / (zc[k + 1] - zc[k]), (w[v] - wL) / (zc[k] - zc[k - 1]), zf[k], zf[k + 1]);
+        }
+    }
+    corrector(w, dwt, dwx, dwy, dwz, gm);
+    double h = (D == 0 ? xf[i + 1] - xf[i] : (D == 1 ? yf[j + 1] - yf[j] : zf[k + 1] - zf[k]));
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        double dw = (D == 0 ? dwx[v] : (D == 1 ? dwy[v] : dwz[v]));
+        WR[v] = w[v] - dw + dwt[v] * dt / h;
+        WL[v] = w[v] + dw + dwt[v] * dt / h;
+    }
+}
+// compute_fluxes<D> (src/hydro.cpp:683-751)
+template <int D>
+__device__ __forceinline__ void fb_face(const double* __restrict__ W, double* __restrict__ F, FV sF,
+                                        const double* __restrict__ th, FV s, const Geo& g, int k, int j, int i, double dt,
+                                        int ay, int az, double gm) {
+    double wL0[NVP], wR0[NVP], wL1[NVP], wR1[NVP], uL[NVP], uR[NVP], fL[NVP];
+    slopes_d<D>(W, s, g, k - (D == 2), j - (D == 1), i - (D == 0), wL0, wR0, dt, ay, az, gm);
+    slopes_d<D>(W, s, g, k, j, i, wL1, wR1, dt, ay, az, gm);
+    cons(wL0, uL, gm);
+    cons(wR1, uR, gm);
+    riemann_hllc<D>(uL, uR, fL, gm);
+    size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    size_t c1 = ((size_t)(k - (D == 2)) * s.Ny + (j - (D == 1))) * s.Nx + (i - (D == 0));
+    double t = smax(th[c0], th[c1]);
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        size_t o = sF.off(v, k, j, i);
+        double f = F[o];
+        f = f + t * (fL[v] - f);
+        F[o] = f;
+    }
+    for (int v = NVP; v < sF.nv; v++) {  // slot: HLLC slot taken as zero
+        size_t o = sF.off(v, k, j, i);
+        double f = F[o];
+        f = f + t * (0.0 - f);
+        F[o] = f;
+    }
+}
+__global__ void __launch_bounds__(BS) k_fallback(const double* __restrict__ W, const double* __restrict__ th,
+                                                 double* __restrict__ Fx, double* __restrict__ Fy, double* __restrict__ Fz,
+                                                 FV s, FV sx, FV sy, FV sz, Geo g, double dt, int ax, int ay, int az,
+                                                 double gm, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Mx = s.Nx - 4 * ax + ax, My = s.Ny - 4 * ay + ay;
+    int k, j, i;
+    split3(idx, My, Mx, k, j, i);
+    k += 2 * az; j += 2 * ay; i += 2 * ax;
+    fb_face<0>(W, Fx, sx, th, s, g, k, j, i, dt, ay, az, gm);
+    if (ay) fb_face<1>(W, Fy, sy, th, s, g, k, j, i, dt, ay, az, gm);
+    if (az) fb_face<2>(W, Fz, sz, th, s, g, k, j, i, dt, ay, az, gm);
+}
+
+// ===========================================================================
+static Geo geo_of(spdk_ctx* c) {
+    Geo g;
+    for (int d = 0; d < 3; d++) {
+        g.f[d] = c->fvf[d] ? c->fvf[d] : c->fvf[0];
+        g.c[d] = c->fvc[d] ? c->fvc[d] : c->fvc[0];
+    }
+    return g;
+}
+static FV fv_stag(const spdk_fv_shape& s, int d, int active) {
+    FV f(s);
+    if (active) {
+        if (d == 0) f.Nx += 1;
+        if (d == 1) f.Ny += 1;
+        if (d == 2) f.Nz += 1;
+    }
+    return f;
+}
+
+extern "C" {
+
+int spdk_fv_face_integral(spdk_ctx* c, const double* Ffp, spdk_sd_shape s_fp, double* F, spdk_fv_shape sf, double* T,
+                          int t_id, int dim) {
+    int dim1 = dim == 0 ? 1 : (dim == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
+    const int* act = c->prm.active;
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    int px = s_fp.nx - (dim == 0), py = s_fp.ny - (dim == 1), pz = s_fp.nz - (dim == 2);
+    int Nx = s_fp.Nx + (dim == 0), Ny = s_fp.Ny + (dim == 1), Nz = s_fp.Nz + (dim == 2);
+    uint32_t total = (uint32_t)((size_t)(Nz - 2 * gz) * (Ny - 2 * gy) * (Nx - 2 * gx) * pz * py * px);
+    const double* S = c->mat[SPDK_M_SP_TO_CV];
+    int lds = c->mcols[SPDK_M_SP_TO_CV];
+    if (!act[dim1] || !act[dim2]) {
+        hipLaunchKernelGGL(k_face_int_ref, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds,
+                           t_id, dim, dim1, dim2, act[dim1], act[dim2], gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
+        SPDK_LAUNCH_CHECK("face_integral_ref");
+        return SPDK_OK;
+    }
+    spdk_sd_shape st = s_fp;
+    st.nt = 1;
+    int rc = spdk_transform_1d(c, Ffp, s_fp, T, st, SPDK_M_SP_TO_CV, dim1, t_id);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_face_int_scatter, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, T, SD(st), F, FV(sf), S, lds,
+                       dim, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
+    SPDK_LAUNCH_CHECK("face_integral");
+    return SPDK_OK;
+}
+
+int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_shape s_cv, const double* Fx, const double* Fy,
+                   const double* Fz, spdk_fv_shape sc, int t_id, double dt, int update) {
+    FvUpd a;
+    for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
+    if (t_id < 0 || t_id >= c->T) { set_error("fv_update: slice"); return SPDK_E_ARG; }
+    a.wdt = c->wt_host[t_id] * dt;
+    a.update = update;
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);
+    hipLaunchKernelGGL(k_fv_update, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Un, Uo, Ucv, SD(s_cv), Fx, Fy ? Fy : Fx,
+                       Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]), fv_stag(sc, 2, a.act[2]),
+                       geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
+    SPDK_LAUNCH_CHECK("fv_update");
+    return SPDK_OK;
+}
+
+int spdk_fill_fv_ghosts(spdk_ctx* c, double* U, spdk_fv_shape s) {
+    for (int d = 0; d < 3; d++) {
+        if (!c->prm.active[d]) continue;
+        uint32_t total = (uint32_t)((size_t)(d == 2 ? 2 : s.Nz) * (d == 1 ? 2 : s.Ny) * (d == 0 ? 2 : s.Nx));
+        hipLaunchKernelGGL(k_fv_bc, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, FV(s), d, c->prm.bc[d], total);
+        SPDK_LAUNCH_CHECK("fill_fv_ghosts");
+    }
+    return SPDK_OK;
+}
+
+int spdk_fv_primitives(spdk_ctx* c, const double* U, double* W, spdk_fv_shape s) {
+    uint32_t total = (uint32_t)FV(s).plane();
+    hipLaunchKernelGGL(k_fv_prim, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, W, FV(s), c->prm.gamma, total);
+    SPDK_LAUNCH_CHECK("fv_primitives");
+    return SPDK_OK;
+}
+
+int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double* tr, double* alx, double* aly, double* alz,
+                         spdk_fv_shape s, int pad, int mask) {
+    Det a;
+    for (int d = 0; d < 3; d++) a.act[d] = c->prm.active[d];
+    a.moore = c->prm.nad_moore;
+    a.delta = c->prm.nad_delta;
+    a.use_sed = c->prm.sed && c->p > 1;
+    a.pad = pad;
+    a.mask = mask;
+    a.tol = c->prm.nad_tol;
+    uint32_t total = (uint32_t)FV(s).plane();
+    hipLaunchKernelGGL(k_nad_sed, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Wn, Wo, tr, alx, aly ? aly : alx,
+                       alz ? alz : alx, FV(s), a, geo_of(c), total);
+    SPDK_LAUNCH_CHECK("nad_sed");
+    uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * a.act[2]) * (s.Ny - 2 * a.act[1]) * (s.Nx - 2 * a.act[0]));
+    hipLaunchKernelGGL(k_relax_pad, dim3(nblocks(t2, BS)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx,
+                       alz ? alz : alx, FV(s), a, t2);
+    SPDK_LAUNCH_CHECK("relax_pad");
+    return SPDK_OK;
+}
+
+int spdk_blend_theta(spdk_ctx* c, const double* tr, double* th_tmp, double* th, spdk_fv_shape s) {
+    int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
+    if (c->prm.blending) {
+        uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * az) * (s.Ny - 2 * ay) * (s.Nx - 2 * ax));
+        hipLaunchKernelGGL(k_blend, dim3(nblocks(t2, BS)), dim3(BS), 0, c->stream, tr, th_tmp, FV(s), ax, ay, az, t2);
+        SPDK_LAUNCH_CHECK("blend");
+        hipLaunchKernelGGL(k_ring, dim3(nblocks(t2, BS)), dim3(BS), 0, c->stream, th_tmp, th, FV(s), ax, ay, az, t2);
+        SPDK_LAUNCH_CHECK("ring");
+    } else {
+        uint32_t t = (uint32_t)FV(s).plane();
+        hipLaunchKernelGGL(k_theta_raw, dim3(nblocks(t, BS)), dim3(BS), 0, c->stream, tr, th, FV(s), t);
+        SPDK_LAUNCH_CHECK("theta_raw");
+    }
+    return SPDK_OK;
+}
+
+int spdk_muscl_hllc_blend(spdk_ctx* c, const double* W, const double* th, double* Fx, double* Fy, double* Fz,
+                          spdk_fv_shape s, double dt_sub) {
+    int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
+    uint32_t total = (uint32_t)((size_t)(s.Nz - 4 * az + az) * (s.Ny - 4 * ay + ay) * (s.Nx - 4 * ax + ax));
+    hipLaunchKernelGGL(k_fallback, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, W, th, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx,
+                       FV(s), fv_stag(s, 0, ax), fv_stag(s, 1, ay), fv_stag(s, 2, az), geo_of(c), dt_sub, ax, ay, az,
+                       c->prm.gamma, total);
+    SPDK_LAUNCH_CHECK("muscl_hllc");
+    return SPDK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,390 @@
+// Host driver: the counterpart of src/main.cpp and the integrators of
+// src/hydro_ader.hpp / src/induction_ader.hpp. It parses the run, builds
+// geometry, operators and initial conditions on the host, and sequences the
+// device kernels through the C ABI of include/spdk.h.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+#include "../../../include/spdk.h"
+#include "params.hpp"
+#include "setup.hpp"
+#include "driver.hpp"
+
+namespace spdk_host {
+
+void check(int rc, const char* what) {
+    if (rc != SPDK_OK) throw std::runtime_error(std::string(what) + ": " + spdk_last_error());
+}
+#define CK(call) check((call), #call)
+
+double* DeviceArena::alloc(spdk_ctx* ctx, size_t count) {
+    void* p = nullptr;
+    CK(spdk_malloc(&p, count * sizeof(double)));
+    ptrs.push_back(p);
+    CK(spdk_memset0(ctx, p, count * sizeof(double)));
+    return (double*)p;
+}
+void DeviceArena::release() {
+    for (void* p : ptrs) spdk_free(p);
+    ptrs.clear();
+}
+void fill_kernel_timing(spdk_ctx* ctx, spdk_run_result* res) {
+    spdk_timing_read(ctx, &res->kernel_ms_total, &res->kernel_launches, res->kernel_name, sizeof(res->kernel_name));
+}
+
+size_t sd_count(const spdk_sd_shape& s) { return (size_t)s.nt * s.nv * s.Nz * s.Ny * s.Nx * s.nz * s.ny * s.nx; }
+size_t fv_count(const spdk_fv_shape& s) { return (size_t)s.nv * s.Nz * s.Ny * s.Nx; }
+
+spdk_ctx* make_context(const RunConfig& cfg, const Operators& op, const Axis* ax[3]) {
+    spdk_params prm;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.p = cfg.p;
+    for (int d = 0; d < 3; d++) {
+        prm.active[d] = cfg.act[d];
+        prm.bc[d] = cfg.bc[d];
+        prm.g[d] = cfg.g[d];
+        prm.h[d] = ax[d]->h;
+        prm.vel[d] = cfg.vel[d];
+    }
+    prm.n_ader = op.T;
+    prm.gamma = cfg.gamma;
+    prm.cfl = cfg.cfl;
+    prm.nu = cfg.nu;
+    prm.nad_tol = cfg.nad_tol;
+    prm.nad_delta = cfg.nad_delta;
+    prm.nad_moore = cfg.nad_moore;
+    prm.sed = cfg.sed;
+    prm.blending = cfg.blending;
+    spdk_operators ops;
+    std::memset(&ops, 0, sizeof(ops));
+    ops.mat[SPDK_M_SP_TO_FP] = op.sp_to_fp.v.data();
+    ops.mat[SPDK_M_FP_TO_SP] = op.fp_to_sp.v.data();
+    ops.mat[SPDK_M_DFP_TO_SP] = op.dfp_to_sp.v.data();
+    ops.mat[SPDK_M_SP_TO_CV] = op.sp_to_cv.v.data();
+    ops.mat[SPDK_M_CV_TO_SP] = op.cv_to_sp.v.data();
+    ops.mat[SPDK_M_FP_TO_CV] = op.fp_to_cv.v.data();
+    ops.mat[SPDK_M_INVADER] = op.invader.v.data();
+    ops.wt = op.wt.data();
+    for (int d = 0; d < 3; d++) {
+        ops.fv_faces[d] = ax[d]->fv_faces.data();
+        ops.fv_centers[d] = ax[d]->fv_centers.data();
+        ops.fv_nfaces[d] = (int)ax[d]->fv_faces.size();
+        ops.fv_ncells[d] = (int)ax[d]->fv_centers.size();
+    }
+    spdk_ctx* ctx = nullptr;
+    CK(spdk_ctx_create(&prm, &ops, nullptr, &ctx));
+    return ctx;
+}
+
+// ===========================================================================
+// Hydro integrator: src/hydro_ader.hpp:76-586
+struct HydroRun {
+    RunConfig& cfg;
+    std::string outdir;
+    Axis X, Y, Z;
+    Operators op;
+    spdk_ctx* ctx = nullptr;
+    DeviceArena mem;
+    int T = 1, n_stages = 1, n_output = 0, n_step = 0;
+    double rk_a[3] = {0, 0, 0};
+    double t = 0, dt = 0, Dt = 0;
+    bool fused = true;
+    spdk_sd_shape s_sp{}, s_ader{}, s_fp[3]{};
+    spdk_fv_shape s_fv{}, s_th{}, s_fvd[3]{};
+    double *U_sp = nullptr, *W_sp = nullptr, *W_cv = nullptr, *U_cv = nullptr, *T_sweep = nullptr, *U_ader = nullptr,
+           *U0 = nullptr;
+    double *U_fp[3] = {}, *F_fp[3] = {};
+    double *U_new = nullptr, *W_new = nullptr, *U_old = nullptr, *W_old = nullptr, *troubles = nullptr, *theta = nullptr,
+           *theta_tmp = nullptr;
+    double *F_fv[3] = {}, *alpha[3] = {}, *T_fp[3] = {};
+    void* ws = nullptr;
+    std::vector<double> host;  // staging for outputs
+    bool timing = false;
+
+    HydroRun(RunConfig& c, const std::string& od) : cfg(c), outdir(od) {}
+    ~HydroRun() {
+        if (ctx) spdk_ctx_sync(ctx);
+        mem.release();
+        if (ctx) spdk_ctx_destroy(ctx);
+    }
+
+    void setup() {
+        int p = cfg.p;
+        op.build(p, cfg.integrator == INT_ADER);
+        X.build(0, cfg.nx[0], cfg.act[0] ? cfg.nx[0] : 1, cfg.act[0] ? p : 0, 0, cfg.len[0], op.x_fp.data(), cfg.act[0]);
+        Y.build(1, cfg.nx[1], cfg.act[1] ? cfg.nx[1] : 1, cfg.act[1] ? p : 0, 0, cfg.len[1], op.x_fp.data(), cfg.act[1]);
+        Z.build(2, cfg.nx[2], cfg.act[2] ? cfg.nx[2] : 1, cfg.act[2] ? p : 0, 0, cfg.len[2], op.x_fp.data(), cfg.act[2]);
+        cfg.pp.hx = X.h;
+        cfg.pp.hy = Y.h;
+        cfg.pp.hz = Z.h;
+        if (cfg.outputs) write_dims(outdir, X, Y, Z);
+        T = op.T;
+        if (cfg.integrator == INT_RK) {
+            n_stages = cfg.rk_order;
+            if (cfg.rk_order == 2) rk_a[1] = 0.5;
+            if (cfg.rk_order == 3) { rk_a[1] = 0.75; rk_a[2] = 1.0 / 3.0; }
+        }
+        fused = cfg.path == "fused" && !cfg.fallback && cfg.nu == 0.0;
+        const Axis* ax[3] = {&X, &Y, &Z};
+        ctx = make_context(cfg, op, ax);
+        spdk_timing_enable(ctx, timing ? 1 : 0);
+
+        s_sp = spdk_sd_shape{1, kNVAR, Z.n_total, Y.n_total, X.n_total, Z.n_sp, Y.n_sp, X.n_sp};
+        s_ader = s_sp;
+        s_ader.nt = T;
+        for (int d = 0; d < 3; d++) {
+            s_fp[d] = s_ader;
+            if (cfg.act[d]) (d == 0 ? s_fp[d].nx : d == 1 ? s_fp[d].ny : s_fp[d].nz) = op.p + 2;
+        }
+        size_t nsp = sd_count(s_sp);
+        U_sp = mem.alloc(ctx, nsp);
+        W_cv = mem.alloc(ctx, nsp);
+        T_sweep = mem.alloc(ctx, nsp);
+        W_sp = mem.alloc(ctx, nsp);
+        F_fp[0] = mem.alloc(ctx, sd_count(s_fp[0]));  // output array in every path
+        if (!fused) {
+            U_cv = mem.alloc(ctx, nsp);
+            U_ader = mem.alloc(ctx, sd_count(s_ader));
+            if (cfg.integrator == INT_RK) U0 = mem.alloc(ctx, nsp);
+            for (int d = 0; d < 3; d++) {
+                if (!cfg.act[d]) continue;
+                U_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
+                if (d) F_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
+            }
+        } else {
+            int N[3] = {X.n_total, Y.n_total, Z.n_total};
+            size_t b = spdk_fused_workspace_bytes(ctx, N);
+            if (b == 0) throw std::runtime_error(std::string("fused workspace: ") + spdk_last_error());
+            void* w = nullptr;
+            CK(spdk_malloc(&w, b));
+            mem.ptrs.push_back(w);
+            ws = w;
+            if (cfg.integrator == INT_RK) U0 = nullptr;
+        }
+        if (cfg.fallback) {
+            s_fv = spdk_fv_shape{kNVAR, Z.fv_ncells, Y.fv_ncells, X.fv_ncells};
+            s_th = spdk_fv_shape{1, Z.fv_ncells, Y.fv_ncells, X.fv_ncells};
+            size_t nfv = fv_count(s_fv);
+            U_new = mem.alloc(ctx, nfv);
+            W_new = mem.alloc(ctx, nfv);
+            U_old = mem.alloc(ctx, nfv);
+            W_old = mem.alloc(ctx, nfv);
+            troubles = mem.alloc(ctx, nfv);
+            theta = mem.alloc(ctx, fv_count(s_th));
+            theta_tmp = mem.alloc(ctx, fv_count(s_th));
+            for (int d = 0; d < 3; d++) {
+                s_fvd[d] = s_fv;
+                if (cfg.act[d]) (d == 0 ? s_fvd[d].Nx : d == 1 ? s_fvd[d].Ny : s_fvd[d].Nz) += 1;
+                if (!cfg.act[d]) continue;
+                F_fv[d] = mem.alloc(ctx, fv_count(s_fvd[d]));
+                alpha[d] = mem.alloc(ctx, nfv);
+                spdk_sd_shape st = s_fp[d];
+                st.nt = 1;
+                T_fp[d] = mem.alloc(ctx, sd_count(st));
+            }
+        }
+        // initial conditions (src/hydro_ader.hpp:188-212)
+        initialize_hydro(cfg, X, Y, Z, op, host);
+        CK(spdk_memcpy_h2d(ctx, W_cv, host.data(), nsp * sizeof(double)));
+        CK(spdk_transform(ctx, W_cv, W_sp, T_sweep, s_sp, SPDK_M_CV_TO_SP));
+        CK(spdk_conservatives(ctx, W_sp, U_sp, s_sp));
+        CK(spdk_min_dt(ctx, W_cv, s_sp, &Dt));
+        std::printf("dx = %.17g dt = %.17g\n", X.h, Dt);
+        if (cfg.outputs) write_outputs();
+    }
+
+    void write_outputs() {  // src/hydro_ader.hpp:419-426
+        std::string f = output_folder(outdir);
+        std::printf("\nOUTPUT %d\n", n_output);
+        if (cfg.fallback) {
+            host.resize(fv_count(s_fv));
+            CK(spdk_memcpy_d2h(ctx, host.data(), troubles, host.size() * sizeof(double)));
+            write_raw(f + fv_name("troubles", s_fv.Nx, n_output), host.data(), host.size());
+        }
+        host.resize(sd_count(s_fp[0]));
+        CK(spdk_memcpy_d2h(ctx, host.data(), F_fp[0], host.size() * sizeof(double)));
+        write_raw(f + sd_name("F_ader_fp_x", s_fp[0].Nx, s_fp[0].nx, n_output), host.data(), host.size());
+        host.resize(sd_count(s_sp));
+        CK(spdk_memcpy_d2h(ctx, host.data(), W_cv, host.size() * sizeof(double)));
+        write_raw(f + sd_name("W_cv", s_sp.Nx, s_sp.nx, n_output), host.data(), host.size());
+        n_output++;
+    }
+
+    void solve_fluxes() {  // src/hydro_ader.hpp:278-289 (inviscid)
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_transform_1d(ctx, U_ader, s_ader, U_fp[d], s_fp[d], SPDK_M_SP_TO_FP, d, -1));
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_hydro_flux(ctx, U_fp[d], F_fp[d], s_fp[d], d));
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_fill_sd_traces(ctx, U_fp[d], s_fp[d], d));
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_riemann_llf(ctx, U_fp[d], F_fp[d], s_fp[d], d));
+    }
+    void correct() {
+        if (cfg.fallback) fv_update();
+        else CK(spdk_ader_correct(ctx, U_sp, U_ader, F_fp[0], F_fp[1], F_fp[2], s_ader, op.p + 2, dt));
+    }
+    void ader_step() {  // src/hydro_ader.hpp:291-305
+        CK(spdk_copy_ader(ctx, U_sp, U_ader, s_ader));
+        for (int a = 0; a < T; a++) {
+            solve_fluxes();
+            if (a < T - 1) CK(spdk_ader_predict(ctx, U_sp, U_ader, F_fp[0], F_fp[1], F_fp[2], s_ader, op.p + 2, dt));
+        }
+        correct();
+    }
+    void rk_step() {  // src/hydro_ader.hpp:313-325
+        CK(spdk_memcpy_d2d(ctx, U0, U_sp, sd_count(s_sp) * sizeof(double)));
+        for (int s = 0; s < n_stages; s++) {
+            CK(spdk_copy_ader(ctx, U_sp, U_ader, s_ader));
+            solve_fluxes();
+            correct();
+            if (rk_a[s] > 0) CK(spdk_rk_combine(ctx, U_sp, U0, s_sp, rk_a[s]));
+        }
+    }
+    void fv_update() {  // src/hydro_ader.hpp:500-575
+        const int mask = (1 << 0) | (1 << 4);
+        CK(spdk_transform(ctx, U_sp, U_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
+        for (int a = 0; a < T; a++) {
+            for (int d = 0; d < 3; d++)
+                if (cfg.act[d]) CK(spdk_fv_face_integral(ctx, F_fp[d], s_fp[d], F_fv[d], s_fvd[d], T_fp[d], a, d));
+            CK(spdk_fv_update(ctx, U_new, U_old, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], s_fv, a, dt, 0));
+            CK(spdk_fill_fv_ghosts(ctx, U_old, s_fv));
+            CK(spdk_fill_fv_ghosts(ctx, U_new, s_fv));
+            CK(spdk_fv_primitives(ctx, U_new, W_new, s_fv));
+            CK(spdk_fv_primitives(ctx, U_old, W_old, s_fv));
+            CK(spdk_detect_troubles(ctx, W_new, W_old, troubles, alpha[0], alpha[1], alpha[2], s_fv, 1, mask));
+            CK(spdk_fill_fv_ghosts(ctx, troubles, s_fv));
+            CK(spdk_blend_theta(ctx, troubles, theta_tmp, theta, s_fv));
+            CK(spdk_fill_fv_ghosts(ctx, theta, s_th));
+            CK(spdk_muscl_hllc_blend(ctx, W_old, theta, F_fv[0], F_fv[1], F_fv[2], s_fv, op.wt[a] * dt));
+            CK(spdk_fv_update(ctx, U_new, U_old, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], s_fv, a, dt, 1));
+        }
+        CK(spdk_transform(ctx, U_cv, U_sp, T_sweep, s_sp, SPDK_M_CV_TO_SP));
+    }
+
+    // One step and the dt that follows it (src/hydro_ader.hpp:235-243).
+    double step(bool output_due) {
+        if (fused) {
+            double next = 0;
+            CK(spdk_hydro_step_fused(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr, output_due ? F_fp[0] : nullptr,
+                                     &next));
+            return next;
+        }
+        if (cfg.integrator == INT_RK) rk_step();
+        else ader_step();
+        CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
+        CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
+        double next = 0;
+        CK(spdk_min_dt(ctx, W_cv, s_sp, &next));
+        return next;
+    }
+
+    void evolve(int warmup, spdk_run_result* res) {  // src/hydro_ader.hpp:215-273
+        dt = Dt;
+        double t_output = cfg.dt_output;
+        if (warmup > 0) {  // untimed steps; the state is restored afterwards
+            double* save = mem.alloc(ctx, sd_count(s_sp));
+            CK(spdk_memcpy_d2d(ctx, save, U_sp, sd_count(s_sp) * sizeof(double)));
+            for (int w = 0; w < warmup; w++) step(false);
+            CK(spdk_memcpy_d2d(ctx, U_sp, save, sd_count(s_sp) * sizeof(double)));
+        }
+        CK(spdk_ctx_sync(ctx));
+        spdk_timing_reset(ctx);
+        auto t0 = std::chrono::steady_clock::now();
+        double t_io = 0;
+        int step0 = n_step;
+        while (t < cfg.tlim) {
+            if (cfg.max_steps >= 0 && n_step - step0 >= cfg.max_steps) break;
+            bool out_due = cfg.outputs && (t + dt >= t_output);
+            double next = step(out_due);
+            t += dt;
+            n_step++;
+            dt = next;
+            if (cfg.outputs) {
+                if (t >= t_output) {
+                    t_output = t + cfg.dt_output;
+                    if (fused && !out_due) {  // keep W_cv current for the writer
+                        CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
+                        CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
+                    }
+                    CK(spdk_ctx_sync(ctx));
+                    auto a = std::chrono::steady_clock::now();
+                    write_outputs();
+                    t_io += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                }
+                if (t + dt > t_output) dt = t_output - t;
+            }
+        }
+        CK(spdk_ctx_sync(ctx));
+        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+        res->steps = n_step - step0;
+        res->dofs = (long long)(X.n * X.n_sp) * (Y.n * Y.n_sp) * (Z.n * Z.n_sp);
+        res->evolution_seconds = sec;
+        res->dof_updates_per_s = sec > 0 ? (double)res->dofs * res->steps / sec : 0;
+        res->t_final = t;
+        std::printf("\nevolution: %d steps, %g s (%g ms/step), %g zone-cycles/s\n", res->steps, sec,
+                    res->steps ? sec / res->steps * 1e3 : 0.0, res->dof_updates_per_s);
+    }
+};
+
+int run_hydro(RunConfig& cfg, const std::string& outdir, int warmup, spdk_run_result* res) {
+    HydroRun h(cfg, outdir);
+    h.timing = cfg.timing;
+    h.setup();
+    h.evolve(warmup, res);
+    fill_kernel_timing(h.ctx, res);
+    return 0;
+}
+
+}  // namespace spdk_host
+
+using namespace spdk_host;
+
+extern "C" int spdk_run_argv(int argc, const char* const* argv, const char* outdir, spdk_run_result* res) {
+    spdk_run_result local;
+    if (!res) res = &local;
+    std::memset(res, 0, sizeof(*res));
+    try {
+        Params pin;
+        std::string input;
+        std::vector<std::string> overrides;
+        for (int n = 0; n < argc; n++) {
+            std::string a = argv[n];
+            if (a == "-i" && n + 1 < argc) input = argv[++n];
+            else if (a.find('/') != std::string::npos && a.find('=') != std::string::npos) overrides.push_back(a);
+            else {
+                std::fprintf(stderr, "ERROR: unrecognized argument '%s'\n", a.c_str());
+                return SPDK_E_CONFIG;
+            }
+        }
+        if (!input.empty()) pin.load_file(input);
+        for (auto& o : overrides) pin.apply_override(o);
+        RunConfig cfg;
+        std::string err = configure(pin, cfg);
+        if (!err.empty()) {
+            std::fprintf(stderr, "ERROR: %s\n", err.c_str());
+            return SPDK_E_CONFIG;
+        }
+        std::string od = outdir ? std::string(outdir) : std::string("output");
+        std::printf("system = %s, ndim = %d, p = %d, N = (%d,%d,%d), integrator = %s, outputs = %s\n", cfg.system.c_str(),
+                    cfg.ndim, cfg.p, cfg.act[0] ? cfg.nx[0] : 1, cfg.act[1] ? cfg.nx[1] : 1, cfg.act[2] ? cfg.nx[2] : 1,
+                    cfg.integrator == INT_ADER ? "ader" : ("rk" + std::to_string(cfg.rk_order)).c_str(),
+                    cfg.outputs ? "on" : "off");
+        if (cfg.outputs) {
+            std::ofstream dump(output_folder(od) + "parameters.txt");
+            pin.dump(dump);
+        }
+        int warmup = pin.has("job", "warmup") ? pin.get_int("job", "warmup") : 0;
+        if (cfg.system == "hydro") return run_hydro(cfg, od, warmup, res);
+        if (cfg.system == "induction") return run_induction(cfg, od, warmup, res);
+        std::fprintf(stderr, "ERROR: unknown system '%s'\n", cfg.system.c_str());
+        return SPDK_E_CONFIG;
+    } catch (std::exception& e) {
+        std::fprintf(stderr, "ERROR: %s\n", e.what());
+        return SPDK_E_CONFIG;
+    }
+}

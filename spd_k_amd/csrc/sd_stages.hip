@@ -1,0 +1,650 @@
+// SD stage kernels: one thread per output (element, point), looping over
+// time slices and variables inside, in the reference's C-order layout.
+// Each kernel restates the arithmetic of the reference function cited at
+// its launcher; compiled with -ffp-contract=off they are bit-identical to it.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include "ctx.hpp"
+#include "physics.hpp"
+
+using namespace spdk;
+
+namespace spdk {
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return SPDK_E_HIP;
+}
+}  // namespace spdk
+
+static constexpr int BS = 256;
+
+// ---------------------------------------------------------------------------
+// transform_a_to_b_1d / _slice: src/transforms.cpp:120-177
+__global__ void __launch_bounds__(BS) k_transform_1d(const double* __restrict__ A, SD sa, double* __restrict__ B, SD sb,
+                                                     const double* __restrict__ M, int ldm, int dim, int t_src,
+                                                     uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i, kk, jj, ii;
+    split6(idx, sb.Nz, sb.Ny, sb.Nx, sb.nz, sb.ny, sb.nx, k, j, i, kk, jj, ii);
+    int q = dim == 0 ? sa.nx : (dim == 1 ? sa.ny : sa.nz);
+    int id = dim == 0 ? ii : (dim == 1 ? jj : kk);
+    int nt = t_src < 0 ? sb.nt : 1;
+    const double* Mrow = M + (size_t)id * ldm;
+    // source point stride along `dim` inside an element
+    size_t step = dim == 0 ? 1 : (dim == 1 ? (size_t)sa.nx : (size_t)sa.nx * sa.ny);
+    for (int tt = 0; tt < nt; tt++) {
+        int ts = t_src < 0 ? tt : t_src;
+        for (int v = 0; v < sb.nv; v++) {
+            size_t base = sa.off(ts, v, k, j, i, dim == 2 ? 0 : kk, dim == 1 ? 0 : jj, dim == 0 ? 0 : ii);
+            double u = 0;
+            for (int ll = 0; ll < q; ll++) u += A[base + ll * step] * Mrow[ll];
+            B[sb.off(tt, v, k, j, i, kk, jj, ii)] = u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compute_conservatives / compute_primitives: src/hydro.cpp:47-72, :108-132
+__global__ void __launch_bounds__(BS) k_cons_prim(const double* __restrict__ A, double* __restrict__ B, SD s, double gm,
+                                                  int to_prim, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    size_t slab = s.slab();
+    for (int t = 0; t < s.nt; t++) {
+        size_t o = (size_t)t * s.nv * slab + idx;
+        double a[NVP], b[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) a[v] = A[o + v * slab];
+        if (to_prim) prim(a, b, gm);
+        else cons(a, b, gm);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) B[o + v * slab] = b[v];
+        if (s.nv > NVP) B[o + NVP * slab] = 0.0;  // slot: zero-initialised local in the reference
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compute_fluxes_t<V1,V2,V3>: src/hydro.cpp:167-206
+template <int D>
+__global__ void __launch_bounds__(BS) k_flux(const double* __restrict__ U, double* __restrict__ F, SD s, double gm,
+                                             uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    size_t slab = s.slab();
+    for (int t = 0; t < s.nt; t++) {
+        size_t o = (size_t)t * s.nv * slab + idx;
+        double u[NVP], f[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) u[v] = U[o + v * slab];
+        state_flux<D>(u, f, gm);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) F[o + v * slab] = f[v];
+        if (s.nv > NVP) F[o + NVP * slab] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundaries (SD interface traces): src/boundary.cpp:48-113. One thread per
+// transverse (element, point) location of the ghost slab.
+__global__ void __launch_bounds__(BS) k_sd_bc(double* __restrict__ U, SD s, int dim, int type, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Mx = dim == 0 ? 1 : s.Nx, My = dim == 1 ? 1 : s.Ny, Mz = dim == 2 ? 1 : s.Nz;
+    int px = dim == 0 ? 1 : s.nx, py = dim == 1 ? 1 : s.ny, pz = dim == 2 ? 1 : s.nz;
+    int k, j, i, kk, jj, ii;
+    split6(idx, Mz, My, Mx, pz, py, px, k, j, i, kk, jj, ii);
+    int N = dim == 0 ? s.Nx : (dim == 1 ? s.Ny : s.Nz);
+    int n = dim == 0 ? s.nx : (dim == 1 ? s.ny : s.nz);
+    auto at = [&](int t, int v, int l, int ll) -> size_t {
+        if (dim == 0) return s.off(t, v, k, j, l, kk, jj, ll);
+        if (dim == 1) return s.off(t, v, k, l, i, kk, ll, ii);
+        return s.off(t, v, l, j, i, ll, jj, ii);
+    };
+    for (int t = 0; t < s.nt; t++)
+        for (int v = 0; v < s.nv; v++) {
+            double BL, BR;
+            if (type == SPDK_BC_PERIODIC) {
+                BL = U[at(t, v, N - 2, n - 1)];
+                BR = U[at(t, v, 1, 0)];
+            } else if (type == SPDK_BC_GRADFREE) {
+                BL = U[at(t, v, 1, 0)];
+                BR = U[at(t, v, N - 2, n - 1)];
+            } else {
+                double sgn = (v == 1 + dim) ? -1.0 : 1.0;
+                BL = sgn * U[at(t, v, 1, 0)];
+                BR = sgn * U[at(t, v, N - 2, n - 1)];
+            }
+            U[at(t, v, 0, n - 1)] = BL;
+            U[at(t, v, N - 1, 0)] = BR;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// sd_riemann_solver_t<D,...>: src/hydro.cpp:368-418
+template <int D>
+__global__ void __launch_bounds__(BS) k_riemann(const double* __restrict__ U, double* __restrict__ F, SD s, double gm,
+                                                uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Nx = s.Nx - (D == 0), Ny = s.Ny - (D == 1), Nz = s.Nz - (D == 2);
+    int px = D == 0 ? 1 : s.nx, py = D == 1 ? 1 : s.ny, pz = D == 2 ? 1 : s.nz;
+    int k, j, i, kk, jj, ii;
+    split6(idx, Nz, Ny, Nx, pz, py, px, k, j, i, kk, jj, ii);
+    int n = D == 0 ? s.nx : (D == 1 ? s.ny : s.nz);
+    size_t oL, oR;
+    if (D == 0) { oL = s.off(0, 0, k, j, i, kk, jj, n - 1); oR = s.off(0, 0, k, j, i + 1, kk, jj, 0); }
+    else if (D == 1) { oL = s.off(0, 0, k, j, i, kk, n - 1, ii); oR = s.off(0, 0, k, j + 1, i, kk, 0, ii); }
+    else { oL = s.off(0, 0, k, j, i, n - 1, jj, ii); oR = s.off(0, 0, k + 1, j, i, 0, jj, ii); }
+    size_t slab = s.slab();
+    for (int t = 0; t < s.nt; t++) {
+        size_t tb = (size_t)t * s.nv * slab;
+        double uL[NVP], uR[NVP], f[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) { uL[v] = U[tb + v * slab + oL]; uR[v] = U[tb + v * slab + oR]; }
+        riemann_llf<D>(uL, uR, f, gm);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) { F[tb + v * slab + oL] = f[v]; F[tb + v * slab + oR] = f[v]; }
+        if (s.nv > NVP) {
+            // slot: 0.5*(0+0) - 0.5*c*(uR5-uL5) with the wave speed of this face
+            double wL[NVP], wR[NVP];
+            prim(uL, wL, gm);
+            prim(uR, wR, gm);
+            double c = smax(sound_speed(wL[0], wL[4], gm) + fabs(wL[1 + D]), sound_speed(wR[0], wR[4], gm) + fabs(wR[1 + D]));
+            double s5 = 0.5 * (0.0 + 0.0) - 0.5 * c * (U[tb + NVP * slab + oR] - U[tb + NVP * slab + oL]);
+            F[tb + NVP * slab + oL] = s5;
+            F[tb + NVP * slab + oR] = s5;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// copy_ader: src/hydro_ader.hpp:428-444
+__global__ void __launch_bounds__(BS) k_copy_ader(const double* __restrict__ U, double* __restrict__ Ua, SD s,
+                                                  uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    size_t slab = s.slab();
+    for (int t = 0; t < s.nt; t++)
+        for (int v = 0; v < s.nv; v++) Ua[((size_t)t * s.nv + v) * slab + idx] = U[(size_t)v * slab + idx];
+}
+
+// gravity_source: src/transforms.cpp:17-27 (U is the ADER state, slice t)
+__device__ __forceinline__ double grav_src(const double* U, size_t tb, size_t slab, size_t o, int var, const double* g) {
+    if (var == 1) return U[tb + o] * g[0];
+    if (var == 2) return U[tb + o] * g[1];
+    if (var == 3) return U[tb + o] * g[2];
+    if (var == 4) return U[tb + slab + o] * g[0] + U[tb + 2 * slab + o] * g[1] + U[tb + 3 * slab + o] * g[2];
+    return 0.0;
+}
+
+struct PredArgs {
+    double g[3];
+    double h[3];
+    int act[3];
+    int grav;
+    double dt;
+};
+
+// dudt of one (point, var, t): the flux divergence with x, y, z interleaved
+// per flux point (src/transforms.cpp:301-307)
+__device__ __forceinline__ double divergence(const double* Fx, const double* Fy, const double* Fz, SD fx, SD fy, SD fz,
+                                             const double* Dm, int ldd, int q, const PredArgs& a, int t, int v, int k,
+                                             int j, int i, int kk, int jj, int ii) {
+    double d = 0;
+    for (int ll = 0; ll < q; ll++) {
+        if (a.act[0]) d += Fx[fx.off(t, v, k, j, i, kk, jj, ll)] * Dm[ii * ldd + ll] / a.h[0];
+        if (a.act[1]) d += Fy[fy.off(t, v, k, j, i, kk, ll, ii)] * Dm[jj * ldd + ll] / a.h[1];
+        if (a.act[2]) d += Fz[fz.off(t, v, k, j, i, ll, jj, ii)] * Dm[kk * ldd + ll] / a.h[2];
+    }
+    return d;
+}
+
+// update_prediction: src/transforms.cpp:263-325
+__global__ void __launch_bounds__(BS) k_predict(const double* __restrict__ U, double* __restrict__ Ua,
+                                                const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                const double* __restrict__ Fz, SD sa, SD fx, SD fy, SD fz,
+                                                const double* __restrict__ Dm, int ldd, const double* __restrict__ inv,
+                                                const double* __restrict__ w, int q, PredArgs a, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i, kk, jj, ii;
+    split6(idx, sa.Nz, sa.Ny, sa.Nx, sa.nz, sa.ny, sa.nx, k, j, i, kk, jj, ii);
+    int T = sa.nt;
+    size_t slab = sa.slab();
+    for (int v = 0; v < sa.nv; v++) {
+        double dudt[10];
+        double u_old = U[(size_t)v * slab + idx];
+        for (int t = 0; t < T; t++) {
+            dudt[t] = divergence(Fx, Fy, Fz, fx, fy, fz, Dm, ldd, q, a, t, v, k, j, i, kk, jj, ii);
+            if (a.grav) dudt[t] -= grav_src(Ua, (size_t)t * sa.nv * slab, slab, idx, v, a.g);
+        }
+        for (int t = 0; t < T; t++) {
+            double du = 0;
+            for (int ll = 0; ll < T; ll++) du += dudt[ll] * inv[t * T + ll] * w[ll] * a.dt;
+            Ua[((size_t)t * sa.nv + v) * slab + idx] = u_old - du;
+        }
+    }
+}
+
+// update_solution: src/transforms.cpp:327-374
+__global__ void __launch_bounds__(BS) k_correct(double* __restrict__ U, const double* __restrict__ Ua,
+                                                const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                const double* __restrict__ Fz, SD sa, SD fx, SD fy, SD fz,
+                                                const double* __restrict__ Dm, int ldd, const double* __restrict__ w,
+                                                int q, PredArgs a, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i, kk, jj, ii;
+    split6(idx, sa.Nz, sa.Ny, sa.Nx, sa.nz, sa.ny, sa.nx, k, j, i, kk, jj, ii);
+    int T = sa.nt;
+    size_t slab = sa.slab();
+    for (int v = 0; v < sa.nv; v++) {
+        double du = 0;
+        for (int t = 0; t < T; t++) {
+            double dudt = divergence(Fx, Fy, Fz, fx, fy, fz, Dm, ldd, q, a, t, v, k, j, i, kk, jj, ii);
+            if (a.grav) dudt -= grav_src(Ua, (size_t)t * sa.nv * slab, slab, idx, v, a.g);
+            du += dudt * w[t] * a.dt;
+        }
+        U[(size_t)v * slab + idx] -= du;
+    }
+}
+
+// combine_solution: src/transforms.cpp:252-261
+__global__ void __launch_bounds__(BS) k_combine(double* __restrict__ U, const double* __restrict__ U0, SD s, double a,
+                                                uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    size_t slab = s.slab();
+    for (int v = 0; v < s.nv; v++) {
+        size_t o = (size_t)v * slab + idx;
+        U[o] = a * U0[o] + (1.0 - a) * U[o];
+    }
+}
+
+// compute_dt: src/hydro.cpp:208-262, per-block min then a one-block pass.
+__global__ void __launch_bounds__(BS) k_dt_partial(const double* __restrict__ W, SD s, int gx, int gy, int gz,
+                                                   double gm, double cfl, double dx, double dy, double dz, double nu,
+                                                   int ax, int ay, int az, double* __restrict__ part, uint32_t total) {
+    __shared__ double red[BS];
+    double r = __builtin_inf();
+    int Mz = s.Nz - 2 * gz, My = s.Ny - 2 * gy, Mx = s.Nx - 2 * gx;
+    size_t slab = s.slab();
+    for (uint32_t idx = blockIdx.x * BS + threadIdx.x; idx < total; idx += gridDim.x * BS) {
+        int k, j, i, kk, jj, ii;
+        split6(idx, Mz, My, Mx, s.nz, s.ny, s.nx, k, j, i, kk, jj, ii);
+        size_t o = s.off(0, 0, k + gz, j + gy, i + gx, kk, jj, ii);
+        double c_s = sound_speed(W[o], W[4 * slab + o], gm);
+        double c_max = 0, dx_min = 1, dt_min;
+        if (ax) { c_max += (fabs(W[slab + o]) + c_s); dx_min = smin(dx_min, dx); }
+        if (ay) { c_max += (fabs(W[2 * slab + o]) + c_s); dx_min = smin(dx_min, dy); }
+        if (az) { c_max += (fabs(W[3 * slab + o]) + c_s); dx_min = smin(dx_min, dz); }
+        dt_min = cfl * dx_min / c_max / s.nx;
+        if (nu > 0.0) {
+            double dx_sub = dx_min / s.nx;
+            double dt_visc = 0.25 * cfl * dx_sub * dx_sub / nu;
+            dt_min = dt_min < dt_visc ? dt_min : dt_visc;
+        }
+        r = r < dt_min ? r : dt_min;
+    }
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(BS) k_min_final(double* __restrict__ part, int n) {
+    __shared__ double red[BS];
+    double r = __builtin_inf();
+    for (int i = threadIdx.x; i < n; i += BS) r = r < part[i] ? r : part[i];
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[0] = red[0];
+}
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char* spdk_last_error(void) { return spdk::g_err.c_str(); }
+
+int spdk_device_info(char* name, size_t len, int* cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { set_error("no HIP device"); return SPDK_E_NOGPU; }
+    hipDeviceProp_t p;
+    SPDK_CHECK(hipGetDeviceProperties(&p, dev));
+    if (name && len) { std::strncpy(name, p.gcnArchName, len - 1); name[len - 1] = 0; }
+    if (cus) *cus = p.multiProcessorCount;
+    return SPDK_OK;
+}
+
+int spdk_ctx_create(const spdk_params* prm, const spdk_operators* ops, void* stream, spdk_ctx** out) {
+    if (!prm || !ops || !out) { set_error("null argument"); return SPDK_E_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return SPDK_E_NOGPU; }
+    hipDeviceProp_t dp;
+    int dev = 0;
+    SPDK_CHECK(hipGetDevice(&dev));
+    SPDK_CHECK(hipGetDeviceProperties(&dp, dev));
+    if (std::strncmp(dp.gcnArchName, "gfx950", 6) != 0) {
+        set_error(std::string("libspdk is built for gfx950, device is ") + dp.gcnArchName);
+        return SPDK_E_NOGPU;
+    }
+    if (prm->p < 1 || prm->p > 8 || prm->n_ader < 1 || prm->n_ader > 10) { set_error("bad degree"); return SPDK_E_CONFIG; }
+    spdk_ctx* c = new spdk_ctx();
+    c->prm = *prm;
+    c->p = prm->p;
+    c->n = prm->p + 1;
+    c->m = prm->p + 2;
+    c->T = prm->n_ader;
+    for (int d = 0; d < 3; d++) {
+        c->ngh[d] = prm->active[d] ? 1 : 0;
+        c->fgh[d] = prm->active[d] ? 2 : 0;
+    }
+    if (stream) c->stream = (hipStream_t)stream;
+    else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; set_error("stream"); return SPDK_E_HIP; }
+        c->own_stream = true;
+    }
+    int n = c->n, m = c->m, T = c->T;
+    int rows[SPDK_M_COUNT] = {m, n, n, n, n, n, T}, cols[SPDK_M_COUNT] = {n, m, m, n, n, m, T};
+    for (int q = 0; q < SPDK_M_COUNT; q++) {
+        c->mrows[q] = rows[q];
+        c->mcols[q] = cols[q];
+        size_t b = sizeof(double) * rows[q] * cols[q];
+        SPDK_CHECK(hipMalloc(&c->mat[q], b));
+        if (ops->mat[q]) SPDK_CHECK(hipMemcpy(c->mat[q], ops->mat[q], b, hipMemcpyHostToDevice));
+        else SPDK_CHECK(hipMemset(c->mat[q], 0, b));
+    }
+    SPDK_CHECK(hipMalloc(&c->wt, sizeof(double) * T));
+    for (int t = 0; t < T; t++) c->wt_host[t] = ops->wt[t];
+    SPDK_CHECK(hipMemcpy(c->wt, ops->wt, sizeof(double) * T, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++) {
+        c->fvnf[d] = ops->fv_nfaces[d];
+        c->fvn[d] = ops->fv_ncells[d];
+        if (ops->fv_faces[d] && c->fvnf[d] > 0) {
+            SPDK_CHECK(hipMalloc(&c->fvf[d], sizeof(double) * c->fvnf[d]));
+            SPDK_CHECK(hipMemcpy(c->fvf[d], ops->fv_faces[d], sizeof(double) * c->fvnf[d], hipMemcpyHostToDevice));
+        }
+        if (ops->fv_centers[d] && c->fvn[d] > 0) {
+            SPDK_CHECK(hipMalloc(&c->fvc[d], sizeof(double) * c->fvn[d]));
+            SPDK_CHECK(hipMemcpy(c->fvc[d], ops->fv_centers[d], sizeof(double) * c->fvn[d], hipMemcpyHostToDevice));
+        }
+    }
+    SPDK_CHECK(hipMalloc(&c->red, sizeof(double) * (c->red_blocks + 8)));
+    SPDK_CHECK(hipHostMalloc(&c->red_host, sizeof(double) * 8, hipHostMallocDefault));
+    SPDK_CHECK(hipEventCreate(&c->ev0));
+    SPDK_CHECK(hipEventCreate(&c->ev1));
+    *out = c;
+    return SPDK_OK;
+}
+
+int spdk_ctx_destroy(spdk_ctx* c) {
+    if (!c) return SPDK_OK;
+    hipStreamSynchronize(c->stream);
+    for (int q = 0; q < SPDK_M_COUNT; q++) hipFree(c->mat[q]);
+    hipFree(c->wt);
+    for (int d = 0; d < 3; d++) { hipFree(c->fvf[d]); hipFree(c->fvc[d]); }
+    hipFree(c->red);
+    hipHostFree(c->red_host);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    delete c;
+    return SPDK_OK;
+}
+
+int spdk_ctx_sync(spdk_ctx* c) {
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    return SPDK_OK;
+}
+void* spdk_ctx_stream(spdk_ctx* c) { return (void*)c->stream; }
+
+int spdk_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return SPDK_E_ARG;
+    SPDK_CHECK(hipMalloc(ptr, bytes ? bytes : 8));
+    return SPDK_OK;
+}
+int spdk_free(void* ptr) {
+    SPDK_CHECK(hipFree(ptr));
+    return SPDK_OK;
+}
+int spdk_memset0(spdk_ctx* c, void* ptr, size_t bytes) {
+    SPDK_CHECK(hipMemsetAsync(ptr, 0, bytes, c->stream));
+    return SPDK_OK;
+}
+int spdk_memcpy_h2d(spdk_ctx* c, void* dst, const void* src, size_t bytes) {
+    SPDK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    return SPDK_OK;
+}
+int spdk_memcpy_d2h(spdk_ctx* c, void* dst, const void* src, size_t bytes) {
+    SPDK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    return SPDK_OK;
+}
+int spdk_memcpy_d2d(spdk_ctx* c, void* dst, const void* src, size_t bytes) {
+    SPDK_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return SPDK_OK;
+}
+
+static bool shape_ok(const spdk_sd_shape& s) {
+    return s.nt > 0 && s.nv > 0 && s.Nz > 0 && s.Ny > 0 && s.Nx > 0 && s.nz > 0 && s.ny > 0 && s.nx > 0 &&
+           (uint64_t)s.Nz * s.Ny * s.Nx * s.nz * s.ny * s.nx < (1ull << 32);
+}
+#define REQUIRE_SHAPE(s)                                                  \
+    do {                                                                  \
+        if (!shape_ok(s)) { set_error("bad SD shape"); return SPDK_E_SHAPE; } \
+    } while (0)
+
+int spdk_transform_1d(spdk_ctx* c, const double* A, spdk_sd_shape sa, double* B, spdk_sd_shape sb, int mat, int dim,
+                      int t_src) {
+    REQUIRE_SHAPE(sa);
+    REQUIRE_SHAPE(sb);
+    if (mat < 0 || mat >= SPDK_M_COUNT || dim < 0 || dim > 2 || !A || !B) { set_error("bad transform args"); return SPDK_E_ARG; }
+    int q = dim == 0 ? sa.nx : (dim == 1 ? sa.ny : sa.nz);
+    int nb = dim == 0 ? sb.nx : (dim == 1 ? sb.ny : sb.nz);
+    if (q != c->mcols[mat] || nb > c->mrows[mat] || sa.Nx != sb.Nx || sa.Ny != sb.Ny || sa.Nz != sb.Nz || sa.nv != sb.nv ||
+        (t_src >= sa.nt) || (t_src < 0 && sa.nt != sb.nt)) {
+        set_error("transform shape mismatch");
+        return SPDK_E_SHAPE;
+    }
+    for (int d = 0; d < 3; d++) {
+        int pa = d == 0 ? sa.nx : (d == 1 ? sa.ny : sa.nz), pb = d == 0 ? sb.nx : (d == 1 ? sb.ny : sb.nz);
+        if (d != dim && pa != pb) { set_error("transform transverse mismatch"); return SPDK_E_SHAPE; }
+    }
+    uint32_t total = (uint32_t)((size_t)sb.Nz * sb.Ny * sb.Nx * sb.nz * sb.ny * sb.nx);
+    hipLaunchKernelGGL(k_transform_1d, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat],
+                       c->mcols[mat], dim, t_src, total);
+    SPDK_LAUNCH_CHECK("transform_1d");
+    return SPDK_OK;
+}
+
+int spdk_transform(spdk_ctx* c, const double* A, double* B, double* T, spdk_sd_shape s, int mat) {
+    int dims[3], nd = 0;
+    for (int d = 0; d < 3; d++)
+        if (c->prm.active[d]) dims[nd++] = d;
+    const double* src = A;
+    for (int q = 0; q < nd; q++) {
+        double* dst = ((nd - 1 - q) % 2 == 0) ? B : T;
+        int rc = spdk_transform_1d(c, src, s, dst, s, mat, dims[q], -1);
+        if (rc) return rc;
+        src = dst;
+    }
+    return SPDK_OK;
+}
+
+int spdk_conservatives(spdk_ctx* c, const double* W, double* U, spdk_sd_shape s) {
+    REQUIRE_SHAPE(s);
+    if (s.nv < NVP) return SPDK_E_SHAPE;
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    hipLaunchKernelGGL(k_cons_prim, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, W, U, SD(s), c->prm.gamma, 0, total);
+    SPDK_LAUNCH_CHECK("conservatives");
+    return SPDK_OK;
+}
+int spdk_primitives(spdk_ctx* c, const double* U, double* W, spdk_sd_shape s) {
+    REQUIRE_SHAPE(s);
+    if (s.nv < NVP) return SPDK_E_SHAPE;
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    hipLaunchKernelGGL(k_cons_prim, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, W, SD(s), c->prm.gamma, 1, total);
+    SPDK_LAUNCH_CHECK("primitives");
+    return SPDK_OK;
+}
+
+int spdk_hydro_flux(spdk_ctx* c, const double* U, double* F, spdk_sd_shape s, int dim) {
+    REQUIRE_SHAPE(s);
+    if (s.nv < NVP || dim < 0 || dim > 2) return SPDK_E_ARG;
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    dim3 g(nblocks(total, BS));
+    if (dim == 0) hipLaunchKernelGGL(k_flux<0>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    if (dim == 1) hipLaunchKernelGGL(k_flux<1>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    if (dim == 2) hipLaunchKernelGGL(k_flux<2>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    SPDK_LAUNCH_CHECK("hydro_flux");
+    return SPDK_OK;
+}
+
+int spdk_fill_sd_traces(spdk_ctx* c, double* U, spdk_sd_shape s, int dim) {
+    REQUIRE_SHAPE(s);
+    if (dim < 0 || dim > 2) return SPDK_E_ARG;
+    int N = dim == 0 ? s.Nx : (dim == 1 ? s.Ny : s.Nz);
+    if (N < 3) { set_error("trace fill needs ghost elements"); return SPDK_E_SHAPE; }
+    uint32_t total = (uint32_t)((size_t)(dim == 0 ? 1 : s.Nx) * (dim == 1 ? 1 : s.Ny) * (dim == 2 ? 1 : s.Nz) *
+                                (dim == 0 ? 1 : s.nx) * (dim == 1 ? 1 : s.ny) * (dim == 2 ? 1 : s.nz));
+    hipLaunchKernelGGL(k_sd_bc, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, SD(s), dim, c->prm.bc[dim], total);
+    SPDK_LAUNCH_CHECK("fill_sd_traces");
+    return SPDK_OK;
+}
+
+int spdk_riemann_llf(spdk_ctx* c, const double* U, double* F, spdk_sd_shape s, int dim) {
+    REQUIRE_SHAPE(s);
+    if (s.nv < NVP || dim < 0 || dim > 2) return SPDK_E_ARG;
+    int Nx = s.Nx - (dim == 0), Ny = s.Ny - (dim == 1), Nz = s.Nz - (dim == 2);
+    int px = dim == 0 ? 1 : s.nx, py = dim == 1 ? 1 : s.ny, pz = dim == 2 ? 1 : s.nz;
+    uint32_t total = (uint32_t)((size_t)Nz * Ny * Nx * pz * py * px);
+    if (total == 0) return SPDK_OK;
+    dim3 g(nblocks(total, BS));
+    if (dim == 0) hipLaunchKernelGGL(k_riemann<0>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    if (dim == 1) hipLaunchKernelGGL(k_riemann<1>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    if (dim == 2) hipLaunchKernelGGL(k_riemann<2>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    SPDK_LAUNCH_CHECK("riemann_llf");
+    return SPDK_OK;
+}
+
+int spdk_copy_ader(spdk_ctx* c, const double* U, double* Ua, spdk_sd_shape s) {
+    REQUIRE_SHAPE(s);
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    hipLaunchKernelGGL(k_copy_ader, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, Ua, SD(s), total);
+    SPDK_LAUNCH_CHECK("copy_ader");
+    return SPDK_OK;
+}
+
+static void flux_shapes(spdk_ctx* c, const spdk_sd_shape& s, SD* f) {
+    for (int d = 0; d < 3; d++) {
+        f[d] = SD(s);
+        bool a = c->prm.active[d];
+        if (d == 0) f[d].nx = a ? c->m : s.nx;
+        if (d == 1) f[d].ny = a ? c->m : s.ny;
+        if (d == 2) f[d].nz = a ? c->m : s.nz;
+    }
+}
+static PredArgs pred_args(spdk_ctx* c, double dt) {
+    PredArgs a;
+    for (int d = 0; d < 3; d++) {
+        a.g[d] = c->prm.g[d];
+        a.h[d] = c->prm.h[d];
+        a.act[d] = c->prm.active[d];
+    }
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
+    a.dt = dt;
+    return a;
+}
+
+int spdk_ader_predict(spdk_ctx* c, const double* U, double* Ua, const double* Fx, const double* Fy, const double* Fz,
+                      spdk_sd_shape s, int q, double dt) {
+    REQUIRE_SHAPE(s);
+    if (s.nt > 10 || s.nt != c->T) { set_error("predict: slice count"); return SPDK_E_SHAPE; }
+    SD f[3];
+    flux_shapes(c, s, f);
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    hipLaunchKernelGGL(k_predict, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, Ua, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx,
+                       SD(s), f[0], f[1], f[2], c->mat[SPDK_M_DFP_TO_SP], c->mcols[SPDK_M_DFP_TO_SP],
+                       c->mat[SPDK_M_INVADER], c->wt, q, pred_args(c, dt), total);
+    SPDK_LAUNCH_CHECK("ader_predict");
+    return SPDK_OK;
+}
+
+int spdk_ader_correct(spdk_ctx* c, double* U, const double* Ua, const double* Fx, const double* Fy, const double* Fz,
+                      spdk_sd_shape s, int q, double dt) {
+    REQUIRE_SHAPE(s);
+    if (s.nt != c->T) { set_error("correct: slice count"); return SPDK_E_SHAPE; }
+    SD f[3];
+    flux_shapes(c, s, f);
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    hipLaunchKernelGGL(k_correct, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, Ua, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx,
+                       SD(s), f[0], f[1], f[2], c->mat[SPDK_M_DFP_TO_SP], c->mcols[SPDK_M_DFP_TO_SP], c->wt, q,
+                       pred_args(c, dt), total);
+    SPDK_LAUNCH_CHECK("ader_correct");
+    return SPDK_OK;
+}
+
+int spdk_rk_combine(spdk_ctx* c, double* U, const double* U0, spdk_sd_shape s, double a) {
+    REQUIRE_SHAPE(s);
+    uint32_t total = (uint32_t)SD(s).total_cells();
+    hipLaunchKernelGGL(k_combine, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, U0, SD(s), a, total);
+    SPDK_LAUNCH_CHECK("rk_combine");
+    return SPDK_OK;
+}
+
+int spdk_min_dt(spdk_ctx* c, const double* W, spdk_sd_shape s, double* dt_host) {
+    REQUIRE_SHAPE(s);
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    uint32_t total = (uint32_t)((size_t)(s.Nz - 2 * gz) * (s.Ny - 2 * gy) * (s.Nx - 2 * gx) * s.nz * s.ny * s.nx);
+    int nb = (int)std::min<size_t>(c->red_blocks, nblocks(total, BS));
+    if (nb < 1) nb = 1;
+    const spdk_params& P = c->prm;
+    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(BS), 0, c->stream, W, SD(s), gx, gy, gz, P.gamma, P.cfl, P.h[0], P.h[1],
+                       P.h[2], P.nu, P.active[0], P.active[1], P.active[2], c->red, total);
+    SPDK_LAUNCH_CHECK("dt_partial");
+    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb);
+    SPDK_LAUNCH_CHECK("min_final");
+    SPDK_CHECK(hipMemcpyAsync(c->red_host, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    if (dt_host) *dt_host = c->red_host[0];
+    return SPDK_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+int spdk_timing_enable(spdk_ctx* c, int on) {
+    if (!c) return SPDK_E_ARG;
+    c->timing = on != 0;
+    return SPDK_OK;
+}
+int spdk_timing_reset(spdk_ctx* c) {
+    if (!c) return SPDK_E_ARG;
+    c->kernel_ms = 0;
+    c->kernel_launches = 0;
+    return SPDK_OK;
+}
+int spdk_timing_read(spdk_ctx* c, double* ms, int* launches, char* name, size_t len) {
+    if (!c) return SPDK_E_ARG;
+    if (ms) *ms = c->kernel_ms;
+    if (launches) *launches = c->kernel_launches;
+    if (name && len) {
+        std::strncpy(name, c->kernel_name, len - 1);
+        name[len - 1] = 0;
+    }
+    return SPDK_OK;
+}
+}  // extern "C"

@@ -64,6 +64,7 @@ typedef struct spdk_params {
     int active[3];      /* axis x,y,z active                                   */
     int bc[3];          /* SPDK_BC_* per axis                                  */
     int n_ader;         /* time slices T: p+1 (ADER) or 1 (SSP-RK)             */
+    int rk_order;       /* SSP-RK stages (1..3) when n_ader == 1               */
     double gamma, cfl;
     double g[3];        /* constant gravity (source term)                      */
     double nu;          /* kinematic viscosity (dt clause only)               */

@@ -1,16 +1,633 @@
-// Fused element-tile path (placeholder until the sweep kernel lands).
+// Fused element-tile path for the hydro SD step (plain FP64).
+//
+// One Picard sweep of ADER_step / one RK stage (src/hydro_ader.hpp:291-325)
+// is one kernel over element tiles. A workgroup owns EPB whole elements and
+// keeps their state, flux-point values and fluxes in LDS; the only data that
+// crosses elements are the face traces (the solution interpolated to the
+// first and last flux point of every pencil), which each sweep writes for the
+// next one. The Riemann flux of a face is evaluated by both elements that
+// share it, from the same two traces, so both see identical bits.
+//
+// The arithmetic of every value (interpolation, Euler flux, LLF, the
+// x/y/z-interleaved flux divergence, predictor and corrector sums, cv
+// transform, dt) is performed in the same order as the stage kernels and the
+// reference, so this path is bit-identical to the stage path.
 #include <hip/hip_runtime.h>
+#include <cstring>
 #include "ctx.hpp"
+#include "physics.hpp"
+
+using namespace spdk;
+
+namespace {
+
+constexpr int kMaxT = 8;
+
+template <int N, int DIM>
+struct Tile {
+    static constexpr int M = N + 1;                                // flux points per axis
+    static constexpr int NT = DIM == 1 ? 1 : (DIM == 2 ? N : N * N);  // points per element face
+    static constexpr int TPE = N * NT;                             // solution points per element
+    static constexpr int EPB = TPE >= 256 ? 1 : 256 / TPE;         // elements per workgroup
+    static constexpr int BLOCK = ((EPB * TPE + 63) / 64) * 64;
+    static constexpr int FSZ = M * NT;                             // flux points per axis per element
+    // LDS per element (doubles): state + fluxes of every axis
+    static constexpr int LDS_U = NVP * TPE;
+    static constexpr int LDS_F = DIM * NVP * FSZ;
+    static constexpr int LDS_ELEM = LDS_U + LDS_F;
+    static_assert(BLOCK == 256, "tile workgroups are 256 threads");
+    static_assert(LDS_F >= LDS_U, "cv ping-pong reuses the flux area");
+};
+
+struct SweepArgs {
+    const double* U_sp;   // (1,6,E,TPE) solution (reference layout)
+    double* U_sp_out;     // corrector output (same array)
+    const double* U_in;   // (T,5,E,TPE) predictor state in (IN_ADER)
+    double* U_out;        // (T,5,E,TPE) predictor state out (OUT_PRED)
+    const double* tr_in[3];  // per axis (Tin,5,E,2,NT); Tin = 1 when traces are slice-independent
+    double* tr_out[3];       // per axis (T,5,E,2,NT)
+    double* U0;           // (5,E,TPE) RK step-start state
+    double* W_cv;         // optional (1,6,E,TPE)
+    double* F_x;          // optional (T,6,E,NT,M) x flux-point fluxes
+    double* part;         // per-block dt minima
+    const double* Dm;     // dfp_to_sp (N x M)
+    const double* S2F;    // sp_to_fp  (M x N)
+    const double* S2C;    // sp_to_cv  (N x N)
+    const double* inv;    // invader (T x T)
+    const double* w;      // time weights (T)
+    int Ne[3];            // elements per axis incl. ghosts (x,y,z)
+    int act[3], bc[3];
+    int E;
+    double h[3];
+    double dt, gm, cfl, rk_a;
+    int tin_slices;       // slices present in tr_in (1 or T)
+    int in_ader;          // state from U_in (else U_sp broadcast to every slice)
+    int out_pred;         // write predictor U_out + tr_out, else corrector
+    int rk_first;         // RK: write U0 = U_sp
+    int want_dt;          // corrector: compute cv primitives and dt partials
+};
+
+__device__ __forceinline__ int fidx(int d, int ll, int q, int N, int NT, int M) {
+    // LDS index of flux point ll of pencil q along axis d (see header comment)
+    if (d == 0) return q * M + ll;
+    if (d == 1) return ((q / N) * M + ll) * N + (q % N);
+    return ll * NT + q;
+}
+
+// Solution-point index of position ll along axis d of pencil q.
+template <int N, int DIM>
+__device__ __forceinline__ int spidx(int d, int ll, int q) {
+    if (DIM == 1) return ll;
+    if (DIM == 2) return d == 0 ? q * N + ll : ll * N + q;
+    if (d == 0) return q * N + ll;                          // q = kk*N + jj
+    if (d == 1) return ((q / N) * N + ll) * N + (q % N);   // q = kk*N + ii
+    return ll * N * N + q;                                  // q = jj*N + ii
+}
+
+template <int N, int DIM, int T>
+__global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
+    using G = Tile<N, DIM>;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB;
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int el = tid / TPE;      // element slot in the tile
+    const int s = tid % TPE;       // solution point within the element
+    const int e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < a.E;
+    double* us = lds + (el < EPB ? el : 0) * G::LDS_ELEM;
+    double* Fs = us + G::LDS_U;
+    const size_t E = (size_t)a.E;
+    const size_t slabE = E * TPE;  // one (t,v) plane of a state array
+
+    // element coordinates
+    int ei = 0, ej = 0, ek = 0;
+    if (live) {
+        ei = e % a.Ne[0];
+        ej = (e / a.Ne[0]) % a.Ne[1];
+        ek = e / (a.Ne[0] * a.Ne[1]);
+    }
+    const int ii = s % N, jj = DIM >= 2 ? (s / N) % N : 0, kk = DIM == 3 ? s / (N * N) : 0;
+
+    // derivative rows of this solution point
+    double Dx[M], Dy[M], Dz[M];
+#pragma unroll
+    for (int l = 0; l < M; l++) {
+        Dx[l] = a.Dm[ii * M + l];
+        Dy[l] = DIM >= 2 ? a.Dm[jj * M + l] : 0.0;
+        Dz[l] = DIM == 3 ? a.Dm[kk * M + l] : 0.0;
+    }
+    double u_old[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) u_old[v] = live ? a.U_sp[(size_t)v * slabE + (size_t)e * TPE + s] : 0.0;
+
+    double dudt[T][NVP];
+    const int nslices = a.in_ader ? T : 1;  // a broadcast state has identical slices
+
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        if (t >= nslices) break;
+        // ---- stage the slice in LDS
+        if (live) {
+#pragma unroll
+            for (int v = 0; v < NVP; v++)
+                us[v * TPE + s] = a.in_ader ? a.U_in[((size_t)t * NVP + v) * slabE + (size_t)e * TPE + s] : u_old[v];
+        }
+        __syncthreads();
+        // ---- pencils: interpolation to flux points, fluxes, face Riemann solves
+        if (live && s < DIM * NT) {
+            const int d = s / NT, q = s % NT;
+            const int ed = d == 0 ? ei : (d == 1 ? ej : ek);
+            const int Nd = a.Ne[d];
+            const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)a.Ne[0] : (size_t)a.Ne[0] * a.Ne[1]);
+            const size_t ebase = (size_t)e - (size_t)ed * stride;  // element with ed = 0 on this line
+            const int ts = a.tin_slices > 1 ? t : 0;
+            const double* tr = a.tr_in[d];
+            auto trace = [&](int ed_src, int side, int v) -> double {
+                size_t src = ebase + (size_t)ed_src * stride;
+                return tr[((((size_t)ts * NVP + v) * E + src) * 2 + side) * NT + q];
+            };
+            double uf[M][NVP];
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                double p[N];
+#pragma unroll
+                for (int l = 0; l < N; l++) p[l] = us[v * TPE + spidx<N, DIM>(d, l, q)];
+#pragma unroll
+                for (int f = 0; f < M; f++) {
+                    double x = 0;
+#pragma unroll
+                    for (int l = 0; l < N; l++) x += p[l] * a.S2F[f * N + l];
+                    uf[f][v] = x;
+                }
+            }
+            double* Fd = Fs + d * NVP * G::FSZ;
+            // interior flux points
+#pragma unroll
+            for (int f = 1; f < M - 1; f++) {
+                double fl[NVP];
+                if (d == 0) state_flux<0>(uf[f], fl, a.gm);
+                else if (d == 1) state_flux<1>(uf[f], fl, a.gm);
+                else state_flux<2>(uf[f], fl, a.gm);
+#pragma unroll
+                for (int v = 0; v < NVP; v++) Fd[v * G::FSZ + fidx(d, f, q, N, NT, M)] = fl[v];
+            }
+            // the two faces: ghost values follow src/boundary.cpp:48-113
+            const int type = a.bc[d];
+            double bl[NVP], br[NVP];  // values the trace fill writes at (0, M-1) and (Nd-1, 0)
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                double sg = (type == SPDK_BC_REFLECTIVE && v == 1 + d) ? -1.0 : 1.0;
+                if (type == SPDK_BC_PERIODIC) {
+                    bl[v] = (ed == 1 || ed == 0) ? trace(Nd - 2, 1, v) : 0.0;
+                    br[v] = (ed == Nd - 2 || ed == Nd - 1) ? trace(1, 0, v) : 0.0;
+                } else {
+                    bl[v] = (ed == 1 || ed == 0) ? sg * trace(1, 0, v) : 0.0;
+                    br[v] = (ed == Nd - 2 || ed == Nd - 1) ? sg * trace(Nd - 2, 1, v) : 0.0;
+                }
+            }
+            for (int side = 0; side < 2; side++) {
+                double fl[NVP];
+                const int f = side ? M - 1 : 0;
+                bool raw = side ? (ed == Nd - 1) : (ed == 0);
+                if (raw) {
+                    if (d == 0) state_flux<0>(uf[f], fl, a.gm);
+                    else if (d == 1) state_flux<1>(uf[f], fl, a.gm);
+                    else state_flux<2>(uf[f], fl, a.gm);
+                } else {
+                    double uL[NVP], uR[NVP];
+#pragma unroll
+                    for (int v = 0; v < NVP; v++) {
+                        if (side == 0) {
+                            uL[v] = (ed - 1 == 0) ? bl[v] : trace(ed - 1, 1, v);
+                            uR[v] = (ed == Nd - 1) ? br[v] : uf[0][v];
+                        } else {
+                            uL[v] = (ed == 0) ? bl[v] : uf[M - 1][v];
+                            uR[v] = (ed + 1 == Nd - 1) ? br[v] : trace(ed + 1, 0, v);
+                        }
+                    }
+                    if (d == 0) riemann_llf<0>(uL, uR, fl, a.gm);
+                    else if (d == 1) riemann_llf<1>(uL, uR, fl, a.gm);
+                    else riemann_llf<2>(uL, uR, fl, a.gm);
+                }
+#pragma unroll
+                for (int v = 0; v < NVP; v++) Fd[v * G::FSZ + fidx(d, f, q, N, NT, M)] = fl[v];
+            }
+        }
+        __syncthreads();
+        // ---- flux divergence at the solution points (src/transforms.cpp:301-307)
+        if (live) {
+            const int qx = DIM == 3 ? kk * N + jj : jj;
+            const int qy = DIM == 3 ? kk * N + ii : ii;
+            const int qz = jj * N + ii;
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                double acc = 0;
+                const double* Fx = Fs + v * G::FSZ;
+                const double* Fy = Fs + (NVP + v) * G::FSZ;
+                const double* Fz = Fs + (2 * NVP + v) * G::FSZ;
+#pragma unroll
+                for (int l = 0; l < M; l++) {
+                    acc += Fx[fidx(0, l, qx, N, NT, M)] * Dx[l] / a.h[0];
+                    if (DIM >= 2) acc += Fy[fidx(1, l, qy, N, NT, M)] * Dy[l] / a.h[1];
+                    if (DIM == 3) acc += Fz[fidx(2, l, qz, N, NT, M)] * Dz[l] / a.h[2];
+                }
+                dudt[t][v] = acc;
+            }
+            if (a.F_x && !a.out_pred) {  // x flux-point fluxes of the final sweep (outputs)
+                for (int idx = s; idx < G::FSZ; idx += TPE) {
+                    size_t o = (((size_t)t * NV) * E + e) * G::FSZ + idx;
+#pragma unroll
+                    for (int v = 0; v < NVP; v++) a.F_x[o + (size_t)v * E * G::FSZ] = Fs[v * G::FSZ + idx];
+                    a.F_x[o + (size_t)NVP * E * G::FSZ] = 0.0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!a.in_ader) {
+#pragma unroll
+        for (int t = 1; t < T; t++)
+#pragma unroll
+            for (int v = 0; v < NVP; v++) dudt[t][v] = dudt[0][v];
+    }
+
+    if (a.out_pred) {
+        // ---- predictor (src/transforms.cpp:316-322) and traces of the new slices
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            if (live) {
+#pragma unroll
+                for (int v = 0; v < NVP; v++) {
+                    double du = 0;
+#pragma unroll
+                    for (int l = 0; l < T; l++) du += dudt[l][v] * a.inv[t * T + l] * a.w[l] * a.dt;
+                    double un = u_old[v] - du;
+                    a.U_out[((size_t)t * NVP + v) * slabE + (size_t)e * TPE + s] = un;
+                    us[v * TPE + s] = un;
+                }
+            }
+            __syncthreads();
+            if (live && s < DIM * NT) {
+                const int d = s / NT, q = s % NT;
+#pragma unroll
+                for (int v = 0; v < NVP; v++) {
+                    double p[N];
+#pragma unroll
+                    for (int l = 0; l < N; l++) p[l] = us[v * TPE + spidx<N, DIM>(d, l, q)];
+                    double x0 = 0, x1 = 0;
+#pragma unroll
+                    for (int l = 0; l < N; l++) x0 += p[l] * a.S2F[l];
+#pragma unroll
+                    for (int l = 0; l < N; l++) x1 += p[l] * a.S2F[(M - 1) * N + l];
+                    double* to = a.tr_out[d] + ((((size_t)t * NVP + v) * E + e) * 2) * NT + q;
+                    to[0] = x0;
+                    to[NT] = x1;
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    // ---- corrector (src/transforms.cpp:354-372), optional SSP combine
+    double un[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        double du = 0;
+#pragma unroll
+        for (int t = 0; t < T; t++) du += dudt[t][v] * a.w[t] * a.dt;
+        un[v] = u_old[v] - du;
+    }
+    if (a.rk_a != 0.0 && live) {  // combine_solution (src/transforms.cpp:252-261)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            double u0 = a.U0[(size_t)v * slabE + (size_t)e * TPE + s];
+            un[v] = a.rk_a * u0 + (1.0 - a.rk_a) * un[v];
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            if (a.rk_first) a.U0[(size_t)v * slabE + (size_t)e * TPE + s] = u_old[v];
+            a.U_sp_out[(size_t)v * slabE + (size_t)e * TPE + s] = un[v];
+        }
+    }
+    if (!a.want_dt) return;
+    // ---- cv primitives: compute_primitives then sp->cv sweeps x, y, z
+    double* wa = us;  // ping (state area, free by now)
+    double* wb = Fs;  // pong
+    if (live) {
+        double wv[NVP];
+        prim(un, wv, a.gm);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) wa[v * TPE + s] = wv[v];
+    }
+    __syncthreads();
+    double* src = wa;
+    double* dst = wb;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        if (live) {
+            const int id = d == 0 ? ii : (d == 1 ? jj : kk);
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                double x = 0;
+#pragma unroll
+                for (int l = 0; l < N; l++) {
+                    int sl = d == 0 ? s - ii + l : (d == 1 ? s + (l - jj) * N : s + (l - kk) * N * N);
+                    x += src[v * TPE + sl] * a.S2C[id * N + l];
+                }
+                dst[v * TPE + s] = x;
+            }
+        }
+        __syncthreads();
+        double* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+    // src now holds W_cv of the element
+    double r = __builtin_inf();
+    if (live) {
+        double wc[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) wc[v] = src[v * TPE + s];
+        if (a.W_cv) {
+#pragma unroll
+            for (int v = 0; v < NVP; v++) a.W_cv[(size_t)v * slabE + (size_t)e * TPE + s] = wc[v];
+            a.W_cv[(size_t)NVP * slabE + (size_t)e * TPE + s] = 0.0;
+        }
+        bool active = (!a.act[0] || (ei >= 1 && ei <= a.Ne[0] - 2)) && (!a.act[1] || (ej >= 1 && ej <= a.Ne[1] - 2)) &&
+                      (!a.act[2] || (ek >= 1 && ek <= a.Ne[2] - 2));
+        if (active) {  // src/hydro.cpp:226-256 (inviscid)
+            double c_s = sound_speed(wc[0], wc[4], a.gm);
+            double c_max = 0, dx_min = 1;
+            if (a.act[0]) { c_max += (fabs(wc[1]) + c_s); dx_min = smin(dx_min, a.h[0]); }
+            if (a.act[1]) { c_max += (fabs(wc[2]) + c_s); dx_min = smin(dx_min, a.h[1]); }
+            if (a.act[2]) { c_max += (fabs(wc[3]) + c_s); dx_min = smin(dx_min, a.h[2]); }
+            r = a.cfl * dx_min / c_max / N;
+        }
+    }
+    __syncthreads();
+    double* red = lds;
+    red[tid] = r;
+    __syncthreads();
+    for (int wdt = G::BLOCK / 2; wdt > 0; wdt >>= 1) {
+        if (tid < wdt) red[tid] = red[tid] < red[tid + wdt] ? red[tid] : red[tid + wdt];
+        __syncthreads();
+    }
+    if (tid == 0) a.part[blockIdx.x] = red[0];
+}
+
+// Face traces of the solution U_sp (slice-independent), the input of a
+// step's first sweep.
+template <int N, int DIM>
+__global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U_sp, double* tr0,
+                                                               double* tr1, double* tr2, const double* __restrict__ S2F,
+                                                               int Eint) {
+    using G = Tile<N, DIM>;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB;
+    __shared__ double us[EPB * NVP * TPE];
+    const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
+    const int e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < Eint;
+    const size_t E = (size_t)Eint, slabE = E * TPE;
+    if (live)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) us[(el * NVP + v) * TPE + s] = U_sp[(size_t)v * slabE + (size_t)e * TPE + s];
+    __syncthreads();
+    if (live && s < DIM * NT) {
+        const int d = s / NT, q = s % NT;
+        double* trd = d == 0 ? tr0 : (d == 1 ? tr1 : tr2);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            double x0 = 0, x1 = 0;
+#pragma unroll
+            for (int l = 0; l < N; l++) {
+                double p = us[(el * NVP + v) * TPE + spidx<N, DIM>(d, l, q)];
+                x0 += p * S2F[l];
+            }
+#pragma unroll
+            for (int l = 0; l < N; l++) {
+                double p = us[(el * NVP + v) * TPE + spidx<N, DIM>(d, l, q)];
+                x1 += p * S2F[(M - 1) * N + l];
+            }
+            double* to = trd + (((size_t)v * E + e) * 2) * NT + q;
+            to[0] = x0;
+            to[NT] = x1;
+        }
+    }
+}
+
+__global__ void k_partial_min(double* part, int n) {
+    __shared__ double red[1024];
+    double r = __builtin_inf();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) r = r < part[i] ? r : part[i];
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+struct Layout {
+    size_t E = 0, tpe = 0, nt = 0, T = 0, dim = 0;
+    size_t off_uader = 0, off_trA = 0, off_trB = 0, off_u0 = 0, off_part = 0, total = 0;
+    size_t tr_axis = 0;  // doubles per axis per trace buffer
+    unsigned blocks = 0;
+};
+
+template <int N, int DIM>
+Layout layout_for(size_t E, int T) {
+    using G = Tile<N, DIM>;
+    Layout L;
+    L.E = E;
+    L.tpe = G::TPE;
+    L.nt = G::NT;
+    L.T = T;
+    L.dim = DIM;
+    L.blocks = (unsigned)((E + G::EPB - 1) / G::EPB);
+    size_t st = (size_t)T * NVP * E * G::TPE;
+    L.tr_axis = (size_t)T * NVP * E * 2 * G::NT;
+    size_t o = 0;
+    L.off_uader = o; o += st;
+    L.off_trA = o; o += DIM * L.tr_axis;
+    L.off_trB = o; o += DIM * L.tr_axis;
+    L.off_u0 = o; o += (size_t)NVP * E * G::TPE;
+    L.off_part = o; o += L.blocks + 64;
+    L.total = o;
+    return L;
+}
+
+template <int N, int DIM, int T>
+int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, double* W_cv, double* F_x,
+             double* dt_next) {
+    using G = Tile<N, DIM>;
+    size_t E = (size_t)Ne[0] * Ne[1] * Ne[2];
+    Layout L = layout_for<N, DIM>(E, T);
+    double* W = (double*)ws;
+    double* trA[3] = {nullptr, nullptr, nullptr};
+    double* trB[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < DIM; d++) {
+        trA[d] = W + L.off_trA + d * L.tr_axis;
+        trB[d] = W + L.off_trB + d * L.tr_axis;
+    }
+    SweepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.U_sp = U_sp;
+    a.U_sp_out = U_sp;
+    a.U0 = W + L.off_u0;
+    a.part = W + L.off_part;
+    a.Dm = c->mat[SPDK_M_DFP_TO_SP];
+    a.S2F = c->mat[SPDK_M_SP_TO_FP];
+    a.S2C = c->mat[SPDK_M_SP_TO_CV];
+    a.inv = c->mat[SPDK_M_INVADER];
+    a.w = c->wt;
+    for (int d = 0; d < 3; d++) {
+        a.Ne[d] = Ne[d];
+        a.act[d] = c->prm.active[d];
+        a.bc[d] = c->prm.bc[d];
+        a.h[d] = c->prm.h[d];
+    }
+    a.E = (int)E;
+    a.dt = dt;
+    a.gm = c->prm.gamma;
+    a.cfl = c->prm.cfl;
+    size_t lds = sizeof(double) * (size_t)G::EPB * G::LDS_ELEM;
+    if (lds < sizeof(double) * G::BLOCK) lds = sizeof(double) * G::BLOCK;
+    dim3 grid(L.blocks), block(G::BLOCK);
+    auto launch_sweep = [&](const SweepArgs& args) -> int {
+        if (c->timing) SPDK_CHECK(hipEventRecord(c->ev0, c->stream));
+        hipLaunchKernelGGL((k_sweep<N, DIM, T>), grid, block, lds, c->stream, args);
+        SPDK_LAUNCH_CHECK("k_sweep");
+        if (c->timing) {
+            SPDK_CHECK(hipEventRecord(c->ev1, c->stream));
+            SPDK_CHECK(hipEventSynchronize(c->ev1));
+            float ms = 0;
+            SPDK_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->kernel_ms += ms;
+            c->kernel_launches++;
+        }
+        return SPDK_OK;
+    };
+    auto traces0 = [&]() -> int {
+        hipLaunchKernelGGL((k_trace0<N, DIM>), grid, block, 0, c->stream, U_sp, trA[0], trA[1], trA[2], a.S2F, (int)E);
+        SPDK_LAUNCH_CHECK("k_trace0");
+        return SPDK_OK;
+    };
+    int rc;
+    if (T > 1) {  // ADER: K = T sweeps, the last one with the corrector
+        if ((rc = traces0())) return rc;
+        for (int k = 0; k < T; k++) {
+            SweepArgs s = a;
+            bool last = k == T - 1;
+            double** tin = (k % 2 == 0) ? trA : trB;
+            double** tout = (k % 2 == 0) ? trB : trA;
+            for (int d = 0; d < 3; d++) { s.tr_in[d] = tin[d]; s.tr_out[d] = tout[d]; }
+            s.tin_slices = k == 0 ? 1 : T;
+            s.in_ader = k > 0;
+            s.U_in = W + L.off_uader;
+            s.U_out = W + L.off_uader;
+            s.out_pred = !last;
+            s.want_dt = last;
+            s.W_cv = last ? W_cv : nullptr;
+            s.F_x = last ? F_x : nullptr;
+            if ((rc = launch_sweep(s))) return rc;
+        }
+    } else {  // SSP-RK stages (src/hydro_ader.hpp:313-325)
+        int order = c->prm.rk_order;
+        double rk[3] = {0, 0, 0};
+        if (order == 2) rk[1] = 0.5;
+        if (order == 3) { rk[1] = 0.75; rk[2] = 1.0 / 3.0; }
+        for (int st = 0; st < order; st++) {
+            if ((rc = traces0())) return rc;
+            SweepArgs s = a;
+            for (int d = 0; d < 3; d++) { s.tr_in[d] = trA[d]; s.tr_out[d] = nullptr; }
+            s.tin_slices = 1;
+            s.in_ader = 0;
+            s.out_pred = 0;
+            s.rk_a = rk[st];
+            s.rk_first = st == 0 && order > 1;
+            bool last = st == order - 1;
+            s.want_dt = last;
+            s.W_cv = last ? W_cv : nullptr;
+            s.F_x = last ? F_x : nullptr;
+            if ((rc = launch_sweep(s))) return rc;
+        }
+    }
+    hipLaunchKernelGGL(k_partial_min, dim3(1), dim3(1024), 0, c->stream, a.part, (int)L.blocks);
+    SPDK_LAUNCH_CHECK("k_partial_min");
+    SPDK_CHECK(hipMemcpyAsync(c->red_host, a.part, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    if (dt_next) *dt_next = c->red_host[0];
+    return SPDK_OK;
+}
+
+typedef int (*step_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, double*);
+typedef Layout (*layout_fn)(size_t, int);
+
+struct Variant {
+    int n, dim;
+    step_fn ader, rk;
+    layout_fn layout;
+};
+#define SPDK_VARIANT(Nn, D) {Nn, D, run_step<Nn, D, Nn>, run_step<Nn, D, 1>, layout_for<Nn, D>}
+const Variant kVariants[] = {SPDK_VARIANT(2, 1), SPDK_VARIANT(3, 1), SPDK_VARIANT(4, 1), SPDK_VARIANT(5, 1),
+                             SPDK_VARIANT(2, 2), SPDK_VARIANT(3, 2), SPDK_VARIANT(4, 2), SPDK_VARIANT(5, 2),
+                             SPDK_VARIANT(2, 3), SPDK_VARIANT(3, 3), SPDK_VARIANT(4, 3)};
+
+const Variant* find_variant(spdk_ctx* c) {
+    int dim = c->prm.active[0] + c->prm.active[1] + c->prm.active[2];
+    // the fused tile assumes active axes are x, x-y or x-y-z
+    if (!c->prm.active[0] || (c->prm.active[2] && !c->prm.active[1])) return nullptr;
+    for (const Variant& v : kVariants)
+        if (v.n == c->n && v.dim == dim) return &v;
+    return nullptr;
+}
+
+}  // namespace
+
 extern "C" {
-size_t spdk_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]) {
-    (void)ctx; (void)N;
-    spdk::set_error("fused path not available");
-    return 0;
+
+size_t spdk_fused_workspace_bytes(spdk_ctx* c, const int N[3]) {
+    const Variant* v = c ? find_variant(c) : nullptr;
+    if (!v) {
+        set_error("fused path: no tile variant for this degree/dimensionality");
+        return 0;
+    }
+    if (c->T > kMaxT || c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
+        set_error("fused path covers the inviscid, source-free update");
+        return 0;
+    }
+    Layout L = v->layout((size_t)N[0] * N[1] * N[2], c->T);
+    return L.total * sizeof(double);
 }
-int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
+
+int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
                           double* dt_next) {
-    (void)ctx; (void)U_sp; (void)s; (void)ws; (void)dt; (void)W_cv; (void)F_x; (void)dt_next;
-    spdk::set_error("fused path not available");
-    return SPDK_E_ARG;
+    if (!c || !U_sp || !ws) { set_error("fused: null argument"); return SPDK_E_ARG; }
+    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
+        set_error("fused path covers the inviscid, source-free update");
+        return SPDK_E_CONFIG;
+    }
+    const Variant* v = find_variant(c);
+    if (!v) { set_error("fused path: no tile variant for this degree/dimensionality"); return SPDK_E_CONFIG; }
+    if (s.nv != NV || s.nt != 1 || s.nx != c->n || s.ny != (c->prm.active[1] ? c->n : 1) ||
+        s.nz != (c->prm.active[2] ? c->n : 1)) {
+        set_error("fused: U_sp shape");
+        return SPDK_E_SHAPE;
+    }
+    for (int d = 0; d < 3; d++) {
+        int Nd = d == 0 ? s.Nx : (d == 1 ? s.Ny : s.Nz);
+        if (c->prm.active[d] && Nd < 3) { set_error("fused: axis needs ghost elements"); return SPDK_E_SHAPE; }
+    }
+    if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
+    if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next);
+    return v->rk(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next);
 }
-}
+
+}  // extern "C"

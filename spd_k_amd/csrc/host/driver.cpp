@@ -51,6 +51,7 @@ spdk_ctx* make_context(const RunConfig& cfg, const Operators& op, const Axis* ax
         prm.vel[d] = cfg.vel[d];
     }
     prm.n_ader = op.T;
+    prm.rk_order = cfg.integrator == INT_RK ? cfg.rk_order : 0;
     prm.gamma = cfg.gamma;
     prm.cfl = cfg.cfl;
     prm.nu = cfg.nu;
@@ -128,9 +129,13 @@ struct HydroRun {
             if (cfg.rk_order == 2) rk_a[1] = 0.5;
             if (cfg.rk_order == 3) { rk_a[1] = 0.75; rk_a[2] = 1.0 / 3.0; }
         }
-        fused = cfg.path == "fused" && !cfg.fallback && cfg.nu == 0.0;
         const Axis* ax[3] = {&X, &Y, &Z};
         ctx = make_context(cfg, op, ax);
+        fused = false;
+        if (cfg.path == "fused" && !cfg.fallback) {
+            int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
+            fused = spdk_fused_workspace_bytes(ctx, Nq) > 0;  // else the stage sequence runs
+        }
         spdk_timing_enable(ctx, timing ? 1 : 0);
 
         s_sp = spdk_sd_shape{1, kNVAR, Z.n_total, Y.n_total, X.n_total, Z.n_sp, Y.n_sp, X.n_sp};

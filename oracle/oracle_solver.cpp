@@ -1676,3 +1676,188 @@ int run_induction(Config& cfg, const char* outdir, double* evol_seconds, int* st
 }
 
 }  // namespace orc
+
+// ===========================================================================
+// Stage-level C entry points for the parity tests. Arrays are flat float64
+// in the reference layouts; shapes are int[8] (SD) or int[4] (FV).
+namespace {
+struct OrcCfg {  // mirrored by tests/oracle_py.py
+    int act[3];
+    int bc[3];
+    double gamma;
+    double g[3];
+    double nad_tol;
+    int nad_delta, nad_moore, sed, blending, p;
+};
+orc::Config to_cfg(const OrcCfg& c) {
+    orc::Config k;
+    for (int d = 0; d < 3; d++) {
+        k.act[d] = c.act[d] != 0;
+        k.ngh[d] = k.act[d] ? orc::NGH : 0;
+        k.fgh[d] = k.act[d] ? orc::nGH : 0;
+        k.bc[d] = c.bc[d];
+        k.g[d] = c.g[d];
+    }
+    k.ndim = (int)k.act[0] + (int)k.act[1] + (int)k.act[2];
+    k.gamma = c.gamma;
+    k.nad_tol = c.nad_tol;
+    k.nad_delta = c.nad_delta;
+    k.nad_moore = c.nad_moore;
+    k.sed = c.sed;
+    k.blending = c.blending;
+    k.p = c.p;
+    return k;
+}
+orc::SDA sda(const int* s, const double* data) {
+    orc::SDA a;
+    a.nt = s[0]; a.nv = s[1]; a.Nz = s[2]; a.Ny = s[3]; a.Nx = s[4]; a.nz = s[5]; a.ny = s[6]; a.nx = s[7];
+    a.a.assign(data, data + a.size());
+    return a;
+}
+orc::FVA fva(const int* s, const double* data) {
+    orc::FVA a;
+    a.nv = s[0]; a.Nz = s[1]; a.Ny = s[2]; a.Nx = s[3];
+    a.a.assign(data, data + a.size());
+    return a;
+}
+orc::Mat mat(const double* m, int r, int c) {
+    orc::Mat x;
+    x.init(r, c);
+    x.a.assign(m, m + (size_t)r * c);
+    return x;
+}
+void out(const std::vector<double>& v, double* o) { std::copy(v.begin(), v.end(), o); }
+}  // namespace
+
+extern "C" {
+
+int orc_transform_1d(const double* A, const int* sa, double* B, const int* sb, const double* M, int mr, int mc, int dim,
+                     int t_src) {
+    orc::SDA a = sda(sa, A), b = sda(sb, B);
+    orc::transform_1d(a, b, mat(M, mr, mc), dim, t_src);
+    out(b.a, B);
+    return 0;
+}
+int orc_fluxes(const OrcCfg* c, const double* U, const int* s, double* F, int dim) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s, U), f = sda(s, F);
+    orc::compute_fluxes(k, u, f, dim);
+    out(f.a, F);
+    return 0;
+}
+int orc_riemann(const OrcCfg* c, const double* U, double* F, const int* s, int dim) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s, U), f = sda(s, F);
+    orc::sd_riemann(k, u, f, dim);
+    out(f.a, F);
+    return 0;
+}
+int orc_sd_bc(const OrcCfg* c, double* U, const int* s, int dim) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s, U);
+    orc::sd_boundaries(k, u, dim);
+    out(u.a, U);
+    return 0;
+}
+int orc_predict(const OrcCfg* c, const double* U, const int* s_u, double* Ua, const int* s_a, const double* Fx,
+                const int* sx, const double* Fy, const int* sy, const double* Fz, const int* sz, const double* D, int dr,
+                int dc, const double* inv, const double* w, const double* h, double dt) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s_u, U), ua = sda(s_a, Ua), fx = sda(sx, Fx), fy = sda(sy, Fy), fz = sda(sz, Fz);
+    std::vector<double> wv(w, w + s_a[0]);
+    orc::update_prediction(k, u, ua, fx, fy, fz, mat(D, dr, dc), mat(inv, s_a[0], s_a[0]), wv, h[0], h[1], h[2], dt);
+    out(ua.a, Ua);
+    return 0;
+}
+int orc_correct(const OrcCfg* c, double* U, const int* s_u, const double* Ua, const int* s_a, const double* Fx,
+                const int* sx, const double* Fy, const int* sy, const double* Fz, const int* sz, const double* D, int dr,
+                int dc, const double* w, const double* h, double dt) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s_u, U), ua = sda(s_a, Ua), fx = sda(sx, Fx), fy = sda(sy, Fy), fz = sda(sz, Fz);
+    std::vector<double> wv(w, w + s_a[0]);
+    orc::update_solution(k, u, ua, fx, fy, fz, mat(D, dr, dc), wv, h[0], h[1], h[2], dt);
+    out(u.a, U);
+    return 0;
+}
+double orc_dt(const OrcCfg* c, const double* W, const int* s, double cfl, const double* h, double nu) {
+    orc::Config k = to_cfg(*c);
+    k.cfl = cfl;
+    orc::SDA w = sda(s, W);
+    return orc::compute_dt(k, w, h[0], h[1], h[2], nu);
+}
+int orc_detect(const OrcCfg* c, const double* Wn, const double* Wo, double* tr, double* ax, double* ay, double* az,
+               const int* sf, const double* const* fvc, const double* const* fvf, const int* nfc, int pad, int mask) {
+    orc::Config k = to_cfg(*c);
+    orc::FVA wn = fva(sf, Wn), wo = fva(sf, Wo), t = fva(sf, tr), a0 = fva(sf, ax), a1 = fva(sf, ay), a2 = fva(sf, az);
+    orc::Dim D[3];
+    for (int d = 0; d < 3; d++) {
+        D[d].p = c->p;
+        D[d].fvc.assign(fvc[d], fvc[d] + nfc[d]);
+        D[d].fvf.assign(fvf[d], fvf[d] + nfc[d] + 1);
+    }
+    orc::detect_troubles(k, wn, wo, t, a0, a1, a2, D[0], D[1], D[2], pad != 0, mask);
+    out(t.a, tr);
+    out(a0.a, ax);
+    out(a1.a, ay);
+    out(a2.a, az);
+    return 0;
+}
+int orc_fv_bc(const OrcCfg* c, double* U, const int* sf) {
+    orc::Config k = to_cfg(*c);
+    orc::FVA u = fva(sf, U);
+    orc::apply_fv_boundaries(k, u);
+    out(u.a, U);
+    return 0;
+}
+int orc_blend(const OrcCfg* c, const double* tr, double* th_tmp, double* th, const int* sf) {
+    orc::Config k = to_cfg(*c);
+    int s1[4] = {1, sf[1], sf[2], sf[3]};
+    orc::FVA t = fva(sf, tr), a = fva(s1, th_tmp), b = fva(s1, th);
+    if (k.blending) {
+        orc::apply_blending(k, t, a);
+        orc::blending_ring(k, a, b);
+    } else
+        orc::theta_from_troubles(t, b);
+    out(a.a, th_tmp);
+    out(b.a, th);
+    return 0;
+}
+int orc_fallback(const OrcCfg* c, const double* W, const double* th, double* Fx, double* Fy, double* Fz, const int* sf,
+                 const double* const* fvc, const double* const* fvf, const int* nfc, double dt_sub) {
+    orc::Config k = to_cfg(*c);
+    int s1[4] = {1, sf[1], sf[2], sf[3]};
+    int sx[4] = {sf[0], sf[1], sf[2], sf[3] + (k.act[0] ? 1 : 0)};
+    int sy[4] = {sf[0], sf[1], sf[2] + (k.act[1] ? 1 : 0), sf[3]};
+    int sz[4] = {sf[0], sf[1] + (k.act[2] ? 1 : 0), sf[2], sf[3]};
+    orc::FVA w = fva(sf, W), t = fva(s1, th), fx = fva(sx, Fx), fy = fva(sy, Fy), fz = fva(sz, Fz);
+    std::vector<double> C[3], F[3];
+    for (int d = 0; d < 3; d++) {
+        C[d].assign(fvc[d], fvc[d] + nfc[d]);
+        F[d].assign(fvf[d], fvf[d] + nfc[d] + 1);
+    }
+    orc::FVGeom g{&C[0], &F[0], &C[1], &F[1], &C[2], &F[2]};
+    orc::fallback_fluxes(k, w, t, g, fx, fy, fz, dt_sub);
+    out(fx.a, Fx);
+    out(fy.a, Fy);
+    out(fz.a, Fz);
+    return 0;
+}
+
+// point functions with closed forms checked by the tests
+double orc_minmod(double SL, double SR, double xL, double xR) { return orc::minmod(SL, SR, xL, xR); }
+double orc_alpha(double dv, double dUm, double dU, double dUp) { return orc::Alpha(dv, dUm, dU, dUp); }
+int orc_llf(const double* UL, const double* UR, double* F, int dim, double gm) {
+    int v1, v2, v3;
+    orc::vel_idx(dim, v1, v2, v3);
+    orc::riemann_llf(F, UL, UR, v1, v2, v3, gm);
+    return 0;
+}
+int orc_hllc(const double* UL, const double* UR, double* F, int dim, double gm) {
+    int v1, v2, v3;
+    orc::vel_idx(dim, v1, v2, v3);
+    orc::riemann_hllc(F, UL, UR, v1, v2, v3, gm);
+    return 0;
+}
+double orc_normal(unsigned long long seed, unsigned long long cell) { return orc::normal_sample(seed, cell); }
+
+}  // extern "C"

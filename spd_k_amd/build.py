@@ -22,7 +22,7 @@ HOST_SOURCES = ["csrc/host/params.cpp", "csrc/host/setup.cpp", "csrc/host/driver
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-Wall", "-Wno-unused-function"]
 
 
 def _run(cmd):
@@ -52,7 +52,7 @@ def build(verbose=False):
     objs = [c[-1] for c in jobs]
     lib = os.path.join(BUILD, "libspdk.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, lib)
     exe = os.path.join(BUILD, "spdk")
     _run([CXX, "-O2", os.path.join(HERE, "csrc/host/main.cpp"), "-o", exe, "-L" + BUILD, "-lspdk",

@@ -28,6 +28,9 @@ struct spdk_ctx {
     int kernel_launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     char kernel_name[64] = "";
+    static constexpr int kEvPool = 64;
+    hipEvent_t evs[kEvPool][2] = {};
+    int ev_used = 0;  // event pairs recorded since the last harvest
 };
 
 namespace spdk {

@@ -498,18 +498,13 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     size_t lds = sizeof(double) * (size_t)G::EPB * G::LDS_ELEM;
     if (lds < sizeof(double) * G::BLOCK) lds = sizeof(double) * G::BLOCK;
     dim3 grid(L.blocks), block(G::BLOCK);
+    // timing: one event pair per sweep launch, harvested after the step's own sync
     auto launch_sweep = [&](const SweepArgs& args) -> int {
-        if (c->timing) SPDK_CHECK(hipEventRecord(c->ev0, c->stream));
+        bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
+        if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
         hipLaunchKernelGGL((k_sweep<N, DIM, T>), grid, block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_sweep");
-        if (c->timing) {
-            SPDK_CHECK(hipEventRecord(c->ev1, c->stream));
-            SPDK_CHECK(hipEventSynchronize(c->ev1));
-            float ms = 0;
-            SPDK_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            c->kernel_ms += ms;
-            c->kernel_launches++;
-        }
+        if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
         return SPDK_OK;
     };
     auto traces0 = [&]() -> int {
@@ -562,6 +557,13 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     SPDK_CHECK(hipMemcpyAsync(c->red_host, a.part, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SPDK_CHECK(hipStreamSynchronize(c->stream));
     if (dt_next) *dt_next = c->red_host[0];
+    for (int i = 0; i < c->ev_used; i++) {
+        float ms = 0;
+        SPDK_CHECK(hipEventElapsedTime(&ms, c->evs[i][0], c->evs[i][1]));
+        c->kernel_ms += ms;
+        c->kernel_launches++;
+    }
+    c->ev_used = 0;
     return SPDK_OK;
 }
 

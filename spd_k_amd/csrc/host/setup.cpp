@@ -1,8 +1,10 @@
 #include "setup.hpp"
 #include <sys/stat.h>
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <stdexcept>
+#include <thread>
 
 namespace spdk_host {
 
@@ -428,8 +430,12 @@ void initialize_hydro(const RunConfig& c, const Axis& X, const Axis& Y, const Ax
     size_t pts = (size_t)px * py * pz, cells = (size_t)X.n_total * Y.n_total * Z.n_total * pts;
     W.assign(cells * kNVAR, 0.0);
     long long ncx = (long long)X.n * X.n_sp, ncy = (long long)Y.n * Y.n_sp;
-    for (int k = 0; k < Z.n_total; k++)
-        for (int j = 0; j < Y.n_total; j++)
+    // every (element, point) is independent: element rows are shared out
+    // over host threads
+    const int rows = Z.n_total * Y.n_total;
+    auto work = [&](int r0, int r1) {
+    for (int r = r0; r < r1; r++) {
+        const int k = r / Y.n_total, j = r % Y.n_total;
             for (int i = 0; i < X.n_total; i++)
                 for (int kk = 0; kk < pz; kk++)
                     for (int jj = 0; jj < py; jj++)
@@ -464,6 +470,13 @@ void initialize_hydro(const RunConfig& c, const Axis& X, const Axis& Y, const Ax
                                 W[(size_t)var * cells + cell] = value;
                             }
                         }
+    }
+    };
+    unsigned nth = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    if (rows < 64) nth = 1;
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++) pool.emplace_back(work, (int)((long long)rows * t / nth), (int)((long long)rows * (t + 1) / nth));
+    for (auto& th : pool) th.join();
 }
 
 // ---------------------------------------------------------------------------

@@ -385,6 +385,8 @@ int spdk_ctx_create(const spdk_params* prm, const spdk_operators* ops, void* str
     SPDK_CHECK(hipHostMalloc(&c->red_host, sizeof(double) * 8, hipHostMallocDefault));
     SPDK_CHECK(hipEventCreate(&c->ev0));
     SPDK_CHECK(hipEventCreate(&c->ev1));
+    for (int i = 0; i < spdk_ctx::kEvPool; i++)
+        for (int j = 0; j < 2; j++) SPDK_CHECK(hipEventCreate(&c->evs[i][j]));
     *out = c;
     return SPDK_OK;
 }
@@ -397,6 +399,9 @@ int spdk_ctx_destroy(spdk_ctx* c) {
     for (int d = 0; d < 3; d++) { hipFree(c->fvf[d]); hipFree(c->fvc[d]); }
     hipFree(c->red);
     hipHostFree(c->red_host);
+    for (int i = 0; i < spdk_ctx::kEvPool; i++)
+        for (int j = 0; j < 2; j++)
+            if (c->evs[i][j]) hipEventDestroy(c->evs[i][j]);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->own_stream) hipStreamDestroy(c->stream);

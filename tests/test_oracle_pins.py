@@ -1,0 +1,173 @@
+"""Pins of the CPU oracle against what fixes it independently of its own code:
+the reference's golden files, the analytic limits and conservation checks of
+the reference suite (tests/run_tests.py:44-143 in the reference), closed
+forms of the basis/quadrature, and hand-derived values of the point
+functions (minmod, SED alpha, LLF, HLLC)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as op
+
+S = os.path.join(op.INPUTS, "sine_wave.athinput")
+
+
+# ---------------------------------------------------------------- goldens
+def test_hydro_golden_bit_exact(tmp_path):
+    """3-D sine wave N=8 p=3 ADER to t=0.1 (run_tests.py:45-55): the oracle
+    reproduces the reference golden bit for bit (the reference's own check is
+    rtol 1e-6)."""
+    op.run_oracle(["-i", S], str(tmp_path))
+    a = np.fromfile(os.path.join(op.GOLDEN, "hydro_sd", "W_cv_N8p3_1_0.dat"))
+    b = np.fromfile(str(tmp_path / "W_cv_N8p3_1_0.dat"))
+    assert a.shape == b.shape
+    assert np.abs(a - b).max() / np.abs(a).max() < 1e-6
+    assert int((a != b).sum()) == 0
+
+
+def test_induction_golden_bit_exact(tmp_path):
+    """3-D field loop with the FV update, t=0.2 (run_tests.py:133-142)."""
+    op.run_oracle(["-i", os.path.join(op.INPUTS, "induction_loop.athinput")], str(tmp_path))
+    a = np.fromfile(os.path.join(op.GOLDEN, "induction_fv", "B2_cv_N8p3_2_0.dat"))
+    b = np.fromfile(str(tmp_path / "B2_cv_N8p3_2_0.dat"))
+    assert np.abs(a - b).max() / np.abs(a).max() < 1e-8
+    assert int((a != b).sum()) == 0
+
+
+# ---------------------------------------------------------------- analytic L1 and conservation
+@pytest.mark.parametrize("name,over,ndim,limit", [
+    ("sd_2d", ["mesh/nx3=1"], 2, 3.0e-5),
+    ("sd_1d", ["mesh/nx2=1", "mesh/nx3=1"], 1, 2.0e-5),
+    ("sd_3d_rk3", ["time/integrator=rk3"], 3, 3.0e-5),
+])
+def test_analytic_l1(tmp_path, name, over, ndim, limit):
+    op.run_oracle(["-i", S] + over, str(tmp_path))
+    err = op.l1_vs_exact_sine(str(tmp_path), 1, 0.1, ndim)
+    assert err < limit, err
+    assert op.mass_drift(str(tmp_path)) < 1e-12
+
+
+@pytest.mark.parametrize("name,over", [
+    ("fv_2d", ["job/fallback=true", "mesh/nx3=1"]),
+    ("fv_blast_2d", ["job/fallback=true", "mesh/nx3=1", "problem/problem=spherical_blast",
+                     "hydro/gamma=1.6666666666667", "time/tlim=0.02", "output/dt=0.02"]),
+    ("fv_blast_2d_rk3", ["time/integrator=rk3", "job/fallback=true", "mesh/nx3=1", "problem/problem=spherical_blast",
+                         "hydro/gamma=1.6666666666667", "time/tlim=0.02", "output/dt=0.02"]),
+])
+def test_mass_conservation(tmp_path, name, over):
+    op.run_oracle(["-i", S] + over, str(tmp_path))
+    assert op.mass_drift(str(tmp_path)) < 1e-12
+
+
+def test_reflective_mass_and_trouble_flags(tmp_path):
+    """Implosion with reflective walls (run_tests.py:123-132): mass conserved;
+    the detector fires on the shock."""
+    op.run_oracle(["-i", os.path.join(op.INPUTS, "implosion.athinput"), "time/tlim=0.1", "output/dt=0.05"], str(tmp_path))
+    assert op.mass_drift(str(tmp_path)) < 1e-12
+    tr = np.fromfile(str(tmp_path / "troubles_N132_1_0.dat")).reshape(6, 1, 132, 132)
+    assert tr[5, 0, 2:-2, 2:-2].sum() > 0
+
+
+def test_uniform_state_is_exact(tmp_path):
+    """A uniform state with uniform velocity is an exact steady solution of
+    the discrete scheme: every output must equal the initial state exactly
+    up to the round-off of the basis transforms."""
+    op.run_oracle(["-i", S, "problem/amp=0", "mesh/nx3=1", "time/tlim=0.02", "output/dt=0.02"], str(tmp_path))
+    g = op.grid(str(tmp_path))
+    r0 = op.load_sd(str(tmp_path), g, 0, 0)
+    r1 = op.load_sd(str(tmp_path), g, 1, 0)
+    assert np.abs(r1 - 1.0).max() < 1e-13 and np.abs(r0 - 1.0).max() < 1e-13
+
+
+# ---------------------------------------------------------------- basis and matrices
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 7])
+def test_matrices_closed_forms(p):
+    m = op.matrices(p)
+    n = p + 1
+    # GL nodes/weights on [0,1] against numpy's Legendre-Gauss rule
+    xg, wg = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(m["xt"], 0.5 * (xg + 1), atol=1e-14)
+    assert np.allclose(m["wt"], 0.5 * wg, atol=1e-14)
+    # flux points [0, GL_p, 1], Chebyshev-Gauss solution points
+    xp, _ = np.polynomial.legendre.leggauss(p)
+    assert np.allclose(m["x_fp"], np.r_[0, 0.5 * (xp + 1), 1], atol=1e-14)
+    assert np.allclose(m["x_sp"], 0.5 * (1 - np.cos((2 * np.arange(n) + 1) / (2 * n) * np.pi)), atol=1e-15)
+    # interpolation and derivative exact on polynomials of degree <= p
+    for k in range(n):
+        f = m["x_sp"] ** k
+        assert np.allclose(m["sp_to_fp"] @ f, m["x_fp"] ** k, atol=1e-12)
+        g = m["x_fp"] ** min(k + 1, p + 1)
+        kk = min(k + 1, p + 1)
+        assert np.allclose(m["dfp_to_sp"] @ g, kk * m["x_sp"] ** (kk - 1), atol=1e-10)
+        # sp_to_cv: exact cell averages over [x_fp[c], x_fp[c+1]]
+        a, b = m["x_fp"][:-1], m["x_fp"][1:]
+        avg = (b ** (k + 1) - a ** (k + 1)) / ((k + 1) * (b - a))
+        assert np.allclose(m["sp_to_cv"] @ f, avg, atol=1e-12)
+    assert np.allclose(m["cv_to_sp"] @ m["sp_to_cv"], np.eye(n), atol=1e-11)
+    assert np.allclose(m["invader"] @ m["ader"], np.eye(n), atol=1e-10)
+    # fp_to_cv integrates the degree p+1 fp basis exactly
+    for k in range(p + 2):
+        a, b = m["x_fp"][:-1], m["x_fp"][1:]
+        assert np.allclose(m["fp_to_cv"] @ m["x_fp"] ** k, (b ** (k + 1) - a ** (k + 1)) / ((k + 1) * (b - a)), atol=1e-12)
+
+
+# ---------------------------------------------------------------- point functions
+def test_minmod_cases():
+    f = op.orc().orc_minmod
+    assert f(2.0, 1.0, 0.0, 1.0) == 0.5 * 1.0 * 1.0          # ratio 1/2 -> slope 1
+    assert f(1.0, 3.0, 0.0, 2.0) == 0.5 * 1.0 * 2.0          # ratio clipped to 1
+    assert f(1.0, -1.0, 0.0, 1.0) == 0.0                     # opposite signs
+    assert f(0.0, 0.0, 0.0, 1.0) == 0.0                      # 0/0 -> 0
+    assert f(0.0, 5.0, 0.0, 1.0) == 0.0                      # S_L = 0
+
+
+def test_sed_alpha_cases():
+    f = op.orc().orc_alpha
+    assert f(0.0, 1.0, 2.0, 3.0) == 1.0                  # dv == 0 -> smooth
+    # dv > 0: alphaL = min(1, max(dU-dUm,0)/dv), alphaR = min(1, max(dUp-dU,0)/dv)
+    assert f(4.0, 1.0, 2.0, 3.0) == min(1.0, 1.0 / 4.0)
+    assert f(0.5, 1.0, 2.0, 3.0) == 1.0
+    # dv < 0: uses min(v,0)/dv
+    assert f(-4.0, 3.0, 2.0, 1.0) == min(1.0, -1.0 / -4.0)
+
+
+def _cons(rho, v, p, gm, d=0):
+    u = np.zeros(6)
+    u[0] = rho
+    u[1 + d] = rho * v
+    u[4] = p / (gm - 1) + 0.5 * rho * v * v
+    return u
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_llf_and_hllc_consistency(d):
+    """For equal left/right states both fluxes reduce to the physical flux;
+    HLLC resolves a stationary contact exactly (zero mass flux)."""
+    gm = 1.4
+    u = _cons(1.3, 0.7, 2.1, gm, d)
+    F1, F2 = np.zeros(6), np.zeros(6)
+    op.orc().orc_llf(op.dp(u), op.dp(u), op.dp(F1), d, ctypes_double(gm))
+    op.orc().orc_hllc(op.dp(u), op.dp(u), op.dp(F2), d, ctypes_double(gm))
+    v, p = 0.7, 2.1
+    phys = np.zeros(6)
+    phys[0] = 1.3 * v
+    phys[1 + d] = 1.3 * v * v + p
+    phys[4] = (u[4] + p) * v
+    assert np.allclose(F1[:5], phys[:5], rtol=1e-14) and np.allclose(F2[:5], phys[:5], rtol=1e-14)
+    uL, uR = _cons(1.0, 0.0, 1.0, gm, d), _cons(0.2, 0.0, 1.0, gm, d)
+    op.orc().orc_hllc(op.dp(uL), op.dp(uR), op.dp(F2), d, ctypes_double(gm))
+    assert F2[0] == 0.0 and abs(F2[1 + d] - 1.0) < 1e-14
+
+
+def ctypes_double(x):
+    import ctypes
+    return ctypes.c_double(x)
+
+
+def test_normal_sampler_statistics():
+    """Box-Muller over splitmix64: mean 0, variance 1, deterministic."""
+    f = op.orc().orc_normal
+    z = np.array([f(20251202, i) for i in range(20000)])
+    assert abs(z.mean()) < 0.03 and abs(z.var() - 1) < 0.05
+    assert f(20251202, 7) == f(20251202, 7) and f(1, 7) != f(2, 7)

@@ -177,6 +177,40 @@ int spdk_blend_theta(spdk_ctx* ctx, const double* troubles, double* theta_tmp, d
 int spdk_muscl_hllc_blend(spdk_ctx* ctx, const double* W, const double* theta, double* Fx, double* Fy, double* Fz,
                           spdk_fv_shape s, double dt_sub);
 
+/* ---- induction stage kernels (staggered face B, edge E) -------------------- */
+/* rotational_a_to_b (src/induction.cpp:94-137): B_dim = d1 A2 / d1 - d2 A1 / d2 */
+int spdk_B_curl(spdk_ctx* ctx, const double* A1, spdk_sd_shape s1, const double* A2, spdk_sd_shape s2, double* B,
+                spdk_sd_shape sb, double d1, double d2, int dim);
+/* compute_E (src/induction.cpp:139-208): edge E normal to `dim` from face B
+ * interpolated sp->fp; E has 8 variables [E, b1, b2, v1, v2, 0, b1, b2] */
+int spdk_induction_E(spdk_ctx* ctx, double* E, spdk_sd_shape se, const double* B1, spdk_sd_shape s1, const double* B2,
+                     spdk_sd_shape s2, int dim);
+/* E_riemann_solver (src/induction.cpp:509-560): upwind by max(v_L, v_R) of
+ * variable v1, in place on both sides of faces along `dim` */
+int spdk_E_riemann(spdk_ctx* ctx, double* E, spdk_sd_shape s, int dim, int v1);
+/* update_B_prediction (src/induction.cpp:303-365), active elements */
+int spdk_B_predict(spdk_ctx* ctx, const double* B, double* B_ader, spdk_sd_shape s_ader, const double* E1,
+                   spdk_sd_shape s1, const double* E2, spdk_sd_shape s2, double d1, double d2, double dt, int dim);
+/* update_B_solution (src/induction.cpp:367-420), active elements */
+int spdk_B_correct(spdk_ctx* ctx, double* B, spdk_sd_shape sb, const double* E1, spdk_sd_shape s1, const double* E2,
+                   spdk_sd_shape s2, double d1, double d2, double dt, int dim);
+/* compute_B2_cv (src/induction.cpp:422-465): cv averages of B and |B|^2 */
+int spdk_B2_cv(spdk_ctx* ctx, double* B2, spdk_sd_shape s, const double* Bx, spdk_sd_shape sx, const double* By,
+               spdk_sd_shape sy, const double* Bz, spdk_sd_shape sz);
+/* compute_B_cv_from_cf (src/induction.cpp:467-500): FV cell B from face averages */
+int spdk_B_cv_from_cf(spdk_ctx* ctx, double* B, spdk_fv_shape sf, const double* Bx, spdk_sd_shape sx, const double* By,
+                      spdk_sd_shape sy, const double* Bz, spdk_sd_shape sz);
+/* edge_integral (src/induction.cpp:602-654): edge-averaged E of slice t_id */
+int spdk_edge_integral(spdk_ctx* ctx, const double* E_ep, spdk_sd_shape s, double* E, spdk_fv_shape sf, int t_id,
+                       int dim);
+/* fv_update_B_solution (src/induction.cpp:656-720) */
+int spdk_fv_update_B(spdk_ctx* ctx, double* B_new, double* B_old, spdk_fv_shape sn, double* B, spdk_sd_shape sb,
+                     const double* E1, spdk_fv_shape s1, const double* E2, spdk_fv_shape s2, int t_id, double dt, int dim,
+                     int update);
+/* boundaries, FV with alignment (src/boundary.cpp:123-190): alignment 1 and
+ * a_dim mark a field staggered along a_dim; all active axes x->y->z */
+int spdk_fill_fv_ghosts_aligned(spdk_ctx* ctx, double* U, spdk_fv_shape s, int alignment, int a_dim);
+
 /* ---- fused element-tile path (hydro) -------------------------------------- */
 /* Device workspace for the fused path of a mesh with ghost-inclusive element
  * counts N[3]; all scratch lives in caller memory. */

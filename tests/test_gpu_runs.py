@@ -108,3 +108,32 @@ def test_conservation_at_benchmark_size(tmp_path):
     g = op.grid(out)
     for v in range(5):
         assert np.isfinite(op.load_sd(out, g, idx[-1], v)).all()
+
+
+IND = os.path.join(op.INPUTS, "induction_loop.athinput")
+
+
+def test_induction_golden(tmp_path):
+    out = str(tmp_path / "ind")
+    op.run_driver(["-i", IND], out)
+    a = np.fromfile(os.path.join(op.GOLDEN, "induction_fv", "B2_cv_N8p3_2_0.dat"))
+    b = np.fromfile(os.path.join(out, "B2_cv_N8p3_2_0.dat"))
+    assert np.abs(a - b).max() / np.abs(a).max() < 1e-8  # the reference's tolerance
+    assert int((a != b).sum()) == 0
+
+
+IND_CASES = {
+    "ind_fv_golden_run": ["-i", IND],
+    "ind_sd_ader": ["-i", IND, "job/fallback=false", "mesh/nx1=6", "mesh/nx2=7", "mesh/nx3=5", "time/tlim=0.1",
+                    "output/dt=0.05"],
+    "ind_fv_rk3": ["-i", IND, "time/integrator=rk3", "mesh/nx1=6", "mesh/nx2=6", "mesh/nx3=4", "time/tlim=0.05",
+                   "output/dt=0.025"],
+    "ind_velocity_centre": ["-i", IND, "induction/v1=1.0", "induction/v2=0.5", "induction/v3=0.25", "induction/cx=0.4",
+                            "induction/cy=0.6", "mesh/p=2", "time/tlim=0.05", "output/dt=0.025"],
+}
+
+
+@pytest.mark.parametrize("name", sorted(IND_CASES))
+def test_induction_parity_with_oracle(tmp_path, name):
+    a, b = both(tmp_path, IND_CASES[name], "stages")
+    assert_bit_exact(a, b)

@@ -224,6 +224,48 @@ size_t spdk_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
 int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
                           double* F_x, double* dt_next);
 
+/* ---- mixed-precision ADER path (hydro) ------------------------------------ */
+/* The ADER step of src/hydro_ader.hpp:291-305 with the space-time state, its
+ * face traces and the transform / divergence / predictor contractions in
+ * FP32 (as deviations from each element's FP64 mean), and fluxes, Riemann
+ * solves, the corrector accumulation, cv transform and dt in FP64. Not
+ * bit-identical to the reference; its error against the FP64 path is
+ * measured per configuration. No reference counterpart beyond the step it
+ * replaces. */
+#define SPDK_GUESS_ZERO 0     /* U_ader = U at every node (copy_ader)             */
+#define SPDK_GUESS_EXTERNAL 1 /* caller-filled predictor buffer (e.g. the MLP)    */
+typedef struct spdk_picard {
+    int residual;   /* 0: p predictor sweeps then the corrector (the reference's
+                       count); 1: sweep until the element residual test passes */
+    int guess;      /* SPDK_GUESS_*                                            */
+    double tol;     /* residual test: max over elements of
+                       max|d_out - d_in| / max_v|mean_v| < tol                 */
+    int max_sweeps; /* predictor sweeps before the fixed-count zero-guess
+                       fallback of the whole step                              */
+} spdk_picard;
+typedef struct spdk_picard_stats {
+    long long steps, sweeps, fallbacks;  /* accumulated by spdk_hydro_step_mixed */
+    double last_residual;
+} spdk_picard_stats;
+size_t spdk_mixed_workspace_bytes(spdk_ctx* ctx, const int N[3]);
+/* Element means, FP32 deviations and their traces for U_sp; must precede a
+ * step with SPDK_GUESS_EXTERNAL (the guess is a deviation from these means). */
+int spdk_mixed_prepare(spdk_ctx* ctx, const double* U_sp, spdk_sd_shape s, void* workspace);
+/* Pointers into the workspace: the (T,5,E,TPE) FP32 predictor buffer a guess
+ * is written to, the (5,E) FP64 element means and the (5,E,TPE) FP32
+ * deviations of U_sp. Any may be NULL. */
+int spdk_mixed_predictor_buffer(spdk_ctx* ctx, void* workspace, const int N[3], float** D, double** mu, float** d_old);
+/* One step; pc NULL means the fixed count from the zero-order guess. Writes
+ * the next dt to *dt_next (blocking; the residual form also synchronises
+ * once per sweep to read the residual). */
+int spdk_hydro_step_mixed(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
+                          double* F_x, const spdk_picard* pc, spdk_picard_stats* stats, double* dt_next);
+/* Histogram of the predictor sweeps each element needed to pass the residual
+ * test, accumulated over residual-form steps (bin k = k sweeps, bin 31 =
+ * never within max_sweeps). Copies nbins (<= 32) counts; reset clears. */
+int spdk_mixed_histogram(spdk_ctx* ctx, void* workspace, const int N[3], unsigned long long* hist, int nbins,
+                         int reset);
+
 /* ---- whole-run driver ------------------------------------------------------ */
 typedef struct spdk_run_result {
     int steps;                 /* completed time steps                           */
@@ -234,11 +276,18 @@ typedef struct spdk_run_result {
     double kernel_ms_total;    /* device time of the dominant kernel (HIP events)*/
     int kernel_launches;       /* launches of that kernel                        */
     char kernel_name[64];
+    long long picard_steps;    /* mixed path: steps, predictor/corrector sweeps, */
+    long long picard_sweeps;   /*   zero-guess fallbacks and the histogram of    */
+    long long picard_fallbacks;/*   spdk_mixed_histogram (residual form)         */
+    long long picard_hist[32];
 } spdk_run_result;
 
 /* Runs the host driver (src/main.cpp:86-257) on an argv-style command line:
  * "-i file.athinput" plus "block/name=value" overrides; outputs go to outdir
- * (SPD_OUTPUT_DIR semantics). Extra keys: job/path = fused|stages. */
+ * (SPD_OUTPUT_DIR semantics). Extra keys: job/path = fused|stages,
+ * job/precision = fp64|mixed, job/picard = fixed|residual, job/picard_tol,
+ * job/picard_max, job/guess = zero|mlp, job/mlp_weights, time/nlim, job/warmup,
+ * job/timing. */
 int spdk_run_argv(int argc, const char* const* argv, const char* outdir, spdk_run_result* res);
 
 #ifdef __cplusplus

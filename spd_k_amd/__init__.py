@@ -59,7 +59,9 @@ class RunResult(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("dofs", ctypes.c_longlong), ("evolution_seconds", ctypes.c_double),
                 ("dof_updates_per_s", ctypes.c_double), ("t_final", ctypes.c_double),
                 ("kernel_ms_total", ctypes.c_double), ("kernel_launches", ctypes.c_int),
-                ("kernel_name", ctypes.c_char * 64)]
+                ("kernel_name", ctypes.c_char * 64), ("picard_steps", ctypes.c_longlong),
+                ("picard_sweeps", ctypes.c_longlong), ("picard_fallbacks", ctypes.c_longlong),
+                ("picard_hist", ctypes.c_longlong * 32)]
 
 
 _lib = None
@@ -98,7 +100,9 @@ def run(args, outdir="output"):
     return {"steps": res.steps, "dofs": res.dofs, "evolution_seconds": res.evolution_seconds,
             "dof_updates_per_s": res.dof_updates_per_s, "t_final": res.t_final,
             "kernel_ms_total": res.kernel_ms_total, "kernel_launches": res.kernel_launches,
-            "kernel_name": res.kernel_name.decode()}
+            "kernel_name": res.kernel_name.decode(), "picard_steps": res.picard_steps,
+            "picard_sweeps": res.picard_sweeps, "picard_fallbacks": res.picard_fallbacks,
+            "picard_hist": list(res.picard_hist)}
 
 
 def _ptr(t):

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <vector>
 #include "../../include/spdk.h"
 
 struct spdk_ctx {
@@ -13,6 +14,7 @@ struct spdk_ctx {
     int ngh[3] = {0, 0, 0};  // ghost elements per axis
     int fgh[3] = {0, 0, 0};  // ghost FV cells per axis
     double* mat[SPDK_M_COUNT] = {};
+    std::vector<double> mat_host[SPDK_M_COUNT];  // host copies, row-major
     int mrows[SPDK_M_COUNT] = {}, mcols[SPDK_M_COUNT] = {};
     double* wt = nullptr;
     double wt_host[10] = {};  // host copy of the time weights
@@ -31,6 +33,9 @@ struct spdk_ctx {
     static constexpr int kEvPool = 64;
     hipEvent_t evs[kEvPool][2] = {};
     int ev_used = 0;  // event pairs recorded since the last harvest
+    // device buffers created on first use by a path and freed with the context
+    float* mixed_mats = nullptr;  // FP32 sp_to_fp and dfp_to_sp / h_d
+    std::vector<void*> owned;
 };
 
 namespace spdk {

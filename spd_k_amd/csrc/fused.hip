@@ -134,8 +134,9 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
         }
         __syncthreads();
         // ---- pencils: interpolation to flux points, fluxes, face Riemann solves
-        if (live && s < DIM * NT) {
-            const int d = s / NT, q = s % NT;
+        // (DIM * NT pencils; more than the element's points only for p = 1 in 3-D)
+        for (int task = s; live && task < DIM * NT; task += TPE) {
+            const int d = task / NT, q = task % NT;
             const int ed = d == 0 ? ei : (d == 1 ? ej : ek);
             const int Nd = a.Ne[d];
             const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)a.Ne[0] : (size_t)a.Ne[0] * a.Ne[1]);
@@ -267,8 +268,8 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
                 }
             }
             __syncthreads();
-            if (live && s < DIM * NT) {
-                const int d = s / NT, q = s % NT;
+            for (int task = s; live && task < DIM * NT; task += TPE) {
+                const int d = task / NT, q = task % NT;
 #pragma unroll
                 for (int v = 0; v < NVP; v++) {
                     double p[N];
@@ -395,8 +396,8 @@ __global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U_sp,
 #pragma unroll
         for (int v = 0; v < NVP; v++) us[(el * NVP + v) * TPE + s] = U_sp[(size_t)v * slabE + (size_t)e * TPE + s];
     __syncthreads();
-    if (live && s < DIM * NT) {
-        const int d = s / NT, q = s % NT;
+    for (int task = s; live && task < DIM * NT; task += TPE) {
+        const int d = task / NT, q = task % NT;
         double* trd = d == 0 ? tr0 : (d == 1 ? tr1 : tr2);
 #pragma unroll
         for (int v = 0; v < NVP; v++) {

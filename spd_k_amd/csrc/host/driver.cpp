@@ -105,6 +105,9 @@ struct HydroRun {
     void* ws = nullptr;
     std::vector<double> host;  // staging for outputs
     bool timing = false;
+    bool mixed = false;        // job/precision=mixed
+    spdk_picard pc{};
+    spdk_picard_stats pst{};
 
     HydroRun(RunConfig& c, const std::string& od) : cfg(c), outdir(od) {}
     ~HydroRun() {
@@ -132,7 +135,8 @@ struct HydroRun {
         const Axis* ax[3] = {&X, &Y, &Z};
         ctx = make_context(cfg, op, ax);
         fused = false;
-        if (cfg.path == "fused" && !cfg.fallback) {
+        mixed = cfg.precision == "mixed";
+        if (!mixed && cfg.path == "fused" && !cfg.fallback) {
             int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
             fused = spdk_fused_workspace_bytes(ctx, Nq) > 0;  // else the stage sequence runs
         }
@@ -151,7 +155,20 @@ struct HydroRun {
         T_sweep = mem.alloc(ctx, nsp);
         W_sp = mem.alloc(ctx, nsp);
         F_fp[0] = mem.alloc(ctx, sd_count(s_fp[0]));  // output array in every path
-        if (!fused) {
+        if (mixed) {
+            int N[3] = {X.n_total, Y.n_total, Z.n_total};
+            size_t b = spdk_mixed_workspace_bytes(ctx, N);
+            if (b == 0) throw std::runtime_error(std::string("mixed workspace: ") + spdk_last_error());
+            void* w = nullptr;
+            CK(spdk_malloc(&w, b));
+            mem.ptrs.push_back(w);
+            CK(spdk_memset0(ctx, w, b));
+            ws = w;
+            pc.residual = cfg.picard == "residual";
+            pc.guess = SPDK_GUESS_ZERO;
+            pc.tol = cfg.picard_tol;
+            pc.max_sweeps = cfg.picard_max;
+        } else if (!fused) {
             U_cv = mem.alloc(ctx, nsp);
             U_ader = mem.alloc(ctx, sd_count(s_ader));
             if (cfg.integrator == INT_RK) U0 = mem.alloc(ctx, nsp);
@@ -273,6 +290,12 @@ struct HydroRun {
 
     // One step and the dt that follows it (src/hydro_ader.hpp:235-243).
     double step(bool output_due) {
+        if (mixed) {
+            double next = 0;
+            CK(spdk_hydro_step_mixed(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr,
+                                     output_due ? F_fp[0] : nullptr, &pc, &pst, &next));
+            return next;
+        }
         if (fused) {
             double next = 0;
             CK(spdk_hydro_step_fused(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr, output_due ? F_fp[0] : nullptr,
@@ -296,6 +319,11 @@ struct HydroRun {
             CK(spdk_memcpy_d2d(ctx, save, U_sp, sd_count(s_sp) * sizeof(double)));
             for (int w = 0; w < warmup; w++) step(false);
             CK(spdk_memcpy_d2d(ctx, U_sp, save, sd_count(s_sp) * sizeof(double)));
+            pst = spdk_picard_stats{};
+            if (mixed && pc.residual) {
+                int N[3] = {X.n_total, Y.n_total, Z.n_total};
+                CK(spdk_mixed_histogram(ctx, ws, N, nullptr, 0, 1));
+            }
         }
         CK(spdk_ctx_sync(ctx));
         spdk_timing_reset(ctx);
@@ -312,7 +340,7 @@ struct HydroRun {
             if (cfg.outputs) {
                 if (t >= t_output) {
                     t_output = t + cfg.dt_output;
-                    if (fused && !out_due) {  // keep W_cv current for the writer
+                    if ((fused || mixed) && !out_due) {  // keep W_cv current for the writer
                         CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
                         CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
                     }
@@ -333,6 +361,23 @@ struct HydroRun {
         res->t_final = t;
         std::printf("\nevolution: %d steps, %g s (%g ms/step), %g zone-cycles/s\n", res->steps, sec,
                     res->steps ? sec / res->steps * 1e3 : 0.0, res->dof_updates_per_s);
+        if (mixed) {
+            res->picard_steps = pst.steps;
+            res->picard_sweeps = pst.sweeps;
+            res->picard_fallbacks = pst.fallbacks;
+            std::printf("picard: %lld steps, %.3f sweeps/step, %lld fallbacks, last residual %.3e\n", pst.steps,
+                        pst.steps ? (double)pst.sweeps / pst.steps : 0.0, pst.fallbacks, pst.last_residual);
+            if (pc.residual) {
+                int N[3] = {X.n_total, Y.n_total, Z.n_total};
+                unsigned long long hist[32];
+                CK(spdk_mixed_histogram(ctx, ws, N, hist, 32, 0));
+                std::printf("picard histogram (elements by predictor sweeps to tolerance):");
+                for (int b = 0; b < 32; b++)
+                    if (hist[b]) std::printf(" %s%d:%llu", b == 31 ? "never" : "", b == 31 ? 0 : b, hist[b]);
+                std::printf("\n");
+                for (int b = 0; b < 32; b++) res->picard_hist[b] = (long long)hist[b];
+            }
+        }
     }
 };
 

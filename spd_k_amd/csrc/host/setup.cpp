@@ -121,6 +121,23 @@ std::string configure(Params& pin, RunConfig& c) {
     c.path = pin.has("job", "path") ? pin.get_string("job", "path") : std::string("fused");
     if (c.path != "fused" && c.path != "stages") return "job/path must be fused or stages";
     c.timing = pin.has("job", "timing") && pin.get_bool("job", "timing");
+    if (pin.has("job", "precision")) c.precision = pin.get_string("job", "precision");
+    if (c.precision != "fp64" && c.precision != "mixed") return "job/precision must be fp64 or mixed";
+    if (pin.has("job", "picard")) c.picard = pin.get_string("job", "picard");
+    if (c.picard != "fixed" && c.picard != "residual") return "job/picard must be fixed or residual";
+    if (pin.has("job", "picard_tol")) c.picard_tol = pin.get_real("job", "picard_tol");
+    if (pin.has("job", "picard_max")) c.picard_max = pin.get_int("job", "picard_max");
+    if (pin.has("job", "guess")) c.guess = pin.get_string("job", "guess");
+    if (c.guess != "zero" && c.guess != "mlp") return "job/guess must be zero or mlp";
+    if (pin.has("job", "mlp_weights")) c.mlp_weights = pin.get_string("job", "mlp_weights");
+    if (c.precision == "mixed" || c.picard == "residual" || c.guess == "mlp") {
+        if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
+            return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step without fallback";
+        if (c.precision != "mixed") return "job/picard=residual and job/guess=mlp run on job/precision=mixed";
+        if (c.picard_tol <= 0 || c.picard_max < 1 || c.picard_max > 30) return "job/picard_tol > 0, 1 <= job/picard_max <= 30";
+        if (c.guess == "mlp" && (c.picard != "residual" || c.mlp_weights.empty()))
+            return "job/guess=mlp needs job/picard=residual and job/mlp_weights";
+    }
     return "";
 }
 

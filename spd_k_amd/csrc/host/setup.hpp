@@ -52,6 +52,12 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     int max_steps = -1;      // time/nlim: stop after this many steps
     std::string path = "fused";  // job/path: fused | stages
     bool timing = false;         // job/timing: HIP-event timing of the dominant kernel
+    std::string precision = "fp64";  // job/precision: fp64 | mixed (FP32 state and contractions)
+    std::string picard = "fixed";    // job/picard: fixed (p predictor sweeps) | residual
+    double picard_tol = 1e-4;        // job/picard_tol: element residual tolerance
+    int picard_max = 8;              // job/picard_max: predictor sweeps before the zero-guess fallback
+    std::string guess = "zero";      // job/guess: zero | mlp
+    std::string mlp_weights;         // job/mlp_weights: trained network file (job/guess=mlp)
 };
 
 // Parses the configuration from the parameter table (src/main.cpp:126-189);

@@ -1,0 +1,1207 @@
+// Mixed-precision element-tile path for the hydro ADER step.
+//
+// Same sweep structure as the FP64 tile path (one kernel per Picard sweep
+// over element tiles; only face traces cross elements, through HBM), with
+// the state carried as FP32 deviations from each element's FP64 mean:
+//
+//     U(t, v, sp) = mu(v) + d(t, v, sp),   mu(v) = mean of U_sp(v) over the element
+//
+//   FP32: the space-time state d and its face traces, the sp->fp
+//         interpolation, the flux-divergence contraction (applied to
+//         F - F(mu) with dfp_to_sp / h premultiplied) and the predictor's
+//         time contraction (dt * invader * w premultiplied);
+//   FP64: the element means, every flux and Riemann evaluation (pressure,
+//         sound speed, positivity), the corrector accumulation of U, the
+//         cv transform of the new state and dt.
+//
+// Working on deviations keeps the FP32 rounding relative to the variation
+// inside an element, not to the size of U; subtracting F(mu) before the
+// divergence keeps it relative to the flux variation.
+//
+// Conservation. In exact arithmetic the quadrature of the flux divergence
+// over an element equals its face-flux balance,
+//     sum_sp w_sp dudt_sp = sum_d sum_q w'_q (F_q(1) - F_q(0)) / h_d,
+// and the face fluxes are single-valued, so totals telescope. The FP32
+// divergence breaks the identity at FP32 rounding. The corrector restores it:
+// the balance I is accumulated in FP64 from the FP64 face fluxes (each face's
+// flux is evaluated from the same inputs, with the same operations, by both
+// elements that share it) and the update du of every point is shifted by
+// (I - J) / sum w, J = sum_sp w_sp du_sp, so each element's total changes by
+// exactly its face balance up to FP64 rounding. w are the quadrature
+// weights of the solution points on [0, 1].
+//
+// A sweep maps work to threads in two ways. The first sweep from the
+// zero-order guess (U_ader = U for every time node, the reference's
+// copy_ader) evaluates one time slice, all axes at once; every later sweep
+// evaluates all T slices one axis at a time, with exactly one pencil
+// (slice, line of points along the axis) per thread (T * N^(d-1) = N^d).
+//
+// Picard control: the fixed count (p predictor sweeps, then the corrector,
+// src/hydro_ader.hpp:291-305) or a residual test. In the residual form every
+// predictor sweep also evaluates the corrector from the fluxes of its input
+// state (a candidate) and the element residual
+//     r_e = max |d_out - d_in| / max_v |mu(v)|;
+// once max_e r_e < tol the candidate is the step's result. A step whose
+// residual does not pass within max_sweeps falls back to the fixed-count
+// sequence from the zero-order guess.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include "ctx.hpp"
+#include "physics.hpp"
+
+#pragma clang fp contract(fast)
+
+using namespace spdk;
+
+namespace {
+
+constexpr int kMaxTm = 8;
+constexpr int kHistBins = 32;
+
+// Minimum waves per SIMD the sweep kernel is compiled for (bounds its VGPRs).
+#ifndef SPDK_MSWEEP_WAVES
+#define SPDK_MSWEEP_WAVES 1
+#endif
+
+template <int N, int DIM>
+struct MT {
+    static constexpr int T = N;  // ADER time nodes
+    static constexpr int M = N + 1;
+    static constexpr int NT = DIM == 1 ? 1 : (DIM == 2 ? N : N * N);
+    static constexpr int TPE = N * NT;
+    static constexpr int FSZ = M * NT;
+    static constexpr int SLOTS = T > DIM ? T : DIM;
+    static constexpr int LDS_D = T * NVP * TPE;      // floats: state slices
+    static constexpr int LDS_F = SLOTS * NVP * FSZ;  // floats: flux slots
+    static constexpr int LDS_X = 6 * NVP;            // doubles: mu, F(mu) per axis, residual, shift
+    static constexpr int ELEM_BYTES = ((LDS_X * 8 + (LDS_D + LDS_F) * 4) + 15) / 16 * 16;
+    static constexpr int EPB0 = TPE >= 256 ? 1 : 256 / TPE;
+    static constexpr int EPB_LDS = 61440 / ELEM_BYTES;
+    static constexpr int EPB = EPB0 < EPB_LDS ? EPB0 : EPB_LDS;
+    static constexpr int BLOCK = ((EPB * TPE + 63) / 64) * 64;
+    static constexpr int LDS_BYTES = EPB * ELEM_BYTES > BLOCK * 8 ? EPB * ELEM_BYTES : BLOCK * 8;
+    static_assert(EPB >= 1 && BLOCK <= 256, "tile does not fit a workgroup");
+    static_assert((LDS_D + LDS_F) * 4 >= 2 * NVP * TPE * 8, "cv ping-pong needs two FP64 state planes");
+    static constexpr int XRES = 4 * NVP;      // X slot: element residual
+    static constexpr int XSHIFT = 4 * NVP + 2; // X slots: conservative shift per variable
+    static_assert(T <= kMaxTm, "time nodes");
+};
+
+struct MArgs {
+    const double* U_sp;     // (1,6,E,TPE) solution at the step start (FP64)
+    double* U_new;          // corrector: the new solution (U_sp itself); predictor: candidate (5,E,TPE) or null
+    const double* mu;       // (5,E) element means of U_sp
+    const float* d_old;     // (5,E,TPE) U_sp - mu
+    float* D;               // (T,5,E,TPE) predictor state, read and rewritten in place
+    const float* tr_in[3];  // per axis (Tin,5,E,2,NT) traces of the input state
+    float* tr_out[3];       // per axis (T,5,E,2,NT) traces of the output state
+    double* W_cv;           // optional (1,6,E,TPE) output primitives at control volumes
+    double* F_x;            // optional (T,6,E,NT,M) x flux-point fluxes
+    double* part;           // per-block dt minima (corrector)
+    double* res_part;       // per-block residual maxima (predictor)
+    unsigned char* conv;    // optional per-element first sweep with r_e < tol
+    const float* S2F;       // (M x N) FP32 sp_to_fp
+    const float* Dh;        // (3 x N x M) FP32 dfp_to_sp / h_d
+    const double* S2C;      // sp_to_cv (N x N) FP64
+    double wq[kMaxTm];      // quadrature weights of the solution points on [0, 1]
+    double wtot;            // sum of the element's point weights
+    double ih[3];           // 1 / h_d
+    float cdt[kMaxTm * kMaxTm];  // dt * invader(t,l) * w(l)
+    double wdt[kMaxTm];          // dt * w(t)
+    int Ne[3], act[3], bc[3];
+    int E;
+    double h[3];
+    double gm, cfl;
+    double tol;
+    int sweep;
+    int path;       // 0: every slice is d_old (zero-order guess); 1: slices from D
+    int tin_slices; // slices present in tr_in (1 or T)
+    int out_pred;   // predictor (else corrector)
+};
+
+// ---- FP64 physics in the frame of the face normal: slot 1 is the normal
+// momentum, slots 2 and 3 the transverse ones (cyclic, as V1, V2, V3 of
+// src/hydro.cpp:156-162). One code path serves every axis; the caller maps
+// slots to variables with vperm.
+__device__ __forceinline__ int vperm(int v, int d) { return (v == 0 || v == 4) ? v : 1 + (v - 1 + d) % 3; }
+
+// refined hardware reciprocal (one Newton step; not correctly rounded)
+__device__ __forceinline__ double frcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+__device__ __forceinline__ void mflux(const double* u, double* f, double gm1) {
+    double ri = frcp(u[0]);
+    double p = (u[4] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * ri) * gm1;
+    double vn = u[1] * ri;
+    f[0] = u[1];
+    f[1] = u[1] * vn + p;
+    f[2] = u[2] * vn;
+    f[3] = u[3] * vn;
+    f[4] = (u[4] + p) * vn;
+}
+
+// LLF (src/hydro.cpp:270-289): fluxes and the jump term in FP64; the wave
+// speed, a dissipation coefficient evaluated identically on both sides of a
+// face, from an FP32 square root.
+__device__ __forceinline__ void mllf(const double* uL, const double* uR, double* F, double gm) {
+    const double gm1 = gm - 1.0;
+    double riL = frcp(uL[0]), riR = frcp(uR[0]);
+    double pL = (uL[4] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2] + uL[3] * uL[3]) * riL) * gm1;
+    double pR = (uR[4] - 0.5 * (uR[1] * uR[1] + uR[2] * uR[2] + uR[3] * uR[3]) * riR) * gm1;
+    double vL = uL[1] * riL, vR = uR[1] * riR;
+    float c2L = fmaxf((float)(gm * pL * riL), (float)kMinC2);
+    float c2R = fmaxf((float)(gm * pR * riR), (float)kMinC2);
+    double cL = (double)__builtin_sqrtf(c2L) + fabs(vL);
+    double cR = (double)__builtin_sqrtf(c2R) + fabs(vR);
+    double hc = 0.5 * fmax(cL, cR);
+    double fL[NVP], fR[NVP];
+    fL[0] = uL[1];
+    fR[0] = uR[1];
+    fL[1] = uL[1] * vL + pL;
+    fR[1] = uR[1] * vR + pR;
+    fL[2] = uL[2] * vL;
+    fR[2] = uR[2] * vR;
+    fL[3] = uL[3] * vL;
+    fR[3] = uR[3] * vR;
+    fL[4] = (uL[4] + pL) * vL;
+    fR[4] = (uR[4] + pR) * vR;
+#pragma unroll
+    for (int v = 0; v < NVP; v++) F[v] = 0.5 * (fR[v] + fL[v]) - hc * (uR[v] - uL[v]);
+}
+
+// x[i] for a runtime i < K without indexing registers or kernel arguments
+template <typename V, int K>
+__device__ __forceinline__ V pick(const V (&x)[K], int i) {
+    V r = x[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) r = i == k ? x[k] : r;
+    return r;
+}
+
+// LDS index of flux point ll of pencil q along axis d (x: q*M+ll, the
+// reference's fp-x layout; y: (kk, ll, ii); z: (ll, jj, ii)).
+template <int N, int DIM>
+__device__ __forceinline__ int fidx(int d, int ll, int q) {
+    constexpr int M = N + 1, NT = MT<N, DIM>::NT;
+    if (d == 0) return q * M + ll;
+    if (d == 1) return ((q / N) * M + ll) * N + (q % N);
+    return ll * NT + q;
+}
+// Solution-point index of position ll along axis d of pencil q.
+template <int N, int DIM>
+__device__ __forceinline__ int spidx(int d, int ll, int q) {
+    if (DIM == 1) return ll;
+    if (DIM == 2) return d == 0 ? q * N + ll : ll * N + q;
+    if (d == 0) return q * N + ll;
+    if (d == 1) return ((q / N) * N + ll) * N + (q % N);
+    return ll * N * N + q;
+}
+
+// One pencil task: slice Dsl of the LDS state (floats [v][TPE]) along axis
+// d, pencil q. Writes F - F(mu) at the M flux points into the flux slot
+// (floats [v][FSZ]) and, when asked, adds bw * (F(1) - F(0)) of the
+// pencil to the face balance (want_bal). Face values follow src/boundary.cpp:48-113
+// (ghost traces) and src/hydro.cpp:368-418 (LLF at element faces).
+template <int N, int DIM>
+__device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* Fslot, const double* X, int d, int q,
+                                       int ts, int ei, int ej, int ek, size_t e, bool want_bal, double (&bal)[NVP],
+                                       double bw) {
+    using G = MT<N, DIM>;
+    constexpr int M = G::M, NT = G::NT;
+    const size_t E = (size_t)a.E;
+    const double gm1 = a.gm - 1.0;
+    int pv[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) pv[v] = vperm(v, d);
+    // neighbour operands: n0 is the value left of face 0 (or the ghost fill
+    // that replaces the own value at face M-1 of the left ghost), n1 the value
+    // right of face M-1 (or the ghost fill at face 0 of the right ghost).
+    // Ghost fills: periodic copies the opposite interior element, gradfree /
+    // reflective the adjacent one (normal momentum negated). Issued first so
+    // the loads overlap the interpolation and the interior fluxes.
+    const int ed = d == 0 ? ei : (d == 1 ? ej : ek);
+    const int Nd = d == 0 ? a.Ne[0] : (d == 1 ? a.Ne[1] : a.Ne[2]);
+    const int type = d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2]);
+    const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)a.Ne[0] : (size_t)a.Ne[0] * a.Ne[1]);
+    const size_t ebase = e - (size_t)ed * stride;
+    const float* tr = d == 0 ? a.tr_in[0] : (d == 1 ? a.tr_in[1] : a.tr_in[2]);
+    const bool per = type == SPDK_BC_PERIODIC;
+    const size_t el0 = ebase + (size_t)(ed <= 1 ? (per ? Nd - 2 : 1) : ed - 1) * stride;
+    const int sid0 = ed <= 1 ? (per ? 1 : 0) : 1;
+    const size_t el1 = ebase + (size_t)(ed >= Nd - 2 ? (per ? 1 : Nd - 2) : ed + 1) * stride;
+    const int sid1 = ed >= Nd - 2 ? (per ? 0 : 1) : 0;
+    const bool flip0 = ed <= 1 && type == SPDK_BC_REFLECTIVE;
+    const bool flip1 = ed >= Nd - 2 && type == SPDK_BC_REFLECTIVE;
+    double n0[NVP], n1[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        const size_t pvE = (size_t)pv[v] * E;
+        const size_t tb = ((size_t)ts * NVP + pv[v]) * E;
+        n0[v] = a.mu[pvE + el0] + (double)tr[((tb + el0) * 2 + sid0) * NT + q];
+        n1[v] = a.mu[pvE + el1] + (double)tr[((tb + el1) * 2 + sid1) * NT + q];
+    }
+    if (flip0) n0[1] = -n0[1];
+    if (flip1) n1[1] = -n1[1];
+    double mu[NVP], Fb[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        mu[v] = X[pv[v]];
+        Fb[v] = X[NVP + d * NVP + pv[v]];
+    }
+    float uf[M][NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        float p[N];
+#pragma unroll
+        for (int l = 0; l < N; l++) p[l] = Dsl[pv[v] * G::TPE + spidx<N, DIM>(d, l, q)];
+#pragma unroll
+        for (int f = 0; f < M; f++) {
+            float x = 0.f;
+#pragma unroll
+            for (int l = 0; l < N; l++) x += p[l] * a.S2F[f * N + l];
+            uf[f][v] = x;
+        }
+    }
+    // interior flux points
+#pragma unroll
+    for (int f = 1; f < M - 1; f++) {
+        double u[NVP], fl[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) u[v] = mu[v] + (double)uf[f][v];
+        mflux(u, fl, gm1);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + fidx<N, DIM>(d, f, q)] = (float)(fl[v] - Fb[v]);
+    }
+    // faces
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const int f = side ? M - 1 : 0;
+        const bool raw = side ? (ed == Nd - 1) : (ed == 0);
+        double own[NVP], fl[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) own[v] = mu[v] + (double)uf[f][v];
+        if (raw) {
+            mflux(own, fl, gm1);
+        } else {
+            double uL[NVP], uR[NVP];
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                if (side == 0) {
+                    uL[v] = n0[v];
+                    uR[v] = ed == Nd - 1 ? n1[v] : own[v];
+                } else {
+                    uL[v] = ed == 0 ? n0[v] : own[v];
+                    uR[v] = n1[v];
+                }
+            }
+            mllf(uL, uR, fl, a.gm);
+        }
+#pragma unroll
+        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + fidx<N, DIM>(d, f, q)] = (float)(fl[v] - Fb[v]);
+        if (want_bal) {
+            // bal is indexed by variable: un-permute the momentum slots
+            double c[NVP];
+#pragma unroll
+            for (int v = 0; v < NVP; v++) c[v] = side ? bw * fl[v] : -(bw * fl[v]);
+            bal[0] += c[0];
+            bal[4] += c[4];
+            bal[1] += d == 0 ? c[1] : (d == 1 ? c[3] : c[2]);
+            bal[2] += d == 0 ? c[2] : (d == 1 ? c[1] : c[3]);
+            bal[3] += d == 0 ? c[3] : (d == 1 ? c[2] : c[1]);
+        }
+    }
+}
+
+// Transverse quadrature weight of pencil q (product of the 1-D weights of
+// its transverse coordinates; the same formula for every axis).
+template <int N, int DIM>
+__device__ __forceinline__ double pencil_weight(const MArgs& a, int q) {
+    if (DIM == 1) return 1.0;
+    if (DIM == 2) return pick(a.wq, q);
+    return pick(a.wq, q / N) * pick(a.wq, q % N);
+}
+
+// New state at the control volumes and the element's dt candidate (FP64):
+// compute_primitives, sp->cv sweeps x, y, z (src/transforms.cpp:99-117) and
+// src/hydro.cpp:226-256. `area` is the element's LDS (>= 2 planes of FP64).
+template <int N, int DIM>
+__device__ __forceinline__ double cv_and_dt(const double* un, double* area, bool live, int s, int ii, int jj, int kk,
+                                            int ei, int ej, int ek, size_t e, double* W_cv, size_t slabE,
+                                            const double* S2C, const int* Ne, const int* act, const double* h,
+                                            double gm, double cfl) {
+    using G = MT<N, DIM>;
+    constexpr int TPE = G::TPE;
+    double* wa = area;
+    double* wb = area + NVP * TPE;
+    if (live) {
+        double wv[NVP];
+        prim(un, wv, gm);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) wa[v * TPE + s] = wv[v];
+    }
+    __syncthreads();
+    double* src = wa;
+    double* dst = wb;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        if (live) {
+            const int id = d == 0 ? ii : (d == 1 ? jj : kk);
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                double x = 0;
+#pragma unroll
+                for (int l = 0; l < N; l++) {
+                    int sl = d == 0 ? s - ii + l : (d == 1 ? s + (l - jj) * N : s + (l - kk) * N * N);
+                    x += src[v * TPE + sl] * S2C[id * N + l];
+                }
+                dst[v * TPE + s] = x;
+            }
+        }
+        __syncthreads();
+        double* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+    double r = __builtin_inf();
+    if (live) {
+        double wc[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) wc[v] = src[v * TPE + s];
+        if (W_cv) {
+#pragma unroll
+            for (int v = 0; v < NVP; v++) W_cv[(size_t)v * slabE + e * TPE + s] = wc[v];
+            W_cv[(size_t)NVP * slabE + e * TPE + s] = 0.0;
+        }
+        bool active = (!act[0] || (ei >= 1 && ei <= Ne[0] - 2)) && (!act[1] || (ej >= 1 && ej <= Ne[1] - 2)) &&
+                      (!act[2] || (ek >= 1 && ek <= Ne[2] - 2));
+        if (active) {
+            double c_s = sound_speed(wc[0], wc[4], gm);
+            double c_max = 0, dx_min = 1;
+            if (act[0]) { c_max += (fabs(wc[1]) + c_s); dx_min = smin(dx_min, h[0]); }
+            if (act[1]) { c_max += (fabs(wc[2]) + c_s); dx_min = smin(dx_min, h[1]); }
+            if (act[2]) { c_max += (fabs(wc[3]) + c_s); dx_min = smin(dx_min, h[2]); }
+            r = cfl * dx_min / c_max / N;
+        }
+    }
+    return r;
+}
+
+// Block minimum of r into part[blockIdx.x]; `red` holds BLOCK doubles.
+template <int BLOCK>
+__device__ __forceinline__ void block_min(double r, double* red, double* part) {
+    __syncthreads();
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Conservative shift of the point updates du (see the header): red holds
+// 2 * NVP * TPE doubles of the element's LDS, X_shift NVP doubles.
+template <int N, int DIM>
+__device__ __forceinline__ void cons_shift(double* du, const double* bal, double wsp, double wtot, double* red,
+                                           double* X_shift, bool live, int s) {
+    constexpr int TPE = MT<N, DIM>::TPE;
+    if (live)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            red[v * TPE + s] = bal[v];
+            red[(NVP + v) * TPE + s] = wsp * du[v];
+        }
+    __syncthreads();
+    if (live)
+        for (int v = s; v < NVP; v += TPE) {  // an element may have fewer points than variables
+            double I = 0.0, J = 0.0;
+            for (int j = 0; j < TPE; j++) {
+                I += red[v * TPE + j];
+                J += red[(NVP + v) * TPE + j];
+            }
+            X_shift[v] = (I - J) / wtot;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NVP; v++) du[v] += X_shift[v];
+}
+
+template <int N, int DIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_MSWEEP_WAVES))) k_msweep(MArgs a) {
+    using G = MT<N, DIM>;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T, FSZ = G::FSZ;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int tid = threadIdx.x;
+    const int el = tid / TPE, s = tid % TPE;
+    const int e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < a.E;
+    unsigned char* base = lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES;
+    double* X = (double*)base;  // mu[5], F(mu) per axis [3][5], residual
+    float* Ds = (float*)(X + G::LDS_X);
+    float* Fs = Ds + G::LDS_D;
+    const size_t E = (size_t)a.E;
+    const size_t slabE = E * TPE;
+    const size_t eo = (size_t)e * TPE + s;
+
+    int ei = 0, ej = 0, ek = 0;
+    if (live) {
+        ei = e % a.Ne[0];
+        ej = (e / a.Ne[0]) % a.Ne[1];
+        ek = e / (a.Ne[0] * a.Ne[1]);
+    }
+    const int ii = s % N, jj = DIM >= 2 ? (s / N) % N : 0, kk = DIM == 3 ? s / (N * N) : 0;
+    // pencil through this point and the point's position along it, per axis
+    const int qx = DIM == 3 ? kk * N + jj : jj, qy = DIM == 3 ? kk * N + ii : ii, qz = jj * N + ii;
+    const double wsp = pick(a.wq, ii) * (DIM >= 2 ? pick(a.wq, jj) : 1.0) * (DIM == 3 ? pick(a.wq, kk) : 1.0);
+
+    if (live)
+        for (int v = s; v < NVP; v += TPE) X[v] = a.mu[(size_t)v * E + e];
+    __syncthreads();
+    if (live && s < DIM) {  // F(mu) along axis s
+        double um[NVP], fb[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) um[v] = X[vperm(v, s)];
+        mflux(um, fb, a.gm - 1.0);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) X[NVP + s * NVP + vperm(v, s)] = fb[v];
+    }
+    float dold[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) dold[v] = live ? a.d_old[(size_t)v * slabE + eo] : 0.f;
+    if (live) {
+        if (a.path) {
+#pragma unroll
+            for (int t = 0; t < T; t++)
+#pragma unroll
+                for (int v = 0; v < NVP; v++) Ds[(t * NVP + v) * TPE + s] = a.D[((size_t)t * NVP + v) * slabE + eo];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NVP; v++) Ds[v * TPE + s] = dold[v];
+        }
+    }
+    __syncthreads();
+
+    float dudt[T][NVP];
+#pragma unroll
+    for (int t = 0; t < T; t++)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) dudt[t][v] = 0.f;
+    // the corrector and the candidates conserve: accumulate the face balance
+    const bool cons = !a.out_pred || a.U_new;
+    double bal[NVP] = {0.0, 0.0, 0.0, 0.0, 0.0};
+
+    // Rounds of pencil tasks, then the flux divergence of the round's fluxes.
+    // Zero-order guess (path 0): one round, one slice, task s = d * NT + q in
+    // flux slot d. Otherwise DIM rounds, round d: task s = t * NT + q in flux
+    // slot t. The round loop is not unrolled: one copy of its code.
+    const int nrounds = a.path ? DIM : 1;
+    double wsum = 0.0;
+#pragma unroll
+    for (int t = 0; t < T; t++) wsum += a.wdt[t];
+#pragma unroll 1
+    for (int r = 0; r < nrounds; r++) {
+        // tasks of the round: TPE (= T * NT) or DIM * NT (more than TPE only for p = 1 in 3-D)
+        const int ntask = a.path ? TPE : DIM * NT;
+        for (int task = s; live && task < ntask; task += TPE) {
+            const int d = a.path ? r : task / NT, t = a.path ? task / NT : 0, q = task % NT;
+            const int slot = a.path ? t : d;
+            const int ts = a.tin_slices > 1 ? t : 0;
+            const double bw = (a.path ? pick(a.wdt, t) : wsum) * pencil_weight<N, DIM>(a, q) * pick(a.ih, d);
+            pencil<N, DIM>(a, Ds + t * NVP * TPE, Fs + slot * NVP * FSZ, X, d, q, ts, ei, ej, ek, (size_t)e, cons,
+                           bal, bw);
+        }
+        __syncthreads();
+        if (live) {
+            const int d0 = a.path ? r : 0, d1 = a.path ? r + 1 : DIM;
+#pragma unroll 1
+            for (int d = d0; d < d1; d++) {
+                const int qdd = d == 0 ? qx : (d == 1 ? qy : qz), idd = d == 0 ? ii : (d == 1 ? jj : kk);
+                float Dr[M];
+#pragma unroll
+                for (int l = 0; l < M; l++) Dr[l] = a.Dh[(d * N + idd) * M + l];
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    if (!a.path && t > 0) break;
+                    const float* Ft = Fs + (a.path ? t : d) * NVP * FSZ;
+#pragma unroll
+                    for (int v = 0; v < NVP; v++) {
+                        const float* F = Ft + v * FSZ;
+                        float acc = 0.f;
+#pragma unroll
+                        for (int l = 0; l < M; l++) acc += F[fidx<N, DIM>(d, l, qdd)] * Dr[l];
+                        dudt[t][v] += acc;
+                    }
+                }
+            }
+            if (a.F_x && (a.path ? r == 0 : true)) {  // x fluxes at the flux points (outputs)
+                for (int idx = s; idx < FSZ; idx += TPE)
+                    for (int t = 0; t < T; t++) {
+                        size_t o = (((size_t)t * NV) * E + e) * FSZ + idx;
+                        const float* Ft = Fs + (a.path ? t : 0) * NVP * FSZ;  // path 0: slot 0 is x
+#pragma unroll
+                        for (int v = 0; v < NVP; v++) a.F_x[o + (size_t)v * E * FSZ] = X[NVP + v] + (double)Ft[v * FSZ + idx];
+                        a.F_x[o + (size_t)NVP * E * FSZ] = 0.0;
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (!a.path)  // the zero-order guess has identical slices
+#pragma unroll
+        for (int t = 1; t < T; t++)
+#pragma unroll
+            for (int v = 0; v < NVP; v++) dudt[t][v] = dudt[0][v];
+
+    if (a.out_pred) {
+        // ---- predictor: d_new(t) = d_old - sum_l cdt(t,l) dudt(l); residual vs the input state
+        float res = 0.f;
+        double du[NVP];
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < T; t++)
+#pragma unroll
+                for (int v = 0; v < NVP; v++) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int l = 0; l < T; l++) acc += a.cdt[t * kMaxTm + l] * dudt[l][v];
+                    float dn = dold[v] - acc;
+                    // input state at this point: its own LDS slot (path 1) or d_old
+                    float din = a.path ? Ds[(t * NVP + v) * TPE + s] : dold[v];
+                    res = fmaxf(res, fabsf(dn - din));
+                    a.D[((size_t)t * NVP + v) * slabE + eo] = dn;
+                    Ds[(t * NVP + v) * TPE + s] = dn;
+                }
+        }
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {  // corrector candidate from the fluxes of the input state
+            du[v] = 0.0;
+#pragma unroll
+            for (int t = 0; t < T; t++) du[v] += a.wdt[t] * (double)dudt[t][v];
+        }
+        __syncthreads();
+        // traces of the new state: task = (t, d, q)
+        if (live) {
+            for (int task = s; task < T * DIM * NT; task += TPE) {
+                const int t = task / (DIM * NT), r = task % (DIM * NT), d = r / NT, q = r % NT;
+#pragma unroll
+                for (int v = 0; v < NVP; v++) {
+                    float x0 = 0.f, x1 = 0.f;
+#pragma unroll
+                    for (int l = 0; l < N; l++) {
+                        float p = Ds[(t * NVP + v) * TPE + spidx<N, DIM>(d, l, q)];
+                        x0 += p * a.S2F[l];
+                        x1 += p * a.S2F[(M - 1) * N + l];
+                    }
+                    float* to = a.tr_out[d] + ((((size_t)t * NVP + v) * E + e) * 2) * NT + q;
+                    to[0] = x0;
+                    to[NT] = x1;
+                }
+            }
+        }
+        __syncthreads();
+        if (a.U_new) {
+            cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, live, s);
+            if (live)
+#pragma unroll
+                for (int v = 0; v < NVP; v++) a.U_new[(size_t)v * slabE + eo] = a.U_sp[(size_t)v * slabE + eo] - du[v];
+        }
+        if (live) Fs[s] = res;
+        __syncthreads();
+        if (live && s == 0) {
+            float re = 0.f;
+            for (int j = 0; j < TPE; j++) re = fmaxf(re, Fs[j]);
+            double sc = 0.0;
+#pragma unroll
+            for (int v = 0; v < NVP; v++) sc = fmax(sc, fabs(X[v]));
+            double rr = (double)re / (sc > 0.0 ? sc : 1.0);
+            X[G::XRES] = rr;
+            if (a.conv && rr < a.tol && a.conv[e] == 0xff) a.conv[e] = (unsigned char)a.sweep;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double rb = 0.0;
+            for (int j = 0; j < EPB; j++)
+                if (blockIdx.x * EPB + j < a.E) {
+                    double v = ((const double*)(lds_raw + (size_t)j * G::ELEM_BYTES))[G::XRES];
+                    rb = v > rb ? v : rb;
+                }
+            a.res_part[blockIdx.x] = rb;
+        }
+        return;
+    }
+
+    // ---- corrector (src/transforms.cpp:354-372), FP64 accumulation, conservative shift
+    double un[NVP], du[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        du[v] = 0.0;
+#pragma unroll
+        for (int t = 0; t < T; t++) du[v] += a.wdt[t] * (double)dudt[t][v];
+    }
+    __syncthreads();
+    cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, live, s);
+#pragma unroll
+    for (int v = 0; v < NVP; v++) un[v] = live ? a.U_sp[(size_t)v * slabE + eo] - du[v] : 1.0;
+    if (live)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) a.U_new[(size_t)v * slabE + eo] = un[v];
+    __syncthreads();
+    double r = cv_and_dt<N, DIM>(un, (double*)Ds, live, s, ii, jj, kk, ei, ej, ek, (size_t)e, a.W_cv, slabE, a.S2C,
+                                 a.Ne, a.act, a.h, a.gm, a.cfl);
+    block_min<G::BLOCK>(r, (double*)lds_raw, a.part);
+}
+
+// Step start: element means, deviations and their face traces (one slice);
+// resets the per-element convergence marks.
+template <int N, int DIM>
+__global__ void __launch_bounds__(256) k_mstart(const double* __restrict__ U_sp, double* __restrict__ mu,
+                                                float* __restrict__ d_old, float* tr0, float* tr1, float* tr2,
+                                                const float* __restrict__ S2F, unsigned char* conv, int Eint) {
+    using G = MT<N, DIM>;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
+    const int e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < Eint;
+    unsigned char* base = lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES;
+    double* mloc = (double*)base;          // [v] element means
+    double* us = mloc + 8;                 // [v][TPE] FP64
+    float* ds = (float*)(us + NVP * TPE);  // [v][TPE] FP32
+    const size_t E = (size_t)Eint, slabE = E * TPE;
+    const size_t eo = (size_t)e * TPE + s;
+    double u[NVP];
+    if (live)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            u[v] = U_sp[(size_t)v * slabE + eo];
+            us[v * TPE + s] = u[v];
+        }
+    __syncthreads();
+    if (live)
+        for (int v = s; v < NVP; v += TPE) {  // an element may have fewer points than variables
+            double acc = 0.0;
+            for (int j = 0; j < TPE; j++) acc += us[v * TPE + j];
+            double m = acc * (1.0 / TPE);
+            mloc[v] = m;
+            mu[(size_t)v * E + e] = m;
+        }
+    if (live && s == 0 && conv) conv[e] = 0xff;
+    __syncthreads();
+    if (live)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            float d = (float)(u[v] - mloc[v]);
+            d_old[(size_t)v * slabE + eo] = d;
+            ds[v * TPE + s] = d;
+        }
+    __syncthreads();
+    for (int task = s; live && task < DIM * NT; task += TPE) {
+        const int d = task / NT, q = task % NT;
+        float* trd = d == 0 ? tr0 : (d == 1 ? tr1 : tr2);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            float x0 = 0.f, x1 = 0.f;
+#pragma unroll
+            for (int l = 0; l < N; l++) {
+                float p = ds[v * TPE + spidx<N, DIM>(d, l, q)];
+                x0 += p * S2F[l];
+                x1 += p * S2F[(M - 1) * N + l];
+            }
+            float* to = trd + (((size_t)v * E + e) * 2) * NT + q;
+            to[0] = x0;
+            to[NT] = x1;
+        }
+    }
+}
+
+// Face traces of a T-slice state (after an externally supplied guess).
+template <int N, int DIM>
+__global__ void __launch_bounds__(256) k_mtrace(const float* __restrict__ D, float* tr0, float* tr1, float* tr2,
+                                                const float* __restrict__ S2F, int Eint) {
+    using G = MT<N, DIM>;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
+    const int e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < Eint;
+    float* Ds = (float*)(lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES);
+    const size_t E = (size_t)Eint, slabE = E * TPE;
+    if (live)
+        for (int t = 0; t < T; t++)
+#pragma unroll
+            for (int v = 0; v < NVP; v++) Ds[(t * NVP + v) * TPE + s] = D[((size_t)t * NVP + v) * slabE + (size_t)e * TPE + s];
+    __syncthreads();
+    if (!live) return;
+    for (int task = s; task < T * DIM * NT; task += TPE) {
+        const int t = task / (DIM * NT), r = task % (DIM * NT), d = r / NT, q = r % NT;
+        float* trd = d == 0 ? tr0 : (d == 1 ? tr1 : tr2);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            float x0 = 0.f, x1 = 0.f;
+#pragma unroll
+            for (int l = 0; l < N; l++) {
+                float p = Ds[(t * NVP + v) * TPE + spidx<N, DIM>(d, l, q)];
+                x0 += p * S2F[l];
+                x1 += p * S2F[(M - 1) * N + l];
+            }
+            float* to = trd + ((((size_t)t * NVP + v) * E + e) * 2) * NT + q;
+            to[0] = x0;
+            to[NT] = x1;
+        }
+    }
+}
+
+// Accept a corrector candidate: copy it into U_sp, then cv primitives and dt.
+template <int N, int DIM>
+__global__ void __launch_bounds__(256) k_mfinish(const double* __restrict__ cand, double* U_sp, double* W_cv,
+                                                 double* part, const double* __restrict__ S2C, int Ne0, int Ne1,
+                                                 int Ne2, int act0, int act1, int act2, double h0, double h1,
+                                                 double h2, double gm, double cfl, int Eint) {
+    using G = MT<N, DIM>;
+    constexpr int TPE = G::TPE, EPB = G::EPB;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
+    const int e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < Eint;
+    double* area = (double*)(lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES);
+    const size_t slabE = (size_t)Eint * TPE;
+    const int Ne[3] = {Ne0, Ne1, Ne2}, act[3] = {act0, act1, act2};
+    const double h[3] = {h0, h1, h2};
+    int ei = 0, ej = 0, ek = 0;
+    if (live) {
+        ei = e % Ne0;
+        ej = (e / Ne0) % Ne1;
+        ek = e / (Ne0 * Ne1);
+    }
+    const int ii = s % N, jj = DIM >= 2 ? (s / N) % N : 0, kk = DIM == 3 ? s / (N * N) : 0;
+    double un[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        un[v] = live ? cand[(size_t)v * slabE + (size_t)e * TPE + s] : 1.0;
+        if (live) U_sp[(size_t)v * slabE + (size_t)e * TPE + s] = un[v];
+    }
+    double r = cv_and_dt<N, DIM>(un, area, live, s, ii, jj, kk, ei, ej, ek, (size_t)e, W_cv, slabE, S2C, Ne, act, h,
+                                 gm, cfl);
+    block_min<G::BLOCK>(r, (double*)lds_raw, part);
+}
+
+// out[0] = min of a[0..n), out[1] = max of b[0..n)
+__global__ void k_mreduce(const double* a, const double* b, int n, double* out) {
+    __shared__ double ra[1024], rb[1024];
+    double x = __builtin_inf(), y = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        x = x < a[i] ? x : a[i];
+        if (b) y = y > b[i] ? y : b[i];
+    }
+    ra[threadIdx.x] = x;
+    rb[threadIdx.x] = y;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ra[threadIdx.x] = ra[threadIdx.x] < ra[threadIdx.x + w] ? ra[threadIdx.x] : ra[threadIdx.x + w];
+            rb[threadIdx.x] = rb[threadIdx.x] > rb[threadIdx.x + w] ? rb[threadIdx.x] : rb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = ra[0];
+        out[1] = rb[0];
+    }
+}
+
+// Histogram of the per-element convergence sweep (bin k = k predictor
+// sweeps needed; the last bin counts elements that never passed).
+__global__ void k_mhist(const unsigned char* conv, int E, int nb, unsigned long long* hist) {
+    __shared__ unsigned int loc[kHistBins];
+    if (threadIdx.x < kHistBins) loc[threadIdx.x] = 0;
+    __syncthreads();
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        int c = conv[e];
+        int bin = c == 0xff ? nb - 1 : (c + 1 < nb - 1 ? c + 1 : nb - 1);
+        atomicAdd(&loc[bin], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)nb && loc[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+struct MLayout {
+    size_t E = 0, tpe = 0, nt = 0;
+    unsigned blocks = 0;
+    // byte offsets
+    size_t mu = 0, cand = 0, part = 0, resp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0, conv = 0,
+           total = 0;
+    size_t tr_axis = 0;  // bytes per axis per trace buffer
+};
+
+inline size_t al(size_t x) { return (x + 255) / 256 * 256; }
+
+template <int N, int DIM>
+MLayout mlayout(size_t E) {
+    using G = MT<N, DIM>;
+    MLayout L;
+    L.E = E;
+    L.tpe = G::TPE;
+    L.nt = G::NT;
+    L.blocks = (unsigned)((E + G::EPB - 1) / G::EPB);
+    size_t o = 0;
+    L.mu = o; o = al(o + sizeof(double) * NVP * E);
+    L.cand = o; o = al(o + sizeof(double) * NVP * E * G::TPE);
+    L.part = o; o = al(o + sizeof(double) * (L.blocks + 8));
+    L.resp = o; o = al(o + sizeof(double) * (L.blocks + 8));
+    L.red2 = o; o = al(o + sizeof(double) * 2);
+    L.hist = o; o = al(o + sizeof(unsigned long long) * kHistBins);
+    L.dold = o; o = al(o + sizeof(float) * NVP * E * G::TPE);
+    L.D = o; o = al(o + sizeof(float) * G::T * NVP * E * G::TPE);
+    L.tr_axis = al(sizeof(float) * G::T * NVP * E * 2 * G::NT);
+    L.trA = o; o += DIM * L.tr_axis;
+    L.trB = o; o += DIM * L.tr_axis;
+    L.conv = o; o = al(o + E);
+    L.total = o;
+    return L;
+}
+
+// Quadrature weights of the N Chebyshev-Gauss solution points on [0, 1]
+// (x_i = (1 - cos((2i+1) pi / 2N)) / 2, src/polynomials.cpp:89-93): the
+// exact integrals of their Lagrange basis polynomials, expanded in powers of
+// y = x - 1/2 (the points are symmetric about 1/2, which keeps the
+// expansion well conditioned); odd powers integrate to zero.
+void sp_weights(int n, double* w) {
+    double y[kMaxTm];
+    for (int i = 0; i < n; i++) y[i] = -0.5 * std::cos((2 * i + 1) / (2.0 * n) * 3.141592653589793);
+    for (int i = 0; i < n; i++) {
+        double c[kMaxTm + 1] = {1.0};  // coefficients of prod_{j != i} (y - y_j), ascending powers
+        int deg = 0;
+        double den = 1.0;
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            for (int k = deg + 1; k > 0; k--) c[k] = c[k - 1] - y[j] * c[k];
+            c[0] = -y[j] * c[0];
+            deg++;
+            den *= y[i] - y[j];
+        }
+        double integral = 0.0;
+        for (int k = 0; k <= deg; k += 2) integral += c[k] * 2.0 * std::pow(0.5, k + 1) / (k + 1);
+        w[i] = integral / den;
+    }
+}
+
+int ensure_mixed_mats(spdk_ctx* c) {
+    if (c->mixed_mats) return SPDK_OK;
+    const int n = c->n, m = c->m;
+    std::vector<float> h((size_t)m * n + 3 * (size_t)n * m, 0.f);
+    const std::vector<double>& s2f = c->mat_host[SPDK_M_SP_TO_FP];
+    const std::vector<double>& dm = c->mat_host[SPDK_M_DFP_TO_SP];
+    for (int i = 0; i < m * n; i++) h[i] = (float)s2f[i];
+    for (int d = 0; d < 3; d++)
+        for (int i = 0; i < n * m; i++) h[(size_t)m * n + d * n * m + i] = (float)(dm[i] / c->prm.h[d]);
+    void* p = nullptr;
+    SPDK_CHECK(hipMalloc(&p, h.size() * sizeof(float)));
+    c->owned.push_back(p);
+    SPDK_CHECK(hipMemcpy(p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    c->mixed_mats = (float*)p;
+    return SPDK_OK;
+}
+
+template <int N, int DIM>
+int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, double* W_cv, double* F_x,
+              const spdk_picard* pc, spdk_picard_stats* st, double* dt_next) {
+    using G = MT<N, DIM>;
+    constexpr int T = G::T;
+    if (c->T != T) { set_error("mixed path: ADER with T = p+1 time nodes only"); return SPDK_E_CONFIG; }
+    int rc;
+    if ((rc = ensure_mixed_mats(c))) return rc;
+    const size_t E = (size_t)Ne[0] * Ne[1] * Ne[2];
+    MLayout L = mlayout<N, DIM>(E);
+    char* W = (char*)ws;
+    float* trA[3] = {nullptr, nullptr, nullptr};
+    float* trB[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < DIM; d++) {
+        trA[d] = (float*)(W + L.trA + d * L.tr_axis);
+        trB[d] = (float*)(W + L.trB + d * L.tr_axis);
+    }
+    MArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.U_sp = U_sp;
+    a.mu = (const double*)(W + L.mu);
+    a.d_old = (const float*)(W + L.dold);
+    a.D = (float*)(W + L.D);
+    a.part = (double*)(W + L.part);
+    a.res_part = (double*)(W + L.resp);
+    a.S2F = c->mixed_mats;
+    a.Dh = c->mixed_mats + (size_t)c->m * c->n;
+    a.S2C = c->mat[SPDK_M_SP_TO_CV];
+    const std::vector<double>& inv = c->mat_host[SPDK_M_INVADER];
+    for (int t = 0; t < T; t++) {
+        for (int l = 0; l < T; l++) a.cdt[t * kMaxTm + l] = (float)(dt * inv[t * T + l] * c->wt_host[l]);
+        a.wdt[t] = dt * c->wt_host[t];
+    }
+    for (int d = 0; d < 3; d++) {
+        a.Ne[d] = Ne[d];
+        a.act[d] = c->prm.active[d];
+        a.bc[d] = c->prm.bc[d];
+        a.h[d] = c->prm.h[d];
+        a.ih[d] = 1.0 / c->prm.h[d];
+    }
+    double wsum = 0.0;
+    sp_weights(N, a.wq);
+    for (int i = 0; i < N; i++) wsum += a.wq[i];
+    a.wtot = wsum;
+    for (int d = 1; d < DIM; d++) a.wtot *= wsum;
+    a.E = (int)E;
+    a.gm = c->prm.gamma;
+    a.cfl = c->prm.cfl;
+    a.tol = pc ? pc->tol : 0.0;
+    unsigned char* conv = (unsigned char*)(W + L.conv);
+    double* cand = (double*)(W + L.cand);
+    double* red2 = (double*)(W + L.red2);
+    dim3 grid(L.blocks), block(G::BLOCK);
+    const size_t lds = G::LDS_BYTES;
+
+    auto sweep = [&](const MArgs& args) -> int {
+        bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
+        if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
+        hipLaunchKernelGGL((k_msweep<N, DIM>), grid, block, lds, c->stream, args);
+        SPDK_LAUNCH_CHECK("k_msweep");
+        if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
+        return SPDK_OK;
+    };
+    auto start = [&]() -> int {
+        hipLaunchKernelGGL((k_mstart<N, DIM>), grid, block, lds, c->stream, (const double*)U_sp, (double*)(W + L.mu),
+                           (float*)(W + L.dold), trA[0], trA[1], trA[2], (const float*)a.S2F, conv, (int)E);
+        SPDK_LAUNCH_CHECK("k_mstart");
+        return SPDK_OK;
+    };
+    auto reduce = [&](bool with_res) -> int {
+        hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, (const double*)a.part,
+                           with_res ? (const double*)a.res_part : nullptr, (int)L.blocks, red2);
+        SPDK_LAUNCH_CHECK("k_mreduce");
+        SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SPDK_CHECK(hipStreamSynchronize(c->stream));
+        return SPDK_OK;
+    };
+    // fixed count: p predictor sweeps from the zero-order guess, then the corrector
+    auto fixed_count = [&]() -> int {
+        for (int k = 0; k < T; k++) {
+            MArgs s = a;
+            const bool last = k == T - 1;
+            float** tin = (k % 2 == 0) ? trA : trB;
+            float** tout = (k % 2 == 0) ? trB : trA;
+            for (int d = 0; d < 3; d++) { s.tr_in[d] = tin[d]; s.tr_out[d] = tout[d]; }
+            s.path = k > 0;
+            s.tin_slices = k == 0 ? 1 : T;
+            s.out_pred = !last;
+            s.U_new = last ? U_sp : nullptr;
+            s.W_cv = last ? W_cv : nullptr;
+            s.F_x = last ? F_x : nullptr;
+            s.sweep = k;
+            int r;
+            if ((r = sweep(s))) return r;
+        }
+        return SPDK_OK;
+    };
+
+    const bool external = pc && pc->guess == SPDK_GUESS_EXTERNAL;
+    if (!external && (rc = start())) return rc;  // an external guess follows spdk_mixed_prepare
+    long long sweeps = 0;
+    bool done = false;
+    if (pc && pc->residual) {
+        const int kmax = pc->max_sweeps > 0 ? pc->max_sweeps : 8;
+        if (external) {
+            // the caller's guess is already in the predictor buffer: traces of it
+            hipLaunchKernelGGL((k_mtrace<N, DIM>), grid, block, lds, c->stream, (const float*)a.D, trA[0], trA[1],
+                               trA[2], (const float*)a.S2F, (int)E);
+            SPDK_LAUNCH_CHECK("k_mtrace");
+        }
+        for (int k = 0; k < kmax && k < 254; k++) {
+            MArgs s = a;
+            float** tin = (k % 2 == 0) ? trA : trB;
+            float** tout = (k % 2 == 0) ? trB : trA;
+            for (int d = 0; d < 3; d++) { s.tr_in[d] = tin[d]; s.tr_out[d] = tout[d]; }
+            s.path = (k > 0 || external) ? 1 : 0;
+            s.tin_slices = s.path ? T : 1;
+            s.out_pred = 1;
+            s.U_new = cand;
+            s.F_x = F_x;
+            s.conv = conv;
+            s.sweep = k;
+            if ((rc = sweep(s))) return rc;
+            sweeps++;
+            // the candidate's dt partials are not computed yet: reduce the residual only
+            hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, (const double*)a.res_part,
+                               (const double*)a.res_part, (int)L.blocks, red2);
+            SPDK_LAUNCH_CHECK("k_mreduce");
+            SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            SPDK_CHECK(hipStreamSynchronize(c->stream));
+            double r = c->red_host[1];
+            if (st) st->last_residual = r;
+            if (r < pc->tol) {
+                hipLaunchKernelGGL((k_mfinish<N, DIM>), grid, block, lds, c->stream, (const double*)cand, U_sp, W_cv,
+                                   a.part, a.S2C, Ne[0], Ne[1], Ne[2], a.act[0], a.act[1], a.act[2], a.h[0], a.h[1],
+                                   a.h[2], a.gm, a.cfl, (int)E);
+                SPDK_LAUNCH_CHECK("k_mfinish");
+                done = true;
+                break;
+            }
+        }
+        hipLaunchKernelGGL(k_mhist, dim3(256), dim3(256), 0, c->stream, (const unsigned char*)conv, (int)E,
+                           kHistBins, (unsigned long long*)(W + L.hist));
+        SPDK_LAUNCH_CHECK("k_mhist");
+        if (!done) {
+            if (st) st->fallbacks++;
+            if ((rc = start())) return rc;  // fresh traces of the zero-order guess
+        }
+    }
+    if (!done) {
+        if ((rc = fixed_count())) return rc;
+        sweeps += T;
+    }
+    if ((rc = reduce(false))) return rc;
+    if (dt_next) *dt_next = c->red_host[0];
+    if (st) {
+        st->steps++;
+        st->sweeps += sweeps;
+    }
+    for (int i = 0; i < c->ev_used; i++) {
+        float ms = 0;
+        SPDK_CHECK(hipEventElapsedTime(&ms, c->evs[i][0], c->evs[i][1]));
+        c->kernel_ms += ms;
+        c->kernel_launches++;
+    }
+    c->ev_used = 0;
+    return SPDK_OK;
+}
+
+template <int N, int DIM>
+int run_prepare(spdk_ctx* c, const double* U_sp, const int Ne[3], void* ws) {
+    using G = MT<N, DIM>;
+    int rc;
+    if ((rc = ensure_mixed_mats(c))) return rc;
+    const size_t E = (size_t)Ne[0] * Ne[1] * Ne[2];
+    MLayout L = mlayout<N, DIM>(E);
+    char* W = (char*)ws;
+    float* tr[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < DIM; d++) tr[d] = (float*)(W + L.trA + d * L.tr_axis);
+    hipLaunchKernelGGL((k_mstart<N, DIM>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, U_sp,
+                       (double*)(W + L.mu), (float*)(W + L.dold), tr[0], tr[1], tr[2], (const float*)c->mixed_mats,
+                       (unsigned char*)(W + L.conv), (int)E);
+    SPDK_LAUNCH_CHECK("k_mstart");
+    return SPDK_OK;
+}
+
+typedef int (*mstep_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, const spdk_picard*,
+                        spdk_picard_stats*, double*);
+typedef MLayout (*mlayout_fn)(size_t);
+typedef int (*mprep_fn)(spdk_ctx*, const double*, const int*, void*);
+struct MVariant {
+    int n, dim;
+    mstep_fn step;
+    mlayout_fn layout;
+    mprep_fn prepare;
+};
+#define SPDK_MVARIANT(Nn, D) {Nn, D, run_mixed<Nn, D>, mlayout<Nn, D>, run_prepare<Nn, D>}
+const MVariant kMVariants[] = {SPDK_MVARIANT(2, 1), SPDK_MVARIANT(3, 1), SPDK_MVARIANT(4, 1), SPDK_MVARIANT(5, 1),
+                               SPDK_MVARIANT(2, 2), SPDK_MVARIANT(3, 2), SPDK_MVARIANT(4, 2), SPDK_MVARIANT(5, 2),
+                               SPDK_MVARIANT(2, 3), SPDK_MVARIANT(3, 3), SPDK_MVARIANT(4, 3)};
+
+const MVariant* find_mvariant(spdk_ctx* c) {
+    int dim = c->prm.active[0] + c->prm.active[1] + c->prm.active[2];
+    if (!c->prm.active[0] || (c->prm.active[2] && !c->prm.active[1])) return nullptr;
+    if (c->prm.rk_order != 0 || c->T != c->n) return nullptr;
+    for (const MVariant& v : kMVariants)
+        if (v.n == c->n && v.dim == dim) return &v;
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t spdk_mixed_workspace_bytes(spdk_ctx* c, const int N[3]) {
+    const MVariant* v = c ? find_mvariant(c) : nullptr;
+    if (!v) {
+        set_error("mixed path: ADER hydro with a tile variant for this degree/dimensionality only");
+        return 0;
+    }
+    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
+        set_error("mixed path covers the inviscid, source-free update");
+        return 0;
+    }
+    return v->layout((size_t)N[0] * N[1] * N[2]).total;
+}
+
+int spdk_mixed_predictor_buffer(spdk_ctx* c, void* ws, const int N[3], float** D, double** mu, float** d_old) {
+    const MVariant* v = c ? find_mvariant(c) : nullptr;
+    if (!v || !ws) { set_error("mixed path: no variant or workspace"); return SPDK_E_CONFIG; }
+    MLayout L = v->layout((size_t)N[0] * N[1] * N[2]);
+    if (D) *D = (float*)((char*)ws + L.D);
+    if (mu) *mu = (double*)((char*)ws + L.mu);
+    if (d_old) *d_old = (float*)((char*)ws + L.dold);
+    return SPDK_OK;
+}
+
+int spdk_mixed_histogram(spdk_ctx* c, void* ws, const int N[3], unsigned long long* hist, int nbins, int reset) {
+    const MVariant* v = c ? find_mvariant(c) : nullptr;
+    if (!v || !ws || nbins < 0) { set_error("mixed histogram: bad argument"); return SPDK_E_ARG; }
+    MLayout L = v->layout((size_t)N[0] * N[1] * N[2]);
+    unsigned long long* d = (unsigned long long*)((char*)ws + L.hist);
+    if (hist && nbins > 0) {
+        unsigned long long tmp[kHistBins] = {};
+        SPDK_CHECK(hipMemcpyAsync(tmp, d, sizeof(tmp), hipMemcpyDeviceToHost, c->stream));
+        SPDK_CHECK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < nbins; i++) hist[i] = i < kHistBins ? tmp[i] : 0;
+    }
+    if (reset) SPDK_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long) * kHistBins, c->stream));
+    return SPDK_OK;
+}
+
+static int check_mixed(spdk_ctx* c, const void* U_sp, const spdk_sd_shape& s, const void* ws, const MVariant** out) {
+    if (!c || !U_sp || !ws) { set_error("mixed: null argument"); return SPDK_E_ARG; }
+    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
+        set_error("mixed path covers the inviscid, source-free update");
+        return SPDK_E_CONFIG;
+    }
+    const MVariant* v = find_mvariant(c);
+    if (!v) { set_error("mixed path: ADER hydro with a tile variant for this degree/dimensionality only"); return SPDK_E_CONFIG; }
+    if (s.nv != NV || s.nt != 1 || s.nx != c->n || s.ny != (c->prm.active[1] ? c->n : 1) ||
+        s.nz != (c->prm.active[2] ? c->n : 1)) {
+        set_error("mixed: U_sp shape");
+        return SPDK_E_SHAPE;
+    }
+    for (int d = 0; d < 3; d++) {
+        int Nd = d == 0 ? s.Nx : (d == 1 ? s.Ny : s.Nz);
+        if (c->prm.active[d] && Nd < 3) { set_error("mixed: axis needs ghost elements"); return SPDK_E_SHAPE; }
+    }
+    if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("mixed: too many elements"); return SPDK_E_SHAPE; }
+    *out = v;
+    return SPDK_OK;
+}
+
+int spdk_mixed_prepare(spdk_ctx* c, const double* U_sp, spdk_sd_shape s, void* ws) {
+    const MVariant* v = nullptr;
+    int rc = check_mixed(c, U_sp, s, ws, &v);
+    if (rc) return rc;
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    return v->prepare(c, U_sp, Ne, ws);
+}
+
+int spdk_hydro_step_mixed(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
+                          const spdk_picard* pc, spdk_picard_stats* stats, double* dt_next) {
+    const MVariant* v = nullptr;
+    int rc0 = check_mixed(c, U_sp, s, ws, &v);
+    if (rc0) return rc0;
+    if (pc && pc->residual && !(pc->tol > 0.0)) { set_error("mixed: residual tolerance must be positive"); return SPDK_E_ARG; }
+    if (pc && pc->guess != SPDK_GUESS_ZERO && pc->guess != SPDK_GUESS_EXTERNAL) { set_error("mixed: unknown guess"); return SPDK_E_ARG; }
+    if (pc && pc->guess == SPDK_GUESS_EXTERNAL && !pc->residual) {
+        set_error("mixed: an external guess needs the residual test");
+        return SPDK_E_ARG;
+    }
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);
+    return v->step(c, U_sp, Ne, ws, dt, W_cv, F_x, pc, stats, dt_next);
+}
+
+}  // extern "C"

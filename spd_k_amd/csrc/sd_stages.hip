@@ -365,6 +365,8 @@ int spdk_ctx_create(const spdk_params* prm, const spdk_operators* ops, void* str
         SPDK_CHECK(hipMalloc(&c->mat[q], b));
         if (ops->mat[q]) SPDK_CHECK(hipMemcpy(c->mat[q], ops->mat[q], b, hipMemcpyHostToDevice));
         else SPDK_CHECK(hipMemset(c->mat[q], 0, b));
+        if (ops->mat[q]) c->mat_host[q].assign(ops->mat[q], ops->mat[q] + rows[q] * cols[q]);
+        else c->mat_host[q].assign((size_t)rows[q] * cols[q], 0.0);
     }
     SPDK_CHECK(hipMalloc(&c->wt, sizeof(double) * T));
     for (int t = 0; t < T; t++) c->wt_host[t] = ops->wt[t];
@@ -399,6 +401,7 @@ int spdk_ctx_destroy(spdk_ctx* c) {
     for (int d = 0; d < 3; d++) { hipFree(c->fvf[d]); hipFree(c->fvc[d]); }
     hipFree(c->red);
     hipHostFree(c->red_host);
+    for (void* p : c->owned) hipFree(p);
     for (int i = 0; i < spdk_ctx::kEvPool; i++)
         for (int j = 0; j < 2; j++)
             if (c->evs[i][j]) hipEventDestroy(c->evs[i][j]);

@@ -74,6 +74,11 @@ CASES = {
                             "fused"),
     "rectangular2d": (["-i", S, "mesh/nx1=8", "mesh/nx2=4", "mesh/nx3=1", "mesh/x1len=2.0", "mesh/x2len=0.5",
                        "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    # p = 1 in 3-D: more pencils (3 * 4) than points per element (8)
+    "sine3d_p1": (["-i", S, "mesh/p=1", "mesh/nx1=6", "mesh/nx2=5", "mesh/nx3=4", "time/tlim=0.02", "output/dt=0.01"],
+                  "fused"),
+    "sine3d_p1_rk2": (["-i", S, "mesh/p=1", "mesh/nx1=4", "mesh/nx2=4", "mesh/nx3=4", "time/integrator=rk2",
+                       "time/tlim=0.02", "output/dt=0.01"], "fused"),
 }
 
 
@@ -137,3 +142,60 @@ IND_CASES = {
 def test_induction_parity_with_oracle(tmp_path, name):
     a, b = both(tmp_path, IND_CASES[name], "stages")
     assert_bit_exact(a, b)
+
+
+# ---- mixed-precision path (FP32 state and contractions, FP64 fluxes and accumulation)
+def _mixed_vs_fp64(tmp_path, args, extra=()):
+    a, b = str(tmp_path / "fp64"), str(tmp_path / "mixed")
+    op.run_driver(args, a)
+    print(op.run_driver(args + ["job/precision=mixed"] + list(extra), b)[-400:])
+    return a, b
+
+
+def _rel(a, b, i, v):
+    g = op.grid(a)
+    x, y = op.load_sd(a, g, i, v), op.load_sd(b, g, i, v)
+    return float(np.abs(x - y).mean() / np.abs(x).mean()), float(np.abs(x - y).max() / np.abs(x).max())
+
+
+def test_mixed_sine2d_accuracy(tmp_path):
+    """2-D sine advection, 32^2, p=3: the mixed path's L1 error against the
+    exact solution is within 1.1x of the FP64 path's, its density differs
+    from FP64 by FP32-level amounts, and mass is conserved to round-off."""
+    args = ["-i", S, "mesh/nx1=32", "mesh/nx2=32", "mesh/nx3=1", "problem/amp=0.1", "time/tlim=0.2", "output/dt=0.1"]
+    a, b = _mixed_vs_fp64(tmp_path, args)
+    i = op.output_indices(a)[-1]
+    l1a = op.l1_vs_exact_sine(a, i, 0.2, 2, amp=0.1)
+    l1b = op.l1_vs_exact_sine(b, i, 0.2, 2, amp=0.1)
+    assert l1b <= 1.1 * l1a, (l1a, l1b)
+    mean, mx = _rel(a, b, i, 0)
+    assert mean < 1e-7 and mx < 1e-6, (mean, mx)
+    assert op.mass_drift(b) < 1e-13
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16, 3), (12, 10, 1, 3), (6, 6, 6, 1), (8, 8, 8, 2), (10, 12, 1, 4)])
+def test_mixed_kh_close_to_fp64(tmp_path, dims):
+    """3-D/2-D KH with random perturbation: every conserved variable stays
+    within FP32-level relative distance of the FP64 path and totals are
+    conserved to round-off (the face-balance correction)."""
+    nx, ny, nz, p = dims
+    args = ["-i", KH, "job/fallback=false", f"mesh/nx1={nx}", f"mesh/nx2={ny}", f"mesh/nx3={nz}", f"mesh/p={p}",
+            "problem/perturb=random", "problem/amp=0.01", "time/tlim=0.05", "output/dt=0.025"]
+    a, b = _mixed_vs_fp64(tmp_path, args)
+    i = op.output_indices(a)[-1]
+    for v in (0, 1, 4):
+        mean, mx = _rel(a, b, i, v)
+        assert mean < 1e-6 and mx < 1e-5, (v, mean, mx)
+    assert op.mass_drift(b) < 1e-13
+
+
+def test_mixed_residual_converges_to_fixed_point(tmp_path):
+    """Residual-driven Picard from the zero-order guess: with a tight
+    tolerance the step uses the converged space-time predictor, which stays
+    close to the p-iteration predictor's result; totals are conserved."""
+    args = ["-i", S, "mesh/nx1=16", "mesh/nx2=16", "mesh/nx3=1", "time/tlim=0.05", "output/dt=0.025"]
+    a, b = _mixed_vs_fp64(tmp_path, args, ["job/picard=residual", "job/picard_tol=1e-9", "job/picard_max=30"])
+    i = op.output_indices(a)[-1]
+    mean, mx = _rel(a, b, i, 0)
+    assert mean < 1e-5, (mean, mx)
+    assert op.mass_drift(b) < 1e-13

@@ -72,7 +72,9 @@ struct MT {
     static constexpr int TPE = N * NT;
     static constexpr int FSZ = M * NT;
     static constexpr int SLOTS = T > DIM ? T : DIM;
-    static constexpr int LDS_D = T * NVP * TPE;      // floats: state slices
+    static constexpr int ROW = N + 1;                // padded x-row of an LDS state plane
+    static constexpr int PSZ = DIM == 1 ? N : TPE / N * ROW;  // padded state plane
+    static constexpr int LDS_D = T * NVP * PSZ;      // floats: state slices
     static constexpr int LDS_F = SLOTS * NVP * FSZ;  // floats: flux slots
     static constexpr int LDS_X = 6 * NVP;            // doubles: mu, F(mu) per axis, residual, shift
     static constexpr int ELEM_BYTES = ((LDS_X * 8 + (LDS_D + LDS_F) * 4) + 15) / 16 * 16;
@@ -124,7 +126,12 @@ struct MArgs {
 // momentum, slots 2 and 3 the transverse ones (cyclic, as V1, V2, V3 of
 // src/hydro.cpp:156-162). One code path serves every axis; the caller maps
 // slots to variables with vperm.
-__device__ __forceinline__ int vperm(int v, int d) { return (v == 0 || v == 4) ? v : 1 + (v - 1 + d) % 3; }
+__device__ __forceinline__ int vperm(int v, int d) {
+    if (v == 1) return d == 0 ? 1 : (d == 1 ? 2 : 3);
+    if (v == 2) return d == 0 ? 2 : (d == 1 ? 3 : 1);
+    if (v == 3) return d == 0 ? 3 : (d == 1 ? 1 : 2);
+    return v;
+}
 
 // refined hardware reciprocal (one Newton step; not correctly rounded)
 __device__ __forceinline__ double frcp(double x) {
@@ -200,7 +207,37 @@ __device__ __forceinline__ int spidx(int d, int ll, int q) {
     return ll * N * N + q;
 }
 
-// One pencil task: slice Dsl of the LDS state (floats [v][TPE]) along axis
+// LDS state planes pad every x-row of N points to N+1 floats, so the pencil
+// reads along x, y and z (one pencil per lane) fall in distinct banks.
+template <int N, int DIM>
+__device__ __forceinline__ int lpad(int s) { return DIM == 1 ? s : s + s / N; }
+
+// Pencil q along axis d: point l at state offset sb + l*ss (padded plane),
+// flux point f at slot offset fb + f*fs (the fidx layout). Selects only: d
+// may differ between lanes.
+struct Geo { int sb, ss, fb, fs; };
+template <int N, int DIM>
+__device__ __forceinline__ Geo geo(int d, int q) {
+    constexpr int M = N + 1, R = N + 1, NT = MT<N, DIM>::NT;
+    Geo g;
+    if (DIM == 1) {
+        g.sb = 0; g.ss = 1; g.fb = 0; g.fs = 1;
+    } else if (DIM == 2) {
+        g.sb = d == 0 ? q * R : q;
+        g.ss = d == 0 ? 1 : R;
+        g.fb = d == 0 ? q * M : q;
+        g.fs = d == 0 ? 1 : N;
+    } else {
+        const int qh = q / N, ql = q % N;
+        g.sb = d == 0 ? q * R : (d == 1 ? qh * N * R + ql : q + qh);
+        g.ss = d == 0 ? 1 : (d == 1 ? R : N * R);
+        g.fb = d == 0 ? q * M : (d == 1 ? qh * M * N + ql : q);
+        g.fs = d == 0 ? 1 : (d == 1 ? N : NT);
+    }
+    return g;
+}
+
+// One pencil task: slice Dsl of the LDS state (floats [v][PSZ]) along axis
 // d, pencil q. Writes F - F(mu) at the M flux points into the flux slot
 // (floats [v][FSZ]) and, when asked, adds bw * (F(1) - F(0)) of the
 // pencil to the face balance (want_bal). Face values follow src/boundary.cpp:48-113
@@ -213,6 +250,7 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
     constexpr int M = G::M, NT = G::NT;
     const size_t E = (size_t)a.E;
     const double gm1 = a.gm - 1.0;
+    const Geo g = geo<N, DIM>(d, q);
     int pv[NVP];
 #pragma unroll
     for (int v = 0; v < NVP; v++) pv[v] = vperm(v, d);
@@ -256,7 +294,7 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
     for (int v = 0; v < NVP; v++) {
         float p[N];
 #pragma unroll
-        for (int l = 0; l < N; l++) p[l] = Dsl[pv[v] * G::TPE + spidx<N, DIM>(d, l, q)];
+        for (int l = 0; l < N; l++) p[l] = Dsl[pv[v] * G::PSZ + g.sb + l * g.ss];
 #pragma unroll
         for (int f = 0; f < M; f++) {
             float x = 0.f;
@@ -273,34 +311,30 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
         for (int v = 0; v < NVP; v++) u[v] = mu[v] + (double)uf[f][v];
         mflux(u, fl, gm1);
 #pragma unroll
-        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + fidx<N, DIM>(d, f, q)] = (float)(fl[v] - Fb[v]);
+        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + g.fb + f * g.fs] = (float)(fl[v] - Fb[v]);
     }
-    // faces
+    // faces: LLF at every face; at the outer face of a ghost element
+    // (no neighbour) both states are the element's own, which gives its
+    // physical flux exactly
 #pragma unroll
     for (int side = 0; side < 2; side++) {
         const int f = side ? M - 1 : 0;
         const bool raw = side ? (ed == Nd - 1) : (ed == 0);
-        double own[NVP], fl[NVP];
+        double own[NVP], fl[NVP], uL[NVP], uR[NVP];
 #pragma unroll
-        for (int v = 0; v < NVP; v++) own[v] = mu[v] + (double)uf[f][v];
-        if (raw) {
-            mflux(own, fl, gm1);
-        } else {
-            double uL[NVP], uR[NVP];
-#pragma unroll
-            for (int v = 0; v < NVP; v++) {
-                if (side == 0) {
-                    uL[v] = n0[v];
-                    uR[v] = ed == Nd - 1 ? n1[v] : own[v];
-                } else {
-                    uL[v] = ed == 0 ? n0[v] : own[v];
-                    uR[v] = n1[v];
-                }
+        for (int v = 0; v < NVP; v++) {
+            own[v] = mu[v] + (double)uf[f][v];
+            if (side == 0) {
+                uL[v] = raw ? own[v] : n0[v];
+                uR[v] = ed == Nd - 1 ? n1[v] : own[v];
+            } else {
+                uL[v] = ed == 0 ? n0[v] : own[v];
+                uR[v] = raw ? own[v] : n1[v];
             }
-            mllf(uL, uR, fl, a.gm);
         }
+        mllf(uL, uR, fl, a.gm);
 #pragma unroll
-        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + fidx<N, DIM>(d, f, q)] = (float)(fl[v] - Fb[v]);
+        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + g.fb + f * g.fs] = (float)(fl[v] - Fb[v]);
         if (want_bal) {
             // bal is indexed by variable: un-permute the momentum slots
             double c[NVP];
@@ -445,6 +479,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
     const size_t E = (size_t)a.E;
     const size_t slabE = E * TPE;
     const size_t eo = (size_t)e * TPE + s;
+    constexpr int PSZ = G::PSZ;
+    const int so = lpad<N, DIM>(s);  // this point in a padded state plane
 
     int ei = 0, ej = 0, ek = 0;
     if (live) {
@@ -476,10 +512,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
 #pragma unroll
             for (int t = 0; t < T; t++)
 #pragma unroll
-                for (int v = 0; v < NVP; v++) Ds[(t * NVP + v) * TPE + s] = a.D[((size_t)t * NVP + v) * slabE + eo];
+                for (int v = 0; v < NVP; v++) Ds[(t * NVP + v) * PSZ + so] = a.D[((size_t)t * NVP + v) * slabE + eo];
         } else {
 #pragma unroll
-            for (int v = 0; v < NVP; v++) Ds[v * TPE + s] = dold[v];
+            for (int v = 0; v < NVP; v++) Ds[v * PSZ + so] = dold[v];
         }
     }
     __syncthreads();
@@ -510,7 +546,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             const int slot = a.path ? t : d;
             const int ts = a.tin_slices > 1 ? t : 0;
             const double bw = (a.path ? pick(a.wdt, t) : wsum) * pencil_weight<N, DIM>(a, q) * pick(a.ih, d);
-            pencil<N, DIM>(a, Ds + t * NVP * TPE, Fs + slot * NVP * FSZ, X, d, q, ts, ei, ej, ek, (size_t)e, cons,
+            pencil<N, DIM>(a, Ds + t * NVP * PSZ, Fs + slot * NVP * FSZ, X, d, q, ts, ei, ej, ek, (size_t)e, cons,
                            bal, bw);
         }
         __syncthreads();
@@ -519,6 +555,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
 #pragma unroll 1
             for (int d = d0; d < d1; d++) {
                 const int qdd = d == 0 ? qx : (d == 1 ? qy : qz), idd = d == 0 ? ii : (d == 1 ? jj : kk);
+                const Geo gd = geo<N, DIM>(d, qdd);
                 float Dr[M];
 #pragma unroll
                 for (int l = 0; l < M; l++) Dr[l] = a.Dh[(d * N + idd) * M + l];
@@ -531,7 +568,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                         const float* F = Ft + v * FSZ;
                         float acc = 0.f;
 #pragma unroll
-                        for (int l = 0; l < M; l++) acc += F[fidx<N, DIM>(d, l, qdd)] * Dr[l];
+                        for (int l = 0; l < M; l++) acc += F[gd.fb + l * gd.fs] * Dr[l];
                         dudt[t][v] += acc;
                     }
                 }
@@ -569,10 +606,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                     for (int l = 0; l < T; l++) acc += a.cdt[t * kMaxTm + l] * dudt[l][v];
                     float dn = dold[v] - acc;
                     // input state at this point: its own LDS slot (path 1) or d_old
-                    float din = a.path ? Ds[(t * NVP + v) * TPE + s] : dold[v];
+                    float din = a.path ? Ds[(t * NVP + v) * PSZ + so] : dold[v];
                     res = fmaxf(res, fabsf(dn - din));
                     a.D[((size_t)t * NVP + v) * slabE + eo] = dn;
-                    Ds[(t * NVP + v) * TPE + s] = dn;
+                    Ds[(t * NVP + v) * PSZ + so] = dn;
                 }
         }
 #pragma unroll
@@ -586,12 +623,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         if (live) {
             for (int task = s; task < T * DIM * NT; task += TPE) {
                 const int t = task / (DIM * NT), r = task % (DIM * NT), d = r / NT, q = r % NT;
+                const Geo gt = geo<N, DIM>(d, q);
 #pragma unroll
                 for (int v = 0; v < NVP; v++) {
                     float x0 = 0.f, x1 = 0.f;
 #pragma unroll
                     for (int l = 0; l < N; l++) {
-                        float p = Ds[(t * NVP + v) * TPE + spidx<N, DIM>(d, l, q)];
+                        float p = Ds[(t * NVP + v) * PSZ + gt.sb + l * gt.ss];
                         x0 += p * a.S2F[l];
                         x1 += p * a.S2F[(M - 1) * N + l];
                     }

@@ -236,7 +236,8 @@ int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
 #define SPDK_GUESS_EXTERNAL 1 /* caller-filled predictor buffer (e.g. the MLP)    */
 typedef struct spdk_picard {
     int residual;   /* 0: p predictor sweeps then the corrector (the reference's
-                       count); 1: sweep until the element residual test passes */
+                       count); 1: sweep the elements that have not passed the
+                       residual test (and their neighbours) until all pass   */
     int guess;      /* SPDK_GUESS_*                                            */
     double tol;     /* residual test: max over elements of
                        max|d_out - d_in| / max_v|mean_v| < tol                 */
@@ -244,7 +245,9 @@ typedef struct spdk_picard {
                        fallback of the whole step                              */
 } spdk_picard;
 typedef struct spdk_picard_stats {
-    long long steps, sweeps, fallbacks;  /* accumulated by spdk_hydro_step_mixed */
+    long long steps, fallbacks; /* accumulated by spdk_hydro_step_mixed           */
+    double sweeps;              /* sweeps, each weighted by the fraction of
+                                   elements it updated                          */
     double last_residual;
 } spdk_picard_stats;
 size_t spdk_mixed_workspace_bytes(spdk_ctx* ctx, const int N[3]);
@@ -276,8 +279,8 @@ typedef struct spdk_run_result {
     double kernel_ms_total;    /* device time of the dominant kernel (HIP events)*/
     int kernel_launches;       /* launches of that kernel                        */
     char kernel_name[64];
-    long long picard_steps;    /* mixed path: steps, predictor/corrector sweeps, */
-    long long picard_sweeps;   /*   zero-guess fallbacks and the histogram of    */
+    long long picard_steps;    /* mixed path: steps, element-weighted sweeps,    */
+    double picard_sweeps;      /*   zero-guess fallbacks and the histogram of    */
     long long picard_fallbacks;/*   spdk_mixed_histogram (residual form)         */
     long long picard_hist[32];
 } spdk_run_result;

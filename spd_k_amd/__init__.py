@@ -60,7 +60,7 @@ class RunResult(ctypes.Structure):
                 ("dof_updates_per_s", ctypes.c_double), ("t_final", ctypes.c_double),
                 ("kernel_ms_total", ctypes.c_double), ("kernel_launches", ctypes.c_int),
                 ("kernel_name", ctypes.c_char * 64), ("picard_steps", ctypes.c_longlong),
-                ("picard_sweeps", ctypes.c_longlong), ("picard_fallbacks", ctypes.c_longlong),
+                ("picard_sweeps", ctypes.c_double), ("picard_fallbacks", ctypes.c_longlong),
                 ("picard_hist", ctypes.c_longlong * 32)]
 
 

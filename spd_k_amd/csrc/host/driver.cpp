@@ -366,7 +366,7 @@ struct HydroRun {
             res->picard_sweeps = pst.sweeps;
             res->picard_fallbacks = pst.fallbacks;
             std::printf("picard: %lld steps, %.3f sweeps/step, %lld fallbacks, last residual %.3e\n", pst.steps,
-                        pst.steps ? (double)pst.sweeps / pst.steps : 0.0, pst.fallbacks, pst.last_residual);
+                        pst.steps ? pst.sweeps / pst.steps : 0.0, pst.fallbacks, pst.last_residual);
             if (pc.residual) {
                 int N[3] = {X.n_total, Y.n_total, Z.n_total};
                 unsigned long long hist[32];

@@ -37,13 +37,22 @@
 // (slice, line of points along the axis) per thread (T * N^(d-1) = N^d).
 //
 // Picard control: the fixed count (p predictor sweeps, then the corrector,
-// src/hydro_ader.hpp:291-305) or a residual test. In the residual form every
-// predictor sweep also evaluates the corrector from the fluxes of its input
-// state (a candidate) and the element residual
-//     r_e = max |d_out - d_in| / max_v |mu(v)|;
-// once max_e r_e < tol the candidate is the step's result. A step whose
-// residual does not pass within max_sweeps falls back to the fixed-count
-// sequence from the zero-order guess.
+// src/hydro_ader.hpp:291-305) or a residual test with the element residual
+//     r_e = max |d_out - d_in| / max_v |mu(v)|.
+// Residual form: the first sweep updates every element and also evaluates
+// the corrector from the fluxes of its input state (a candidate); if every
+// r_e < tol the candidate is the step's result. Otherwise later sweeps update
+// only the active elements: those whose own r_e, or the r_e of an element
+// whose traces they read, was >= tol in the previous sweep; the others keep
+// their state and traces. When no element is active a corrector sweep over
+// every element ends the step (its face fluxes come from one set of traces,
+// so conservation holds). A step not converged within max_sweeps falls back
+// to the fixed-count sequence from the zero-order guess.
+//
+// Trace buffers. Each element writes its traces to the older of two buffers
+// and records the sweep number (ver); readers take, per neighbour, the
+// newest buffer written before the current sweep. A buffer being written is
+// therefore never read, whether or not the element was active before.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstring>
@@ -86,6 +95,7 @@ struct MT {
     static_assert(EPB >= 1 && BLOCK <= 256, "tile does not fit a workgroup");
     static_assert((LDS_D + LDS_F) * 4 >= 2 * NVP * TPE * 8, "cv ping-pong needs two FP64 state planes");
     static constexpr int XRES = 4 * NVP;      // X slot: element residual
+    static constexpr int XACT = 4 * NVP + 1;  // X slot: element active this sweep
     static constexpr int XSHIFT = 4 * NVP + 2; // X slots: conservative shift per variable
     static_assert(T <= kMaxTm, "time nodes");
 };
@@ -96,8 +106,12 @@ struct MArgs {
     const double* mu;       // (5,E) element means of U_sp
     const float* d_old;     // (5,E,TPE) U_sp - mu
     float* D;               // (T,5,E,TPE) predictor state, read and rewritten in place
-    const float* tr_in[3];  // per axis (Tin,5,E,2,NT) traces of the input state
-    float* tr_out[3];       // per axis (T,5,E,2,NT) traces of the output state
+    float* trb[2][3];       // two trace buffers per axis, (T,5,E,2,NT) (slice 0 only for the zero-order guess)
+    int* ver;               // (2,E): sweep number of each element's last write to each buffer
+    int cur;                // this sweep's number
+    const unsigned char* big_in;  // per element: r_e >= tol in the previous sweep (null: every element active)
+    unsigned char* big_out;       // per element, this sweep (null: not recorded)
+    double* cnt_part;       // per-block count of active elements (predictor)
     double* W_cv;           // optional (1,6,E,TPE) output primitives at control volumes
     double* F_x;            // optional (T,6,E,NT,M) x flux-point fluxes
     double* part;           // per-block dt minima (corrector)
@@ -118,7 +132,7 @@ struct MArgs {
     double tol;
     int sweep;
     int path;       // 0: every slice is d_old (zero-order guess); 1: slices from D
-    int tin_slices; // slices present in tr_in (1 or T)
+    int tin_slices; // slices present in the readable trace buffers (1 or T)
     int out_pred;   // predictor (else corrector)
 };
 
@@ -237,6 +251,30 @@ __device__ __forceinline__ Geo geo(int d, int q) {
     return g;
 }
 
+// Trace buffer of axis d holding element el's newest traces written before
+// this sweep.
+__device__ __forceinline__ const float* read_buffer(const MArgs& a, int d, size_t el) {
+    const int v0 = a.ver[el], v1 = a.ver[(size_t)a.E + el];
+    const int b = (v1 < a.cur && (v1 > v0 || v0 >= a.cur)) ? 1 : 0;
+    const float* t0 = d == 0 ? a.trb[0][0] : (d == 1 ? a.trb[0][1] : a.trb[0][2]);
+    const float* t1 = d == 0 ? a.trb[1][0] : (d == 1 ? a.trb[1][1] : a.trb[1][2]);
+    return b ? t1 : t0;
+}
+
+// Elements whose traces element e reads across its two faces along axis d
+// (ghost fills: periodic copies the opposite interior element, gradfree /
+// reflective the adjacent one; src/boundary.cpp:48-113).
+__device__ __forceinline__ void face_sources(const MArgs& a, int d, int ei, int ej, int ek, size_t e, size_t& el0,
+                                             size_t& el1) {
+    const int ed = d == 0 ? ei : (d == 1 ? ej : ek);
+    const int Nd = d == 0 ? a.Ne[0] : (d == 1 ? a.Ne[1] : a.Ne[2]);
+    const bool per = (d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2])) == SPDK_BC_PERIODIC;
+    const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)a.Ne[0] : (size_t)a.Ne[0] * a.Ne[1]);
+    const size_t ebase = e - (size_t)ed * stride;
+    el0 = ebase + (size_t)(ed <= 1 ? (per ? Nd - 2 : 1) : ed - 1) * stride;
+    el1 = ebase + (size_t)(ed >= Nd - 2 ? (per ? 1 : Nd - 2) : ed + 1) * stride;
+}
+
 // One pencil task: slice Dsl of the LDS state (floats [v][PSZ]) along axis
 // d, pencil q. Writes F - F(mu) at the M flux points into the flux slot
 // (floats [v][FSZ]) and, when asked, adds bw * (F(1) - F(0)) of the
@@ -265,7 +303,6 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
     const int type = d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2]);
     const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)a.Ne[0] : (size_t)a.Ne[0] * a.Ne[1]);
     const size_t ebase = e - (size_t)ed * stride;
-    const float* tr = d == 0 ? a.tr_in[0] : (d == 1 ? a.tr_in[1] : a.tr_in[2]);
     const bool per = type == SPDK_BC_PERIODIC;
     const size_t el0 = ebase + (size_t)(ed <= 1 ? (per ? Nd - 2 : 1) : ed - 1) * stride;
     const int sid0 = ed <= 1 ? (per ? 1 : 0) : 1;
@@ -273,13 +310,15 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
     const int sid1 = ed >= Nd - 2 ? (per ? 0 : 1) : 0;
     const bool flip0 = ed <= 1 && type == SPDK_BC_REFLECTIVE;
     const bool flip1 = ed >= Nd - 2 && type == SPDK_BC_REFLECTIVE;
+    const float* tr0 = read_buffer(a, d, el0);
+    const float* tr1 = read_buffer(a, d, el1);
     double n0[NVP], n1[NVP];
 #pragma unroll
     for (int v = 0; v < NVP; v++) {
         const size_t pvE = (size_t)pv[v] * E;
         const size_t tb = ((size_t)ts * NVP + pv[v]) * E;
-        n0[v] = a.mu[pvE + el0] + (double)tr[((tb + el0) * 2 + sid0) * NT + q];
-        n1[v] = a.mu[pvE + el1] + (double)tr[((tb + el1) * 2 + sid1) * NT + q];
+        n0[v] = a.mu[pvE + el0] + (double)tr0[((tb + el0) * 2 + sid0) * NT + q];
+        n1[v] = a.mu[pvE + el1] + (double)tr1[((tb + el1) * 2 + sid1) * NT + q];
     }
     if (flip0) n0[1] = -n0[1];
     if (flip1) n1[1] = -n1[1];
@@ -495,8 +534,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
 
     if (live)
         for (int v = s; v < NVP; v += TPE) X[v] = a.mu[(size_t)v * E + e];
+    if (live && s == 0) {  // active: own residual or a trace source's was >= tol last sweep
+        bool act = true;
+        if (a.big_in) {
+            act = a.big_in[e] != 0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                size_t el0, el1;
+                face_sources(a, d, ei, ej, ek, (size_t)e, el0, el1);
+                act = act || a.big_in[el0] || a.big_in[el1];
+            }
+        }
+        X[G::XACT] = act ? 1.0 : 0.0;
+    }
     __syncthreads();
-    if (live && s < DIM) {  // F(mu) along axis s
+    const bool act_e = live && X[G::XACT] != 0.0;
+    if (!__syncthreads_or(act_e)) {  // nothing to do in this block
+        if (live && s == 0 && a.big_out) a.big_out[e] = 0;
+        if (tid == 0) {
+            if (a.res_part) a.res_part[blockIdx.x] = 0.0;
+            if (a.cnt_part) a.cnt_part[blockIdx.x] = 0.0;
+        }
+        return;
+    }
+    if (act_e && s < DIM) {  // F(mu) along axis s
         double um[NVP], fb[NVP];
 #pragma unroll
         for (int v = 0; v < NVP; v++) um[v] = X[vperm(v, s)];
@@ -506,8 +567,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
     }
     float dold[NVP];
 #pragma unroll
-    for (int v = 0; v < NVP; v++) dold[v] = live ? a.d_old[(size_t)v * slabE + eo] : 0.f;
-    if (live) {
+    for (int v = 0; v < NVP; v++) dold[v] = act_e ? a.d_old[(size_t)v * slabE + eo] : 0.f;
+    if (act_e) {
         if (a.path) {
 #pragma unroll
             for (int t = 0; t < T; t++)
@@ -541,7 +602,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
     for (int r = 0; r < nrounds; r++) {
         // tasks of the round: TPE (= T * NT) or DIM * NT (more than TPE only for p = 1 in 3-D)
         const int ntask = a.path ? TPE : DIM * NT;
-        for (int task = s; live && task < ntask; task += TPE) {
+        for (int task = s; act_e && task < ntask; task += TPE) {
             const int d = a.path ? r : task / NT, t = a.path ? task / NT : 0, q = task % NT;
             const int slot = a.path ? t : d;
             const int ts = a.tin_slices > 1 ? t : 0;
@@ -550,7 +611,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                            bal, bw);
         }
         __syncthreads();
-        if (live) {
+        if (act_e) {
             const int d0 = a.path ? r : 0, d1 = a.path ? r + 1 : DIM;
 #pragma unroll 1
             for (int d = d0; d < d1; d++) {
@@ -596,7 +657,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         // ---- predictor: d_new(t) = d_old - sum_l cdt(t,l) dudt(l); residual vs the input state
         float res = 0.f;
         double du[NVP];
-        if (live) {
+        if (act_e) {
 #pragma unroll
             for (int t = 0; t < T; t++)
 #pragma unroll
@@ -619,8 +680,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             for (int t = 0; t < T; t++) du[v] += a.wdt[t] * (double)dudt[t][v];
         }
         __syncthreads();
-        // traces of the new state: task = (t, d, q)
-        if (live) {
+        // traces of the new state into this element's older buffer: task = (t, d, q)
+        const int wb = act_e ? (a.ver[(size_t)a.E + e] < a.ver[e] ? 1 : 0) : 0;
+        if (act_e) {
             for (int task = s; task < T * DIM * NT; task += TPE) {
                 const int t = task / (DIM * NT), r = task % (DIM * NT), d = r / NT, q = r % NT;
                 const Geo gt = geo<N, DIM>(d, q);
@@ -633,7 +695,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                         x0 += p * a.S2F[l];
                         x1 += p * a.S2F[(M - 1) * N + l];
                     }
-                    float* to = a.tr_out[d] + ((((size_t)t * NVP + v) * E + e) * 2) * NT + q;
+                    float* tb = wb ? (d == 0 ? a.trb[1][0] : (d == 1 ? a.trb[1][1] : a.trb[1][2]))
+                                   : (d == 0 ? a.trb[0][0] : (d == 1 ? a.trb[0][1] : a.trb[0][2]));
+                    float* to = tb + ((((size_t)t * NVP + v) * E + e) * 2) * NT + q;
                     to[0] = x0;
                     to[NT] = x1;
                 }
@@ -641,14 +705,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         }
         __syncthreads();
         if (a.U_new) {
-            cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, live, s);
-            if (live)
+            cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, act_e, s);
+            if (act_e)
 #pragma unroll
                 for (int v = 0; v < NVP; v++) a.U_new[(size_t)v * slabE + eo] = a.U_sp[(size_t)v * slabE + eo] - du[v];
         }
-        if (live) Fs[s] = res;
+        if (act_e) Fs[s] = res;
         __syncthreads();
-        if (live && s == 0) {
+        if (act_e && s == 0) {
             float re = 0.f;
             for (int j = 0; j < TPE; j++) re = fmaxf(re, Fs[j]);
             double sc = 0.0;
@@ -657,16 +721,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             double rr = (double)re / (sc > 0.0 ? sc : 1.0);
             X[G::XRES] = rr;
             if (a.conv && rr < a.tol && a.conv[e] == 0xff) a.conv[e] = (unsigned char)a.sweep;
+            if (a.big_out) a.big_out[e] = rr >= a.tol ? 1 : 0;
+            a.ver[(size_t)wb * E + e] = a.cur;
+        }
+        if (live && !act_e && s == 0) {  // kept state and traces
+            X[G::XRES] = 0.0;
+            if (a.big_out) a.big_out[e] = 0;
         }
         __syncthreads();
         if (tid == 0) {
-            double rb = 0.0;
+            double rb = 0.0, cnt = 0.0;
             for (int j = 0; j < EPB; j++)
                 if (blockIdx.x * EPB + j < a.E) {
-                    double v = ((const double*)(lds_raw + (size_t)j * G::ELEM_BYTES))[G::XRES];
-                    rb = v > rb ? v : rb;
+                    const double* Xj = (const double*)(lds_raw + (size_t)j * G::ELEM_BYTES);
+                    rb = Xj[G::XRES] > rb ? Xj[G::XRES] : rb;
+                    cnt += Xj[G::XACT];
                 }
             a.res_part[blockIdx.x] = rb;
+            if (a.cnt_part) a.cnt_part[blockIdx.x] = cnt;
         }
         return;
     }
@@ -680,14 +752,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         for (int t = 0; t < T; t++) du[v] += a.wdt[t] * (double)dudt[t][v];
     }
     __syncthreads();
-    cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, live, s);
+    cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, act_e, s);
 #pragma unroll
-    for (int v = 0; v < NVP; v++) un[v] = live ? a.U_sp[(size_t)v * slabE + eo] - du[v] : 1.0;
-    if (live)
+    for (int v = 0; v < NVP; v++) un[v] = act_e ? a.U_sp[(size_t)v * slabE + eo] - du[v] : 1.0;
+    if (act_e)
 #pragma unroll
         for (int v = 0; v < NVP; v++) a.U_new[(size_t)v * slabE + eo] = un[v];
     __syncthreads();
-    double r = cv_and_dt<N, DIM>(un, (double*)Ds, live, s, ii, jj, kk, ei, ej, ek, (size_t)e, a.W_cv, slabE, a.S2C,
+    double r = cv_and_dt<N, DIM>(un, (double*)Ds, act_e, s, ii, jj, kk, ei, ej, ek, (size_t)e, a.W_cv, slabE, a.S2C,
                                  a.Ne, a.act, a.h, a.gm, a.cfl);
     block_min<G::BLOCK>(r, (double*)lds_raw, a.part);
 }
@@ -697,7 +769,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
 template <int N, int DIM>
 __global__ void __launch_bounds__(256) k_mstart(const double* __restrict__ U_sp, double* __restrict__ mu,
                                                 float* __restrict__ d_old, float* tr0, float* tr1, float* tr2,
-                                                const float* __restrict__ S2F, unsigned char* conv, int Eint) {
+                                                const float* __restrict__ S2F, unsigned char* conv, int* ver,
+                                                int Eint) {
     using G = MT<N, DIM>;
     constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -726,7 +799,11 @@ __global__ void __launch_bounds__(256) k_mstart(const double* __restrict__ U_sp,
             mloc[v] = m;
             mu[(size_t)v * E + e] = m;
         }
-    if (live && s == 0 && conv) conv[e] = 0xff;
+    if (live && s == 0) {  // traces of this state are in buffer 0, "written" before sweep 1
+        if (conv) conv[e] = 0xff;
+        ver[e] = 0;
+        ver[(size_t)Eint + e] = -1;
+    }
     __syncthreads();
     if (live)
 #pragma unroll
@@ -758,7 +835,7 @@ __global__ void __launch_bounds__(256) k_mstart(const double* __restrict__ U_sp,
 // Face traces of a T-slice state (after an externally supplied guess).
 template <int N, int DIM>
 __global__ void __launch_bounds__(256) k_mtrace(const float* __restrict__ D, float* tr0, float* tr1, float* tr2,
-                                                const float* __restrict__ S2F, int Eint) {
+                                                const float* __restrict__ S2F, int* ver, int Eint) {
     using G = MT<N, DIM>;
     constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -773,6 +850,10 @@ __global__ void __launch_bounds__(256) k_mtrace(const float* __restrict__ D, flo
             for (int v = 0; v < NVP; v++) Ds[(t * NVP + v) * TPE + s] = D[((size_t)t * NVP + v) * slabE + (size_t)e * TPE + s];
     __syncthreads();
     if (!live) return;
+    if (s == 0) {
+        ver[e] = 0;
+        ver[(size_t)Eint + e] = -1;
+    }
     for (int task = s; task < T * DIM * NT; task += TPE) {
         const int t = task / (DIM * NT), r = task % (DIM * NT), d = r / NT, q = r % NT;
         float* trd = d == 0 ? tr0 : (d == 1 ? tr1 : tr2);
@@ -826,27 +907,32 @@ __global__ void __launch_bounds__(256) k_mfinish(const double* __restrict__ cand
     block_min<G::BLOCK>(r, (double*)lds_raw, part);
 }
 
-// out[0] = min of a[0..n), out[1] = max of b[0..n)
-__global__ void k_mreduce(const double* a, const double* b, int n, double* out) {
-    __shared__ double ra[1024], rb[1024];
-    double x = __builtin_inf(), y = 0.0;
+// out[0] = min of a[0..n) (a may be null: +inf), out[1] = max of b[0..n),
+// out[2] = sum of c[0..n) (b, c may be null: 0)
+__global__ void k_mreduce(const double* a, const double* b, const double* c, int n, double* out) {
+    __shared__ double ra[1024], rb[1024], rc[1024];
+    double x = __builtin_inf(), y = 0.0, z = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        x = x < a[i] ? x : a[i];
+        if (a) x = x < a[i] ? x : a[i];
         if (b) y = y > b[i] ? y : b[i];
+        if (c) z += c[i];
     }
     ra[threadIdx.x] = x;
     rb[threadIdx.x] = y;
+    rc[threadIdx.x] = z;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             ra[threadIdx.x] = ra[threadIdx.x] < ra[threadIdx.x + w] ? ra[threadIdx.x] : ra[threadIdx.x + w];
             rb[threadIdx.x] = rb[threadIdx.x] > rb[threadIdx.x + w] ? rb[threadIdx.x] : rb[threadIdx.x + w];
+            rc[threadIdx.x] += rc[threadIdx.x + w];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         out[0] = ra[0];
         out[1] = rb[0];
+        out[2] = rc[0];
     }
 }
 
@@ -870,8 +956,8 @@ struct MLayout {
     size_t E = 0, tpe = 0, nt = 0;
     unsigned blocks = 0;
     // byte offsets
-    size_t mu = 0, cand = 0, part = 0, resp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0, conv = 0,
-           total = 0;
+    size_t mu = 0, cand = 0, part = 0, resp = 0, cntp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0,
+           conv = 0, ver = 0, big = 0, total = 0;
     size_t tr_axis = 0;  // bytes per axis per trace buffer
 };
 
@@ -890,7 +976,8 @@ MLayout mlayout(size_t E) {
     L.cand = o; o = al(o + sizeof(double) * NVP * E * G::TPE);
     L.part = o; o = al(o + sizeof(double) * (L.blocks + 8));
     L.resp = o; o = al(o + sizeof(double) * (L.blocks + 8));
-    L.red2 = o; o = al(o + sizeof(double) * 2);
+    L.cntp = o; o = al(o + sizeof(double) * (L.blocks + 8));
+    L.red2 = o; o = al(o + sizeof(double) * 4);
     L.hist = o; o = al(o + sizeof(unsigned long long) * kHistBins);
     L.dold = o; o = al(o + sizeof(float) * NVP * E * G::TPE);
     L.D = o; o = al(o + sizeof(float) * G::T * NVP * E * G::TPE);
@@ -898,6 +985,8 @@ MLayout mlayout(size_t E) {
     L.trA = o; o += DIM * L.tr_axis;
     L.trB = o; o += DIM * L.tr_axis;
     L.conv = o; o = al(o + E);
+    L.ver = o; o = al(o + sizeof(int) * 2 * E);
+    L.big = o; o = al(o + 2 * E);
     L.total = o;
     return L;
 }
@@ -994,8 +1083,12 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     a.cfl = c->prm.cfl;
     a.tol = pc ? pc->tol : 0.0;
     unsigned char* conv = (unsigned char*)(W + L.conv);
+    unsigned char* big[2] = {(unsigned char*)(W + L.big), (unsigned char*)(W + L.big) + E};
     double* cand = (double*)(W + L.cand);
     double* red2 = (double*)(W + L.red2);
+    a.ver = (int*)(W + L.ver);
+    a.cnt_part = (double*)(W + L.cntp);
+    for (int d = 0; d < 3; d++) { a.trb[0][d] = trA[d]; a.trb[1][d] = trB[d]; }
     dim3 grid(L.blocks), block(G::BLOCK);
     const size_t lds = G::LDS_BYTES;
 
@@ -1009,15 +1102,15 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     };
     auto start = [&]() -> int {
         hipLaunchKernelGGL((k_mstart<N, DIM>), grid, block, lds, c->stream, (const double*)U_sp, (double*)(W + L.mu),
-                           (float*)(W + L.dold), trA[0], trA[1], trA[2], (const float*)a.S2F, conv, (int)E);
+                           (float*)(W + L.dold), trA[0], trA[1], trA[2], (const float*)a.S2F, conv, a.ver, (int)E);
         SPDK_LAUNCH_CHECK("k_mstart");
         return SPDK_OK;
     };
-    auto reduce = [&](bool with_res) -> int {
-        hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, (const double*)a.part,
-                           with_res ? (const double*)a.res_part : nullptr, (int)L.blocks, red2);
+    // reductions of the per-block partials; blocking read into red_host[0..2]
+    auto reduce = [&](const double* mn, const double* mx, const double* sm) -> int {
+        hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, mn, mx, sm, (int)L.blocks, red2);
         SPDK_LAUNCH_CHECK("k_mreduce");
-        SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         SPDK_CHECK(hipStreamSynchronize(c->stream));
         return SPDK_OK;
     };
@@ -1026,9 +1119,7 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         for (int k = 0; k < T; k++) {
             MArgs s = a;
             const bool last = k == T - 1;
-            float** tin = (k % 2 == 0) ? trA : trB;
-            float** tout = (k % 2 == 0) ? trB : trA;
-            for (int d = 0; d < 3; d++) { s.tr_in[d] = tin[d]; s.tr_out[d] = tout[d]; }
+            s.cur = k + 1;
             s.path = k > 0;
             s.tin_slices = k == 0 ? 1 : T;
             s.out_pred = !last;
@@ -1044,51 +1135,58 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
 
     const bool external = pc && pc->guess == SPDK_GUESS_EXTERNAL;
     if (!external && (rc = start())) return rc;  // an external guess follows spdk_mixed_prepare
-    long long sweeps = 0;
+    double sweeps = 0;  // element-weighted: a sweep over a fraction f of the elements counts f
     bool done = false;
     if (pc && pc->residual) {
         const int kmax = pc->max_sweeps > 0 ? pc->max_sweeps : 8;
         if (external) {
-            // the caller's guess is already in the predictor buffer: traces of it
+            // the caller's guess is in the predictor buffer: its traces, version 0
             hipLaunchKernelGGL((k_mtrace<N, DIM>), grid, block, lds, c->stream, (const float*)a.D, trA[0], trA[1],
-                               trA[2], (const float*)a.S2F, (int)E);
+                               trA[2], (const float*)a.S2F, a.ver, (int)E);
             SPDK_LAUNCH_CHECK("k_mtrace");
         }
-        for (int k = 0; k < kmax && k < 254; k++) {
+        int k = 0;
+        for (; k < kmax && k < 250; k++) {
             MArgs s = a;
-            float** tin = (k % 2 == 0) ? trA : trB;
-            float** tout = (k % 2 == 0) ? trB : trA;
-            for (int d = 0; d < 3; d++) { s.tr_in[d] = tin[d]; s.tr_out[d] = tout[d]; }
+            s.cur = k + 1;
             s.path = (k > 0 || external) ? 1 : 0;
             s.tin_slices = s.path ? T : 1;
             s.out_pred = 1;
-            s.U_new = cand;
-            s.F_x = F_x;
+            s.U_new = k == 0 ? cand : nullptr;  // every element is updated in the first sweep
+            s.F_x = k == 0 ? F_x : nullptr;
             s.conv = conv;
             s.sweep = k;
+            s.big_in = k == 0 ? nullptr : big[(k - 1) % 2];
+            s.big_out = big[k % 2];
             if ((rc = sweep(s))) return rc;
-            sweeps++;
-            // the candidate's dt partials are not computed yet: reduce the residual only
-            hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, (const double*)a.res_part,
-                               (const double*)a.res_part, (int)L.blocks, red2);
-            SPDK_LAUNCH_CHECK("k_mreduce");
-            SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            SPDK_CHECK(hipStreamSynchronize(c->stream));
-            double r = c->red_host[1];
-            if (st) st->last_residual = r;
-            if (r < pc->tol) {
-                hipLaunchKernelGGL((k_mfinish<N, DIM>), grid, block, lds, c->stream, (const double*)cand, U_sp, W_cv,
-                                   a.part, a.S2C, Ne[0], Ne[1], Ne[2], a.act[0], a.act[1], a.act[2], a.h[0], a.h[1],
-                                   a.h[2], a.gm, a.cfl, (int)E);
-                SPDK_LAUNCH_CHECK("k_mfinish");
-                done = true;
-                break;
-            }
+            if ((rc = reduce(nullptr, a.res_part, a.cnt_part))) return rc;
+            sweeps += c->red_host[2] / (double)E;
+            if (st) st->last_residual = c->red_host[1];
+            if (c->red_host[1] < pc->tol) break;
         }
         hipLaunchKernelGGL(k_mhist, dim3(256), dim3(256), 0, c->stream, (const unsigned char*)conv, (int)E,
                            kHistBins, (unsigned long long*)(W + L.hist));
         SPDK_LAUNCH_CHECK("k_mhist");
-        if (!done) {
+        if (k == 0) {  // converged in the first sweep: its candidate is the result
+            hipLaunchKernelGGL((k_mfinish<N, DIM>), grid, block, lds, c->stream, (const double*)cand, U_sp, W_cv,
+                               a.part, a.S2C, Ne[0], Ne[1], Ne[2], a.act[0], a.act[1], a.act[2], a.h[0], a.h[1],
+                               a.h[2], a.gm, a.cfl, (int)E);
+            SPDK_LAUNCH_CHECK("k_mfinish");
+            done = true;
+        } else if (k < kmax && k < 250) {  // converged: the corrector over every element
+            MArgs s = a;
+            s.cur = k + 2;
+            s.path = 1;
+            s.tin_slices = T;
+            s.out_pred = 0;
+            s.U_new = U_sp;
+            s.W_cv = W_cv;
+            s.F_x = F_x;
+            s.sweep = k + 1;
+            if ((rc = sweep(s))) return rc;
+            sweeps += 1.0;
+            done = true;
+        } else {
             if (st) st->fallbacks++;
             if ((rc = start())) return rc;  // fresh traces of the zero-order guess
         }
@@ -1097,7 +1195,7 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         if ((rc = fixed_count())) return rc;
         sweeps += T;
     }
-    if ((rc = reduce(false))) return rc;
+    if ((rc = reduce(a.part, nullptr, nullptr))) return rc;
     if (dt_next) *dt_next = c->red_host[0];
     if (st) {
         st->steps++;
@@ -1125,7 +1223,7 @@ int run_prepare(spdk_ctx* c, const double* U_sp, const int Ne[3], void* ws) {
     for (int d = 0; d < DIM; d++) tr[d] = (float*)(W + L.trA + d * L.tr_axis);
     hipLaunchKernelGGL((k_mstart<N, DIM>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, U_sp,
                        (double*)(W + L.mu), (float*)(W + L.dold), tr[0], tr[1], tr[2], (const float*)c->mixed_mats,
-                       (unsigned char*)(W + L.conv), (int)E);
+                       (unsigned char*)(W + L.conv), (int*)(W + L.ver), (int)E);
     SPDK_LAUNCH_CHECK("k_mstart");
     return SPDK_OK;
 }

@@ -269,6 +269,29 @@ int spdk_hydro_step_mixed(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
 int spdk_mixed_histogram(spdk_ctx* ctx, void* workspace, const int N[3], unsigned long long* hist, int nbins,
                          int reset);
 
+/* ---- per-element MLP for the Picard initial guess -------------------------- */
+/* No reference counterpart: the learned guess that replaces copy_ader
+ * (src/hydro_ader.hpp:428-444) where the caller asks for it. An MLP
+ * in -> h1 -> h2 -> out with GELU on the hidden layers, BF16 MFMA products and
+ * FP32 accumulation, FP32 master weights and Adam. Dimensions are padded to
+ * multiples of 64 internally; all row arrays below are device FP32 with the
+ * given leading dimension (elements). Runs on `stream` (hipStream_t, NULL =
+ * the null stream). */
+typedef struct spdk_mlp spdk_mlp;
+int spdk_mlp_create(const int dims[4], int batch_cap, void* stream, spdk_mlp** out);
+int spdk_mlp_destroy(spdk_mlp* net);
+int spdk_mlp_dims(spdk_mlp* net, int padded[4]);
+/* Parameters as one host FP32 array: W1 (h1 x in, row-major), b1, W2, b2, W3, b3. */
+size_t spdk_mlp_param_count(spdk_mlp* net);
+int spdk_mlp_set_params(spdk_mlp* net, const float* host); /* also resets Adam */
+int spdk_mlp_get_params(spdk_mlp* net, float* host);
+int spdk_mlp_forward(spdk_mlp* net, const float* X, long ldx, int B, float* Y, long ldy);
+/* Gradients of mean((f(X) - T)^2) (mean over B x out) in the parameter layout. */
+int spdk_mlp_gradients(spdk_mlp* net, const float* X, long ldx, const float* T, long ldt, int B, float* host_grad,
+                       float* loss);
+int spdk_mlp_train_step(spdk_mlp* net, const float* X, long ldx, const float* T, long ldt, int B, float lr,
+                        float* loss);
+
 /* ---- whole-run driver ------------------------------------------------------ */
 typedef struct spdk_run_result {
     int steps;                 /* completed time steps                           */

@@ -77,6 +77,17 @@ def lib():
         L.spdk_last_error.restype = ctypes.c_char_p
         L.spdk_fused_workspace_bytes.restype = ctypes.c_size_t
         L.spdk_ctx_stream.restype = ctypes.c_void_p
+        L.spdk_mlp_param_count.restype = ctypes.c_size_t
+        for f in ("spdk_mlp_set_params", "spdk_mlp_get_params", "spdk_mlp_destroy", "spdk_mlp_dims"):
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.spdk_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.spdk_mlp_param_count.argtypes = [ctypes.c_void_p]
+        L.spdk_mlp_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_long]
+        L.spdk_mlp_gradients.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p,
+                                         ctypes.c_long, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.spdk_mlp_train_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p,
+                                          ctypes.c_long, ctypes.c_int, ctypes.c_float, ctypes.c_void_p]
         _lib = L
     return _lib
 

@@ -53,6 +53,9 @@ def orc():
         L.orc_alpha.argtypes = [ctypes.c_double] * 4
         L.orc_normal.argtypes = [ctypes.c_ulonglong, ctypes.c_ulonglong]
         L.orc_dt.restype = ctypes.c_double
+        L.orc_mlp_grad.restype = ctypes.c_double
+        L.orc_bf16_round.restype = ctypes.c_double
+        L.orc_bf16_round.argtypes = [ctypes.c_double]
         _orc = L
     return _orc
 
@@ -208,3 +211,35 @@ def l1_vs_exact_sine(outdir, i, t, ndim, amp=0.125):
 def mass_drift(outdir):
     m = [total_mass(outdir, i) for i in output_indices(outdir)]
     return max(abs(x - m[0]) / abs(m[0]) for x in m)
+
+
+# ---- MLP oracle ------------------------------------------------------------
+def mlp_param_count(dims):
+    return sum(dims[l + 1] * dims[l] + dims[l + 1] for l in range(3))
+
+
+def mlp_init(dims, seed=0, scale=1.0):
+    """Seeded parameters (He-like scaling) in the spdk_mlp_set_params layout."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for l in range(3):
+        out.append(rng.standard_normal(dims[l + 1] * dims[l]) * scale * np.sqrt(2.0 / dims[l]))
+        out.append(rng.standard_normal(dims[l + 1]) * 0.1 * scale)
+    return np.concatenate(out).astype(np.float32)
+
+
+def orc_mlp_forward(dims, params, X, bf16):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.zeros((X.shape[0], dims[3]))
+    orc().orc_mlp_forward(ip(dims), params.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), dp(X), X.shape[0],
+                          int(bf16), dp(Y))
+    return Y
+
+
+def orc_mlp_grad(dims, params, X, T, bf16):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    g = np.zeros(mlp_param_count(dims))
+    loss = orc().orc_mlp_grad(ip(dims), params.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), dp(X), dp(T),
+                              X.shape[0], int(bf16), dp(g))
+    return g, loss

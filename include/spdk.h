@@ -91,6 +91,7 @@ typedef struct spdk_operators {
 } spdk_operators;
 
 typedef struct spdk_ctx spdk_ctx;
+typedef struct spdk_mlp spdk_mlp; /* per-element MLP of the learned Picard guess */
 
 /* ---- context ------------------------------------------------------------ */
 /* Creates a context on the current HIP device. `stream` is a hipStream_t (NULL
@@ -233,7 +234,8 @@ int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
  * measured per configuration. No reference counterpart beyond the step it
  * replaces. */
 #define SPDK_GUESS_ZERO 0     /* U_ader = U at every node (copy_ader)             */
-#define SPDK_GUESS_EXTERNAL 1 /* caller-filled predictor buffer (e.g. the MLP)    */
+#define SPDK_GUESS_EXTERNAL 1 /* caller-filled predictor buffer                   */
+#define SPDK_GUESS_MLP 2      /* the network attached with spdk_mixed_attach_mlp  */
 typedef struct spdk_picard {
     int residual;   /* 0: p predictor sweeps then the corrector (the reference's
                        count); 1: sweep the elements that have not passed the
@@ -268,6 +270,20 @@ int spdk_hydro_step_mixed(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
  * never within max_sweeps). Copies nbins (<= 32) counts; reset clears. */
 int spdk_mixed_histogram(spdk_ctx* ctx, void* workspace, const int N[3], unsigned long long* hist, int nbins,
                          int reset);
+/* Learned guess. Feature and target sizes of this degree and dimension:
+ * din = (d+2) n^d + 6 and dout = T (d+2) n^d (the density, the active
+ * momenta and the energy of every point; targets for every time node). */
+int spdk_mixed_guess_dims(spdk_ctx* ctx, int* din, int* dout);
+/* Attach (NULL: detach) the network used by SPDK_GUESS_MLP; it is moved to the
+ * context's stream. Sizes must match spdk_mixed_guess_dims. */
+int spdk_mixed_attach_mlp(spdk_ctx* ctx, spdk_mlp* net);
+/* Training data: nsweeps Picard sweeps from the zero-order guess without the
+ * corrector (the predictor buffer then holds the nsweeps-th iterate), and the
+ * rows [e0, e0 + nrows) of features (X, nrows x din) and of targets from the
+ * current predictor buffer (Y, nrows x dout); X or Y may be NULL. */
+int spdk_mixed_iterate(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, int nsweeps);
+int spdk_mixed_guess_data(spdk_ctx* ctx, void* workspace, const int N[3], double dt, int e0, int nrows, float* X,
+                          long ldx, float* Y, long ldy);
 
 /* ---- per-element MLP for the Picard initial guess -------------------------- */
 /* No reference counterpart: the learned guess that replaces copy_ader
@@ -277,10 +293,10 @@ int spdk_mixed_histogram(spdk_ctx* ctx, void* workspace, const int N[3], unsigne
  * multiples of 64 internally; all row arrays below are device FP32 with the
  * given leading dimension (elements). Runs on `stream` (hipStream_t, NULL =
  * the null stream). */
-typedef struct spdk_mlp spdk_mlp;
 int spdk_mlp_create(const int dims[4], int batch_cap, void* stream, spdk_mlp** out);
 int spdk_mlp_destroy(spdk_mlp* net);
 int spdk_mlp_dims(spdk_mlp* net, int padded[4]);
+int spdk_mlp_set_stream(spdk_mlp* net, void* stream);
 /* Parameters as one host FP32 array: W1 (h1 x in, row-major), b1, W2, b2, W3, b3. */
 size_t spdk_mlp_param_count(spdk_mlp* net);
 int spdk_mlp_set_params(spdk_mlp* net, const float* host); /* also resets Adam */
@@ -291,6 +307,10 @@ int spdk_mlp_gradients(spdk_mlp* net, const float* X, long ldx, const float* T, 
                        float* loss);
 int spdk_mlp_train_step(spdk_mlp* net, const float* X, long ldx, const float* T, long ldt, int B, float lr,
                         float* loss);
+/* dst[r][:] = src[idx[r]][:] (device rows of `cols` floats; idx device ints). */
+int spdk_gather_rows(const float* src, long ld, const int* idx, int n, int cols, float* dst, void* stream);
+/* flags[r] (device bytes) = 1 if every value of row r is finite and below maxabs in magnitude. */
+int spdk_rows_finite(const float* Y, long ld, int rows, int cols, float maxabs, unsigned char* flags, void* stream);
 
 /* ---- whole-run driver ------------------------------------------------------ */
 typedef struct spdk_run_result {

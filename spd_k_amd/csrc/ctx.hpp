@@ -35,6 +35,9 @@ struct spdk_ctx {
     int ev_used = 0;  // event pairs recorded since the last harvest
     // device buffers created on first use by a path and freed with the context
     float* mixed_mats = nullptr;  // FP32 sp_to_fp and dfp_to_sp / h_d
+    spdk_mlp* guess_net = nullptr;  // learned Picard guess (not owned)
+    float *guess_x = nullptr, *guess_y = nullptr;  // feature / output chunks
+    int guess_cap = 0;
     std::vector<void*> owned;
 };
 

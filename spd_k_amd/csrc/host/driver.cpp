@@ -3,10 +3,14 @@
 // geometry, operators and initial conditions on the host, and sequences the
 // device kernels through the C ABI of include/spdk.h.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <algorithm>
 #include <memory>
+#include <numeric>
+#include <random>
 #include <stdexcept>
 #include <vector>
 #include "../../../include/spdk.h"
@@ -81,6 +85,43 @@ spdk_ctx* make_context(const RunConfig& cfg, const Operators& op, const Axis* ax
     return ctx;
 }
 
+// Weights file of the learned Picard guess: "SPDKMLP1", int32 dims[4]
+// (in, h1, h2, out), int32 degree and dimensionality it was trained for,
+// uint64 parameter count, FP32 parameters (the spdk_mlp_set_params layout).
+struct MlpFile {
+    int dims[4] = {0, 0, 0, 0};
+    int p = 0, dim = 0;
+    std::vector<float> params;
+};
+void save_mlp(const std::string& path, const MlpFile& f) {
+    std::ofstream o(path, std::ios::binary);
+    if (!o) throw std::runtime_error("cannot write " + path);
+    o.write("SPDKMLP1", 8);
+    o.write((const char*)f.dims, sizeof(f.dims));
+    o.write((const char*)&f.p, 4);
+    o.write((const char*)&f.dim, 4);
+    uint64_t n = f.params.size();
+    o.write((const char*)&n, 8);
+    o.write((const char*)f.params.data(), n * 4);
+}
+MlpFile load_mlp(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot read " + path);
+    char magic[8];
+    MlpFile f;
+    uint64_t n = 0;
+    in.read(magic, 8);
+    in.read((char*)f.dims, sizeof(f.dims));
+    in.read((char*)&f.p, 4);
+    in.read((char*)&f.dim, 4);
+    in.read((char*)&n, 8);
+    if (!in || std::memcmp(magic, "SPDKMLP1", 8) != 0 || n > (1ull << 28)) throw std::runtime_error(path + ": not an MLP weights file");
+    f.params.resize(n);
+    in.read((char*)f.params.data(), n * 4);
+    if (!in) throw std::runtime_error(path + ": truncated");
+    return f;
+}
+
 // ===========================================================================
 // Hydro integrator: src/hydro_ader.hpp:76-586
 struct HydroRun {
@@ -108,10 +149,13 @@ struct HydroRun {
     bool mixed = false;        // job/precision=mixed
     spdk_picard pc{};
     spdk_picard_stats pst{};
+    spdk_mlp* net = nullptr;   // learned Picard guess (job/guess=mlp)
 
     HydroRun(RunConfig& c, const std::string& od) : cfg(c), outdir(od) {}
     ~HydroRun() {
         if (ctx) spdk_ctx_sync(ctx);
+        if (ctx) spdk_mixed_attach_mlp(ctx, nullptr);
+        if (net) spdk_mlp_destroy(net);
         mem.release();
         if (ctx) spdk_ctx_destroy(ctx);
     }
@@ -168,6 +212,16 @@ struct HydroRun {
             pc.guess = SPDK_GUESS_ZERO;
             pc.tol = cfg.picard_tol;
             pc.max_sweeps = cfg.picard_max;
+            if (cfg.guess == "mlp") {
+                MlpFile f = load_mlp(cfg.mlp_weights);
+                if (f.p != cfg.p || f.dim != cfg.ndim)
+                    throw std::runtime_error(cfg.mlp_weights + ": trained for another degree or dimensionality");
+                CK(spdk_mlp_create(f.dims, 65536, spdk_ctx_stream(ctx), &net));
+                if (spdk_mlp_param_count(net) != f.params.size()) throw std::runtime_error(cfg.mlp_weights + ": size");
+                CK(spdk_mlp_set_params(net, f.params.data()));
+                CK(spdk_mixed_attach_mlp(ctx, net));
+                pc.guess = SPDK_GUESS_MLP;
+            }
         } else if (!fused) {
             U_cv = mem.alloc(ctx, nsp);
             U_ader = mem.alloc(ctx, sd_count(s_ader));
@@ -288,8 +342,17 @@ struct HydroRun {
         CK(spdk_transform(ctx, U_cv, U_sp, T_sweep, s_sp, SPDK_M_CV_TO_SP));
     }
 
-    // One step and the dt that follows it (src/hydro_ader.hpp:235-243).
+    // One step and the dt that follows it (src/hydro_ader.hpp:235-243); a dt
+    // that is not a positive finite number stops the run with an error (the
+    // state has left the physical range) instead of stalling the time loop.
     double step(bool output_due) {
+        double next = step_raw(output_due);
+        if (!(next > 0.0) || !std::isfinite(next))
+            throw std::runtime_error("the next time step is " + std::to_string(next) +
+                                     ": the state has left the physical range (t = " + std::to_string(t) + ")");
+        return next;
+    }
+    double step_raw(bool output_due) {
         if (mixed) {
             double next = 0;
             CK(spdk_hydro_step_mixed(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr,
@@ -309,6 +372,126 @@ struct HydroRun {
         double next = 0;
         CK(spdk_min_dt(ctx, W_cv, s_sp, &next));
         return next;
+    }
+
+    // Learned-guess training (job/mode=train_guess): snapshots of this run, each
+    // paired with its Picard fixed point (mlp/sweeps sweeps from the zero-order
+    // guess), then Adam on the device. The last snapshot is held out.
+    void train_guess() {
+        int din = 0, dout = 0;
+        CK(spdk_mixed_guess_dims(ctx, &din, &dout));
+        const int N[3] = {X.n_total, Y.n_total, Z.n_total};
+        const size_t E = (size_t)N[0] * N[1] * N[2];
+        const int S = cfg.mlp_snapshots;
+        float *Xs = nullptr, *Ys = nullptr;
+        void* p = nullptr;
+        CK(spdk_malloc(&p, (size_t)S * E * din * 4));
+        mem.ptrs.push_back(p);
+        Xs = (float*)p;
+        CK(spdk_malloc(&p, (size_t)S * E * dout * 4));
+        mem.ptrs.push_back(p);
+        Ys = (float*)p;
+        dt = Dt;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < S; k++) {
+            for (int st = 0; st < cfg.mlp_every; st++) {
+                double next = step(false);
+                t += dt;
+                n_step++;
+                dt = next;
+            }
+            CK(spdk_mixed_iterate(ctx, U_sp, s_sp, ws, dt, cfg.mlp_sweeps));
+            CK(spdk_mixed_guess_data(ctx, ws, N, dt, 0, (int)E, Xs + (size_t)k * E * din, din,
+                                     Ys + (size_t)k * E * dout, dout));
+            CK(spdk_ctx_sync(ctx));
+            std::printf("snapshot %d: t = %.6g, dt = %.6g\n", k, t, dt);
+        }
+        const double gen_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const int dims[4] = {din, cfg.mlp_hidden, cfg.mlp_hidden, dout};
+        const int B = cfg.mlp_batch;
+        CK(spdk_mlp_create(dims, B, spdk_ctx_stream(ctx), &net));
+        // initial parameters: N(0, 2/fan_in) weights, zero biases
+        std::mt19937_64 rng(cfg.mlp_seed);
+        std::normal_distribution<float> nd(0.f, 1.f);
+        MlpFile f;
+        for (int i = 0; i < 4; i++) f.dims[i] = dims[i];
+        f.p = cfg.p;
+        f.dim = cfg.ndim;
+        f.params.resize(spdk_mlp_param_count(net));
+        {
+            size_t o = 0;
+            for (int l = 0; l < 3; l++) {
+                const float sc = std::sqrt(2.0f / dims[l]) * (l == 2 ? 0.1f : 1.0f);
+                for (size_t i = 0; i < (size_t)dims[l + 1] * dims[l]; i++) f.params[o++] = nd(rng) * sc;
+                for (int i = 0; i < dims[l + 1]; i++) f.params[o++] = 0.f;
+            }
+        }
+        CK(spdk_mlp_set_params(net, f.params.data()));
+        // rows whose targets are not finite (Picard diverged there within
+        // mlp/sweeps sweeps) are left out of training and of the held-out loss
+        void* strm = spdk_ctx_stream(ctx);
+        std::vector<int> train_rows, held_rows;
+        {
+            const size_t rows = (size_t)S * E;
+            void* fp = nullptr;
+            CK(spdk_malloc(&fp, 2 * rows));
+            mem.ptrs.push_back(fp);
+            std::vector<unsigned char> ok(2 * rows);
+            CK(spdk_rows_finite(Ys, dout, (int)rows, dout, 1e6f, (unsigned char*)fp, strm));
+            CK(spdk_rows_finite(Xs, din, (int)rows, din, 1e6f, (unsigned char*)fp + rows, strm));
+            CK(spdk_memcpy_d2h(ctx, ok.data(), fp, 2 * rows));
+            for (size_t r = 0; r < rows; r++)
+                if (ok[r] && ok[rows + r]) (r < (size_t)(S - 1) * E ? train_rows : held_rows).push_back((int)r);
+            std::printf("training rows: %zu, held out: %zu, dropped (non-finite or |value| >= 1e6): %zu\n",
+                        train_rows.size(), held_rows.size(), rows - train_rows.size() - held_rows.size());
+        }
+        const size_t ntrain = train_rows.size();
+        float *Xb = nullptr, *Yb = nullptr;
+        int *idx = nullptr, *hidx = nullptr;
+        CK(spdk_malloc(&p, (size_t)B * din * 4)); mem.ptrs.push_back(p); Xb = (float*)p;
+        CK(spdk_malloc(&p, (size_t)B * dout * 4)); mem.ptrs.push_back(p); Yb = (float*)p;
+        CK(spdk_malloc(&p, ntrain * 4 + 4)); mem.ptrs.push_back(p); idx = (int*)p;
+        CK(spdk_malloc(&p, held_rows.size() * 4 + 4)); mem.ptrs.push_back(p); hidx = (int*)p;
+        if (!held_rows.empty()) CK(spdk_memcpy_h2d(ctx, hidx, held_rows.data(), held_rows.size() * 4));
+        auto t1 = std::chrono::steady_clock::now();
+        auto eval = [&]() -> double {  // mean loss over the held-out rows, batches of B
+            double acc = 0;
+            size_t cnt = 0;
+            for (size_t r = 0; r + B <= held_rows.size(); r += B) {
+                float loss = 0;
+                CK(spdk_gather_rows(Xs, din, hidx + r, B, din, Xb, strm));
+                CK(spdk_gather_rows(Ys, dout, hidx + r, B, dout, Yb, strm));
+                CK(spdk_mlp_gradients(net, Xb, din, Yb, dout, B, nullptr, &loss));
+                acc += loss;
+                cnt++;
+            }
+            return cnt ? acc / cnt : 0.0;
+        };
+        std::printf("training: %zu rows (%d snapshots), in %d hidden %d out %d\n", ntrain, S - 1, din,
+                    cfg.mlp_hidden, dout);
+        std::printf("held-out loss before training %.4e\n", eval());
+        for (int ep = 0; ep < cfg.mlp_epochs; ep++) {
+            std::shuffle(train_rows.begin(), train_rows.end(), rng);
+            CK(spdk_memcpy_h2d(ctx, idx, train_rows.data(), ntrain * 4));
+            double acc = 0;
+            size_t nb = 0, nrd = 0;
+            const float lr = (float)(cfg.mlp_lr * (ep + 1 == cfg.mlp_epochs ? 0.3 : 1.0));
+            for (size_t b = 0; b + B <= ntrain; b += B) {
+                CK(spdk_gather_rows(Xs, din, idx + b, B, din, Xb, strm));
+                CK(spdk_gather_rows(Ys, dout, idx + b, B, dout, Yb, strm));
+                float loss = 0;
+                const bool rd = (nb % 64) == 0;
+                CK(spdk_mlp_train_step(net, Xb, din, Yb, dout, B, lr, rd ? &loss : nullptr));
+                if (rd) { acc += loss; nrd++; }
+                nb++;
+            }
+            std::printf("epoch %d: train loss %.4e (sampled), held-out loss %.4e\n", ep, nrd ? acc / nrd : 0.0, eval());
+        }
+        CK(spdk_ctx_sync(ctx));
+        const double train_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        CK(spdk_mlp_get_params(net, f.params.data()));
+        save_mlp(cfg.mlp_out, f);
+        std::printf("data %.2f s, training %.2f s; wrote %s\n", gen_s, train_s, cfg.mlp_out.c_str());
     }
 
     void evolve(int warmup, spdk_run_result* res) {  // src/hydro_ader.hpp:215-273
@@ -354,6 +537,15 @@ struct HydroRun {
         }
         CK(spdk_ctx_sync(ctx));
         double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+        if (cfg.final_output) {  // the state after the last step (outside the timed loop)
+            if (fused || mixed || cfg.fallback) {
+                CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
+                CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
+            }
+            CK(spdk_ctx_sync(ctx));
+            write_dims(outdir, X, Y, Z);
+            write_outputs();
+        }
         res->steps = n_step - step0;
         res->dofs = (long long)(X.n * X.n_sp) * (Y.n * Y.n_sp) * (Z.n * Z.n_sp);
         res->evolution_seconds = sec;
@@ -385,6 +577,10 @@ int run_hydro(RunConfig& cfg, const std::string& outdir, int warmup, spdk_run_re
     HydroRun h(cfg, outdir);
     h.timing = cfg.timing;
     h.setup();
+    if (cfg.mode == "train_guess") {
+        h.train_guess();
+        return 0;
+    }
     h.evolve(warmup, res);
     fill_kernel_timing(h.ctx, res);
     return 0;

@@ -130,6 +130,27 @@ std::string configure(Params& pin, RunConfig& c) {
     if (pin.has("job", "guess")) c.guess = pin.get_string("job", "guess");
     if (c.guess != "zero" && c.guess != "mlp") return "job/guess must be zero or mlp";
     if (pin.has("job", "mlp_weights")) c.mlp_weights = pin.get_string("job", "mlp_weights");
+    if (pin.has("job", "mode")) c.mode = pin.get_string("job", "mode");
+    c.final_output = pin.has("job", "final_output") && pin.get_bool("job", "final_output");
+    if (c.mode != "run" && c.mode != "train_guess") return "job/mode must be run or train_guess";
+    if (c.mode == "train_guess") {
+        if (!pin.has("mlp", "out")) return "job/mode=train_guess needs mlp/out";
+        c.mlp_out = pin.get_string("mlp", "out");
+        if (pin.has("mlp", "snapshots")) c.mlp_snapshots = pin.get_int("mlp", "snapshots");
+        if (pin.has("mlp", "every")) c.mlp_every = pin.get_int("mlp", "every");
+        if (pin.has("mlp", "sweeps")) c.mlp_sweeps = pin.get_int("mlp", "sweeps");
+        if (pin.has("mlp", "epochs")) c.mlp_epochs = pin.get_int("mlp", "epochs");
+        if (pin.has("mlp", "batch")) c.mlp_batch = pin.get_int("mlp", "batch");
+        if (pin.has("mlp", "hidden")) c.mlp_hidden = pin.get_int("mlp", "hidden");
+        if (pin.has("mlp", "lr")) c.mlp_lr = pin.get_real("mlp", "lr");
+        if (pin.has("mlp", "seed")) c.mlp_seed = (uint64_t)std::stoull(pin.get_string("mlp", "seed"));
+        if (c.mlp_snapshots < 2 || c.mlp_every < 1 || c.mlp_sweeps < 1 || c.mlp_epochs < 1 || c.mlp_batch < 64 ||
+            c.mlp_hidden < 64)
+            return "mlp/: snapshots >= 2, every >= 1, sweeps >= 1, epochs >= 1, batch >= 64, hidden >= 64";
+        if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
+            return "job/mode=train_guess covers the ADER hydro step without fallback";
+        c.precision = "mixed";
+    }
     if (c.precision == "mixed" || c.picard == "residual" || c.guess == "mlp") {
         if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
             return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step without fallback";

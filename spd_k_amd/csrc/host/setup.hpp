@@ -58,6 +58,13 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     int picard_max = 8;              // job/picard_max: predictor sweeps before the zero-guess fallback
     std::string guess = "zero";      // job/guess: zero | mlp
     std::string mlp_weights;         // job/mlp_weights: trained network file (job/guess=mlp)
+    std::string mode = "run";        // job/mode: run | train_guess
+    bool final_output = false;       // job/final_output: write the state after the last step
+    // <mlp> block (job/mode=train_guess): fixed-point data from this run, then training
+    std::string mlp_out;             // mlp/out: weights file written
+    int mlp_snapshots = 8, mlp_every = 4, mlp_sweeps = 20, mlp_epochs = 6, mlp_batch = 4096, mlp_hidden = 256;
+    double mlp_lr = 1e-3;
+    uint64_t mlp_seed = 1;
 };
 
 // Parses the configuration from the parameter table (src/main.cpp:126-189);

@@ -54,6 +54,7 @@
 // newest buffer written before the current sweep. A buffer being written is
 // therefore never read, whether or not the element was active before.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include "ctx.hpp"
@@ -67,6 +68,9 @@ namespace {
 
 constexpr int kMaxTm = 8;
 constexpr int kHistBins = 32;
+constexpr int kGS = 12;          // per-element floats of the learned-guess scales
+constexpr int kGExtra = 6;       // extra features per element
+constexpr float kAmin = 1e-6f;   // floor of the element amplitude
 
 // Minimum waves per SIMD the sweep kernel is compiled for (bounds its VGPRs).
 #ifndef SPDK_MSWEEP_WAVES
@@ -668,7 +672,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                     float dn = dold[v] - acc;
                     // input state at this point: its own LDS slot (path 1) or d_old
                     float din = a.path ? Ds[(t * NVP + v) * PSZ + so] : dold[v];
-                    res = fmaxf(res, fabsf(dn - din));
+                    const float df = fabsf(dn - din);
+                    if (!(df <= res)) res = (df == df) ? df : __builtin_inff();  // a NaN never passes
                     a.D[((size_t)t * NVP + v) * slabE + eo] = dn;
                     Ds[(t * NVP + v) * PSZ + so] = dn;
                 }
@@ -714,7 +719,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         __syncthreads();
         if (act_e && s == 0) {
             float re = 0.f;
-            for (int j = 0; j < TPE; j++) re = fmaxf(re, Fs[j]);
+            for (int j = 0; j < TPE; j++) re = Fs[j] > re ? Fs[j] : re;
             double sc = 0.0;
 #pragma unroll
             for (int v = 0; v < NVP; v++) sc = fmax(sc, fabs(X[v]));
@@ -952,12 +957,122 @@ __global__ void k_mhist(const unsigned char* conv, int E, int nb, unsigned long 
 }
 
 // ---------------------------------------------------------------------------
+// Learned guess. Each element is made dimensionless with its mean state:
+// density by rho, momenta by rho c, energy by rho c^2 (c the sound speed of
+// the mean state); its amplitude A = max |d / scale| (floored at kAmin) sets
+// the unit of the features, and A times the acoustic Courant number
+// kappa = dt c / h that of the targets:
+//   features  x = d(v, sp) / (scale_v A)  (variables rho, the active momenta, E),
+//             then v/c (3 components), kappa, log10 A, 1;
+//   targets   y = (U*(t, v, sp) - U(v, sp)) / (scale_v kappa A),
+// with U* the converged space-time predictor. The momentum of an inactive
+// axis is not in the network (its guess is the zero-order one).
+template <int DIM>
+__device__ __forceinline__ int gvar(int vi) { return vi <= DIM ? vi : 4; }
+
+// per element: scale_0..4, kappa, A, kappa A, v/c (3)
+template <int N, int DIM>
+__global__ void k_gscale(const double* __restrict__ mu, const float* __restrict__ d_old, int Eint, int e0, int nrows,
+                         double dt, double hmin, double gm, float* __restrict__ S) {
+    constexpr int TPE = MT<N, DIM>::TPE;
+    const int r0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r0 >= nrows) return;
+    const int e = e0 + r0;
+    const size_t E = (size_t)Eint;
+    const double r = mu[e], ri = 1.0 / r;
+    const double v1 = mu[E + e] * ri, v2 = mu[2 * E + e] * ri, v3 = mu[3 * E + e] * ri;
+    const double p = (mu[4 * E + e] - 0.5 * r * (v1 * v1 + v2 * v2 + v3 * v3)) * (gm - 1.0);
+    const double c = sqrt(fmax(gm * p * ri, kMinC2));
+    const double sc[NVP] = {r, r * c, r * c, r * c, r * c * c};
+    float a = 0.f;
+    for (int vi = 0; vi < DIM + 2; vi++) {
+        const int v = gvar<DIM>(vi);
+        const float inv = (float)(1.0 / sc[v]);
+        for (int s = 0; s < TPE; s++) a = fmaxf(a, fabsf(d_old[((size_t)v * E + e) * TPE + s] * inv));
+    }
+    a = fmaxf(a, kAmin);
+    const double kap = dt * c / hmin;
+    float* o = S + (size_t)kGS * e;
+    for (int v = 0; v < NVP; v++) o[v] = (float)sc[v];
+    o[5] = (float)kap;
+    o[6] = a;
+    o[7] = (float)(kap * a);
+    o[8] = (float)(v1 / c);
+    o[9] = (float)(v2 / c);
+    o[10] = (float)(v3 / c);
+    o[11] = 0.f;
+}
+
+template <int N, int DIM>
+__global__ void k_gfeat(const float* __restrict__ d_old, const float* __restrict__ S, int Eint, int e0, int nrows,
+                        float* __restrict__ X, long ldx) {
+    constexpr int TPE = MT<N, DIM>::TPE, NF = (DIM + 2) * TPE;
+    const long total = (long)nrows * (NF + kGExtra);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(i / (NF + kGExtra)), j = (int)(i % (NF + kGExtra));
+        const size_t e = (size_t)e0 + row;
+        const float* sc = S + (size_t)kGS * e;
+        float x;
+        if (j < NF) {
+            const int v = gvar<DIM>(j / TPE), s = j % TPE;
+            x = d_old[((size_t)v * Eint + e) * TPE + s] / (sc[v] * sc[6]);
+        } else {
+            const int q = j - NF;
+            x = q < 3 ? sc[8 + q] : (q == 3 ? sc[5] : (q == 4 ? log10f(sc[6]) : 1.0f));
+        }
+        X[(long)row * ldx + j] = x;
+    }
+}
+
+// targets from the predictor state D (deviations) of the elements [e0, e0 + nrows)
+template <int N, int DIM>
+__global__ void k_gtarget(const float* __restrict__ D, const float* __restrict__ d_old, const float* __restrict__ S,
+                          int Eint, int e0, int nrows, float* __restrict__ Y, long ldy) {
+    constexpr int TPE = MT<N, DIM>::TPE, NV = DIM + 2, NO = MT<N, DIM>::T * NV * TPE;
+    const size_t E = (size_t)Eint, slabE = E * TPE;
+    const long total = (long)nrows * NO;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(i / NO), j = (int)(i % NO);
+        const int t = j / (NV * TPE), vi = (j / TPE) % NV, s = j % TPE;
+        const int v = gvar<DIM>(vi);
+        const size_t e = (size_t)e0 + row;
+        const float* sc = S + (size_t)kGS * e;
+        const float dn = D[((size_t)t * NVP + v) * slabE + e * TPE + s];
+        const float d0 = d_old[(size_t)v * slabE + e * TPE + s];
+        Y[(long)row * ldy + j] = (dn - d0) / (sc[v] * sc[7]);
+    }
+}
+
+// guess from network outputs: D(t, v, e, sp) for the elements [e0, e0 + nrows)
+template <int N, int DIM>
+__global__ void k_gdecode(const float* __restrict__ Y, long ldy, const float* __restrict__ d_old,
+                          const float* __restrict__ S, int Eint, int e0, int nrows, float* __restrict__ D) {
+    constexpr int TPE = MT<N, DIM>::TPE, NV = DIM + 2, T = MT<N, DIM>::T;
+    const size_t E = (size_t)Eint, slabE = E * TPE;
+    const long total = (long)T * NVP * nrows * TPE;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int s = (int)(i % TPE);
+        const long r2 = i / TPE;
+        const int row = (int)(r2 % nrows);
+        const int tv = (int)(r2 / nrows), t = tv / NVP, v = tv % NVP;
+        const size_t e = (size_t)e0 + row;
+        const float d0 = d_old[(size_t)v * slabE + e * TPE + s];
+        float dn = d0;
+        const int vi = v == 4 ? DIM + 1 : v;
+        if (v == 4 || v <= DIM) {
+            const float* sc = S + (size_t)kGS * e;
+            dn = d0 + Y[(long)row * ldy + (t * NV + vi) * TPE + s] * (sc[v] * sc[7]);
+        }
+        D[((size_t)t * NVP + v) * slabE + e * TPE + s] = dn;
+    }
+}
+
 struct MLayout {
     size_t E = 0, tpe = 0, nt = 0;
     unsigned blocks = 0;
     // byte offsets
     size_t mu = 0, cand = 0, part = 0, resp = 0, cntp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0,
-           conv = 0, ver = 0, big = 0, total = 0;
+           conv = 0, ver = 0, big = 0, gsc = 0, total = 0;
     size_t tr_axis = 0;  // bytes per axis per trace buffer
 };
 
@@ -987,6 +1102,7 @@ MLayout mlayout(size_t E) {
     L.conv = o; o = al(o + E);
     L.ver = o; o = al(o + sizeof(int) * 2 * E);
     L.big = o; o = al(o + 2 * E);
+    L.gsc = o; o = al(o + sizeof(float) * kGS * E);
     L.total = o;
     return L;
 }
@@ -1030,6 +1146,83 @@ int ensure_mixed_mats(spdk_ctx* c) {
     c->owned.push_back(p);
     SPDK_CHECK(hipMemcpy(p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     c->mixed_mats = (float*)p;
+    return SPDK_OK;
+}
+
+inline unsigned gblocks(long n) { long b = (n + 255) / 256; return (unsigned)(b > 262144 ? 262144 : b); }
+
+double hmin_of(spdk_ctx* c) {
+    double h = 0.0;
+    for (int d = 0; d < 3; d++)
+        if (c->prm.active[d]) h = (h == 0.0 || c->prm.h[d] < h) ? c->prm.h[d] : h;
+    return h;
+}
+
+// features (X) and / or targets of the current predictor state (Y) for the
+// elements [e0, e0 + nrows); the scales are refreshed for that range
+template <int N, int DIM>
+int guess_data(spdk_ctx* c, void* ws, size_t E, double dt, int e0, int nrows, float* X, long ldx, float* Y, long ldy) {
+    MLayout L = mlayout<N, DIM>(E);
+    char* W = (char*)ws;
+    float* S = (float*)(W + L.gsc);
+    const double* mu = (const double*)(W + L.mu);
+    const float* d_old = (const float*)(W + L.dold);
+    hipLaunchKernelGGL((k_gscale<N, DIM>), dim3((nrows + 255) / 256), dim3(256), 0, c->stream, mu, d_old, (int)E, e0,
+                       nrows, dt, hmin_of(c), c->prm.gamma, S);
+    SPDK_LAUNCH_CHECK("k_gscale");
+    constexpr int TPE = MT<N, DIM>::TPE;
+    if (X) {
+        hipLaunchKernelGGL((k_gfeat<N, DIM>), dim3(gblocks((long)nrows * ((DIM + 2) * TPE + kGExtra))), dim3(256), 0,
+                           c->stream, d_old, (const float*)S, (int)E, e0, nrows, X, ldx);
+        SPDK_LAUNCH_CHECK("k_gfeat");
+    }
+    if (Y) {
+        hipLaunchKernelGGL((k_gtarget<N, DIM>), dim3(gblocks((long)nrows * MT<N, DIM>::T * (DIM + 2) * TPE)),
+                           dim3(256), 0, c->stream, (const float*)(W + L.D), d_old, (const float*)S, (int)E, e0, nrows,
+                           Y, ldy);
+        SPDK_LAUNCH_CHECK("k_gtarget");
+    }
+    return SPDK_OK;
+}
+
+// the attached network's guess for every element, written to the predictor buffer
+template <int N, int DIM>
+int mlp_guess(spdk_ctx* c, void* ws, size_t E, double dt) {
+    constexpr int TPE = MT<N, DIM>::TPE, din = (DIM + 2) * TPE + kGExtra, dout = MT<N, DIM>::T * (DIM + 2) * TPE;
+    spdk_mlp* net = c->guess_net;
+    int pd[4];
+    spdk_mlp_dims(net, pd);
+    if (!c->guess_x) {
+        int cap = 65536;
+        while (cap > 1024 && (size_t)cap * pd[0] > (size_t)1 << 27) cap /= 2;
+        c->guess_cap = cap;
+        void *x = nullptr, *y = nullptr;
+        SPDK_CHECK(hipMalloc(&x, (size_t)cap * din * sizeof(float)));
+        SPDK_CHECK(hipMalloc(&y, (size_t)cap * dout * sizeof(float)));
+        c->owned.push_back(x);
+        c->owned.push_back(y);
+        c->guess_x = (float*)x;
+        c->guess_y = (float*)y;
+    }
+    MLayout L = mlayout<N, DIM>(E);
+    char* W = (char*)ws;
+    hipLaunchKernelGGL((k_gscale<N, DIM>), dim3((unsigned)((E + 255) / 256)), dim3(256), 0, c->stream,
+                       (const double*)(W + L.mu), (const float*)(W + L.dold), (int)E, 0, (int)E, dt, hmin_of(c),
+                       c->prm.gamma, (float*)(W + L.gsc));
+    SPDK_LAUNCH_CHECK("k_gscale");
+    for (size_t e0 = 0; e0 < E; e0 += c->guess_cap) {
+        const int nrows = (int)std::min((size_t)c->guess_cap, E - e0);
+        hipLaunchKernelGGL((k_gfeat<N, DIM>), dim3(gblocks((long)nrows * din)), dim3(256), 0, c->stream,
+                           (const float*)(W + L.dold), (const float*)(W + L.gsc), (int)E, (int)e0, nrows, c->guess_x,
+                           (long)din);
+        SPDK_LAUNCH_CHECK("k_gfeat");
+        int rc = spdk_mlp_forward(net, c->guess_x, din, nrows, c->guess_y, dout);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_gdecode<N, DIM>), dim3(gblocks((long)MT<N, DIM>::T * NVP * nrows * TPE)), dim3(256), 0,
+                           c->stream, (const float*)c->guess_y, (long)dout, (const float*)(W + L.dold),
+                           (const float*)(W + L.gsc), (int)E, (int)e0, nrows, (float*)(W + L.D));
+        SPDK_LAUNCH_CHECK("k_gdecode");
+    }
     return SPDK_OK;
 }
 
@@ -1133,8 +1326,10 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         return SPDK_OK;
     };
 
-    const bool external = pc && pc->guess == SPDK_GUESS_EXTERNAL;
-    if (!external && (rc = start())) return rc;  // an external guess follows spdk_mixed_prepare
+    const bool learned = pc && pc->guess == SPDK_GUESS_MLP;
+    const bool external = pc && (pc->guess == SPDK_GUESS_EXTERNAL || learned);
+    if (!(external && !learned) && (rc = start())) return rc;  // an external guess follows spdk_mixed_prepare
+    if (learned && (rc = mlp_guess<N, DIM>(c, ws, E, dt))) return rc;
     double sweeps = 0;  // element-weighted: a sweep over a fraction f of the elements counts f
     bool done = false;
     if (pc && pc->residual) {
@@ -1228,17 +1423,81 @@ int run_prepare(spdk_ctx* c, const double* U_sp, const int Ne[3], void* ws) {
     return SPDK_OK;
 }
 
+// p' predictor sweeps from the zero-order guess (no corrector): the
+// predictor buffer then holds the p'-th Picard iterate.
+template <int N, int DIM>
+int run_iterate(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, int nsweeps) {
+    spdk_picard pc{0, SPDK_GUESS_ZERO, 0.0, nsweeps};
+    (void)pc;
+    using G = MT<N, DIM>;
+    int rc;
+    if ((rc = run_prepare<N, DIM>(c, U_sp, Ne, ws))) return rc;
+    const size_t E = (size_t)Ne[0] * Ne[1] * Ne[2];
+    MLayout L = mlayout<N, DIM>(E);
+    char* W = (char*)ws;
+    MArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.U_sp = U_sp;
+    a.mu = (const double*)(W + L.mu);
+    a.d_old = (const float*)(W + L.dold);
+    a.D = (float*)(W + L.D);
+    a.part = (double*)(W + L.part);
+    a.res_part = (double*)(W + L.resp);
+    a.S2F = c->mixed_mats;
+    a.Dh = c->mixed_mats + (size_t)c->m * c->n;
+    a.S2C = c->mat[SPDK_M_SP_TO_CV];
+    a.ver = (int*)(W + L.ver);
+    const std::vector<double>& inv = c->mat_host[SPDK_M_INVADER];
+    for (int t = 0; t < G::T; t++) {
+        for (int l = 0; l < G::T; l++) a.cdt[t * kMaxTm + l] = (float)(dt * inv[t * G::T + l] * c->wt_host[l]);
+        a.wdt[t] = dt * c->wt_host[t];
+    }
+    for (int d = 0; d < 3; d++) {
+        a.Ne[d] = Ne[d];
+        a.act[d] = c->prm.active[d];
+        a.bc[d] = c->prm.bc[d];
+        a.h[d] = c->prm.h[d];
+        a.ih[d] = 1.0 / c->prm.h[d];
+        a.trb[0][d] = DIM > d ? (float*)(W + L.trA + d * L.tr_axis) : nullptr;
+        a.trb[1][d] = DIM > d ? (float*)(W + L.trB + d * L.tr_axis) : nullptr;
+    }
+    double wsum = 0.0;
+    sp_weights(N, a.wq);
+    for (int i = 0; i < N; i++) wsum += a.wq[i];
+    a.wtot = wsum;
+    for (int d = 1; d < DIM; d++) a.wtot *= wsum;
+    a.E = (int)E;
+    a.gm = c->prm.gamma;
+    a.cfl = c->prm.cfl;
+    for (int k = 0; k < nsweeps; k++) {
+        MArgs s = a;
+        s.cur = k + 1;
+        s.path = k > 0;
+        s.tin_slices = k == 0 ? 1 : G::T;
+        s.out_pred = 1;
+        s.sweep = k;
+        hipLaunchKernelGGL((k_msweep<N, DIM>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, s);
+        SPDK_LAUNCH_CHECK("k_msweep");
+    }
+    return SPDK_OK;
+}
+
 typedef int (*mstep_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, const spdk_picard*,
                         spdk_picard_stats*, double*);
 typedef MLayout (*mlayout_fn)(size_t);
 typedef int (*mprep_fn)(spdk_ctx*, const double*, const int*, void*);
+typedef int (*miter_fn)(spdk_ctx*, double*, const int*, void*, double, int);
+typedef int (*mdata_fn)(spdk_ctx*, void*, size_t, double, int, int, float*, long, float*, long);
 struct MVariant {
     int n, dim;
     mstep_fn step;
     mlayout_fn layout;
     mprep_fn prepare;
+    miter_fn iterate;
+    mdata_fn data;
 };
-#define SPDK_MVARIANT(Nn, D) {Nn, D, run_mixed<Nn, D>, mlayout<Nn, D>, run_prepare<Nn, D>}
+#define SPDK_MVARIANT(Nn, D) \
+    {Nn, D, run_mixed<Nn, D>, mlayout<Nn, D>, run_prepare<Nn, D>, run_iterate<Nn, D>, guess_data<Nn, D>}
 const MVariant kMVariants[] = {SPDK_MVARIANT(2, 1), SPDK_MVARIANT(3, 1), SPDK_MVARIANT(4, 1), SPDK_MVARIANT(5, 1),
                                SPDK_MVARIANT(2, 2), SPDK_MVARIANT(3, 2), SPDK_MVARIANT(4, 2), SPDK_MVARIANT(5, 2),
                                SPDK_MVARIANT(2, 3), SPDK_MVARIANT(3, 3), SPDK_MVARIANT(4, 3)};
@@ -1277,6 +1536,55 @@ int spdk_mixed_predictor_buffer(spdk_ctx* c, void* ws, const int N[3], float** D
     if (mu) *mu = (double*)((char*)ws + L.mu);
     if (d_old) *d_old = (float*)((char*)ws + L.dold);
     return SPDK_OK;
+}
+
+static int check_mixed(spdk_ctx* c, const void* U_sp, const spdk_sd_shape& s, const void* ws, const MVariant** out);
+
+int spdk_mixed_guess_dims(spdk_ctx* c, int* din, int* dout) {
+    const MVariant* v = c ? find_mvariant(c) : nullptr;
+    if (!v) { set_error("mixed path: no variant"); return SPDK_E_CONFIG; }
+    const int dim = v->dim, tpe = dim == 1 ? v->n : (dim == 2 ? v->n * v->n : v->n * v->n * v->n);
+    if (din) *din = (dim + 2) * tpe + kGExtra;
+    if (dout) *dout = v->n * (dim + 2) * tpe;
+    return SPDK_OK;
+}
+
+int spdk_mixed_attach_mlp(spdk_ctx* c, spdk_mlp* net) {
+    if (!c) return SPDK_E_ARG;
+    if (net) {
+        int din = 0, dout = 0, rc = spdk_mixed_guess_dims(c, &din, &dout);
+        if (rc) return rc;
+        int pd[4];
+        spdk_mlp_dims(net, pd);
+        if (pd[0] != (din + 63) / 64 * 64 || pd[3] != (dout + 63) / 64 * 64) {
+            set_error("mixed: the network's input/output sizes do not match this degree and dimension");
+            return SPDK_E_SHAPE;
+        }
+    }
+    if (net) {
+        int rc = spdk_mlp_set_stream(net, c->stream);  // ordered with the step's kernels
+        if (rc) return rc;
+    }
+    c->guess_net = net;
+    return SPDK_OK;
+}
+
+int spdk_mixed_iterate(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, int nsweeps) {
+    const MVariant* v = nullptr;
+    int rc = check_mixed(c, U_sp, s, ws, &v);
+    if (rc) return rc;
+    if (nsweeps < 1 || nsweeps > 1000) { set_error("mixed: sweeps outside 1..1000"); return SPDK_E_ARG; }
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    return v->iterate(c, U_sp, Ne, ws, dt, nsweeps);
+}
+
+int spdk_mixed_guess_data(spdk_ctx* c, void* ws, const int N[3], double dt, int e0, int nrows, float* X, long ldx,
+                          float* Y, long ldy) {
+    const MVariant* v = c ? find_mvariant(c) : nullptr;
+    if (!v || !ws || !N) { set_error("mixed guess data: bad argument"); return SPDK_E_ARG; }
+    const size_t E = (size_t)N[0] * N[1] * N[2];
+    if (e0 < 0 || nrows < 0 || (size_t)e0 + nrows > E) { set_error("mixed guess data: rows outside the mesh"); return SPDK_E_ARG; }
+    return v->data(c, ws, E, dt, e0, nrows, X, ldx, Y, ldy);
 }
 
 int spdk_mixed_histogram(spdk_ctx* c, void* ws, const int N[3], unsigned long long* hist, int nbins, int reset) {
@@ -1330,10 +1638,17 @@ int spdk_hydro_step_mixed(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
     int rc0 = check_mixed(c, U_sp, s, ws, &v);
     if (rc0) return rc0;
     if (pc && pc->residual && !(pc->tol > 0.0)) { set_error("mixed: residual tolerance must be positive"); return SPDK_E_ARG; }
-    if (pc && pc->guess != SPDK_GUESS_ZERO && pc->guess != SPDK_GUESS_EXTERNAL) { set_error("mixed: unknown guess"); return SPDK_E_ARG; }
-    if (pc && pc->guess == SPDK_GUESS_EXTERNAL && !pc->residual) {
-        set_error("mixed: an external guess needs the residual test");
+    if (pc && pc->guess != SPDK_GUESS_ZERO && pc->guess != SPDK_GUESS_EXTERNAL && pc->guess != SPDK_GUESS_MLP) {
+        set_error("mixed: unknown guess");
         return SPDK_E_ARG;
+    }
+    if (pc && pc->guess != SPDK_GUESS_ZERO && !pc->residual) {
+        set_error("mixed: a guess other than the zero-order one needs the residual test");
+        return SPDK_E_ARG;
+    }
+    if (pc && pc->guess == SPDK_GUESS_MLP && !c->guess_net) {
+        set_error("mixed: no network attached (spdk_mixed_attach_mlp)");
+        return SPDK_E_CONFIG;
     }
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);

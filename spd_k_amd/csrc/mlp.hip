@@ -227,6 +227,30 @@ __global__ void k_pack_input(const float* Xf, long ld, int M, int din, int dpad,
     }
 }
 
+// dst[r][:] = src[idx[r]][:] for r < n (cols values per row)
+__global__ void k_gather(const float* src, long ld, const int* idx, int n, int cols, float* dst) {
+    const long total = (long)n * cols;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(i / cols), c = (int)(i % cols);
+        dst[i] = src[(long)idx[r] * ld + c];
+    }
+}
+
+// flags[r] = 1 if every value of row r is finite with magnitude < maxabs
+__global__ void k_rows_ok(const float* Y, long ld, int rows, int cols, float maxabs, unsigned char* flags) {
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+        const float v = Y[(long)r * ld + c];
+        if (!(fabsf(v) < maxabs)) bad = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) flags[r] = bad ? 0 : 1;
+}
+
 inline unsigned nb(long n, int bs) { long b = (n + bs - 1) / bs; return (unsigned)(b > 65535 * 4 ? 65535 * 4 : b); }
 
 int gemm(hipStream_t st, const GemmP& p, int ksplit = 1) {
@@ -390,6 +414,30 @@ int spdk_mlp_destroy(spdk_mlp* net) {
     (void)hipStreamSynchronize(net->stream);
     for (void* p : net->owned) (void)hipFree(p);
     delete net;
+    return SPDK_OK;
+}
+
+int spdk_gather_rows(const float* src, long ld, const int* idx, int n, int cols, float* dst, void* stream) {
+    if (!src || !idx || !dst || n < 0 || cols < 0) { set_error("gather: bad argument"); return SPDK_E_ARG; }
+    if (n == 0 || cols == 0) return SPDK_OK;
+    hipLaunchKernelGGL(k_gather, dim3(nb((long)n * cols, 256)), dim3(256), 0, (hipStream_t)stream, src, ld, idx, n, cols,
+                       dst);
+    SPDK_LAUNCH_CHECK("k_gather");
+    return SPDK_OK;
+}
+
+int spdk_rows_finite(const float* Y, long ld, int rows, int cols, float maxabs, unsigned char* flags, void* stream) {
+    if (!Y || !flags || rows < 0 || cols < 0) { set_error("rows_finite: bad argument"); return SPDK_E_ARG; }
+    if (rows == 0) return SPDK_OK;
+    hipLaunchKernelGGL(k_rows_ok, dim3(rows), dim3(256), 0, (hipStream_t)stream, Y, ld, rows, cols, maxabs, flags);
+    SPDK_LAUNCH_CHECK("k_rows_ok");
+    return SPDK_OK;
+}
+
+int spdk_mlp_set_stream(spdk_mlp* net, void* stream) {
+    if (!net) return SPDK_E_ARG;
+    SPDK_CHECK(hipStreamSynchronize(net->stream));
+    net->stream = (hipStream_t)stream;
     return SPDK_OK;
 }
 

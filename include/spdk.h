@@ -239,17 +239,24 @@ int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
 typedef struct spdk_picard {
     int residual;   /* 0: p predictor sweeps then the corrector (the reference's
                        count); 1: sweep the elements that have not passed the
-                       residual test (and their neighbours) until all pass   */
+                       residual test (and the elements reading their traces)
+                       until all pass or reach the cap                       */
     int guess;      /* SPDK_GUESS_*                                            */
-    double tol;     /* residual test: max over elements of
+    double tol;     /* residual test, per interior element:
                        max|d_out - d_in| / max_v|mean_v| < tol                 */
-    int max_sweeps; /* predictor sweeps before the fixed-count zero-guess
-                       fallback of the whole step                              */
+    int max_sweeps; /* cap of predictor sweeps per element; <= 0: the fixed
+                       count p. An element reaching the cap keeps its state
+                       (from the zero-order guess and the cap p, that is the
+                       fixed-count predictor); from an external or learned
+                       guess it restarts from the zero-order guess with the
+                       fixed count p                                           */
 } spdk_picard;
 typedef struct spdk_picard_stats {
-    long long steps, fallbacks; /* accumulated by spdk_hydro_step_mixed           */
-    double sweeps;              /* sweeps, each weighted by the fraction of
-                                   elements it updated                          */
+    long long steps;     /* accumulated by spdk_hydro_step_mixed                 */
+    long long fallbacks; /* steps in which some element reached the cap
+                            unconverged (restarted, from a guess)              */
+    double sweeps;       /* sweeps, each weighted by the fraction of elements
+                            it updated (the corrector counts 1)               */
     double last_residual;
 } spdk_picard_stats;
 size_t spdk_mixed_workspace_bytes(spdk_ctx* ctx, const int N[3]);
@@ -266,8 +273,8 @@ int spdk_mixed_predictor_buffer(spdk_ctx* ctx, void* workspace, const int N[3], 
 int spdk_hydro_step_mixed(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
                           double* F_x, const spdk_picard* pc, spdk_picard_stats* stats, double* dt_next);
 /* Histogram of the predictor sweeps each element needed to pass the residual
- * test, accumulated over residual-form steps (bin k = k sweeps, bin 31 =
- * never within max_sweeps). Copies nbins (<= 32) counts; reset clears. */
+ * test, accumulated over residual-form steps (bin k = k sweeps counted from
+ * the step's first sweep, a restart included; bin 31 = never). Copies nbins (<= 32) counts; reset clears. */
 int spdk_mixed_histogram(spdk_ctx* ctx, void* workspace, const int N[3], unsigned long long* hist, int nbins,
                          int reset);
 /* Learned guess. Feature and target sizes of this degree and dimension:

@@ -440,10 +440,21 @@ struct HydroRun {
             CK(spdk_rows_finite(Ys, dout, (int)rows, dout, 1e6f, (unsigned char*)fp, strm));
             CK(spdk_rows_finite(Xs, din, (int)rows, din, 1e6f, (unsigned char*)fp + rows, strm));
             CK(spdk_memcpy_d2h(ctx, ok.data(), fp, 2 * rows));
-            for (size_t r = 0; r < rows; r++)
+            // ghost elements are not part of the solution: left out as well
+            auto interior = [&](size_t e) {
+                const int i = (int)(e % N[0]), j = (int)(e / N[0] % N[1]), k = (int)(e / ((size_t)N[0] * N[1]));
+                const int ijk[3] = {i, j, k};
+                for (int d = 0; d < 3; d++)
+                    if (N[d] > 1 && (ijk[d] < 1 || ijk[d] > N[d] - 2)) return false;
+                return true;
+            };
+            size_t ghosts = 0;
+            for (size_t r = 0; r < rows; r++) {
+                if (!interior(r % E)) { ghosts++; continue; }
                 if (ok[r] && ok[rows + r]) (r < (size_t)(S - 1) * E ? train_rows : held_rows).push_back((int)r);
+            }
             std::printf("training rows: %zu, held out: %zu, dropped (non-finite or |value| >= 1e6): %zu\n",
-                        train_rows.size(), held_rows.size(), rows - train_rows.size() - held_rows.size());
+                        train_rows.size(), held_rows.size(), rows - ghosts - train_rows.size() - held_rows.size());
         }
         const size_t ntrain = train_rows.size();
         float *Xb = nullptr, *Yb = nullptr;
@@ -557,7 +568,8 @@ struct HydroRun {
             res->picard_steps = pst.steps;
             res->picard_sweeps = pst.sweeps;
             res->picard_fallbacks = pst.fallbacks;
-            std::printf("picard: %lld steps, %.3f sweeps/step, %lld fallbacks, last residual %.3e\n", pst.steps,
+            std::printf("picard: %lld steps, %.3f sweeps/step, %lld steps with capped elements, last residual %.3e\n",
+                        pst.steps,
                         pst.steps ? pst.sweeps / pst.steps : 0.0, pst.fallbacks, pst.last_residual);
             if (pc.residual) {
                 int N[3] = {X.n_total, Y.n_total, Z.n_total};
@@ -565,7 +577,10 @@ struct HydroRun {
                 CK(spdk_mixed_histogram(ctx, ws, N, hist, 32, 0));
                 std::printf("picard histogram (elements by predictor sweeps to tolerance):");
                 for (int b = 0; b < 32; b++)
-                    if (hist[b]) std::printf(" %s%d:%llu", b == 31 ? "never" : "", b == 31 ? 0 : b, hist[b]);
+                    if (hist[b]) {
+                        if (b == 31) std::printf(" capped:%llu", hist[b]);
+                        else std::printf(" %d:%llu", b, hist[b]);
+                    }
                 std::printf("\n");
                 for (int b = 0; b < 32; b++) res->picard_hist[b] = (long long)hist[b];
             }

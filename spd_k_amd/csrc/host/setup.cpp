@@ -155,7 +155,7 @@ std::string configure(Params& pin, RunConfig& c) {
         if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
             return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step without fallback";
         if (c.precision != "mixed") return "job/picard=residual and job/guess=mlp run on job/precision=mixed";
-        if (c.picard_tol <= 0 || c.picard_max < 1 || c.picard_max > 30) return "job/picard_tol > 0, 1 <= job/picard_max <= 30";
+        if (c.picard_tol <= 0 || c.picard_max < 0 || c.picard_max > 30) return "job/picard_tol > 0, 0 <= job/picard_max <= 30";
         if (c.guess == "mlp" && (c.picard != "residual" || c.mlp_weights.empty()))
             return "job/guess=mlp needs job/picard=residual and job/mlp_weights";
     }

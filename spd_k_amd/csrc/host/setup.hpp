@@ -54,8 +54,8 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     bool timing = false;         // job/timing: HIP-event timing of the dominant kernel
     std::string precision = "fp64";  // job/precision: fp64 | mixed (FP32 state and contractions)
     std::string picard = "fixed";    // job/picard: fixed (p predictor sweeps) | residual
-    double picard_tol = 1e-4;        // job/picard_tol: element residual tolerance
-    int picard_max = 8;              // job/picard_max: predictor sweeps before the zero-guess fallback
+    double picard_tol = 1e-6;        // job/picard_tol: element residual tolerance
+    int picard_max = 0;              // job/picard_max: per-element cap of predictor sweeps (0: the fixed count p)
     std::string guess = "zero";      // job/guess: zero | mlp
     std::string mlp_weights;         // job/mlp_weights: trained network file (job/guess=mlp)
     std::string mode = "run";        // job/mode: run | train_guess

@@ -38,16 +38,18 @@
 //
 // Picard control: the fixed count (p predictor sweeps, then the corrector,
 // src/hydro_ader.hpp:291-305) or a residual test with the element residual
-//     r_e = max |d_out - d_in| / max_v |mu(v)|.
-// Residual form: the first sweep updates every element and also evaluates
-// the corrector from the fluxes of its input state (a candidate); if every
-// r_e < tol the candidate is the step's result. Otherwise later sweeps update
+//     r_e = max |d_out - d_in| / max_v |mu(v)|   (ghost elements excluded).
+// Residual form: the first sweep updates every element; later sweeps update
 // only the active elements: those whose own r_e, or the r_e of an element
 // whose traces they read, was >= tol in the previous sweep; the others keep
-// their state and traces. When no element is active a corrector sweep over
+// their state and traces. Each element stops at convergence or at the cap
+// (by default the fixed count, p sweeps: an element that never converges
+// ends with exactly the fixed-count predictor from the zero-order guess,
+// evaluated against its neighbours' latest traces). A corrector sweep over
 // every element ends the step (its face fluxes come from one set of traces,
-// so conservation holds). A step not converged within max_sweeps falls back
-// to the fixed-count sequence from the zero-order guess.
+// so conservation holds). From an external or learned guess, the elements
+// still unconverged at the cap restart from the zero-order guess and get
+// the fixed count again (k_mrestart), the fallback of that form.
 //
 // Trace buffers. Each element writes its traces to the older of two buffers
 // and records the sweep number (ver); readers take, per neighbour, the
@@ -106,7 +108,7 @@ struct MT {
 
 struct MArgs {
     const double* U_sp;     // (1,6,E,TPE) solution at the step start (FP64)
-    double* U_new;          // corrector: the new solution (U_sp itself); predictor: candidate (5,E,TPE) or null
+    double* U_new;          // corrector: the new solution (U_sp itself)
     const double* mu;       // (5,E) element means of U_sp
     const float* d_old;     // (5,E,TPE) U_sp - mu
     float* D;               // (T,5,E,TPE) predictor state, read and rewritten in place
@@ -506,7 +508,11 @@ __device__ __forceinline__ void cons_shift(double* du, const double* bal, double
     for (int v = 0; v < NVP; v++) du[v] += X_shift[v];
 }
 
-template <int N, int DIM>
+// CONS = false: a predictor sweep (out_pred set); CONS = true: the
+// corrector (face balance, conservative update, cv state and dt). Compiled
+// apart, the predictor needs fewer VGPRs (3 waves per SIMD at 3-D p=3
+// instead of 2).
+template <int N, int DIM, bool CONS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_MSWEEP_WAVES))) k_msweep(MArgs a) {
     using G = MT<N, DIM>;
     constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T, FSZ = G::FSZ;
@@ -590,8 +596,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
     for (int t = 0; t < T; t++)
 #pragma unroll
         for (int v = 0; v < NVP; v++) dudt[t][v] = 0.f;
-    // the corrector and the candidates conserve: accumulate the face balance
-    const bool cons = !a.out_pred || a.U_new;
+    // the corrector conserves: accumulate the face balance
+    const bool cons = CONS;
     double bal[NVP] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
     // Rounds of pencil tasks, then the flux divergence of the round's fluxes.
@@ -657,10 +663,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
 #pragma unroll
             for (int v = 0; v < NVP; v++) dudt[t][v] = dudt[0][v];
 
-    if (a.out_pred) {
+    if (!CONS) {
         // ---- predictor: d_new(t) = d_old - sum_l cdt(t,l) dudt(l); residual vs the input state
         float res = 0.f;
-        double du[NVP];
         if (act_e) {
 #pragma unroll
             for (int t = 0; t < T; t++)
@@ -677,12 +682,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                     a.D[((size_t)t * NVP + v) * slabE + eo] = dn;
                     Ds[(t * NVP + v) * PSZ + so] = dn;
                 }
-        }
-#pragma unroll
-        for (int v = 0; v < NVP; v++) {  // corrector candidate from the fluxes of the input state
-            du[v] = 0.0;
-#pragma unroll
-            for (int t = 0; t < T; t++) du[v] += a.wdt[t] * (double)dudt[t][v];
         }
         __syncthreads();
         // traces of the new state into this element's older buffer: task = (t, d, q)
@@ -709,12 +708,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             }
         }
         __syncthreads();
-        if (a.U_new) {
-            cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, act_e, s);
-            if (act_e)
-#pragma unroll
-                for (int v = 0; v < NVP; v++) a.U_new[(size_t)v * slabE + eo] = a.U_sp[(size_t)v * slabE + eo] - du[v];
-        }
         if (act_e) Fs[s] = res;
         __syncthreads();
         if (act_e && s == 0) {
@@ -723,7 +716,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             double sc = 0.0;
 #pragma unroll
             for (int v = 0; v < NVP; v++) sc = fmax(sc, fabs(X[v]));
-            double rr = (double)re / (sc > 0.0 ? sc : 1.0);
+            // Ghost elements feed no interior element (interior faces read
+            // interior traces only, face_sources), so their residual is not
+            // a convergence criterion: a ghost that drifts never blocks it.
+            const bool ghost = (a.act[0] && (ei < 1 || ei > a.Ne[0] - 2)) ||
+                               (a.act[1] && (ej < 1 || ej > a.Ne[1] - 2)) ||
+                               (a.act[2] && (ek < 1 || ek > a.Ne[2] - 2));
+            double rr = ghost ? 0.0 : (double)re / (sc > 0.0 ? sc : 1.0);
             X[G::XRES] = rr;
             if (a.conv && rr < a.tol && a.conv[e] == 0xff) a.conv[e] = (unsigned char)a.sweep;
             if (a.big_out) a.big_out[e] = rr >= a.tol ? 1 : 0;
@@ -878,38 +877,60 @@ __global__ void __launch_bounds__(256) k_mtrace(const float* __restrict__ D, flo
     }
 }
 
-// Accept a corrector candidate: copy it into U_sp, then cv primitives and dt.
+// Restart from the zero-order guess: the elements flagged in `flags` get
+// every time slice of the predictor set to d_old and the traces of d_old
+// written, as version `cur`, into the older of their two buffers.
+// next_flags marks them, so the next sweep updates them and the elements
+// that read their traces.
 template <int N, int DIM>
-__global__ void __launch_bounds__(256) k_mfinish(const double* __restrict__ cand, double* U_sp, double* W_cv,
-                                                 double* part, const double* __restrict__ S2C, int Ne0, int Ne1,
-                                                 int Ne2, int act0, int act1, int act2, double h0, double h1,
-                                                 double h2, double gm, double cfl, int Eint) {
+__global__ void __launch_bounds__(256) k_mrestart(const float* __restrict__ d_old, float* __restrict__ D, float* trA0,
+                                                  float* trA1, float* trA2, float* trB0, float* trB1, float* trB2,
+                                                  const float* __restrict__ S2F, int* ver,
+                                                  const unsigned char* __restrict__ flags, unsigned char* next_flags,
+                                                  int cur, int Eint) {
     using G = MT<N, DIM>;
-    constexpr int TPE = G::TPE, EPB = G::EPB;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
     const int e = blockIdx.x * EPB + el;
     const bool live = el < EPB && e < Eint;
-    double* area = (double*)(lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES);
-    const size_t slabE = (size_t)Eint * TPE;
-    const int Ne[3] = {Ne0, Ne1, Ne2}, act[3] = {act0, act1, act2};
-    const double h[3] = {h0, h1, h2};
-    int ei = 0, ej = 0, ek = 0;
-    if (live) {
-        ei = e % Ne0;
-        ej = (e / Ne0) % Ne1;
-        ek = e / (Ne0 * Ne1);
-    }
-    const int ii = s % N, jj = DIM >= 2 ? (s / N) % N : 0, kk = DIM == 3 ? s / (N * N) : 0;
-    double un[NVP];
+    const bool go = live && flags[e] != 0;
+    float* ds = (float*)(lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES);
+    const size_t E = (size_t)Eint, slabE = E * TPE, eo = (size_t)e * TPE + s;
+    int wb = 0;
+    if (go) {
 #pragma unroll
-    for (int v = 0; v < NVP; v++) {
-        un[v] = live ? cand[(size_t)v * slabE + (size_t)e * TPE + s] : 1.0;
-        if (live) U_sp[(size_t)v * slabE + (size_t)e * TPE + s] = un[v];
+        for (int v = 0; v < NVP; v++) {
+            const float x = d_old[(size_t)v * slabE + eo];
+            ds[v * TPE + s] = x;
+#pragma unroll
+            for (int t = 0; t < T; t++) D[((size_t)t * NVP + v) * slabE + eo] = x;
+        }
+        wb = ver[E + e] < ver[e] ? 1 : 0;
     }
-    double r = cv_and_dt<N, DIM>(un, area, live, s, ii, jj, kk, ei, ej, ek, (size_t)e, W_cv, slabE, S2C, Ne, act, h,
-                                 gm, cfl);
-    block_min<G::BLOCK>(r, (double*)lds_raw, part);
+    __syncthreads();
+    if (live && s == 0) next_flags[e] = go ? 1 : 0;
+    for (int task = s; go && task < DIM * NT; task += TPE) {
+        const int d = task / NT, q = task % NT;
+        float* tb = wb ? (d == 0 ? trB0 : (d == 1 ? trB1 : trB2)) : (d == 0 ? trA0 : (d == 1 ? trA1 : trA2));
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            float x0 = 0.f, x1 = 0.f;
+#pragma unroll
+            for (int l = 0; l < N; l++) {
+                float p = ds[v * TPE + spidx<N, DIM>(d, l, q)];
+                x0 += p * S2F[l];
+                x1 += p * S2F[(M - 1) * N + l];
+            }
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                float* to = tb + ((((size_t)t * NVP + v) * E + e) * 2) * NT + q;
+                to[0] = x0;
+                to[NT] = x1;
+            }
+        }
+    }
+    if (go && s == 0) ver[(size_t)wb * E + e] = cur;
 }
 
 // out[0] = min of a[0..n) (a may be null: +inf), out[1] = max of b[0..n),
@@ -1071,7 +1092,7 @@ struct MLayout {
     size_t E = 0, tpe = 0, nt = 0;
     unsigned blocks = 0;
     // byte offsets
-    size_t mu = 0, cand = 0, part = 0, resp = 0, cntp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0,
+    size_t mu = 0, part = 0, resp = 0, cntp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0,
            conv = 0, ver = 0, big = 0, gsc = 0, total = 0;
     size_t tr_axis = 0;  // bytes per axis per trace buffer
 };
@@ -1088,7 +1109,6 @@ MLayout mlayout(size_t E) {
     L.blocks = (unsigned)((E + G::EPB - 1) / G::EPB);
     size_t o = 0;
     L.mu = o; o = al(o + sizeof(double) * NVP * E);
-    L.cand = o; o = al(o + sizeof(double) * NVP * E * G::TPE);
     L.part = o; o = al(o + sizeof(double) * (L.blocks + 8));
     L.resp = o; o = al(o + sizeof(double) * (L.blocks + 8));
     L.cntp = o; o = al(o + sizeof(double) * (L.blocks + 8));
@@ -1277,7 +1297,6 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     a.tol = pc ? pc->tol : 0.0;
     unsigned char* conv = (unsigned char*)(W + L.conv);
     unsigned char* big[2] = {(unsigned char*)(W + L.big), (unsigned char*)(W + L.big) + E};
-    double* cand = (double*)(W + L.cand);
     double* red2 = (double*)(W + L.red2);
     a.ver = (int*)(W + L.ver);
     a.cnt_part = (double*)(W + L.cntp);
@@ -1288,7 +1307,10 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     auto sweep = [&](const MArgs& args) -> int {
         bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
-        hipLaunchKernelGGL((k_msweep<N, DIM>), grid, block, lds, c->stream, args);
+        if (!args.out_pred || args.U_new)
+            hipLaunchKernelGGL((k_msweep<N, DIM, true>), grid, block, lds, c->stream, args);
+        else
+            hipLaunchKernelGGL((k_msweep<N, DIM, false>), grid, block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_msweep");
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
         return SPDK_OK;
@@ -1331,62 +1353,74 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     if (!(external && !learned) && (rc = start())) return rc;  // an external guess follows spdk_mixed_prepare
     if (learned && (rc = mlp_guess<N, DIM>(c, ws, E, dt))) return rc;
     double sweeps = 0;  // element-weighted: a sweep over a fraction f of the elements counts f
-    bool done = false;
     if (pc && pc->residual) {
-        const int kmax = pc->max_sweeps > 0 ? pc->max_sweeps : 8;
+        const int cap_fixed = T - 1;  // the fixed count of predictor sweeps
+        int cap = pc->max_sweeps > 0 ? std::min(pc->max_sweeps, 250) : cap_fixed;
         if (external) {
             // the caller's guess is in the predictor buffer: its traces, version 0
             hipLaunchKernelGGL((k_mtrace<N, DIM>), grid, block, lds, c->stream, (const float*)a.D, trA[0], trA[1],
                                trA[2], (const float*)a.S2F, a.ver, (int)E);
             SPDK_LAUNCH_CHECK("k_mtrace");
         }
-        int k = 0;
-        for (; k < kmax && k < 250; k++) {
-            MArgs s = a;
-            s.cur = k + 1;
-            s.path = (k > 0 || external) ? 1 : 0;
-            s.tin_slices = s.path ? T : 1;
-            s.out_pred = 1;
-            s.U_new = k == 0 ? cand : nullptr;  // every element is updated in the first sweep
-            s.F_x = k == 0 ? F_x : nullptr;
-            s.conv = conv;
-            s.sweep = k;
-            s.big_in = k == 0 ? nullptr : big[(k - 1) % 2];
-            s.big_out = big[k % 2];
-            if ((rc = sweep(s))) return rc;
+        bool guess_phase = external;
+        int cur = 0;                           // newest trace version
+        int ksw = 0;                           // sweeps this step (histogram index)
+        int kph = 0;                           // sweeps in this phase
+        const unsigned char* flags = nullptr;  // unconverged after the last sweep (null: all)
+        bool capped = false;
+        for (;;) {
+            MArgs sa = a;
+            sa.cur = ++cur;
+            sa.path = (ksw > 0 || external) ? 1 : 0;
+            sa.tin_slices = sa.path ? T : 1;
+            sa.out_pred = 1;
+            sa.conv = conv;
+            sa.sweep = ksw;
+            sa.big_in = flags;
+            sa.big_out = big[ksw % 2];
+            if ((rc = sweep(sa))) return rc;
             if ((rc = reduce(nullptr, a.res_part, a.cnt_part))) return rc;
             sweeps += c->red_host[2] / (double)E;
             if (st) st->last_residual = c->red_host[1];
+            flags = big[ksw % 2];
+            ksw++;
+            kph++;
             if (c->red_host[1] < pc->tol) break;
+            if (kph < cap && ksw < 250) continue;
+            if (guess_phase) {
+                // unconverged from the guess: restart those elements from the
+                // zero-order guess with the fixed count
+                hipLaunchKernelGGL((k_mrestart<N, DIM>), grid, block, lds, c->stream, (const float*)a.d_old, a.D,
+                                   trA[0], trA[1], trA[2], trB[0], trB[1], trB[2], (const float*)a.S2F, a.ver, flags,
+                                   big[ksw % 2], ++cur, (int)E);
+                SPDK_LAUNCH_CHECK("k_mrestart");
+                flags = big[ksw % 2];
+                ksw++;
+                guess_phase = false;
+                kph = 0;
+                cap = cap_fixed;
+                capped = true;
+                continue;
+            }
+            capped = true;  // the rest keep the result of the cap
+            break;
         }
+        if (capped && st) st->fallbacks++;
         hipLaunchKernelGGL(k_mhist, dim3(256), dim3(256), 0, c->stream, (const unsigned char*)conv, (int)E,
                            kHistBins, (unsigned long long*)(W + L.hist));
         SPDK_LAUNCH_CHECK("k_mhist");
-        if (k == 0) {  // converged in the first sweep: its candidate is the result
-            hipLaunchKernelGGL((k_mfinish<N, DIM>), grid, block, lds, c->stream, (const double*)cand, U_sp, W_cv,
-                               a.part, a.S2C, Ne[0], Ne[1], Ne[2], a.act[0], a.act[1], a.act[2], a.h[0], a.h[1],
-                               a.h[2], a.gm, a.cfl, (int)E);
-            SPDK_LAUNCH_CHECK("k_mfinish");
-            done = true;
-        } else if (k < kmax && k < 250) {  // converged: the corrector over every element
-            MArgs s = a;
-            s.cur = k + 2;
-            s.path = 1;
-            s.tin_slices = T;
-            s.out_pred = 0;
-            s.U_new = U_sp;
-            s.W_cv = W_cv;
-            s.F_x = F_x;
-            s.sweep = k + 1;
-            if ((rc = sweep(s))) return rc;
-            sweeps += 1.0;
-            done = true;
-        } else {
-            if (st) st->fallbacks++;
-            if ((rc = start())) return rc;  // fresh traces of the zero-order guess
-        }
-    }
-    if (!done) {
+        MArgs sa = a;  // the corrector over every element
+        sa.cur = ++cur;
+        sa.path = 1;
+        sa.tin_slices = T;
+        sa.out_pred = 0;
+        sa.U_new = U_sp;
+        sa.W_cv = W_cv;
+        sa.F_x = F_x;
+        sa.sweep = ksw;
+        if ((rc = sweep(sa))) return rc;
+        sweeps += 1.0;
+    } else {
         if ((rc = fixed_count())) return rc;
         sweeps += T;
     }
@@ -1476,7 +1510,7 @@ int run_iterate(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt,
         s.tin_slices = k == 0 ? 1 : G::T;
         s.out_pred = 1;
         s.sweep = k;
-        hipLaunchKernelGGL((k_msweep<N, DIM>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, s);
+        hipLaunchKernelGGL((k_msweep<N, DIM, false>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, s);
         SPDK_LAUNCH_CHECK("k_msweep");
     }
     return SPDK_OK;

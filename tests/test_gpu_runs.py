@@ -3,6 +3,7 @@ ABI) against the CPU oracle on the same inputs, every output file compared
 bit for bit (troubled-cell masks included), plus the reference goldens and
 size-independent invariants at the 96^3 benchmark size."""
 import os
+import re
 
 import numpy as np
 import pytest
@@ -182,6 +183,55 @@ def test_mixed_kh_close_to_fp64(tmp_path, dims):
     args = ["-i", KH, "job/fallback=false", f"mesh/nx1={nx}", f"mesh/nx2={ny}", f"mesh/nx3={nz}", f"mesh/p={p}",
             "problem/perturb=random", "problem/amp=0.01", "time/tlim=0.05", "output/dt=0.025"]
     a, b = _mixed_vs_fp64(tmp_path, args)
+    i = op.output_indices(a)[-1]
+    for v in (0, 1, 4):
+        mean, mx = _rel(a, b, i, v)
+        assert mean < 1e-6 and mx < 1e-5, (v, mean, mx)
+    assert op.mass_drift(b) < 1e-13
+
+
+def _picard(text):
+    m = re.search(r"picard: (\d+) steps, ([0-9.]+) sweeps/step, (\d+) steps with capped elements", text)
+    assert m, text[-500:]
+    return int(m.group(1)), float(m.group(2)), int(m.group(3))
+
+
+def test_mixed_residual_matched_l1_3d(tmp_path):
+    """3-D sine advection, 16^3, p=3: residual-driven Picard at tol 1e-6 with
+    the per-element cap at the fixed count keeps the L1 error against the
+    exact solution within 1.1x of the FP64 path's, never sweeps more than the
+    fixed count (3 predictor sweeps + corrector) and conserves mass."""
+    args = ["-i", S, "mesh/nx1=16", "mesh/nx2=16", "mesh/nx3=16", "problem/amp=0.1", "time/tlim=0.1", "output/dt=0.1"]
+    a, b = str(tmp_path / "fp64"), str(tmp_path / "res")
+    op.run_driver(args, a)
+    out = op.run_driver(args + ["job/precision=mixed", "job/picard=residual", "job/picard_tol=1e-6"], b)
+    print(out[-400:])
+    steps, sweeps, _ = _picard(out)
+    i = op.output_indices(a)[-1]
+    l1a = op.l1_vs_exact_sine(a, i, 0.1, 2, amp=0.1)
+    l1b = op.l1_vs_exact_sine(b, i, 0.1, 2, amp=0.1)
+    assert l1b <= 1.1 * l1a, (l1a, l1b)
+    assert steps > 0 and sweeps <= 4.0 + 1e-9, sweeps
+    assert op.mass_drift(b) < 1e-13
+
+
+def test_mixed_learned_guess_restarts(tmp_path):
+    """KH 3-D p=3 from the trained network's guess with a one-sweep cap and a
+    tolerance the guess cannot meet: the unconverged elements restart from
+    the zero-order guess with the fixed count, every step reports them, and
+    the result stays within FP32-level distance of the FP64 path with totals
+    conserved."""
+    w = os.path.join(op.ROOT, "spd_k_amd", "weights", "kh3d_p3.mlp")
+    args = ["-i", KH, "job/fallback=false", "mesh/nx1=10", "mesh/nx2=10", "mesh/nx3=10", "mesh/p=3",
+            "problem/perturb=random", "problem/amp=0.01", "time/tlim=0.02", "output/dt=0.01"]
+    a, b = str(tmp_path / "fp64"), str(tmp_path / "learned")
+    op.run_driver(args, a)
+    out = op.run_driver(args + ["job/precision=mixed", "job/picard=residual", "job/picard_tol=1e-7",
+                                "job/picard_max=1", "job/guess=mlp", f"job/mlp_weights={w}"], b)
+    print(out[-400:])
+    steps, sweeps, capped = _picard(out)
+    assert steps > 0 and capped == steps, (steps, capped)
+    assert sweeps <= 1 + 3 + 1 + 1e-9, sweeps  # guess sweep, the fixed count, the corrector
     i = op.output_indices(a)[-1]
     for v in (0, 1, 4):
         mean, mx = _rel(a, b, i, v)

@@ -151,27 +151,11 @@ def cpu_baseline(sample_n=24, steps=3):
                       f"steps, evolution loop {float(m.group(2)):.2f} s"}
 
 
-def element_means(outdir, var=0):
-    """Volume averages of one variable over each element, from the last
-    output's control-volume values: array (Nz, Ny, Nx)."""
-    import numpy as np
-    import oracle_py as op
-    g = op.grid(outdir)
-    n = g["p"] + 1
-    x = op.load_sd(outdir, g, op.output_indices(outdir)[-1], var)
-    w = [np.diff(g["faces"][d]) for d in range(3)]
-    vol = w[2][:, None, None] * w[1][None, :, None] * w[0][None, None, :]
-    Nz, Ny, Nx = (s // n for s in x.shape)
-    num = (x * vol).reshape(Nz, n, Ny, n, Nx, n).sum(axis=(1, 3, 5))
-    return num / vol.reshape(Nz, n, Ny, n, Nx, n).sum(axis=(1, 3, 5))
-
-
 def accuracy(forms):
     """Matched-L1 check of every form (module docstring)."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py as op
     import spd_k_amd
+    from spd_k_amd.outputs import element_means, l1_vs_exact_sine, output_indices
     sine = ["-i", os.path.join(ROOT, "tests", "inputs", "sine_wave.athinput"), "mesh/nx1=24", "mesh/nx2=24",
             "mesh/nx3=24", "mesh/p=3", "problem/amp=0.1", "time/tlim=0.2", "output/dt=0.2"]
     kh = lambda n: kh_args(n, perturb="sin")[:-2] + ["time/tlim=0.03", "output/dt=0.03"]
@@ -185,7 +169,7 @@ def accuracy(forms):
             sd, kd = os.path.join(d, "s_" + name), os.path.join(d, "k_" + name)
             spd_k_amd.run(sine + FORMS[name], sd)
             spd_k_amd.run(kh(24) + FORMS[name], kd)
-            out[name] = {"sine3d_l1": op.l1_vs_exact_sine(sd, op.output_indices(sd)[-1], 0.2, 2, amp=0.1),
+            out[name] = {"sine3d_l1": l1_vs_exact_sine(sd, output_indices(sd)[-1], 0.2, amp=0.1),
                          "kh3d_l1": float(np.abs(element_means(kd) - ref).mean() / np.abs(ref).mean())}
     base = out.get("plain_fp64")
     for name, r in out.items():

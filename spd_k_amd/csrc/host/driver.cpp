@@ -376,7 +376,7 @@ struct HydroRun {
 
     // Learned-guess training (job/mode=train_guess): snapshots of this run, each
     // paired with its Picard fixed point (mlp/sweeps sweeps from the zero-order
-    // guess), then Adam on the device. The last snapshot is held out.
+    // guess), then Adam on the device.
     void train_guess() {
         int din = 0, dout = 0;
         CK(spdk_mixed_guess_dims(ctx, &din, &dout));
@@ -428,16 +428,21 @@ struct HydroRun {
         }
         CK(spdk_mlp_set_params(net, f.params.data()));
         // rows whose targets are not finite (Picard diverged there within
-        // mlp/sweeps sweeps) are left out of training and of the held-out loss
+        // mlp/sweeps sweeps) or exceed mlp/max_target (elements nearly uniform
+        // inside whose fixed point is driven by their neighbours: the target
+        // is normalised by their tiny amplitude) are left out. Of the rest, a
+        // seeded random eighth of the earlier snapshots' rows is held out
+        // (the loss on unseen rows of the training distribution), and the
+        // last snapshot is never trained on (the loss at a later time).
         void* strm = spdk_ctx_stream(ctx);
-        std::vector<int> train_rows, held_rows;
+        std::vector<int> train_rows, held_rows, last_rows;
         {
             const size_t rows = (size_t)S * E;
             void* fp = nullptr;
             CK(spdk_malloc(&fp, 2 * rows));
             mem.ptrs.push_back(fp);
             std::vector<unsigned char> ok(2 * rows);
-            CK(spdk_rows_finite(Ys, dout, (int)rows, dout, 1e6f, (unsigned char*)fp, strm));
+            CK(spdk_rows_finite(Ys, dout, (int)rows, dout, (float)cfg.mlp_max_target, (unsigned char*)fp, strm));
             CK(spdk_rows_finite(Xs, din, (int)rows, din, 1e6f, (unsigned char*)fp + rows, strm));
             CK(spdk_memcpy_d2h(ctx, ok.data(), fp, 2 * rows));
             // ghost elements are not part of the solution: left out as well
@@ -449,38 +454,47 @@ struct HydroRun {
                 return true;
             };
             size_t ghosts = 0;
+            std::uniform_int_distribution<int> pick8(0, 7);
             for (size_t r = 0; r < rows; r++) {
                 if (!interior(r % E)) { ghosts++; continue; }
-                if (ok[r] && ok[rows + r]) (r < (size_t)(S - 1) * E ? train_rows : held_rows).push_back((int)r);
+                if (!(ok[r] && ok[rows + r])) continue;
+                if (r >= (size_t)(S - 1) * E) last_rows.push_back((int)r);
+                else (pick8(rng) == 0 ? held_rows : train_rows).push_back((int)r);
             }
-            std::printf("training rows: %zu, held out: %zu, dropped (non-finite or |value| >= 1e6): %zu\n",
-                        train_rows.size(), held_rows.size(), rows - ghosts - train_rows.size() - held_rows.size());
+            std::printf("training rows: %zu, held out: %zu random + %zu of the last snapshot, dropped (non-finite "
+                        "or |target| >= %g): %zu\n", train_rows.size(), held_rows.size(), last_rows.size(),
+                        cfg.mlp_max_target, rows - ghosts - train_rows.size() - held_rows.size() - last_rows.size());
         }
         const size_t ntrain = train_rows.size();
         float *Xb = nullptr, *Yb = nullptr;
-        int *idx = nullptr, *hidx = nullptr;
+        int *idx = nullptr, *hidx = nullptr, *lidx = nullptr;
         CK(spdk_malloc(&p, (size_t)B * din * 4)); mem.ptrs.push_back(p); Xb = (float*)p;
         CK(spdk_malloc(&p, (size_t)B * dout * 4)); mem.ptrs.push_back(p); Yb = (float*)p;
         CK(spdk_malloc(&p, ntrain * 4 + 4)); mem.ptrs.push_back(p); idx = (int*)p;
         CK(spdk_malloc(&p, held_rows.size() * 4 + 4)); mem.ptrs.push_back(p); hidx = (int*)p;
+        CK(spdk_malloc(&p, last_rows.size() * 4 + 4)); mem.ptrs.push_back(p); lidx = (int*)p;
         if (!held_rows.empty()) CK(spdk_memcpy_h2d(ctx, hidx, held_rows.data(), held_rows.size() * 4));
+        if (!last_rows.empty()) CK(spdk_memcpy_h2d(ctx, lidx, last_rows.data(), last_rows.size() * 4));
         auto t1 = std::chrono::steady_clock::now();
-        auto eval = [&]() -> double {  // mean loss over the held-out rows, batches of B
+        // mean loss over a row set (device index array, n rows), batches of up to B
+        auto eval = [&](const int* dev_idx, size_t n) -> double {
             double acc = 0;
             size_t cnt = 0;
-            for (size_t r = 0; r + B <= held_rows.size(); r += B) {
+            for (size_t r = 0; r < n; r += B) {
+                const int b = (int)std::min((size_t)B, n - r);
                 float loss = 0;
-                CK(spdk_gather_rows(Xs, din, hidx + r, B, din, Xb, strm));
-                CK(spdk_gather_rows(Ys, dout, hidx + r, B, dout, Yb, strm));
-                CK(spdk_mlp_gradients(net, Xb, din, Yb, dout, B, nullptr, &loss));
-                acc += loss;
-                cnt++;
+                CK(spdk_gather_rows(Xs, din, dev_idx + r, b, din, Xb, strm));
+                CK(spdk_gather_rows(Ys, dout, dev_idx + r, b, dout, Yb, strm));
+                CK(spdk_mlp_gradients(net, Xb, din, Yb, dout, b, nullptr, &loss));
+                acc += (double)loss * b;
+                cnt += b;
             }
             return cnt ? acc / cnt : 0.0;
         };
         std::printf("training: %zu rows (%d snapshots), in %d hidden %d out %d\n", ntrain, S - 1, din,
                     cfg.mlp_hidden, dout);
-        std::printf("held-out loss before training %.4e\n", eval());
+        std::printf("held-out loss before training %.4e, last snapshot %.4e\n", eval(hidx, held_rows.size()),
+                    eval(lidx, last_rows.size()));
         for (int ep = 0; ep < cfg.mlp_epochs; ep++) {
             std::shuffle(train_rows.begin(), train_rows.end(), rng);
             CK(spdk_memcpy_h2d(ctx, idx, train_rows.data(), ntrain * 4));
@@ -496,7 +510,8 @@ struct HydroRun {
                 if (rd) { acc += loss; nrd++; }
                 nb++;
             }
-            std::printf("epoch %d: train loss %.4e (sampled), held-out loss %.4e\n", ep, nrd ? acc / nrd : 0.0, eval());
+            std::printf("epoch %d: train loss %.4e (sampled), held-out loss %.4e, last snapshot %.4e\n", ep,
+                        nrd ? acc / nrd : 0.0, eval(hidx, held_rows.size()), eval(lidx, last_rows.size()));
         }
         CK(spdk_ctx_sync(ctx));
         const double train_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();

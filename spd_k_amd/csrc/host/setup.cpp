@@ -143,6 +143,7 @@ std::string configure(Params& pin, RunConfig& c) {
         if (pin.has("mlp", "batch")) c.mlp_batch = pin.get_int("mlp", "batch");
         if (pin.has("mlp", "hidden")) c.mlp_hidden = pin.get_int("mlp", "hidden");
         if (pin.has("mlp", "lr")) c.mlp_lr = pin.get_real("mlp", "lr");
+        if (pin.has("mlp", "max_target")) c.mlp_max_target = pin.get_real("mlp", "max_target");
         if (pin.has("mlp", "seed")) c.mlp_seed = (uint64_t)std::stoull(pin.get_string("mlp", "seed"));
         if (c.mlp_snapshots < 2 || c.mlp_every < 1 || c.mlp_sweeps < 1 || c.mlp_epochs < 1 || c.mlp_batch < 64 ||
             c.mlp_hidden < 64)

@@ -64,6 +64,7 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     std::string mlp_out;             // mlp/out: weights file written
     int mlp_snapshots = 8, mlp_every = 4, mlp_sweeps = 20, mlp_epochs = 6, mlp_batch = 4096, mlp_hidden = 256;
     double mlp_lr = 1e-3;
+    double mlp_max_target = 50.0;    // mlp/max_target: training rows with a larger |target| are left out
     uint64_t mlp_seed = 1;
 };
 

@@ -140,33 +140,8 @@ def compare_dirs(a, b):
     return res
 
 
-# ---- readers (the formats of tests/spdk_io.py in the reference) -------------
-def grid(outdir):
-    g = {"faces": {}, "N": {}, "p": None}
-    for d, name in enumerate("XYZ"):
-        f = glob.glob(os.path.join(outdir, f"{name}_N*_0.dat"))[0]
-        m = re.match(rf"{name}_N(\d+)p(\d+)_0.dat", os.path.basename(f))
-        N, p = int(m.group(1)), int(m.group(2))
-        xf = np.fromfile(f)
-        g["N"][d] = N
-        if N > 1:
-            g["p"] = p
-            g["faces"][d] = xf[nGH:len(xf) - nGH]
-    return g
-
-
-def load_sd(outdir, g, i, var=0, prefix="W_cv", nvar=6):
-    n = g["p"] + 1
-    act = {d: g["N"][d] > 1 for d in range(3)}
-    Ne = lambda d: g["N"][d] + 2 * NGH if act[d] else 1
-    npt = lambda d: n if act[d] else 1
-    shape = (1, nvar, Ne(2), Ne(1), Ne(0), npt(2), npt(1), npt(0))
-    A = np.fromfile(os.path.join(outdir, f"{prefix}_N{g['N'][0]}p{g['p']}_{i}_0.dat")).reshape(shape)
-    v = A[0, var]
-    sl = lambda d: slice(NGH, -NGH) if act[d] else slice(None)
-    v = v[sl(2), sl(1), sl(0)]
-    Nz, Ny, Nx, nz, ny, nx = v.shape
-    return v.transpose(0, 3, 1, 4, 2, 5).reshape(Nz * nz, Ny * ny, Nx * nx)
+# ---- readers of the output format (shared with the package) ----------------
+from spd_k_amd.outputs import grid, load_sd, output_indices  # noqa: E402
 
 
 def total_mass(outdir, i, prefix="W_cv"):
@@ -180,15 +155,6 @@ def total_mass(outdir, i, prefix="W_cv"):
     if 2 in g["faces"]:
         V = V * np.diff(g["faces"][2])[:, None, None]
     return float((rho * V).sum())
-
-
-def output_indices(outdir, prefix="W_cv"):
-    idx = []
-    for f in glob.glob(os.path.join(outdir, f"{prefix}_*_0.dat")):
-        m = re.match(rf"{prefix}_N\d+(?:p\d+)?_(\d+)_0.dat", os.path.basename(f))
-        if m:
-            idx.append(int(m.group(1)))
-    return sorted(idx)
 
 
 def l1_vs_exact_sine(outdir, i, t, ndim, amp=0.125):

@@ -116,6 +116,7 @@ struct MArgs {
     int* ver;               // (2,E): sweep number of each element's last write to each buffer
     int cur;                // this sweep's number
     const unsigned char* big_in;  // per element: r_e >= tol in the previous sweep (null: every element active)
+    const int* blist;       // blocks to run (grid = its length; null: every block)
     unsigned char* big_out;       // per element, this sweep (null: not recorded)
     double* cnt_part;       // per-block count of active elements (predictor)
     double* W_cv;           // optional (1,6,E,TPE) output primitives at control volumes
@@ -519,7 +520,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int tid = threadIdx.x;
     const int el = tid / TPE, s = tid % TPE;
-    const int e = blockIdx.x * EPB + el;
+    const int blk = a.blist ? a.blist[blockIdx.x] : (int)blockIdx.x;
+    const int e = blk * EPB + el;
     const bool live = el < EPB && e < a.E;
     unsigned char* base = lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES;
     double* X = (double*)base;  // mu[5], F(mu) per axis [3][5], residual
@@ -736,7 +738,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         if (tid == 0) {
             double rb = 0.0, cnt = 0.0;
             for (int j = 0; j < EPB; j++)
-                if (blockIdx.x * EPB + j < a.E) {
+                if (blk * EPB + j < a.E) {
                     const double* Xj = (const double*)(lds_raw + (size_t)j * G::ELEM_BYTES);
                     rb = Xj[G::XRES] > rb ? Xj[G::XRES] : rb;
                     cnt += Xj[G::XACT];
@@ -933,10 +935,16 @@ __global__ void __launch_bounds__(256) k_mrestart(const float* __restrict__ d_ol
     if (go && s == 0) ver[(size_t)wb * E + e] = cur;
 }
 
+__global__ void k_fill_u8(unsigned char* p, size_t n, unsigned char v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 // out[0] = min of a[0..n) (a may be null: +inf), out[1] = max of b[0..n),
-// out[2] = sum of c[0..n) (b, c may be null: 0)
-__global__ void k_mreduce(const double* a, const double* b, const double* c, int n, double* out) {
+// out[2] = sum of c[0..n) (b, c may be null: 0); zeroes *counter (if given)
+// for the k_mblocks that follows
+__global__ void k_mreduce(const double* a, const double* b, const double* c, int n, double* out, int* counter) {
     __shared__ double ra[1024], rb[1024], rc[1024];
+    if (counter && threadIdx.x == 0) *counter = 0;
     double x = __builtin_inf(), y = 0.0, z = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         if (a) x = x < a[i] ? x : a[i];
@@ -960,6 +968,32 @@ __global__ void k_mreduce(const double* a, const double* b, const double* c, int
         out[1] = rb[0];
         out[2] = rc[0];
     }
+}
+
+// List of the blocks that hold an element the next sweep updates (its own
+// flag or a trace source's flag set), appended in any order; *counter must
+// be zero on entry. One thread per block.
+template <int N, int DIM>
+__global__ void __launch_bounds__(256) k_mblocks(MArgs a, const unsigned char* __restrict__ flags, int nblocks,
+                                                 int* list, int* counter) {
+    constexpr int EPB = MT<N, DIM>::EPB;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    bool any = false;
+    for (int j = 0; j < EPB && !any; j++) {
+        const int e = b * EPB + j;
+        if (e >= a.E) break;
+        const int ei = e % a.Ne[0], ej = (e / a.Ne[0]) % a.Ne[1], ek = e / (a.Ne[0] * a.Ne[1]);
+        bool act = flags[e] != 0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            size_t el0, el1;
+            face_sources(a, d, ei, ej, ek, (size_t)e, el0, el1);
+            act = act || flags[el0] || flags[el1];
+        }
+        any = act;
+    }
+    if (any) list[atomicAdd(counter, 1)] = b;
 }
 
 // Histogram of the per-element convergence sweep (bin k = k predictor
@@ -1092,7 +1126,7 @@ struct MLayout {
     size_t E = 0, tpe = 0, nt = 0;
     unsigned blocks = 0;
     // byte offsets
-    size_t mu = 0, part = 0, resp = 0, cntp = 0, red2 = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0,
+    size_t mu = 0, part = 0, resp = 0, cntp = 0, red2 = 0, blist = 0, hist = 0, dold = 0, D = 0, trA = 0, trB = 0,
            conv = 0, ver = 0, big = 0, gsc = 0, total = 0;
     size_t tr_axis = 0;  // bytes per axis per trace buffer
 };
@@ -1112,7 +1146,8 @@ MLayout mlayout(size_t E) {
     L.part = o; o = al(o + sizeof(double) * (L.blocks + 8));
     L.resp = o; o = al(o + sizeof(double) * (L.blocks + 8));
     L.cntp = o; o = al(o + sizeof(double) * (L.blocks + 8));
-    L.red2 = o; o = al(o + sizeof(double) * 4);
+    L.red2 = o; o = al(o + sizeof(double) * 8);  // [0..2] reductions, [4] block-list length (int)
+    L.blist = o; o = al(o + sizeof(int) * (L.blocks + 8));
     L.hist = o; o = al(o + sizeof(unsigned long long) * kHistBins);
     L.dold = o; o = al(o + sizeof(float) * NVP * E * G::TPE);
     L.D = o; o = al(o + sizeof(float) * G::T * NVP * E * G::TPE);
@@ -1304,13 +1339,14 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     dim3 grid(L.blocks), block(G::BLOCK);
     const size_t lds = G::LDS_BYTES;
 
-    auto sweep = [&](const MArgs& args) -> int {
+    // nb blocks: every block, or the length of args.blist
+    auto sweep = [&](const MArgs& args, unsigned nb) -> int {
         bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
         if (!args.out_pred || args.U_new)
-            hipLaunchKernelGGL((k_msweep<N, DIM, true>), grid, block, lds, c->stream, args);
+            hipLaunchKernelGGL((k_msweep<N, DIM, true>), dim3(nb), block, lds, c->stream, args);
         else
-            hipLaunchKernelGGL((k_msweep<N, DIM, false>), grid, block, lds, c->stream, args);
+            hipLaunchKernelGGL((k_msweep<N, DIM, false>), dim3(nb), block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_msweep");
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
         return SPDK_OK;
@@ -1321,12 +1357,28 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         SPDK_LAUNCH_CHECK("k_mstart");
         return SPDK_OK;
     };
-    // reductions of the per-block partials; blocking read into red_host[0..2]
-    auto reduce = [&](const double* mn, const double* mx, const double* sm) -> int {
-        hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, mn, mx, sm, (int)L.blocks, red2);
+    // reductions of the nb per-block partials and, given the flags of the
+    // last sweep, the block list of the next one; blocking read into
+    // red_host[0..2] and the list length into nlist
+    int* blist = (int*)(W + L.blist);
+    int* counter = (int*)(red2 + 4);
+    unsigned nlist = 0;
+    auto reduce = [&](const double* mn, const double* mx, const double* sm, unsigned nb,
+                      const unsigned char* flags) -> int {
+        hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(1024), 0, c->stream, mn, mx, sm, (int)nb, red2,
+                           flags ? counter : nullptr);
         SPDK_LAUNCH_CHECK("k_mreduce");
-        SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (flags) {
+            MArgs fa = a;
+            hipLaunchKernelGGL((k_mblocks<N, DIM>), dim3((L.blocks + 255) / 256), dim3(256), 0, c->stream, fa, flags,
+                               (int)L.blocks, blist, counter);
+            SPDK_LAUNCH_CHECK("k_mblocks");
+        }
+        SPDK_CHECK(hipMemcpyAsync(c->red_host, red2, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         SPDK_CHECK(hipStreamSynchronize(c->stream));
+        int n = 0;
+        std::memcpy(&n, c->red_host + 4, sizeof(int));
+        nlist = flags ? (unsigned)n : 0;
         return SPDK_OK;
     };
     // fixed count: p predictor sweeps from the zero-order guess, then the corrector
@@ -1343,7 +1395,7 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
             s.F_x = last ? F_x : nullptr;
             s.sweep = k;
             int r;
-            if ((r = sweep(s))) return r;
+            if ((r = sweep(s, L.blocks))) return r;
         }
         return SPDK_OK;
     };
@@ -1368,6 +1420,8 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         int kph = 0;                           // sweeps in this phase
         const unsigned char* flags = nullptr;  // unconverged after the last sweep (null: all)
         bool capped = false;
+        unsigned nb = L.blocks;                // blocks of the next sweep: all, or the list
+        const int* list = nullptr;
         for (;;) {
             MArgs sa = a;
             sa.cur = ++cur;
@@ -1378,15 +1432,24 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
             sa.sweep = ksw;
             sa.big_in = flags;
             sa.big_out = big[ksw % 2];
-            if ((rc = sweep(sa))) return rc;
-            if ((rc = reduce(nullptr, a.res_part, a.cnt_part))) return rc;
+            sa.blist = list;
+            if (list) {  // elements of blocks left out keep no stale flag
+                hipLaunchKernelGGL(k_fill_u8, dim3(256), dim3(256), 0, c->stream, big[ksw % 2], E, (unsigned char)0);
+                SPDK_LAUNCH_CHECK("k_fill_u8");
+            }
+            if ((rc = sweep(sa, nb))) return rc;
+            if ((rc = reduce(nullptr, a.res_part, a.cnt_part, nb, big[ksw % 2]))) return rc;
             sweeps += c->red_host[2] / (double)E;
             if (st) st->last_residual = c->red_host[1];
             flags = big[ksw % 2];
             ksw++;
             kph++;
-            if (c->red_host[1] < pc->tol) break;
-            if (kph < cap && ksw < 250) continue;
+            if (c->red_host[1] < pc->tol || nlist == 0) break;
+            if (kph < cap && ksw < 250) {
+                list = blist;
+                nb = nlist;
+                continue;
+            }
             if (guess_phase) {
                 // unconverged from the guess: restart those elements from the
                 // zero-order guess with the fixed count
@@ -1396,6 +1459,10 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
                 SPDK_LAUNCH_CHECK("k_mrestart");
                 flags = big[ksw % 2];
                 ksw++;
+                if ((rc = reduce(nullptr, nullptr, nullptr, 0, flags))) return rc;
+                if (nlist == 0) break;
+                list = blist;
+                nb = nlist;
                 guess_phase = false;
                 kph = 0;
                 cap = cap_fixed;
@@ -1418,13 +1485,13 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         sa.W_cv = W_cv;
         sa.F_x = F_x;
         sa.sweep = ksw;
-        if ((rc = sweep(sa))) return rc;
+        if ((rc = sweep(sa, L.blocks))) return rc;
         sweeps += 1.0;
     } else {
         if ((rc = fixed_count())) return rc;
         sweeps += T;
     }
-    if ((rc = reduce(a.part, nullptr, nullptr))) return rc;
+    if ((rc = reduce(a.part, nullptr, nullptr, L.blocks, nullptr))) return rc;
     if (dt_next) *dt_next = c->red_host[0];
     if (st) {
         st->steps++;

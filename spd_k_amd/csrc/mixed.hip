@@ -91,7 +91,7 @@ struct MT {
     static constexpr int PSZ = DIM == 1 ? N : TPE / N * ROW;  // padded state plane
     static constexpr int LDS_D = T * NVP * PSZ;      // floats: state slices
     static constexpr int LDS_F = SLOTS * NVP * FSZ;  // floats: flux slots
-    static constexpr int LDS_X = 6 * NVP;            // doubles: mu, F(mu) per axis, residual, shift
+    static constexpr int LDS_X = 6 * NVP + 2;        // doubles: mu, F(mu) per axis, residual, shift, mean state
     static constexpr int ELEM_BYTES = ((LDS_X * 8 + (LDS_D + LDS_F) * 4) + 15) / 16 * 16;
     static constexpr int EPB0 = TPE >= 256 ? 1 : 256 / TPE;
     static constexpr int EPB_LDS = 61440 / ELEM_BYTES;
@@ -103,6 +103,7 @@ struct MT {
     static constexpr int XRES = 4 * NVP;      // X slot: element residual
     static constexpr int XACT = 4 * NVP + 1;  // X slot: element active this sweep
     static constexpr int XSHIFT = 4 * NVP + 2; // X slots: conservative shift per variable
+    static constexpr int XMF = 4 * NVP + 2 + NVP + 1;  // 4 X slots = 8 floats: rho, v[3], m[3], H of mu
     static_assert(T <= kMaxTm, "time nodes");
 };
 
@@ -169,6 +170,29 @@ __device__ __forceinline__ void mflux(const double* u, double* f, double gm1) {
     f[2] = u[2] * vn;
     f[3] = u[3] * vn;
     f[4] = (u[4] + p) * vn;
+}
+
+// F(mu + d) - F(mu) in FP32 for a deviation d (normal frame, slot 1 the
+// normal momentum), written in differences so that every term is of the
+// order of d: with m = mu_m + d_m, v = m / rho,
+//   dv_i = (d_m_i - v_i(mu) d_rho) / rho,   d(m_i v_j) = m_i(mu) dv_j + d_m_i v_j,
+//   dp = (gamma-1) (d_E - 1/2 sum_i d(m_i v_i)),  d(H v_n) = H(mu) dv_n + (d_E + dp) v_n.
+// The FP32 rounding is therefore relative to the flux variation, as for the
+// FP64 evaluation rounded to FP32 it replaces. c: rho, v_n, v_t1, v_t2,
+// m_n, m_t1, m_t2, H = E + p of the element mean, in the frame.
+__device__ __forceinline__ void mdflux(const float* d, const float* c, float* df, float gm1) {
+    const float ri = __builtin_amdgcn_rcpf(c[0] + d[0]);
+    const float dvn = (d[1] - c[1] * d[0]) * ri;
+    const float dvt1 = (d[2] - c[2] * d[0]) * ri;
+    const float dvt2 = (d[3] - c[3] * d[0]) * ri;
+    const float vn = c[1] + dvn, vt1 = c[2] + dvt1, vt2 = c[3] + dvt2;
+    const float dk = c[4] * dvn + d[1] * vn + c[5] * dvt1 + d[2] * vt1 + c[6] * dvt2 + d[3] * vt2;
+    const float dp = gm1 * (d[4] - 0.5f * dk);
+    df[0] = d[1];
+    df[1] = c[4] * dvn + d[1] * vn + dp;
+    df[2] = c[5] * dvn + d[2] * vn;
+    df[3] = c[6] * dvn + d[3] * vn;
+    df[4] = c[7] * dvn + (d[4] + dp) * vn;
 }
 
 // LLF (src/hydro.cpp:270-289): fluxes and the jump term in FP64; the wave
@@ -349,15 +373,24 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
             uf[f][v] = x;
         }
     }
-    // interior flux points
+    // interior flux points: F(mu + d) - F(mu) in FP32 difference form
+    {
+        const float* XF = (const float*)(X + G::XMF);
+        float cm[8];
+        cm[0] = XF[0];
 #pragma unroll
-    for (int f = 1; f < M - 1; f++) {
-        double u[NVP], fl[NVP];
+        for (int i = 1; i < 4; i++) {
+            cm[i] = XF[pv[i]];
+            cm[3 + i] = XF[3 + pv[i]];
+        }
+        cm[7] = XF[7];
 #pragma unroll
-        for (int v = 0; v < NVP; v++) u[v] = mu[v] + (double)uf[f][v];
-        mflux(u, fl, gm1);
+        for (int f = 1; f < M - 1; f++) {
+            float df[NVP];
+            mdflux(uf[f], cm, df, (float)gm1);
 #pragma unroll
-        for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + g.fb + f * g.fs] = (float)(fl[v] - Fb[v]);
+            for (int v = 0; v < NVP; v++) Fslot[pv[v] * G::FSZ + g.fb + f * g.fs] = df[v];
+        }
     }
     // faces: LLF at every face; at the outer face of a ghost element
     // (no neighbour) both states are the element's own, which gives its
@@ -576,6 +609,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         mflux(um, fb, a.gm - 1.0);
 #pragma unroll
         for (int v = 0; v < NVP; v++) X[NVP + s * NVP + vperm(v, s)] = fb[v];
+    }
+    if (act_e && s == 0) {  // the mean state in FP32 for the difference-form fluxes
+        float* XF = (float*)(X + G::XMF);
+        const double ri = 1.0 / X[0];
+        const double vx = X[1] * ri, vy = X[2] * ri, vz = X[3] * ri;
+        const double pm = (a.gm - 1.0) * (X[4] - 0.5 * (X[1] * vx + X[2] * vy + X[3] * vz));
+        XF[0] = (float)X[0];
+        XF[1] = (float)vx;
+        XF[2] = (float)vy;
+        XF[3] = (float)vz;
+        XF[4] = (float)X[1];
+        XF[5] = (float)X[2];
+        XF[6] = (float)X[3];
+        XF[7] = (float)(X[4] + pm);
     }
     float dold[NVP];
 #pragma unroll

@@ -73,6 +73,7 @@ constexpr int kHistBins = 32;
 constexpr int kGS = 12;          // per-element floats of the learned-guess scales
 constexpr int kGExtra = 6;       // extra features per element
 constexpr float kAmin = 1e-6f;   // floor of the element amplitude
+constexpr float kAgate = 1e-5f;  // amplitude at or below which the guess is the zero-order one
 
 // Minimum waves per SIMD the sweep kernel is compiled for (bounds its VGPRs).
 #ifndef SPDK_MSWEEP_WAVES
@@ -1161,10 +1162,12 @@ __global__ void k_gdecode(const float* __restrict__ Y, long ldy, const float* __
         const float d0 = d_old[(size_t)v * slabE + e * TPE + s];
         float dn = d0;
         const int vi = v == 4 ? DIM + 1 : v;
-        if (v == 4 || v <= DIM) {
-            const float* sc = S + (size_t)kGS * e;
+        const float* sc = S + (size_t)kGS * e;
+        // an element (nearly) uniform inside keeps the zero-order guess: alone
+        // it is its own fixed point, and the network cannot see the
+        // neighbours that would move it
+        if ((v == 4 || v <= DIM) && sc[6] > kAgate)
             dn = d0 + Y[(long)row * ldy + (t * NV + vi) * TPE + s] * (sc[v] * sc[7]);
-        }
         D[((size_t)t * NVP + v) * slabE + e * TPE + s] = dn;
     }
 }

@@ -62,3 +62,52 @@ def test_train_then_guess(tmp_path):
     for d in (zero, learned):
         assert _rel(ref, d, i, 0) < 1e-6
     assert op.mass_drift(learned) < 1e-13
+
+
+def _hist(text):
+    m = re.search(r"picard histogram \(elements by predictor sweeps to tolerance\):(.*)", text)
+    return m.group(1).strip() if m else ""
+
+
+def test_kh2d_p4_learned_histogram(tmp_path):
+    """The learned-guess configuration: 2-D KH, 512^2 elements, p=4. The
+    network 106 -> 256 -> 256 -> 500 (the element's 5x5 points x 4 conserved
+    deviations plus 6 per-element features in; 5 time nodes x 100 values out)
+    is trained on the run's own element fixed points (8 snapshots of 262,144
+    elements), then Picard runs to the element residual 1e-10 from the
+    zero-order and from the learned guess (cap: the fixed count). Prints the
+    sweep histograms and each form's distance to the plain FP64 run; the FP32
+    deviations floor the residual near 1e-8, so at 1e-10 every element that
+    moves ends at the cap, i.e. the fixed-count result."""
+    # CFL 0.5: at the input's 0.8, p=4 ADER without the FV fallback goes
+    # non-finite within ~12 steps from the grid-scale noise (FP64 and mixed
+    # alike; the fallback is what keeps the reference's KH input stable)
+    base = ["-i", KH, "job/fallback=false", "mesh/nx1=512", "mesh/nx2=512", "mesh/nx3=1", "mesh/p=4",
+            "problem/perturb=random", "problem/amp=0.01", "time/cfl=0.5"]
+    w = str(tmp_path / "kh2d_p4.mlp")
+    out = op.run_driver(base + ["time/tlim=1", "output/dt=-1", "job/mode=train_guess", f"mlp/out={w}",
+                                "mlp/snapshots=9", "mlp/every=4", "mlp/sweeps=20", "mlp/epochs=4", "mlp/seed=5"],
+                        str(tmp_path / "train"))
+    print(out[-700:])
+    assert re.search(r"in 106 hidden 256 out 500", out), out[-700:]
+    before = float(re.search(r"held-out loss before training ([0-9.eE+-]+)", out).group(1))
+    after = float(re.findall(r"held-out loss ([0-9.eE+-]+), last snapshot", out)[-1])
+    assert after < before, (before, after)
+
+    run = base + ["time/tlim=0.002", "output/dt=0.002"]
+    ref = str(tmp_path / "fp64")
+    op.run_driver(run, ref)
+    i = op.output_indices(ref)[-1]
+    res = ["job/precision=mixed", "job/picard=residual"]
+    forms = {"zero 1e-10": res + ["job/picard_tol=1e-10"],
+             "learned 1e-10": res + ["job/picard_tol=1e-10", "job/guess=mlp", f"job/mlp_weights={w}"],
+             "zero 1e-6": res + ["job/picard_tol=1e-6"]}
+    for name, extra in forms.items():
+        d = str(tmp_path / name.replace(" ", "_"))
+        text = op.run_driver(run + extra, d)
+        sweeps, ms = _picard(text)
+        rel = _rel(ref, d, i, 0)
+        print(f"{name}: {sweeps:.3f} sweeps/step, {ms:.3f} ms/step, density L1 vs FP64 {rel:.3e}; "
+              f"histogram {_hist(text)}")
+        assert rel < 1e-6, (name, rel)
+        assert op.mass_drift(d) < 1e-13

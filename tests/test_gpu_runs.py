@@ -90,6 +90,27 @@ def test_parity_with_oracle(tmp_path, name):
     assert_bit_exact(a, b)
 
 
+# The 2-D fallback configuration at its full size (256^2, p=3, NAD + SED +
+# MUSCL-Hancock/HLLC blending): a few steps, every output file (troubled-cell
+# masks included) bit for bit against the oracle.
+CFG2 = {
+    "sod256": ["-i", os.path.join(op.INPUTS, "sod.athinput"), "mesh/nx1=256", "mesh/nx2=256", "mesh/x2_bc=periodic",
+               "time/tlim=0.0007", "output/dt=0.00035"],
+    "sedov256": ["-i", S, "problem/problem=sedov_cells", "job/fallback=true", "mesh/nx1=256", "mesh/nx2=256",
+                 "mesh/nx3=1", "hydro/gamma=1.4", "time/tlim=0.000025", "output/dt=0.0000125"],
+}
+
+
+@pytest.mark.parametrize("name", sorted(CFG2))
+def test_fallback_full_size_masks(tmp_path, name):
+    a, b = both(tmp_path, CFG2[name], "stages")
+    assert_bit_exact(a, b)
+    g = op.grid(a)
+    flagged = [np.fromfile(os.path.join(b, f"troubles_N{4 * g['N'][0] + 4}_{i}_0.dat")).max()
+               for i in op.output_indices(a)]
+    assert max(flagged) > 0  # the detector fired
+
+
 def test_fused_equals_stages_3d(tmp_path):
     args = ["-i", KH, "job/fallback=false", "mesh/nx1=12", "mesh/nx2=12", "mesh/nx3=12", "problem/perturb=random",
             "problem/amp=0.01", "time/tlim=1", "time/nlim=4", "output/dt=-1"]

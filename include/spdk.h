@@ -179,6 +179,11 @@ int spdk_muscl_hllc_blend(spdk_ctx* ctx, const double* W, const double* theta, d
                           spdk_fv_shape s, double dt_sub);
 
 /* ---- induction stage kernels (staggered face B, edge E) -------------------- */
+/* The reference's induction solver is 3-D only (src/induction_ader.hpp:106-110).
+ * These entries also run with z inactive (one point per element along z, no
+ * ghosts): a derivative along an inactive axis is zero and an interpolation
+ * along it takes the value as is; with all axes active the arithmetic is the
+ * reference's. */
 /* rotational_a_to_b (src/induction.cpp:94-137): B_dim = d1 A2 / d1 - d2 A1 / d2 */
 int spdk_B_curl(spdk_ctx* ctx, const double* A1, spdk_sd_shape s1, const double* A2, spdk_sd_shape s2, double* B,
                 spdk_sd_shape sb, double d1, double d2, int dim);
@@ -198,6 +203,13 @@ int spdk_B_correct(spdk_ctx* ctx, double* B, spdk_sd_shape sb, const double* E1,
 /* compute_B2_cv (src/induction.cpp:422-465): cv averages of B and |B|^2 */
 int spdk_B2_cv(spdk_ctx* ctx, double* B2, spdk_sd_shape s, const double* Bx, spdk_sd_shape sx, const double* By,
                spdk_sd_shape sy, const double* Bz, spdk_sd_shape sz);
+/* No reference counterpart (the reference writes B2_cv only): max |div B| at
+ * the solution points of the active elements, each face component
+ * differentiated fp->sp along its own axis, and the magnetic energy, the sum
+ * of |B|^2 (B2 variable 3, from spdk_B2_cv) x cv volume over active elements.
+ * Blocks until out[0] = max |div B| and out[1] = energy are on the host. */
+int spdk_induction_diagnostics(spdk_ctx* ctx, const double* Bx, spdk_sd_shape sx, const double* By, spdk_sd_shape sy,
+                               const double* Bz, spdk_sd_shape sz, const double* B2, spdk_sd_shape s2, double* out);
 /* compute_B_cv_from_cf (src/induction.cpp:467-500): FV cell B from face averages */
 int spdk_B_cv_from_cf(spdk_ctx* ctx, double* B, spdk_fv_shape sf, const double* Bx, spdk_sd_shape sx, const double* By,
                       spdk_sd_shape sy, const double* Bz, spdk_sd_shape sz);

@@ -1183,49 +1183,71 @@ int run_hydro(Config& cfg, const char* outdir, double* evol_seconds, int* steps,
 // induction path are computed but never feed the solution; their values are
 // parity unpinned.
 
-// src/induction.cpp:94-137 (B = curl A at face points)
-static void rotational_a_to_b(const SDA& A1, const SDA& A2, SDA& B, const Mat& Dm, double d1, double d2, int dim) {
-    int q = choose(dim, B.nx, B.ny, B.nz);
+// Axis `a` of (kk, jj, ii) replaced by l.
+static inline void along(int a, int l, int& kk, int& jj, int& ii) {
+    if (a == 0) ii = l;
+    else if (a == 1) jj = l;
+    else kk = l;
+}
+static inline int next_axis(int d) { return d == 2 ? 0 : d + 1; }
+
+// src/induction.cpp:94-137 (B = curl A at face points):
+// B_dim = d_{dim1} A2 / d1 - d_{dim2} A1 / d2 with dim1 = dim+1, dim2 = dim+2
+// (mod 3). Each derivative sums over the flux points of its own axis (the
+// reference's q, B's count along dim, equals it in 3-D). Extension for 2-D
+// runs: a derivative along an inactive axis is zero.
+static void rotational_a_to_b(const Config& cfg, const SDA& A1, const SDA& A2, SDA& B, const Mat& Dm, double d1, double d2,
+                              int dim) {
+    int a1 = next_axis(dim), a2 = next_axis(a1);
+    int q1 = cfg.act[a1] ? cfg.p + 2 : 0, q2 = cfg.act[a2] ? cfg.p + 2 : 0;
     for6(B.Nz, B.Ny, B.Nx, B.nz, B.ny, B.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        int r1 = choose(a1, ii, jj, kk), r2 = choose(a2, ii, jj, kk);
         for (int t = 0; t < B.nt; t++) {
             double d1a2 = 0, d2a1 = 0;
-            for (int ll = 0; ll < q; ll++) {
-                if (dim == 0) {
-                    d1a2 += A2(t, 0, k, j, i, kk, ll, ii) * Dm(jj, ll);
-                    d2a1 += A1(t, 0, k, j, i, ll, jj, ii) * Dm(kk, ll);
-                } else if (dim == 1) {
-                    d1a2 += A2(t, 0, k, j, i, ll, jj, ii) * Dm(kk, ll);
-                    d2a1 += A1(t, 0, k, j, i, kk, jj, ll) * Dm(ii, ll);
-                } else {
-                    d1a2 += A2(t, 0, k, j, i, kk, jj, ll) * Dm(ii, ll);
-                    d2a1 += A1(t, 0, k, j, i, kk, ll, ii) * Dm(jj, ll);
-                }
+            for (int ll = 0; ll < q1; ll++) {
+                int c = kk, b = jj, a = ii;
+                along(a1, ll, c, b, a);
+                d1a2 += A2(t, 0, k, j, i, c, b, a) * Dm(r1, ll);
+            }
+            for (int ll = 0; ll < q2; ll++) {
+                int c = kk, b = jj, a = ii;
+                along(a2, ll, c, b, a);
+                d2a1 += A1(t, 0, k, j, i, c, b, a) * Dm(r2, ll);
             }
             B(t, 0, k, j, i, kk, jj, ii) = d1a2 / d1 - d2a1 / d2;
         }
     });
 }
 
-// src/induction.cpp:139-208: edge E = v1*b2 - v2*b1 with B interpolated
-// sp->fp transversely; stores [E, b1, b2, v1, v2, 0, b1, b2].
+// src/induction.cpp:139-208: edge E = v1*b2 - v2*b1 with b1 = B1
+// interpolated sp->fp along dim2 and b2 = B2 along dim1, each over the
+// solution points of that axis; stores [E, b1, b2, v1, v2, 0, b1, b2].
+// Extension for 2-D runs: along an inactive axis the value is taken as is.
 static void compute_E(const Config& cfg, SDA& E, const SDA& B1, const SDA& B2, const Mat& s2f, int dim) {
-    int q = choose(dim, E.nx, E.ny, E.nz);
-    int dim1 = choose(dim, 1, 2, 0), dim2 = choose(dim1, 1, 2, 0);
+    int dim1 = next_axis(dim), dim2 = next_axis(dim1);
     double v1 = cfg.vel[dim1], v2 = cfg.vel[dim2];
+    int nsp = cfg.p + 1;
     for6(E.Nz, E.Ny, E.Nx, E.nz, E.ny, E.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        int r1 = choose(dim1, ii, jj, kk), r2 = choose(dim2, ii, jj, kk);
         for (int t = 0; t < E.nt; t++) {
             double b1 = 0, b2 = 0;
-            for (int ll = 0; ll < q; ll++) {
-                if (dim == 2) {
-                    b1 += B1(t, 0, k, j, i, kk, ll, ii) * s2f(jj, ll);
-                    b2 += B2(t, 0, k, j, i, kk, jj, ll) * s2f(ii, ll);
-                } else if (dim == 1) {
-                    b1 += B1(t, 0, k, j, i, kk, jj, ll) * s2f(ii, ll);
-                    b2 += B2(t, 0, k, j, i, ll, jj, ii) * s2f(kk, ll);
-                } else {
-                    b1 += B1(t, 0, k, j, i, ll, jj, ii) * s2f(kk, ll);
-                    b2 += B2(t, 0, k, j, i, kk, ll, ii) * s2f(jj, ll);
+            if (cfg.act[dim2]) {
+                for (int ll = 0; ll < nsp; ll++) {
+                    int c = kk, b = jj, a = ii;
+                    along(dim2, ll, c, b, a);
+                    b1 += B1(t, 0, k, j, i, c, b, a) * s2f(r2, ll);
                 }
+            } else {
+                b1 = B1(t, 0, k, j, i, kk, jj, ii);
+            }
+            if (cfg.act[dim1]) {
+                for (int ll = 0; ll < nsp; ll++) {
+                    int c = kk, b = jj, a = ii;
+                    along(dim1, ll, c, b, a);
+                    b2 += B2(t, 0, k, j, i, c, b, a) * s2f(r1, ll);
+                }
+            } else {
+                b2 = B2(t, 0, k, j, i, kk, jj, ii);
             }
             E(t, 0, k, j, i, kk, jj, ii) = v1 * b2 - v2 * b1;
             E(t, 1, k, j, i, kk, jj, ii) = b1;
@@ -1273,38 +1295,39 @@ static void E_riemann(SDA& E, int dim, int v1) {
 }
 
 // dB/dt term of one face-B component: src/induction.cpp:303-365 (dBdt of
-// update_B_prediction) and :367-420 (update_B_solution).
-static inline double dBdt_at(const SDA& E1, const SDA& E2, const Mat& Dm, double d1, double d2, int dim, int t, int k, int j,
-                             int i, int kk, int jj, int ii, int q) {
+// update_B_prediction) and :367-420 (update_B_solution):
+// + d_{dim2} E1 / d2 - d_{dim1} E2 / d1, interleaved per flux point as the
+// reference accumulates them. Extension for 2-D runs: a derivative along an
+// inactive axis is zero.
+static inline double dBdt_at(const Config& cfg, const SDA& E1, const SDA& E2, const Mat& Dm, double d1, double d2, int dim,
+                             int t, int k, int j, int i, int kk, int jj, int ii) {
+    int a1 = next_axis(dim), a2 = next_axis(a1);
+    int r1 = choose(a1, ii, jj, kk), r2 = choose(a2, ii, jj, kk);
     double dBdt = 0, E;
-    for (int ll = 0; ll < q; ll++) {
-        if (dim == 0) {
-            E = E1(t, 0, k, j, i, ll, jj, ii);
-            dBdt += E * Dm(kk, ll) / d2;
-            E = E2(t, 0, k, j, i, kk, ll, ii);
-            dBdt -= E * Dm(jj, ll) / d1;
-        } else if (dim == 1) {
-            E = E1(t, 0, k, j, i, kk, jj, ll);
-            dBdt += E * Dm(ii, ll) / d2;
-            E = E2(t, 0, k, j, i, ll, jj, ii);
-            dBdt -= E * Dm(kk, ll) / d1;
-        } else {
-            E = E1(t, 0, k, j, i, kk, ll, ii);
-            dBdt += E * Dm(jj, ll) / d2;
-            E = E2(t, 0, k, j, i, kk, jj, ll);
-            dBdt -= E * Dm(ii, ll) / d1;
+    for (int ll = 0; ll < cfg.p + 2; ll++) {
+        if (cfg.act[a2]) {
+            int c = kk, b = jj, a = ii;
+            along(a2, ll, c, b, a);
+            E = E1(t, 0, k, j, i, c, b, a);
+            dBdt += E * Dm(r2, ll) / d2;
+        }
+        if (cfg.act[a1]) {
+            int c = kk, b = jj, a = ii;
+            along(a1, ll, c, b, a);
+            E = E2(t, 0, k, j, i, c, b, a);
+            dBdt -= E * Dm(r1, ll) / d1;
         }
     }
     return dBdt;
 }
 static void update_B_prediction(const Config& cfg, const SDA& B, SDA& Ba, const SDA& E1, const SDA& E2, const Mat& Dm,
                                 const Mat& inv, const std::vector<double>& w, double d1, double d2, double dt, int dim) {
-    int q = choose(dim, B.nx, B.ny, B.nz), nader = Ba.nt;
+    int nader = Ba.nt;
     for6(B.Nz - 2 * cfg.ngh[2], B.Ny - 2 * cfg.ngh[1], B.Nx - 2 * cfg.ngh[0], B.nz, B.ny, B.nx, cfg.ngh[2], cfg.ngh[1],
          cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
              double dBdt[10];
              double B_old = B(0, 0, k, j, i, kk, jj, ii);
-             for (int t = 0; t < nader; t++) dBdt[t] = dBdt_at(E1, E2, Dm, d1, d2, dim, t, k, j, i, kk, jj, ii, q);
+             for (int t = 0; t < nader; t++) dBdt[t] = dBdt_at(cfg, E1, E2, Dm, d1, d2, dim, t, k, j, i, kk, jj, ii);
              for (int t = 0; t < nader; t++) {
                  double dB = 0;
                  for (int ll = 0; ll < nader; ll++) dB += dBdt[ll] * inv(t, ll) * w[ll] * dt;
@@ -1314,29 +1337,33 @@ static void update_B_prediction(const Config& cfg, const SDA& B, SDA& Ba, const 
 }
 static void update_B_solution(const Config& cfg, SDA& B, const SDA& E1, const SDA& E2, const Mat& Dm, const std::vector<double>& w,
                               double d1, double d2, double dt, int dim) {
-    int q = choose(dim, B.nx, B.ny, B.nz), nader = E1.nt;
+    int nader = E1.nt;
     for6(B.Nz - 2 * cfg.ngh[2], B.Ny - 2 * cfg.ngh[1], B.Nx - 2 * cfg.ngh[0], B.nz, B.ny, B.nx, cfg.ngh[2], cfg.ngh[1],
          cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
              double dB = 0;
-             for (int t = 0; t < nader; t++) dB += dBdt_at(E1, E2, Dm, d1, d2, dim, t, k, j, i, kk, jj, ii, q) * w[t] * dt;
+             for (int t = 0; t < nader; t++) dB += dBdt_at(cfg, E1, E2, Dm, d1, d2, dim, t, k, j, i, kk, jj, ii) * w[t] * dt;
              B(0, 0, k, j, i, kk, jj, ii) -= dB;
          });
 }
 
-// src/induction.cpp:422-465
-static void compute_B2_cv(SDA& B2, const SDA& Bx, const SDA& By, const SDA& Bz, const Mat& f2c, const Mat& s2c) {
-    int q = Bx.nx, px = B2.nx, py = B2.ny, pz = B2.nz;
+// src/induction.cpp:422-465. Extension for 2-D runs: along an inactive
+// axis the single point is taken with weight 1.
+static void compute_B2_cv(const Config& cfg, SDA& B2, const SDA& Bx, const SDA& By, const SDA& Bz, const Mat& f2c,
+                          const Mat& s2c) {
+    int px = B2.nx, py = B2.ny, pz = B2.nz, qx = Bx.nx, qy = By.ny, qz = Bz.nz;
+    auto sc = [&](int d, int r, int c) { return cfg.act[d] ? s2c(r, c) : 1.0; };
+    auto fc = [&](int d, int r, int c) { return cfg.act[d] ? f2c(r, c) : 1.0; };
     for6(B2.Nz, B2.Ny, B2.Nx, B2.nz, B2.ny, B2.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
         double bx = 0, by = 0, bz = 0;
         for (int nn = 0; nn < pz; nn++)
             for (int mm = 0; mm < py; mm++)
-                for (int ll = 0; ll < q; ll++) bx += Bx(0, 0, k, j, i, nn, mm, ll) * s2c(kk, nn) * s2c(jj, mm) * f2c(ii, ll);
+                for (int ll = 0; ll < qx; ll++) bx += Bx(0, 0, k, j, i, nn, mm, ll) * sc(2, kk, nn) * sc(1, jj, mm) * fc(0, ii, ll);
         for (int nn = 0; nn < pz; nn++)
-            for (int mm = 0; mm < q; mm++)
-                for (int ll = 0; ll < px; ll++) by += By(0, 0, k, j, i, nn, mm, ll) * s2c(kk, nn) * f2c(jj, mm) * s2c(ii, ll);
-        for (int nn = 0; nn < q; nn++)
+            for (int mm = 0; mm < qy; mm++)
+                for (int ll = 0; ll < px; ll++) by += By(0, 0, k, j, i, nn, mm, ll) * sc(2, kk, nn) * fc(1, jj, mm) * sc(0, ii, ll);
+        for (int nn = 0; nn < qz; nn++)
             for (int mm = 0; mm < py; mm++)
-                for (int ll = 0; ll < px; ll++) bz += Bz(0, 0, k, j, i, nn, mm, ll) * f2c(kk, nn) * s2c(jj, mm) * s2c(ii, ll);
+                for (int ll = 0; ll < px; ll++) bz += Bz(0, 0, k, j, i, nn, mm, ll) * fc(2, kk, nn) * sc(1, jj, mm) * sc(0, ii, ll);
         B2(0, 0, k, j, i, kk, jj, ii) = bx;
         B2(0, 1, k, j, i, kk, jj, ii) = by;
         B2(0, 2, k, j, i, kk, jj, ii) = bz;
@@ -1510,9 +1537,9 @@ struct Induction {
     void setup() {
         int p = cfg.p;
         op.build(p, cfg.integrator == INT_ADER);
-        X.init(0, cfg.Nglob[0], cfg.Nglob[0], p, 0, cfg.L[0], op.x_fp.data(), true);
-        Y.init(1, cfg.Nglob[1], cfg.Nglob[1], p, 0, cfg.L[1], op.x_fp.data(), true);
-        Z.init(2, cfg.Nglob[2], cfg.Nglob[2], p, 0, cfg.L[2], op.x_fp.data(), true);
+        X.init(0, cfg.Nglob[0], cfg.Nglob[0], p, 0, cfg.L[0], op.x_fp.data(), cfg.act[0]);
+        Y.init(1, cfg.Nglob[1], cfg.Nglob[1], p, 0, cfg.L[1], op.x_fp.data(), cfg.act[1]);
+        Z.init(2, cfg.Nglob[2], cfg.Nglob[2], p, 0, cfg.L[2], op.x_fp.data(), cfg.act[2]);
         if (cfg.outputs) write_dims(outdir, X, Y, Z);
         if (cfg.integrator == INT_RK) {
             n_ader = 1;
@@ -1536,9 +1563,9 @@ struct Induction {
         Initialize_ep(cfg, Aep[0], X.sdc, Y.sdf, Z.sdf, 0);
         Initialize_ep(cfg, Aep[1], X.sdf, Y.sdc, Z.sdf, 1);
         Initialize_ep(cfg, Aep[2], X.sdf, Y.sdf, Z.sdc, 2);
-        rotational_a_to_b(Aep[1], Aep[2], Bf[0], op.dfp_to_sp, Y.h, Z.h, 0);
-        rotational_a_to_b(Aep[2], Aep[0], Bf[1], op.dfp_to_sp, Z.h, X.h, 1);
-        rotational_a_to_b(Aep[0], Aep[1], Bf[2], op.dfp_to_sp, X.h, Y.h, 2);
+        rotational_a_to_b(cfg, Aep[1], Aep[2], Bf[0], op.dfp_to_sp, Y.h, Z.h, 0);
+        rotational_a_to_b(cfg, Aep[2], Aep[0], Bf[1], op.dfp_to_sp, Z.h, X.h, 1);
+        rotational_a_to_b(cfg, Aep[0], Aep[1], Bf[2], op.dfp_to_sp, X.h, Y.h, 2);
         if (cfg.fallback) {
             const char* cn[3] = {"Bx", "By", "Bz"};
             for (int d = 0; d < 3; d++) {
@@ -1553,38 +1580,90 @@ struct Induction {
             B_new.init("B_new", 4, Z, Y, X, 0, 0, 0);
             troubles.init("troubles", 4, Z, Y, X, 0, 0, 0);
         }
-        // src/induction.cpp:52-92 and induction_ader.hpp:254-268
-        double vmax = std::fabs(cfg.vel[0]) + std::fabs(cfg.vel[1]) + std::fabs(cfg.vel[2]);
-        double dx_min = smin(smin(smin(1.0, X.h), Y.h), Z.h);
+        // src/induction.cpp:52-92 and induction_ader.hpp:254-268 (sums and
+        // minima over the active axes; all three in the reference's 3-D)
+        const Dim* ax[3] = {&X, &Y, &Z};
+        double vmax = 0, dx_min = 1;
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) {
+                vmax += std::fabs(cfg.vel[d]);
+                dx_min = smin(dx_min, ax[d]->h);
+            }
         dt = cfg.cfl * dx_min / vmax / B2_cv.nx;
         if (cfg.eta > 0.0) Dt = 0.5 * cfg.cfl * (X.h * X.h / cfg.eta) / (cfg.ndim) / (cfg.ndim) / (p + 1) / (p + 1);
         else Dt = 1E6;
         Dt = smin(dt, Dt);
         if (cfg.outputs) write_outputs();
+        else diagnostics();
     }
 
     void write_outputs() {
-        compute_B2_cv(B2_cv, Bf[0], Bf[1], Bf[2], op.fp_to_cv, op.sp_to_cv);
+        diagnostics();
         if (cfg.fallback) write_fv(outdir, troubles, n_output);
         write_sd(outdir, B2_cv, n_output++);
+    }
+    // Not in the reference: max |div B| over the solution points of the
+    // active elements (the SD divergence of the face field, each component
+    // differentiated fp->sp along its own axis, terms added x, y, z) and the
+    // magnetic energy sum over active elements and cv cells of |B|^2 * volume
+    // (volumes: products of the FV sub-cell widths along the active axes).
+    // (also at the start and end of a run without outputs)
+    void diagnostics() {
+        compute_B2_cv(cfg, B2_cv, Bf[0], Bf[1], Bf[2], op.fp_to_cv, op.sp_to_cv);
+        const Dim* ax[3] = {&X, &Y, &Z};
+        double divmax = 0, energy = 0;
+        const SDA& S = B2_cv;
+        for (int k = cfg.ngh[2]; k < S.Nz - cfg.ngh[2]; k++)
+            for (int j = cfg.ngh[1]; j < S.Ny - cfg.ngh[1]; j++)
+                for (int i = cfg.ngh[0]; i < S.Nx - cfg.ngh[0]; i++) {
+                    double esum = 0;
+                    for (int kk = 0; kk < S.nz; kk++)
+                        for (int jj = 0; jj < S.ny; jj++)
+                            for (int ii = 0; ii < S.nx; ii++) {
+                                double div = 0;
+                                for (int d = 0; d < 3; d++) {
+                                    if (!cfg.act[d]) continue;
+                                    double s = 0;
+                                    int r = choose(d, ii, jj, kk);
+                                    for (int l = 0; l < cfg.p + 2; l++) {
+                                        int c = kk, b = jj, a = ii;
+                                        along(d, l, c, b, a);
+                                        s += Bf[d](0, 0, k, j, i, c, b, a) * op.dfp_to_sp(r, l);
+                                    }
+                                    div += s / ax[d]->h;
+                                }
+                                divmax = smax(divmax, std::fabs(div));
+                                double vol = 1;
+                                const int el[3] = {i, j, k}, pt[3] = {ii, jj, kk};
+                                for (int d = 0; d < 3; d++) {
+                                    if (!cfg.act[d]) continue;
+                                    int I = cfg.fgh[d] + (el[d] - cfg.ngh[d]) * ax[d]->nsp + pt[d];
+                                    vol *= ax[d]->fvf[I + 1] - ax[d]->fvf[I];
+                                }
+                                esum += S(0, 3, k, j, i, kk, jj, ii) * vol;
+                            }
+                    energy += esum;
+                }
+        std::printf("diagnostics: t = %.17g max|div B| = %.17g magnetic energy = %.17g\n", t, divmax, energy);
     }
 
     void solve_E() {  // induction_ader.hpp:332-341, :386-448
         compute_E(cfg, Eep[2], Ba[0], Ba[1], op.sp_to_fp, 2);
         compute_E(cfg, Eep[1], Ba[2], Ba[0], op.sp_to_fp, 1);
         compute_E(cfg, Eep[0], Ba[1], Ba[2], op.sp_to_fp, 0);
-        sd_boundaries(cfg, Eep[1], 0);
-        sd_boundaries(cfg, Eep[2], 0);
-        sd_boundaries(cfg, Eep[0], 1);
-        sd_boundaries(cfg, Eep[2], 1);
-        sd_boundaries(cfg, Eep[0], 2);
-        sd_boundaries(cfg, Eep[1], 2);
-        E_riemann(Eep[1], 0, 4);
-        E_riemann(Eep[2], 0, 3);
-        E_riemann(Eep[2], 1, 4);
-        E_riemann(Eep[0], 1, 3);
-        E_riemann(Eep[0], 2, 4);
-        E_riemann(Eep[1], 2, 3);
+        // trace fills and face solves along the active axes only
+        static const int fill[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) {
+                sd_boundaries(cfg, Eep[fill[d][0]], d);
+                sd_boundaries(cfg, Eep[fill[d][1]], d);
+            }
+        // along axis d: E_{d+1} upwinded by variable 4, E_{d+2} by variable 3
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) {
+                E_riemann(Eep[next_axis(d)], d, 4);
+                E_riemann(Eep[next_axis(next_axis(d))], d, 3);
+            }
     }
     void update_prediction() {
         update_B_prediction(cfg, Bf[0], Ba[0], Eep[1], Eep[2], op.dfp_to_sp, op.invader, op.wt, Y.h, Z.h, dt, 0);
@@ -1659,13 +1738,21 @@ struct Induction {
             }
         }
         *steps = n_step - step0;
-        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+        if (!cfg.outputs) diagnostics();
+        return sec;
     }
 };
 
 int run_induction(Config& cfg, const char* outdir, double* evol_seconds, int* steps, long long* dofs) {
-    if (cfg.ndim != 3) {
-        std::fprintf(stderr, "oracle: the induction solver requires 3 active dimensions\n");
+    // the reference requires 3 active dimensions; runs with x and y active
+    // and z inactive are this build's extension (SD update only)
+    if (!(cfg.ndim == 3 || (cfg.ndim == 2 && !cfg.act[2]))) {
+        std::fprintf(stderr, "oracle: the induction solver requires 3 active dimensions, or x and y active\n");
+        return 1;
+    }
+    if (cfg.ndim == 2 && cfg.fallback) {
+        std::fprintf(stderr, "oracle: the induction FV update requires 3 active dimensions\n");
         return 1;
     }
     Induction s(cfg, outdir);

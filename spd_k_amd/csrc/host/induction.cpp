@@ -1,6 +1,7 @@
 // Induction integrator: the counterpart of src/induction_ader.hpp (3-D,
 // staggered face B, edge-centred E, ADER or SSP-RK, optional FV update with
-// trouble detection). Kernels run through the C ABI.
+// trouble detection), extended to runs with z inactive (the SD update of the
+// 2-D field loop). Kernels run through the C ABI.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -62,9 +63,9 @@ struct InductionRun {
     void setup() {  // src/induction_ader.hpp:94-272
         int p = cfg.p;
         op.build(p, cfg.integrator == INT_ADER);
-        X.build(0, cfg.nx[0], cfg.nx[0], p, 0, cfg.len[0], op.x_fp.data(), true);
-        Y.build(1, cfg.nx[1], cfg.nx[1], p, 0, cfg.len[1], op.x_fp.data(), true);
-        Z.build(2, cfg.nx[2], cfg.nx[2], p, 0, cfg.len[2], op.x_fp.data(), true);
+        X.build(0, cfg.nx[0], cfg.nx[0], p, 0, cfg.len[0], op.x_fp.data(), cfg.act[0]);
+        Y.build(1, cfg.nx[1], cfg.nx[1], p, 0, cfg.len[1], op.x_fp.data(), cfg.act[1]);
+        Z.build(2, cfg.nx[2], cfg.nx[2], p, 0, cfg.len[2], op.x_fp.data(), cfg.act[2]);
         if (cfg.outputs) write_dims(outdir, X, Y, Z);
         T = op.T;
         if (cfg.integrator == INT_RK) {
@@ -128,24 +129,36 @@ struct InductionRun {
                 alpha[d] = zeros(fv_count(sC));
             }
         }
-        // src/induction.cpp:52-92 and src/induction_ader.hpp:254-268
-        double vmax = std::fabs(cfg.vel[0]) + std::fabs(cfg.vel[1]) + std::fabs(cfg.vel[2]);
-        double dx_min = 1;
-        dx_min = std::min(dx_min, X.h);
-        dx_min = std::min(dx_min, Y.h);
-        dx_min = std::min(dx_min, Z.h);
+        // src/induction.cpp:52-92 and src/induction_ader.hpp:254-268 (sums and
+        // minima over the active axes; all three in the reference's 3-D)
+        double vmax = 0, dx_min = 1;
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) {
+                vmax += std::fabs(cfg.vel[d]);
+                dx_min = std::min(dx_min, ax[d]->h);
+            }
         dt = cfg.cfl * dx_min / vmax / sB2.nx;
         if (cfg.eta > 0.0) Dt = 0.5 * cfg.cfl * (X.h * X.h / cfg.eta) / (cfg.ndim) / (cfg.ndim) / (p + 1) / (p + 1);
         else Dt = 1E6;
         std::printf("dx = %.17g dt_nu = %.17g dt_v = %.17g\n", X.h, Dt, dt);
         Dt = dt < Dt ? dt : Dt;
         if (cfg.outputs) write_outputs();
+        else diagnostics();
+    }
+
+    // B2_cv, then max |div B| and the magnetic energy (not in the reference),
+    // printed; also at the start and end of a run without outputs
+    void diagnostics() {
+        CK(spdk_B2_cv(ctx, B2, sB2, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2]));
+        double diag[2];
+        CK(spdk_induction_diagnostics(ctx, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2], B2, sB2, diag));
+        std::printf("diagnostics: t = %.17g max|div B| = %.17g magnetic energy = %.17g\n", t, diag[0], diag[1]);
     }
 
     void write_outputs() {  // src/induction_ader.hpp:487-494
         std::string f = output_folder(outdir);
         std::printf("\nOUTPUT %d\n", n_output);
-        CK(spdk_B2_cv(ctx, B2, sB2, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2]));
+        diagnostics();
         if (cfg.fallback) {
             host.resize(fv_count(sC));
             CK(spdk_memcpy_d2h(ctx, host.data(), troubles, host.size() * sizeof(double)));
@@ -161,18 +174,20 @@ struct InductionRun {
         CK(spdk_induction_E(ctx, Eep[2], sE[2], Ba[0], sBa[0], Ba[1], sBa[1], 2));
         CK(spdk_induction_E(ctx, Eep[1], sE[1], Ba[2], sBa[2], Ba[0], sBa[0], 1));
         CK(spdk_induction_E(ctx, Eep[0], sE[0], Ba[1], sBa[1], Ba[2], sBa[2], 0));
-        CK(spdk_fill_sd_traces(ctx, Eep[1], sE[1], 0));
-        CK(spdk_fill_sd_traces(ctx, Eep[2], sE[2], 0));
-        CK(spdk_fill_sd_traces(ctx, Eep[0], sE[0], 1));
-        CK(spdk_fill_sd_traces(ctx, Eep[2], sE[2], 1));
-        CK(spdk_fill_sd_traces(ctx, Eep[0], sE[0], 2));
-        CK(spdk_fill_sd_traces(ctx, Eep[1], sE[1], 2));
-        CK(spdk_E_riemann(ctx, Eep[1], sE[1], 0, 4));
-        CK(spdk_E_riemann(ctx, Eep[2], sE[2], 0, 3));
-        CK(spdk_E_riemann(ctx, Eep[2], sE[2], 1, 4));
-        CK(spdk_E_riemann(ctx, Eep[0], sE[0], 1, 3));
-        CK(spdk_E_riemann(ctx, Eep[0], sE[0], 2, 4));
-        CK(spdk_E_riemann(ctx, Eep[1], sE[1], 2, 3));
+        // trace fills and face solves along the active axes only
+        static const int fill[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) {
+                CK(spdk_fill_sd_traces(ctx, Eep[fill[d][0]], sE[fill[d][0]], d));
+                CK(spdk_fill_sd_traces(ctx, Eep[fill[d][1]], sE[fill[d][1]], d));
+            }
+        // along axis d: E_{d+1} upwinded by variable 4, E_{d+2} by variable 3
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) {
+                int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+                CK(spdk_E_riemann(ctx, Eep[e1], sE[e1], d, 4));
+                CK(spdk_E_riemann(ctx, Eep[e2], sE[e2], d, 3));
+            }
     }
     void update_prediction() {  // src/induction_ader.hpp:471-479
         CK(spdk_B_predict(ctx, Bf[0], Ba[0], sBa[0], Eep[1], sE[1], Eep[2], sE[2], Y.h, Z.h, dt, 0));
@@ -252,6 +267,12 @@ struct InductionRun {
         }
         CK(spdk_ctx_sync(ctx));
         double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - t_io;
+        if (cfg.final_output) {  // the state after the last step (outside the timed loop)
+            write_dims(outdir, X, Y, Z);
+            write_outputs();
+        } else if (!cfg.outputs) {
+            diagnostics();
+        }
         res->steps = n_step - step0;
         res->dofs = (long long)(X.n * X.n_sp) * (Y.n * Y.n_sp) * (Z.n * Z.n_sp);
         res->evolution_seconds = sec;
@@ -264,8 +285,14 @@ struct InductionRun {
 
 int run_induction(RunConfig& cfg, const std::string& outdir, int warmup, spdk_run_result* res) {
     (void)warmup;
-    if (cfg.ndim != 3) {
-        std::fprintf(stderr, "ERROR: the induction solver currently requires 3 active dimensions\n");
+    // the reference requires 3 active dimensions; runs with x and y active
+    // and z inactive are this build's extension (SD update only)
+    if (!(cfg.ndim == 3 || (cfg.ndim == 2 && !cfg.act[2]))) {
+        std::fprintf(stderr, "ERROR: the induction solver requires 3 active dimensions, or x and y active\n");
+        return SPDK_E_CONFIG;
+    }
+    if (cfg.ndim == 2 && cfg.fallback) {
+        std::fprintf(stderr, "ERROR: the induction FV update requires 3 active dimensions\n");
         return SPDK_E_CONFIG;
     }
     InductionRun r(cfg, outdir);

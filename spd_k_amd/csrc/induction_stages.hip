@@ -2,6 +2,7 @@
 // Each kernel restates the reference function cited at its launcher, with
 // one thread per output (element, point).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "ctx.hpp"
 #include "physics.hpp"
 
@@ -10,55 +11,77 @@ static constexpr int BS = 256;
 
 __device__ __forceinline__ int pick(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
 
-// rotational_a_to_b: src/induction.cpp:94-137 (B = curl A at face points)
+// Index of axis `a` of (ii, jj, kk) replaced by `l`.
+__device__ __forceinline__ void along(int a, int l, int& kk, int& jj, int& ii) {
+    if (a == 0) ii = l;
+    else if (a == 1) jj = l;
+    else kk = l;
+}
+
+// rotational_a_to_b: src/induction.cpp:94-137 (B = curl A at face points).
+// B_dim = d_{dim1} A2 / d1 - d_{dim2} A1 / d2, dim1 = dim+1, dim2 = dim+2
+// (mod 3). Each derivative sums over the flux points of its own axis (p+2;
+// the reference's count q is the same in 3-D); a derivative along an
+// inactive axis is zero (2-D runs).
 __global__ void __launch_bounds__(BS) k_curl(const double* __restrict__ A1, SD s1, const double* __restrict__ A2, SD s2,
                                              double* __restrict__ B, SD sb, const double* __restrict__ Dm, int ldd,
-                                             double d1, double d2, int dim, uint32_t total) {
+                                             double d1, double d2, int dim, int act, int nfp, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i, kk, jj, ii;
     split6(idx, sb.Nz, sb.Ny, sb.Nx, sb.nz, sb.ny, sb.nx, k, j, i, kk, jj, ii);
-    int q = pick(dim, sb.nx, sb.ny, sb.nz);
+    const int a1 = dim == 2 ? 0 : dim + 1, a2 = a1 == 2 ? 0 : a1 + 1;
+    const int q1 = (act >> a1) & 1 ? nfp : 0, q2 = (act >> a2) & 1 ? nfp : 0;
+    const int r1 = pick(a1, ii, jj, kk), r2 = pick(a2, ii, jj, kk);
     for (int t = 0; t < sb.nt; t++) {
         double d1a2 = 0, d2a1 = 0;
-        for (int ll = 0; ll < q; ll++) {
-            if (dim == 0) {
-                d1a2 += A2[s2.off(t, 0, k, j, i, kk, ll, ii)] * Dm[jj * ldd + ll];
-                d2a1 += A1[s1.off(t, 0, k, j, i, ll, jj, ii)] * Dm[kk * ldd + ll];
-            } else if (dim == 1) {
-                d1a2 += A2[s2.off(t, 0, k, j, i, ll, jj, ii)] * Dm[kk * ldd + ll];
-                d2a1 += A1[s1.off(t, 0, k, j, i, kk, jj, ll)] * Dm[ii * ldd + ll];
-            } else {
-                d1a2 += A2[s2.off(t, 0, k, j, i, kk, jj, ll)] * Dm[ii * ldd + ll];
-                d2a1 += A1[s1.off(t, 0, k, j, i, kk, ll, ii)] * Dm[jj * ldd + ll];
-            }
+        for (int ll = 0; ll < q1; ll++) {
+            int c = kk, b = jj, a = ii;
+            along(a1, ll, c, b, a);
+            d1a2 += A2[s2.off(t, 0, k, j, i, c, b, a)] * Dm[r1 * ldd + ll];
+        }
+        for (int ll = 0; ll < q2; ll++) {
+            int c = kk, b = jj, a = ii;
+            along(a2, ll, c, b, a);
+            d2a1 += A1[s1.off(t, 0, k, j, i, c, b, a)] * Dm[r2 * ldd + ll];
         }
         B[sb.off(t, 0, k, j, i, kk, jj, ii)] = d1a2 / d1 - d2a1 / d2;
     }
 }
 
-// compute_E: src/induction.cpp:139-208
+// compute_E: src/induction.cpp:139-208. b1 is B1 interpolated sp->fp along
+// dim2, b2 is B2 along dim1, each over the solution points of its axis
+// (p+1, the reference's q in 3-D); along an inactive axis the value is
+// taken as is.
 __global__ void __launch_bounds__(BS) k_compute_E(double* __restrict__ E, SD se, const double* __restrict__ B1, SD s1,
                                                   const double* __restrict__ B2, SD s2, const double* __restrict__ S2F,
-                                                  int lds, double v1, double v2, int dim, uint32_t total) {
+                                                  int lds, double v1, double v2, int dim, int act, int nsp, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i, kk, jj, ii;
     split6(idx, se.Nz, se.Ny, se.Nx, se.nz, se.ny, se.nx, k, j, i, kk, jj, ii);
-    int q = pick(dim, se.nx, se.ny, se.nz);
+    const int a1 = dim == 2 ? 0 : dim + 1, a2 = a1 == 2 ? 0 : a1 + 1;
+    const bool on1 = (act >> a1) & 1, on2 = (act >> a2) & 1;
+    const int r1 = pick(a1, ii, jj, kk), r2 = pick(a2, ii, jj, kk);
     for (int t = 0; t < se.nt; t++) {
         double b1 = 0, b2 = 0;
-        for (int ll = 0; ll < q; ll++) {
-            if (dim == 2) {
-                b1 += B1[s1.off(t, 0, k, j, i, kk, ll, ii)] * S2F[jj * lds + ll];
-                b2 += B2[s2.off(t, 0, k, j, i, kk, jj, ll)] * S2F[ii * lds + ll];
-            } else if (dim == 1) {
-                b1 += B1[s1.off(t, 0, k, j, i, kk, jj, ll)] * S2F[ii * lds + ll];
-                b2 += B2[s2.off(t, 0, k, j, i, ll, jj, ii)] * S2F[kk * lds + ll];
-            } else {
-                b1 += B1[s1.off(t, 0, k, j, i, ll, jj, ii)] * S2F[kk * lds + ll];
-                b2 += B2[s2.off(t, 0, k, j, i, kk, ll, ii)] * S2F[jj * lds + ll];
+        if (on2) {
+            for (int ll = 0; ll < nsp; ll++) {
+                int c = kk, b = jj, a = ii;
+                along(a2, ll, c, b, a);
+                b1 += B1[s1.off(t, 0, k, j, i, c, b, a)] * S2F[r2 * lds + ll];
             }
+        } else {
+            b1 = B1[s1.off(t, 0, k, j, i, kk, jj, ii)];
+        }
+        if (on1) {
+            for (int ll = 0; ll < nsp; ll++) {
+                int c = kk, b = jj, a = ii;
+                along(a1, ll, c, b, a);
+                b2 += B2[s2.off(t, 0, k, j, i, c, b, a)] * S2F[r1 * lds + ll];
+            }
+        } else {
+            b2 = B2[s2.off(t, 0, k, j, i, kk, jj, ii)];
         }
         double vals[8] = {v1 * b2 - v2 * b1, b1, b2, v1, v2, 0.0, b1, b2};
         for (int v = 0; v < 8; v++) E[se.off(t, v, k, j, i, kk, jj, ii)] = vals[v];
@@ -99,27 +122,28 @@ __global__ void __launch_bounds__(BS) k_E_riemann(double* __restrict__ E, SD s, 
     }
 }
 
-// dB/dt at one face point (src/induction.cpp:331-356 / :391-415)
+// dB/dt at one face point (src/induction.cpp:331-356 / :391-415):
+// + d_{dim2} E1 / d2 - d_{dim1} E2 / d1, interleaved per flux point l as the
+// reference accumulates them; a derivative along an inactive axis is zero.
 __device__ __forceinline__ double dBdt_at(const double* E1, SD s1, const double* E2, SD s2, const double* Dm, int ldd,
                                           double d1, double d2, int dim, int t, int k, int j, int i, int kk, int jj, int ii,
-                                          int q) {
+                                          int nfp, int act) {
+    const int a1 = dim == 2 ? 0 : dim + 1, a2 = a1 == 2 ? 0 : a1 + 1;
+    const bool on1 = (act >> a1) & 1, on2 = (act >> a2) & 1;
+    const int r1 = pick(a1, ii, jj, kk), r2 = pick(a2, ii, jj, kk);
     double dB = 0, E;
-    for (int ll = 0; ll < q; ll++) {
-        if (dim == 0) {
-            E = E1[s1.off(t, 0, k, j, i, ll, jj, ii)];
-            dB += E * Dm[kk * ldd + ll] / d2;
-            E = E2[s2.off(t, 0, k, j, i, kk, ll, ii)];
-            dB -= E * Dm[jj * ldd + ll] / d1;
-        } else if (dim == 1) {
-            E = E1[s1.off(t, 0, k, j, i, kk, jj, ll)];
-            dB += E * Dm[ii * ldd + ll] / d2;
-            E = E2[s2.off(t, 0, k, j, i, ll, jj, ii)];
-            dB -= E * Dm[kk * ldd + ll] / d1;
-        } else {
-            E = E1[s1.off(t, 0, k, j, i, kk, ll, ii)];
-            dB += E * Dm[jj * ldd + ll] / d2;
-            E = E2[s2.off(t, 0, k, j, i, kk, jj, ll)];
-            dB -= E * Dm[ii * ldd + ll] / d1;
+    for (int ll = 0; ll < nfp; ll++) {
+        if (on2) {
+            int c = kk, b = jj, a = ii;
+            along(a2, ll, c, b, a);
+            E = E1[s1.off(t, 0, k, j, i, c, b, a)];
+            dB += E * Dm[r2 * ldd + ll] / d2;
+        }
+        if (on1) {
+            int c = kk, b = jj, a = ii;
+            along(a1, ll, c, b, a);
+            E = E2[s2.off(t, 0, k, j, i, c, b, a)];
+            dB -= E * Dm[r1 * ldd + ll] / d1;
         }
     }
     return dB;
@@ -131,19 +155,18 @@ __global__ void __launch_bounds__(BS) k_B_update(const double* __restrict__ B, d
                                                  const double* __restrict__ E1, SD s1, const double* __restrict__ E2, SD s2,
                                                  const double* __restrict__ Dm, int ldd, const double* __restrict__ inv,
                                                  const double* __restrict__ w, double d1, double d2, double dt, int dim,
-                                                 int predict, int gx, int gy, int gz, uint32_t total) {
+                                                 int predict, int gx, int gy, int gz, int act, int nfp, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i, kk, jj, ii;
     split6(idx, sb.Nz - 2 * gz, sb.Ny - 2 * gy, sb.Nx - 2 * gx, sb.nz, sb.ny, sb.nx, k, j, i, kk, jj, ii);
     k += gz; j += gy; i += gx;
-    int q = pick(dim, sb.nx, sb.ny, sb.nz);
     size_t o = sb.off(0, 0, k, j, i, kk, jj, ii);
     size_t slab = sb.slab();
     if (predict) {
         double dBdt[10];
         double B_old = B[o];
-        for (int t = 0; t < nader; t++) dBdt[t] = dBdt_at(E1, s1, E2, s2, Dm, ldd, d1, d2, dim, t, k, j, i, kk, jj, ii, q);
+        for (int t = 0; t < nader; t++) dBdt[t] = dBdt_at(E1, s1, E2, s2, Dm, ldd, d1, d2, dim, t, k, j, i, kk, jj, ii, nfp, act);
         for (int t = 0; t < nader; t++) {
             double dB = 0;
             for (int l = 0; l < nader; l++) dB += dBdt[l] * inv[t * nader + l] * w[l] * dt;
@@ -151,34 +174,39 @@ __global__ void __launch_bounds__(BS) k_B_update(const double* __restrict__ B, d
         }
     } else {
         double dB = 0;
-        for (int t = 0; t < nader; t++) dB += dBdt_at(E1, s1, E2, s2, Dm, ldd, d1, d2, dim, t, k, j, i, kk, jj, ii, q) * w[t] * dt;
+        for (int t = 0; t < nader; t++) dB += dBdt_at(E1, s1, E2, s2, Dm, ldd, d1, d2, dim, t, k, j, i, kk, jj, ii, nfp, act) * w[t] * dt;
         Bout[o] -= dB;
     }
 }
 
-// compute_B2_cv: src/induction.cpp:422-465 (all elements)
+// compute_B2_cv: src/induction.cpp:422-465 (all elements). Along an inactive
+// axis the single point is taken with weight 1.
 __global__ void __launch_bounds__(BS) k_B2_cv(double* __restrict__ B2, SD s, const double* __restrict__ Bx, SD sx,
                                               const double* __restrict__ By, SD sy, const double* __restrict__ Bz, SD sz,
                                               const double* __restrict__ F2C, int ldf, const double* __restrict__ S2C,
-                                              int ldc, uint32_t total) {
+                                              int ldc, int act, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i, kk, jj, ii;
     split6(idx, s.Nz, s.Ny, s.Nx, s.nz, s.ny, s.nx, k, j, i, kk, jj, ii);
-    int q = sx.nx, px = s.nx, py = s.ny, pz = s.nz;
+    const bool ax = act & 1, ay = (act >> 1) & 1, az = (act >> 2) & 1;
+    int px = s.nx, py = s.ny, pz = s.nz;
+    int qx = sx.nx, qy = sy.ny, qz = sz.nz;  // flux points along x, y, z
+    auto sc = [&](bool on, int r, int c) { return on ? S2C[r * ldc + c] : 1.0; };
+    auto fc = [&](bool on, int r, int c) { return on ? F2C[r * ldf + c] : 1.0; };
     double bx = 0, by = 0, bz = 0;
     for (int nn = 0; nn < pz; nn++)
         for (int mm = 0; mm < py; mm++)
-            for (int ll = 0; ll < q; ll++)
-                bx += Bx[sx.off(0, 0, k, j, i, nn, mm, ll)] * S2C[kk * ldc + nn] * S2C[jj * ldc + mm] * F2C[ii * ldf + ll];
+            for (int ll = 0; ll < qx; ll++)
+                bx += Bx[sx.off(0, 0, k, j, i, nn, mm, ll)] * sc(az, kk, nn) * sc(ay, jj, mm) * fc(ax, ii, ll);
     for (int nn = 0; nn < pz; nn++)
-        for (int mm = 0; mm < q; mm++)
+        for (int mm = 0; mm < qy; mm++)
             for (int ll = 0; ll < px; ll++)
-                by += By[sy.off(0, 0, k, j, i, nn, mm, ll)] * S2C[kk * ldc + nn] * F2C[jj * ldf + mm] * S2C[ii * ldc + ll];
-    for (int nn = 0; nn < q; nn++)
+                by += By[sy.off(0, 0, k, j, i, nn, mm, ll)] * sc(az, kk, nn) * fc(ay, jj, mm) * sc(ax, ii, ll);
+    for (int nn = 0; nn < qz; nn++)
         for (int mm = 0; mm < py; mm++)
             for (int ll = 0; ll < px; ll++)
-                bz += Bz[sz.off(0, 0, k, j, i, nn, mm, ll)] * F2C[kk * ldf + nn] * S2C[jj * ldc + mm] * S2C[ii * ldc + ll];
+                bz += Bz[sz.off(0, 0, k, j, i, nn, mm, ll)] * fc(az, kk, nn) * sc(ay, jj, mm) * sc(ax, ii, ll);
     B2[s.off(0, 0, k, j, i, kk, jj, ii)] = bx;
     B2[s.off(0, 1, k, j, i, kk, jj, ii)] = by;
     B2[s.off(0, 2, k, j, i, kk, jj, ii)] = bz;
@@ -297,8 +325,94 @@ __global__ void __launch_bounds__(BS) k_fv_bc_aligned(double* __restrict__ U, FV
     }
 }
 
+// Not in the reference: per-block max |div B| over the solution points of the
+// active elements (each face component differentiated fp->sp along its own
+// axis, terms added x, y, z) and the partial sums of |B|^2 (B2 variable 3) x
+// cv volume (products of FV face widths along the active axes). part[b] and
+// part[nb + b] receive block b's max and sum.
+__global__ void __launch_bounds__(BS) k_ind_diag(const double* __restrict__ Bx, SD sx, const double* __restrict__ By, SD sy,
+                                                 const double* __restrict__ Bz, SD sz, const double* __restrict__ B2, SD s2,
+                                                 const double* __restrict__ Dm, int ldd, const double* __restrict__ fx,
+                                                 const double* __restrict__ fy, const double* __restrict__ fz, double hx,
+                                                 double hy, double hz, int act, int nfp, int gx, int gy, int gz, int cx, int cy,
+                                                 int cz, double* __restrict__ part, uint32_t total) {
+    __shared__ double smax_[BS], ssum_[BS];
+    double m = 0, e = 0;
+    const double* Bd[3] = {Bx, By, Bz};
+    const SD sd[3] = {sx, sy, sz};
+    const double* fv[3] = {fx, fy, fz};
+    const double hd[3] = {hx, hy, hz};
+    const int cg[3] = {cx, cy, cz}, eg[3] = {gx, gy, gz};
+    for (uint32_t idx = blockIdx.x * BS + threadIdx.x; idx < total; idx += gridDim.x * BS) {
+        int k, j, i, kk, jj, ii;
+        split6(idx, s2.Nz - 2 * gz, s2.Ny - 2 * gy, s2.Nx - 2 * gx, s2.nz, s2.ny, s2.nx, k, j, i, kk, jj, ii);
+        k += gz; j += gy; i += gx;
+        double div = 0;
+        for (int d = 0; d < 3; d++) {
+            if (!((act >> d) & 1)) continue;
+            int r = pick(d, ii, jj, kk);
+            double sum = 0;
+            for (int l = 0; l < nfp; l++) {
+                int c = kk, b = jj, a = ii;
+                along(d, l, c, b, a);
+                sum += Bd[d][sd[d].off(0, 0, k, j, i, c, b, a)] * Dm[r * ldd + l];
+            }
+            div += sum / hd[d];
+        }
+        m = fmax(m, fabs(div));
+        double vol = 1;
+        const int el[3] = {i, j, k}, pt[3] = {ii, jj, kk}, np[3] = {s2.nx, s2.ny, s2.nz};
+        for (int d = 0; d < 3; d++) {
+            if (!((act >> d) & 1)) continue;
+            int I = cg[d] + (el[d] - eg[d]) * np[d] + pt[d];
+            vol *= fv[d][I + 1] - fv[d][I];
+        }
+        e += B2[s2.off(0, 3, k, j, i, kk, jj, ii)] * vol;
+    }
+    smax_[threadIdx.x] = m;
+    ssum_[threadIdx.x] = e;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smax_[threadIdx.x] = fmax(smax_[threadIdx.x], smax_[threadIdx.x + w]);
+            ssum_[threadIdx.x] += ssum_[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = smax_[0];
+        part[gridDim.x + blockIdx.x] = ssum_[0];
+    }
+}
+__global__ void __launch_bounds__(BS) k_ind_diag_final(double* __restrict__ part, int nb) {
+    __shared__ double smax_[BS], ssum_[BS];
+    double m = 0, e = 0;
+    for (int b = threadIdx.x; b < nb; b += BS) {
+        m = fmax(m, part[b]);
+        e += part[nb + b];
+    }
+    smax_[threadIdx.x] = m;
+    ssum_[threadIdx.x] = e;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smax_[threadIdx.x] = fmax(smax_[threadIdx.x], smax_[threadIdx.x + w]);
+            ssum_[threadIdx.x] += ssum_[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[0] = smax_[0];
+        part[1] = ssum_[0];
+    }
+}
+
 // ===========================================================================
 extern "C" {
+
+static int act_mask(const spdk_ctx* c) {
+    return (c->prm.active[0] ? 1 : 0) | (c->prm.active[1] ? 2 : 0) | (c->prm.active[2] ? 4 : 0);
+}
 
 #define IND_TOTAL(s) ((uint32_t)((size_t)(s).Nz * (s).Ny * (s).Nx * (s).nz * (s).ny * (s).nx))
 
@@ -306,7 +420,7 @@ int spdk_B_curl(spdk_ctx* c, const double* A1, spdk_sd_shape s1, const double* A
                 spdk_sd_shape sb, double d1, double d2, int dim) {
     uint32_t total = IND_TOTAL(sb);
     hipLaunchKernelGGL(k_curl, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, A1, SD(s1), A2, SD(s2), B, SD(sb),
-                       c->mat[SPDK_M_DFP_TO_SP], c->mcols[SPDK_M_DFP_TO_SP], d1, d2, dim, total);
+                       c->mat[SPDK_M_DFP_TO_SP], c->mcols[SPDK_M_DFP_TO_SP], d1, d2, dim, act_mask(c), c->m, total);
     SPDK_LAUNCH_CHECK("B_curl");
     return SPDK_OK;
 }
@@ -317,7 +431,8 @@ int spdk_induction_E(spdk_ctx* c, double* E, spdk_sd_shape se, const double* B1,
     int dim1 = dim == 0 ? 1 : (dim == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
     uint32_t total = IND_TOTAL(se);
     hipLaunchKernelGGL(k_compute_E, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, E, SD(se), B1, SD(s1), B2, SD(s2),
-                       c->mat[SPDK_M_SP_TO_FP], c->mcols[SPDK_M_SP_TO_FP], c->prm.vel[dim1], c->prm.vel[dim2], dim, total);
+                       c->mat[SPDK_M_SP_TO_FP], c->mcols[SPDK_M_SP_TO_FP], c->prm.vel[dim1], c->prm.vel[dim2], dim,
+                       act_mask(c), c->n, total);
     SPDK_LAUNCH_CHECK("induction_E");
     return SPDK_OK;
 }
@@ -338,7 +453,7 @@ static int B_update(spdk_ctx* c, const double* B, double* Bout, spdk_sd_shape sb
     uint32_t total = (uint32_t)((size_t)(sb.Nz - 2 * gz) * (sb.Ny - 2 * gy) * (sb.Nx - 2 * gx) * sb.nz * sb.ny * sb.nx);
     hipLaunchKernelGGL(k_B_update, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, B, Bout, SD(sb), nader, E1, SD(s1), E2,
                        SD(s2), c->mat[SPDK_M_DFP_TO_SP], c->mcols[SPDK_M_DFP_TO_SP], c->mat[SPDK_M_INVADER], c->wt, d1, d2,
-                       dt, dim, predict, gx, gy, gz, total);
+                       dt, dim, predict, gx, gy, gz, act_mask(c), c->m, total);
     SPDK_LAUNCH_CHECK("B_update");
     return SPDK_OK;
 }
@@ -358,7 +473,7 @@ int spdk_B2_cv(spdk_ctx* c, double* B2, spdk_sd_shape s, const double* Bx, spdk_
     uint32_t total = IND_TOTAL(s);
     hipLaunchKernelGGL(k_B2_cv, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, B2, SD(s), Bx, SD(sx), By, SD(sy), Bz,
                        SD(sz), c->mat[SPDK_M_FP_TO_CV], c->mcols[SPDK_M_FP_TO_CV], c->mat[SPDK_M_SP_TO_CV],
-                       c->mcols[SPDK_M_SP_TO_CV], total);
+                       c->mcols[SPDK_M_SP_TO_CV], act_mask(c), total);
     SPDK_LAUNCH_CHECK("B2_cv");
     return SPDK_OK;
 }
@@ -408,6 +523,28 @@ int spdk_fill_fv_ghosts_aligned(spdk_ctx* c, double* U, spdk_fv_shape s, int ali
                            total);
         SPDK_LAUNCH_CHECK("fill_fv_ghosts_aligned");
     }
+    return SPDK_OK;
+}
+
+int spdk_induction_diagnostics(spdk_ctx* c, const double* Bx, spdk_sd_shape sx, const double* By, spdk_sd_shape sy,
+                               const double* Bz, spdk_sd_shape sz, const double* B2, spdk_sd_shape s2, double* out) {
+    if (!out || s2.nv != 4) { set_error("induction_diagnostics: B2 carries 4 variables, out 2 doubles"); return SPDK_E_ARG; }
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    uint32_t total = (uint32_t)((size_t)(s2.Nz - 2 * gz) * (s2.Ny - 2 * gy) * (s2.Nx - 2 * gx) * s2.nz * s2.ny * s2.nx);
+    // part[] holds 2 values per block inside the context's reduction buffer
+    int nb = (int)std::min<size_t>(c->red_blocks / 2, nblocks(total, BS));
+    if (nb < 1) nb = 1;
+    const spdk_params& P = c->prm;
+    hipLaunchKernelGGL(k_ind_diag, dim3(nb), dim3(BS), 0, c->stream, Bx, SD(sx), By, SD(sy), Bz, SD(sz), B2, SD(s2),
+                       c->mat[SPDK_M_DFP_TO_SP], c->mcols[SPDK_M_DFP_TO_SP], c->fvf[0], c->fvf[1], c->fvf[2], P.h[0], P.h[1],
+                       P.h[2], act_mask(c), c->m, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], c->red, total);
+    SPDK_LAUNCH_CHECK("ind_diag");
+    hipLaunchKernelGGL(k_ind_diag_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb);
+    SPDK_LAUNCH_CHECK("ind_diag_final");
+    SPDK_CHECK(hipMemcpyAsync(c->red_host, c->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    out[0] = c->red_host[0];
+    out[1] = c->red_host[1];
     return SPDK_OK;
 }
 

@@ -203,6 +203,19 @@ int spdk_B_correct(spdk_ctx* ctx, double* B, spdk_sd_shape sb, const double* E1,
 /* compute_B2_cv (src/induction.cpp:422-465): cv averages of B and |B|^2 */
 int spdk_B2_cv(spdk_ctx* ctx, double* B2, spdk_sd_shape s, const double* Bx, spdk_sd_shape sx, const double* By,
                spdk_sd_shape sy, const double* Bz, spdk_sd_shape sz);
+/* ---- fused induction step ---------------------------------------------------
+ * One ADER step of the induction solver without the FV update (the stage
+ * sequence copy_ader, p+1 x Solve_E with p x update_B_prediction, then
+ * update_B_solution: src/induction_ader.hpp:343-359) in p+2 launches, bit-exact
+ * with the stage entries: x-y (z inactive) or x-y-z active, p = 1..4 (1..3 in
+ * 3-D), periodic or gradfree boundaries. Face B in the stage layout, updated
+ * in place over the active elements. The workspace (size from
+ * spdk_induction_fused_workspace_bytes, ghost-inclusive element counts; 0 when
+ * the configuration is not covered) holds two copies of the raw edge field of
+ * every slice. */
+size_t spdk_induction_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
+int spdk_induction_step_fused(spdk_ctx* ctx, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz,
+                              spdk_sd_shape sz, void* workspace, double dt);
 /* No reference counterpart (the reference writes B2_cv only): max |div B| at
  * the solution points of the active elements, each face component
  * differentiated fp->sp along its own axis, and the magnetic energy, the sum

@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["csrc/sd_stages.hip", "csrc/fv_stages.hip", "csrc/fused.hip", "csrc/induction_stages.hip",
-               "csrc/mixed.hip", "csrc/mlp.hip"]
+               "csrc/induction_fused.hip", "csrc/mixed.hip", "csrc/mlp.hip"]
 HOST_SOURCES = ["csrc/host/params.cpp", "csrc/host/setup.cpp", "csrc/host/driver.cpp",
                 "csrc/host/induction.cpp", "csrc/host/user_ic.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -49,6 +49,8 @@ struct InductionRun {
     double *Bf[3] = {}, *Ba[3] = {}, *B0[3] = {}, *Eep[3] = {}, *Bcf[3] = {}, *Tfp[3] = {};
     double *Bold[3] = {}, *Bnew[3] = {}, *Efv[3] = {}, *alpha[3] = {};
     double *B_old = nullptr, *B_new = nullptr, *troubles = nullptr, *B2 = nullptr;
+    bool fused = false;    // fused ADER sweeps (spdk_induction_step_fused) instead of the stages
+    double* ws = nullptr;  // their workspace
     std::vector<double> host;
 
     InductionRun(RunConfig& c, const std::string& od) : cfg(c), outdir(od) {}
@@ -75,14 +77,25 @@ struct InductionRun {
         }
         const Axis* ax[3] = {&X, &Y, &Z};
         ctx = make_context(cfg, op, ax);
+        spdk_timing_enable(ctx, cfg.timing ? 1 : 0);
+        if (cfg.path == "fused" && !cfg.fallback && cfg.integrator == INT_ADER) {
+            int Ng[3] = {X.n_total, Y.n_total, Z.n_total};
+            size_t b = spdk_induction_fused_workspace_bytes(ctx, Ng);
+            if (b > 0) {
+                fused = true;
+                ws = zeros((b + sizeof(double) - 1) / sizeof(double));
+            }
+        }
         for (int d = 0; d < 3; d++) {
             sB[d] = sd_of(1, 1, X, Y, Z, d == 2, d == 1, d == 0);
             sBa[d] = sB[d];
             sBa[d].nt = T;
             sE[d] = sd_of(T, 8, X, Y, Z, d != 2, d != 1, d != 0);
             Bf[d] = zeros(sd_count(sB[d]));
-            Ba[d] = zeros(sd_count(sBa[d]));
-            Eep[d] = zeros(sd_count(sE[d]));
+            if (!fused) {  // the stage sequence's slices and edge arrays
+                Ba[d] = zeros(sd_count(sBa[d]));
+                Eep[d] = zeros(sd_count(sE[d]));
+            }
             if (cfg.integrator == INT_RK) B0[d] = zeros(sd_count(sB[d]));
         }
         sB2 = sd_of(1, 4, X, Y, Z, false, false, false);
@@ -221,6 +234,10 @@ struct InductionRun {
         for (int d = 0; d < 3; d++) transform_2d(Bcf[d], Bf[d], Tfp[d], sB[d], SPDK_M_CV_TO_SP, d);
     }
     void ader_step() {  // src/induction_ader.hpp:343-359
+        if (fused) {
+            CK(spdk_induction_step_fused(ctx, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2], ws, dt));
+            return;
+        }
         for (int d = 0; d < 3; d++) CK(spdk_copy_ader(ctx, Bf[d], Ba[d], sBa[d]));
         for (int a = 0; a < T; a++) {
             solve_E();
@@ -240,9 +257,23 @@ struct InductionRun {
                 for (int d = 0; d < 3; d++) CK(spdk_rk_combine(ctx, Bf[d], B0[d], sB[d], rk_a[s]));
         }
     }
-    void evolve(spdk_run_result* res) {  // src/induction_ader.hpp:274-327
+    void evolve(int warmup, spdk_run_result* res) {  // src/induction_ader.hpp:274-327
         dt = Dt;
         double t_output = cfg.dt_output;
+        if (warmup > 0) {  // untimed steps; B is restored afterwards
+            std::vector<double*> save;
+            for (int d = 0; d < 3; d++) {
+                save.push_back(zeros(sd_count(sB[d])));
+                CK(spdk_memcpy_d2d(ctx, save[d], Bf[d], sd_count(sB[d]) * sizeof(double)));
+            }
+            for (int w = 0; w < warmup; w++) {
+                if (cfg.integrator == INT_RK) rk_step();
+                else ader_step();
+            }
+            for (int d = 0; d < 3; d++) CK(spdk_memcpy_d2d(ctx, Bf[d], save[d], sd_count(sB[d]) * sizeof(double)));
+            CK(spdk_ctx_sync(ctx));
+            spdk_timing_reset(ctx);
+        }
         CK(spdk_ctx_sync(ctx));
         auto t0 = std::chrono::steady_clock::now();
         double t_io = 0;
@@ -284,7 +315,6 @@ struct InductionRun {
 };
 
 int run_induction(RunConfig& cfg, const std::string& outdir, int warmup, spdk_run_result* res) {
-    (void)warmup;
     // the reference requires 3 active dimensions; runs with x and y active
     // and z inactive are this build's extension (SD update only)
     if (!(cfg.ndim == 3 || (cfg.ndim == 2 && !cfg.act[2]))) {
@@ -297,7 +327,8 @@ int run_induction(RunConfig& cfg, const std::string& outdir, int warmup, spdk_ru
     }
     InductionRun r(cfg, outdir);
     r.setup();
-    r.evolve(res);
+    std::printf("induction path: %s\n", r.fused ? "fused" : "stages");
+    r.evolve(warmup, res);
     fill_kernel_timing(r.ctx, res);
     return 0;
 }

@@ -167,6 +167,17 @@ IND_CASES.update({
     "ind2d_ader": LOOP2D + ["mesh/nx1=12", "mesh/nx2=10", "time/tlim=0.1", "output/dt=0.05"],
     "ind2d_ader_p2": LOOP2D + ["mesh/p=2", "mesh/nx1=9", "mesh/nx2=11", "time/tlim=0.1", "output/dt=0.05"],
     "ind2d_rk3": LOOP2D + ["time/integrator=rk3", "mesh/nx1=10", "mesh/nx2=10", "time/tlim=0.05", "output/dt=0.025"],
+    # the loop crosses two gradfree walls
+    "ind2d_gradfree": LOOP2D + ["mesh/nx1=10", "mesh/nx2=12", "mesh/x1_bc=gradfree", "mesh/x2_bc=gradfree",
+                                "induction/cx=0.25", "induction/cy=1.8", "time/tlim=0.1", "output/dt=0.05"],
+    "ind2d_p1": LOOP2D + ["mesh/p=1", "mesh/nx1=16", "mesh/nx2=14", "time/tlim=0.1", "output/dt=0.05"],
+    "ind2d_p4": LOOP2D + ["mesh/p=4", "mesh/nx1=7", "mesh/nx2=9", "time/tlim=0.1", "output/dt=0.05"],
+    "ind3d_p2": ["-i", IND, "job/fallback=false", "mesh/p=2", "mesh/nx1=6", "mesh/nx2=5", "mesh/nx3=4",
+                 "induction/v3=0.3", "time/tlim=0.1", "output/dt=0.05"],
+    "ind3d_p1_gradfree_z": ["-i", IND, "job/fallback=false", "mesh/p=1", "mesh/nx1=6", "mesh/nx2=6", "mesh/nx3=5",
+                            "mesh/x3_bc=gradfree", "induction/v3=-0.4", "time/tlim=0.1", "output/dt=0.05"],
+    "ind3d_stages": ["-i", IND, "job/fallback=false", "job/path=stages", "mesh/nx1=6", "mesh/nx2=7", "mesh/nx3=5",
+                     "time/tlim=0.1", "output/dt=0.05"],
 })
 
 
@@ -180,6 +191,7 @@ def test_induction_parity_with_oracle(tmp_path, name):
     a, b = str(tmp_path / "oracle"), str(tmp_path / "gpu")
     ta = op.run_oracle(IND_CASES[name], a)
     tb = op.run_driver(IND_CASES[name], b)
+    print(re.findall(r"induction path: \w+", tb))
     assert_bit_exact(a, b)
     # max|div B| is a max of bit-identical point values; the energy sums the
     # same cell values in another order

@@ -216,6 +216,13 @@ int spdk_B2_cv(spdk_ctx* ctx, double* B2, spdk_sd_shape s, const double* Bx, spd
 size_t spdk_induction_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
 int spdk_induction_step_fused(spdk_ctx* ctx, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz,
                               spdk_sd_shape sz, void* workspace, double dt);
+/* The same step in mixed precision (no reference counterpart): raw edge E,
+ * face solves, interpolations and predictor slices in FP32; dB/dt of the
+ * corrector in FP64 from the FP32 edge values and B accumulated in FP64, so
+ * div B stays at FP64 round-off. Workspace: half the FP64 one. */
+size_t spdk_induction_mixed_workspace_bytes(spdk_ctx* ctx, const int N[3]);
+int spdk_induction_step_mixed(spdk_ctx* ctx, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz,
+                              spdk_sd_shape sz, void* workspace, double dt);
 /* No reference counterpart (the reference writes B2_cv only): max |div B| at
  * the solution points of the active elements, each face component
  * differentiated fp->sp along its own axis, and the magnetic energy, the sum

@@ -50,10 +50,11 @@ struct InductionRun {
     double *Bold[3] = {}, *Bnew[3] = {}, *Efv[3] = {}, *alpha[3] = {};
     double *B_old = nullptr, *B_new = nullptr, *troubles = nullptr, *B2 = nullptr;
     bool fused = false;    // fused ADER sweeps (spdk_induction_step_fused) instead of the stages
+    bool mixed = false;    // their FP32 form (spdk_induction_step_mixed)
     double* ws = nullptr;  // their workspace
     std::vector<double> host;
 
-    InductionRun(RunConfig& c, const std::string& od) : cfg(c), outdir(od) {}
+    InductionRun(RunConfig& c, const std::string& od) : cfg(c), outdir(od), mixed(c.precision == "mixed") {}
     ~InductionRun() {
         if (ctx) spdk_ctx_sync(ctx);
         mem.release();
@@ -78,14 +79,15 @@ struct InductionRun {
         const Axis* ax[3] = {&X, &Y, &Z};
         ctx = make_context(cfg, op, ax);
         spdk_timing_enable(ctx, cfg.timing ? 1 : 0);
-        if (cfg.path == "fused" && !cfg.fallback && cfg.integrator == INT_ADER) {
+        if ((cfg.path == "fused" || mixed) && !cfg.fallback && cfg.integrator == INT_ADER) {
             int Ng[3] = {X.n_total, Y.n_total, Z.n_total};
-            size_t b = spdk_induction_fused_workspace_bytes(ctx, Ng);
+            size_t b = mixed ? spdk_induction_mixed_workspace_bytes(ctx, Ng) : spdk_induction_fused_workspace_bytes(ctx, Ng);
             if (b > 0) {
                 fused = true;
                 ws = zeros((b + sizeof(double) - 1) / sizeof(double));
             }
         }
+        if (mixed && !fused) throw std::runtime_error(std::string("mixed induction step: ") + spdk_last_error());
         for (int d = 0; d < 3; d++) {
             sB[d] = sd_of(1, 1, X, Y, Z, d == 2, d == 1, d == 0);
             sBa[d] = sB[d];
@@ -235,7 +237,8 @@ struct InductionRun {
     }
     void ader_step() {  // src/induction_ader.hpp:343-359
         if (fused) {
-            CK(spdk_induction_step_fused(ctx, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2], ws, dt));
+            if (mixed) CK(spdk_induction_step_mixed(ctx, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2], ws, dt));
+            else CK(spdk_induction_step_fused(ctx, Bf[0], sB[0], Bf[1], sB[1], Bf[2], sB[2], ws, dt));
             return;
         }
         for (int d = 0; d < 3; d++) CK(spdk_copy_ader(ctx, Bf[d], Ba[d], sBa[d]));
@@ -327,7 +330,7 @@ int run_induction(RunConfig& cfg, const std::string& outdir, int warmup, spdk_ru
     }
     InductionRun r(cfg, outdir);
     r.setup();
-    std::printf("induction path: %s\n", r.fused ? "fused" : "stages");
+    std::printf("induction path: %s\n", r.fused ? (r.mixed ? "fused mixed" : "fused") : "stages");
     r.evolve(warmup, res);
     fill_kernel_timing(r.ctx, res);
     return 0;

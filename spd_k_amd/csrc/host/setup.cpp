@@ -152,6 +152,11 @@ std::string configure(Params& pin, RunConfig& c) {
             return "job/mode=train_guess covers the ADER hydro step without fallback";
         c.precision = "mixed";
     }
+    if (c.system == "induction" && c.precision == "mixed") {
+        if (c.integrator != INT_ADER || c.fallback || c.picard != "fixed" || c.guess != "zero")
+            return "job/precision=mixed covers the induction ADER step without the FV update (fixed Picard count)";
+        return "";
+    }
     if (c.precision == "mixed" || c.picard == "residual" || c.guess == "mlp") {
         if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
             return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step without fallback";

@@ -22,6 +22,13 @@
 // The zero-order guess makes every slice equal, so the first launch only
 // computes the raw E of B once per point and writes it to every slice.
 //
+// The same kernels with R = float are the mixed-precision form: raw E, the
+// face solves, the sp->fp interpolations and the predictor slices in FP32
+// (with dfp_to_sp/h and invader*w*dt premultiplied), while the corrector forms
+// dB/dt in FP64 from the FP32 edge values and accumulates B in FP64. Since
+// every face-B update is a discrete curl of one set of edge values, div B
+// stays at FP64 round-off.
+//
 // Only the upwind velocities of the face solves matter: they are the uniform
 // advection velocity of the run (compute_E stores it per point, the face
 // solve takes max(v_L, v_R) = v), and the other variables the stage path
@@ -29,6 +36,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include "ctx.hpp"
 
 using namespace spdk;
@@ -58,14 +66,14 @@ struct IG {
     __host__ __device__ static constexpr int eoff(int e) { return e == 0 ? 0 : (e == 1 ? epts(0) : epts(0) + epts(1)); }
 };
 
-template <int N, int D, int T>
+template <int N, int D, int T, class R>
 struct Tile {
     using G = IG<N, D>;
     // per element: one slice of raw and of face-solved E, one predictor slice
     static constexpr int PER = 2 * G::EP + G::BP;
     // LDS bound, and at most two face-B points per thread (their dB/dt of
     // every slice stay in registers)
-    static constexpr int EPB_LDS = kLdsBudget / (int)(PER * sizeof(double));
+    static constexpr int EPB_LDS = kLdsBudget / (int)(PER * sizeof(R));
     static constexpr int EPB_REG = 2 * BS / G::BP;
     static constexpr int EPB_RAW = EPB_LDS < EPB_REG ? EPB_LDS : EPB_REG;
     static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
@@ -113,8 +121,8 @@ constexpr bool perm_is_permutation() {
 }
 
 struct IArgs {
-    double* Eout;         // raw E written by this launch (T, 3 components, elements, points)
-    const double* Ein;    // raw E of the state this launch reads
+    void* Eout;           // raw E written by this launch (T, 3 components, elements, points)
+    const void* Ein;      // raw E of the state this launch reads (element type R)
     double* B[3];         // face B, ghost-inclusive layout (1, 1, Nz, Ny, Nx, pts)
     const double* S2F;    // sp_to_fp (M x N)
     const double* Dm;     // dfp_to_sp (N x M)
@@ -130,8 +138,9 @@ struct IArgs {
     int nel;              // active elements
 };
 
-__device__ __forceinline__ double upwind(double l, double r, double v) {
-    if (v == 0) return 0.5 * (l + r);
+template <class R>
+__device__ __forceinline__ R upwind(R l, R r, double v) {
+    if (v == 0) return R(0.5) * (l + r);
     return v > 0 ? l : r;
 }
 
@@ -169,45 +178,45 @@ __device__ __forceinline__ int resolve(const IArgs& a, int e, const int el[3], i
 
 // E after the face solve along axis `ax` only (or the raw value off that
 // axis's faces); raw(e, el, pt) reads a raw value.
-template <int N, int D, class Raw>
-__device__ __forceinline__ double solved_1(const IArgs& a, int e, const int el[3], const int pt[3], int ax, Raw raw) {
+template <int N, int D, class R, class Raw>
+__device__ __forceinline__ R solved_1(const IArgs& a, int e, const int el[3], const int pt[3], int ax, Raw raw) {
     using G = IG<N, D>;
     const int last = G::ecnt(e, ax) - 1;
-    const double own = raw(e, el, pt);
+    const R own = raw(e, el, pt);
     if (pt[ax] != 0 && pt[ax] != last) return own;
     int el2[3] = {el[0], el[1], el[2]}, pt2[3] = {pt[0], pt[1], pt[2]};
     if (pt[ax] == 0) {
         el2[ax] -= 1;
         pt2[ax] = last;
-        return upwind(raw(e, el2, pt2), own, a.vel[ax]);
+        return upwind<R>(raw(e, el2, pt2), own, a.vel[ax]);
     }
     el2[ax] += 1;
     pt2[ax] = 0;
-    return upwind(own, raw(e, el2, pt2), a.vel[ax]);
+    return upwind<R>(own, raw(e, el2, pt2), a.vel[ax]);
 }
 
 // Face-solved E of component e: along its first active face axis, then its
 // second (x before y before z, the order of the stage calls).
-template <int N, int D, class Raw>
-__device__ __forceinline__ double solved(const IArgs& a, int e, const int el[3], const int pt[3], Raw raw) {
+template <int N, int D, class R, class Raw>
+__device__ __forceinline__ R solved(const IArgs& a, int e, const int el[3], const int pt[3], Raw raw) {
     using G = IG<N, D>;
     int ax[2] = {0, 0}, nax = 0;
 #pragma unroll
     for (int c = 0; c < 3; c++)
         if (c != e && c < D) ax[nax++] = c;
-    if (nax == 1) return solved_1<N, D>(a, e, el, pt, ax[0], raw);
+    if (nax == 1) return solved_1<N, D, R>(a, e, el, pt, ax[0], raw);
     const int b = ax[1], last = G::ecnt(e, b) - 1;
-    const double own = solved_1<N, D>(a, e, el, pt, ax[0], raw);
+    const R own = solved_1<N, D, R>(a, e, el, pt, ax[0], raw);
     if (pt[b] != 0 && pt[b] != last) return own;
     int el2[3] = {el[0], el[1], el[2]}, pt2[3] = {pt[0], pt[1], pt[2]};
     if (pt[b] == 0) {
         el2[b] -= 1;
         pt2[b] = last;
-        return upwind(solved_1<N, D>(a, e, el2, pt2, ax[0], raw), own, a.vel[b]);
+        return upwind<R>(solved_1<N, D, R>(a, e, el2, pt2, ax[0], raw), own, a.vel[b]);
     }
     el2[b] += 1;
     pt2[b] = 0;
-    return upwind(own, solved_1<N, D>(a, e, el2, pt2, ax[0], raw), a.vel[b]);
+    return upwind<R>(own, solved_1<N, D, R>(a, e, el2, pt2, ax[0], raw), a.vel[b]);
 }
 
 __device__ __forceinline__ void split_pt(int p, int nx, int ny, int pt[3]) {
@@ -221,10 +230,10 @@ __device__ __forceinline__ void split_pt(int p, int nx, int ny, int pt[3]) {
 // given by an accessor bget(d, point[3]): v1*b2 - v2*b1 with b1 = B_{e+1}
 // interpolated sp->fp along e+2 and b2 = B_{e+2} along e+1 (an inactive axis
 // takes the value as is).
-template <int N, int D, class BGet>
-__device__ __forceinline__ double edge_e(int e, const int pt[3], const double* S2F, const double vel[3], BGet bget) {
+template <int N, int D, class R, class BGet>
+__device__ __forceinline__ R edge_e(int e, const int pt[3], const R* S2F, const R vel[3], BGet bget) {
     const int a1 = e == 2 ? 0 : e + 1, a2 = a1 == 2 ? 0 : a1 + 1;
-    double b1 = 0, b2 = 0;
+    R b1 = 0, b2 = 0;
     int q[3] = {pt[0], pt[1], pt[2]};
     if (a2 < D) {
         for (int l = 0; l < N; l++) {
@@ -247,9 +256,15 @@ __device__ __forceinline__ double edge_e(int e, const int pt[3], const double* S
 }
 
 // Zero-order guess: the raw E of B, written to every slice.
-template <int N, int D, int T>
+template <int N, int D, int T, class R>
 __global__ void __launch_bounds__(BS) k_istart(IArgs a) {
     using G = IG<N, D>;
+    constexpr int M = G::M;
+    __shared__ R sS2F[M * N];
+    for (int i = threadIdx.x; i < M * N; i += BS) sS2F[i] = (R)a.S2F[i];
+    __syncthreads();
+    const R vel[3] = {(R)a.vel[0], (R)a.vel[1], (R)a.vel[2]};
+    R* out = (R*)a.Eout;
     const size_t total = (size_t)a.nel * G::EP;
     for (size_t idx = (size_t)blockIdx.x * BS + threadIdx.x; idx < total; idx += (size_t)gridDim.x * BS) {
         int lin = (int)(idx / G::EP), p = (int)(idx % G::EP);
@@ -259,36 +274,46 @@ __global__ void __launch_bounds__(BS) k_istart(IArgs a) {
         int pt[3];
         split_pt(p, G::ecnt(e, 0), G::ecnt(e, 1), pt);
         size_t gb = ((size_t)(el[2] + a.g[2]) * a.G[1] + (el[1] + a.g[1])) * a.G[0] + (el[0] + a.g[0]);
-        auto bget = [&](int d, const int q[3]) {
-            return a.B[d][gb * G::bpts(d) + (q[2] * G::bcnt(d, 1) + q[1]) * G::bcnt(d, 0) + q[0]];
+        auto bget = [&](int d, const int q[3]) -> R {
+            return (R)a.B[d][gb * G::bpts(d) + (q[2] * G::bcnt(d, 1) + q[1]) * G::bcnt(d, 0) + q[0]];
         };
-        double v = edge_e<N, D>(e, pt, a.S2F, a.vel, bget);
+        R v = edge_e<N, D, R>(e, pt, sS2F, vel, bget);
         size_t o = eslab<N, D, T>(e, a.nel) + (size_t)lin * G::epts(e) + p;
-        for (int t = 0; t < T; t++) a.Eout[o + (size_t)t * a.nel * G::epts(e)] = v;
+        for (int t = 0; t < T; t++) out[o + (size_t)t * a.nel * G::epts(e)] = v;
     }
 }
 
 // One Picard sweep (CORR = false: predictor slices and their raw E; CORR =
 // true: the corrector into B). Slice by slice: the tile's raw E of the slice
 // is staged in LDS (face partners inside the tile are read there, others from
-// HBM), face-solved into a second LDS buffer and differentiated into the dB/dt of the
-// thread's face-B points, kept in registers for all slices.
-template <int N, int D, int T, bool CORR>
+// HBM), face-solved into a second LDS buffer and differentiated into the
+// dB/dt of the thread's face-B points, kept in registers for all slices.
+template <int N, int D, int T, bool CORR, class R>
 __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
     using G = IG<N, D>;
-    using TL = Tile<N, D, T>;
+    using TL = Tile<N, D, T, R>;
+    constexpr bool PLAIN = std::is_same<R, double>::value;
+    using Acc = typename std::conditional<PLAIN || CORR, double, float>::type;  // dB/dt
     constexpr int EPB = TL::EPB, M = G::M, NB = TL::NB;
-    __shared__ double sE[EPB * G::EP], sS[EPB * G::EP];
-    __shared__ double sB[CORR ? 1 : EPB * G::BP];
-    __shared__ double sDm[N * M], sS2F[M * N], sInv[T * T], sW[T];
+    __shared__ R sE[EPB * G::EP], sS[EPB * G::EP];
+    __shared__ R sB[CORR ? 1 : EPB * G::BP];
+    __shared__ R sS2F[M * N];
+    __shared__ double sDm[N * M], sInv[T * T], sW[T];
+    __shared__ Acc sDmH[PLAIN ? 1 : D][PLAIN ? 1 : N * M];  // dfp_to_sp / h_axis
+    __shared__ float sP[PLAIN ? 1 : T * T];                  // invader * w * dt
     __shared__ int sEl[EPB][3];
     __shared__ int sPerm[G::EP];
     for (int i = threadIdx.x; i < G::EP; i += BS) sPerm[i] = kPerm<N, D>.p[i];
     for (int i = threadIdx.x; i < N * M; i += BS) {
         sDm[i] = a.Dm[i];
-        sS2F[i] = a.S2F[i];
+        sS2F[i] = (R)a.S2F[i];
+        if (!PLAIN)
+            for (int c = 0; c < D; c++) sDmH[c][i] = (Acc)(a.Dm[i] / a.h[c]);
     }
-    if (threadIdx.x < T * T) sInv[threadIdx.x] = a.inv[threadIdx.x];
+    if (threadIdx.x < T * T) {
+        sInv[threadIdx.x] = a.inv[threadIdx.x];
+        if (!PLAIN) sP[threadIdx.x] = (float)(a.inv[threadIdx.x] * a.w[threadIdx.x % T] * a.dt);
+    }
     if (threadIdx.x < T) sW[threadIdx.x] = a.w[threadIdx.x];
     const int e0 = blockIdx.x * EPB;
     const int ne = min(EPB, a.nel - e0);
@@ -299,13 +324,14 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
         sEl[threadIdx.x][2] = lin / (a.n[0] * a.n[1]);
     }
     __syncthreads();
+    const R* Ein = (const R*)a.Ein;
 
-    double dBdt[NB][T];
+    Acc dBdt[NB][T];
     for (int t = 0; t < T; t++) {
         // A1. the tile's raw E of slice t (contiguous per component)
 #pragma unroll
         for (int e = 0; e < 3; e++) {
-            const double* src = a.Ein + eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + e0) * G::epts(e);
+            const R* src = Ein + eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + e0) * G::epts(e);
             for (int i = threadIdx.x; i < ne * G::epts(e); i += BS) {
                 int q = i / G::epts(e), p = i % G::epts(e);
                 sE[q * G::EP + G::eoff(e) + p] = src[i];
@@ -313,13 +339,13 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
         }
         __syncthreads();
         // A2. face solves
-        auto raw = [&](int e, const int el[3], const int pt[3]) -> double {
+        auto raw = [&](int e, const int el[3], const int pt[3]) -> R {
             int r[3] = {pt[0], pt[1], pt[2]};
             int lin = resolve<N, D>(a, e, el, r);
             int p = (r[2] * G::ecnt(e, 1) + r[1]) * G::ecnt(e, 0) + r[0];
             int q = lin - e0;
             if (q >= 0 && q < ne) return sE[q * G::EP + G::eoff(e) + p];
-            return a.Ein[eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + lin) * G::epts(e) + p];
+            return Ein[eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + lin) * G::epts(e) + p];
         };
         // items run point-major over the tile's elements, points in class order
         for (int idx = threadIdx.x; idx < ne * G::EP; idx += BS) {
@@ -328,14 +354,14 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
             int e = p < G::eoff(1) ? 0 : (p < G::eoff(2) ? 1 : 2);
             int pt[3];
             split_pt(p - G::eoff(e), G::ecnt(e, 0), G::ecnt(e, 1), pt);
-            sS[q * G::EP + p] = solved<N, D>(a, e, sEl[q], pt, raw);
+            sS[q * G::EP + p] = solved<N, D, R>(a, e, sEl[q], pt, raw);
         }
         __syncthreads();
         // B. dB/dt of slice t at the thread's face-B points
 #pragma unroll
         for (int j = 0; j < NB; j++) {
             int idx = threadIdx.x + j * BS;
-            double dB = 0;
+            Acc dB = 0;
             if (idx < ne * G::BP) {
                 // point-major over the tile's elements: a wave's lanes share
                 // the component
@@ -345,21 +371,23 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
                 split_pt(p - G::boff(d), G::bcnt(d, 0), G::bcnt(d, 1), pt);
                 const int a1 = d == 2 ? 0 : d + 1, a2 = a1 == 2 ? 0 : a1 + 1;
                 // E1 = E_{d+1} differentiated along d+2, E2 = E_{d+2} along d+1
-                const double* E1 = sS + q * G::EP + G::eoff(a1);
-                const double* E2 = sS + q * G::EP + G::eoff(a2);
-                double E;
+                const R* E1 = sS + q * G::EP + G::eoff(a1);
+                const R* E2 = sS + q * G::EP + G::eoff(a2);
+                Acc E;
                 for (int l = 0; l < M; l++) {
                     if (a2 < D) {
                         int r[3] = {pt[0], pt[1], pt[2]};
                         r[a2] = l;
-                        E = E1[(r[2] * G::ecnt(a1, 1) + r[1]) * G::ecnt(a1, 0) + r[0]];
-                        dB += E * sDm[pt[a2] * M + l] / a.h[a2];
+                        E = (Acc)E1[(r[2] * G::ecnt(a1, 1) + r[1]) * G::ecnt(a1, 0) + r[0]];
+                        if (PLAIN) dB += E * sDm[pt[a2] * M + l] / a.h[a2];
+                        else dB += E * sDmH[PLAIN ? 0 : a2][pt[a2] * M + l];
                     }
                     if (a1 < D) {
                         int r[3] = {pt[0], pt[1], pt[2]};
                         r[a1] = l;
-                        E = E2[(r[2] * G::ecnt(a2, 1) + r[1]) * G::ecnt(a2, 0) + r[0]];
-                        dB -= E * sDm[pt[a1] * M + l] / a.h[a1];
+                        E = (Acc)E2[(r[2] * G::ecnt(a2, 1) + r[1]) * G::ecnt(a2, 0) + r[0]];
+                        if (PLAIN) dB -= E * sDm[pt[a1] * M + l] / a.h[a1];
+                        else dB -= E * sDmH[PLAIN ? 0 : a1][pt[a1] * M + l];
                     }
                 }
             }
@@ -391,67 +419,74 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
             if (!Bp[j]) continue;
             double dB = 0;
 #pragma unroll
-            for (int t = 0; t < T; t++) dB += dBdt[j][t] * sW[t] * a.dt;
+            for (int t = 0; t < T; t++) dB += (double)dBdt[j][t] * sW[t] * a.dt;
             *Bp[j] = B_old[j] - dB;
         }
         return;
     }
+    R* Eout = (R*)a.Eout;
     for (int t = 0; t < T; t++) {
         // C1. predictor slice t
 #pragma unroll
         for (int j = 0; j < NB; j++) {
             int idx = threadIdx.x + j * BS;
             if (idx < ne * G::BP) {
-                double dB = 0;
+                Acc dB = 0;
+                if (PLAIN) {
 #pragma unroll
-                for (int l = 0; l < T; l++) dB += dBdt[j][l] * sInv[t * T + l] * sW[l] * a.dt;
-                sB[(idx - (idx / ne) * ne) * G::BP + idx / ne] = B_old[j] - dB;
+                    for (int l = 0; l < T; l++) dB += dBdt[j][l] * sInv[t * T + l] * sW[l] * a.dt;
+                } else {
+#pragma unroll
+                    for (int l = 0; l < T; l++) dB += dBdt[j][l] * (Acc)sP[t * T + l];
+                }
+                sB[(idx - (idx / ne) * ne) * G::BP + idx / ne] = (R)B_old[j] - (R)dB;
             }
         }
         __syncthreads();
         // C2. its raw E
+        const R vel[3] = {(R)a.vel[0], (R)a.vel[1], (R)a.vel[2]};
 #pragma unroll
         for (int e = 0; e < 3; e++) {
-            double* dst = a.Eout + eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + e0) * G::epts(e);
+            R* dst = Eout + eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + e0) * G::epts(e);
             for (int i = threadIdx.x; i < ne * G::epts(e); i += BS) {
                 int q = i / G::epts(e), pe = i % G::epts(e);
                 int pt[3];
                 split_pt(pe, G::ecnt(e, 0), G::ecnt(e, 1), pt);
-                const double* Bs = sB + q * G::BP;
-                auto bget = [&](int d, const int r3[3]) {
+                const R* Bs = sB + q * G::BP;
+                auto bget = [&](int d, const int r3[3]) -> R {
                     return Bs[G::boff(d) + (r3[2] * G::bcnt(d, 1) + r3[1]) * G::bcnt(d, 0) + r3[0]];
                 };
-                dst[i] = edge_e<N, D>(e, pt, sS2F, a.vel, bget);
+                dst[i] = edge_e<N, D, R>(e, pt, sS2F, vel, bget);
             }
         }
         __syncthreads();
     }
 }
 
-template <int N, int D, int T>
+template <int N, int D, int T, class R>
 size_t ws_bytes(int nel) {
-    return 2 * (size_t)T * nel * IG<N, D>::EP * sizeof(double);
+    return 2 * (size_t)T * nel * IG<N, D>::EP * sizeof(R);
 }
 
-template <int N, int D, int T>
+template <int N, int D, int T, class R>
 int run_step(spdk_ctx* c, IArgs a, void* ws) {
-    using TL = Tile<N, D, T>;
+    using TL = Tile<N, D, T, R>;
     static_assert(perm_is_permutation<N, D>(), "edge-point order must be a permutation");
     const size_t half = (size_t)T * a.nel * IG<N, D>::EP;
-    double* buf[2] = {(double*)ws, (double*)ws + half};
+    R* buf[2] = {(R*)ws, (R*)ws + half};
     const unsigned nb = (unsigned)((a.nel + TL::EPB - 1) / TL::EPB);
     const unsigned nstart = (unsigned)std::min<size_t>(((size_t)a.nel * IG<N, D>::EP + BS - 1) / BS, 65536);
     a.Eout = buf[0];
     a.Ein = nullptr;
-    hipLaunchKernelGGL((k_istart<N, D, T>), dim3(nstart), dim3(BS), 0, c->stream, a);
+    hipLaunchKernelGGL((k_istart<N, D, T, R>), dim3(nstart), dim3(BS), 0, c->stream, a);
     SPDK_LAUNCH_CHECK("induction istart");
     for (int s = 0; s < T; s++) {
         a.Ein = buf[s & 1];
         a.Eout = buf[(s + 1) & 1];
         bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
-        if (s < T - 1) hipLaunchKernelGGL((k_isweep<N, D, T, false>), dim3(nb), dim3(BS), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_isweep<N, D, T, true>), dim3(nb), dim3(BS), 0, c->stream, a);
+        if (s < T - 1) hipLaunchKernelGGL((k_isweep<N, D, T, false, R>), dim3(nb), dim3(BS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_isweep<N, D, T, true, R>), dim3(nb), dim3(BS), 0, c->stream, a);
         SPDK_LAUNCH_CHECK("induction isweep");
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
     }
@@ -472,10 +507,14 @@ typedef int (*istep_fn)(spdk_ctx*, IArgs, void*);
 typedef size_t (*iws_fn)(int);
 struct IVariant {
     int n, dim, T;
-    istep_fn step;
-    iws_fn ws;
+    istep_fn step[2];  // FP64, mixed
+    iws_fn ws[2];
 };
-#define SPDK_IVARIANT(Nn, D) {Nn, D, Nn, run_step<Nn, D, Nn>, ws_bytes<Nn, D, Nn>}
+#define SPDK_IVARIANT(Nn, D)                                                                  \
+    {                                                                                         \
+        Nn, D, Nn, {run_step<Nn, D, Nn, double>, run_step<Nn, D, Nn, float>},                 \
+            {ws_bytes<Nn, D, Nn, double>, ws_bytes<Nn, D, Nn, float>}                        \
+    }
 const IVariant kIVariants[] = {SPDK_IVARIANT(2, 2), SPDK_IVARIANT(3, 2), SPDK_IVARIANT(4, 2), SPDK_IVARIANT(5, 2),
                                SPDK_IVARIANT(2, 3), SPDK_IVARIANT(3, 3), SPDK_IVARIANT(4, 3)};
 
@@ -494,11 +533,7 @@ bool bc_supported(spdk_ctx* c) {
     return true;
 }
 
-}  // namespace
-
-extern "C" {
-
-size_t spdk_induction_fused_workspace_bytes(spdk_ctx* c, const int N[3]) {
+size_t workspace_bytes(spdk_ctx* c, const int N[3], int mixed) {
     const IVariant* v = c ? find_ivariant(c) : nullptr;
     if (!v || !bc_supported(c)) {
         set_error("fused induction path: ADER, x-y or x-y-z active, p = 1..4 (1..3 in 3-D), periodic or gradfree "
@@ -508,11 +543,11 @@ size_t spdk_induction_fused_workspace_bytes(spdk_ctx* c, const int N[3]) {
     size_t nel = 1;
     for (int d = 0; d < 3; d++) nel *= (size_t)(c->prm.active[d] ? N[d] - 2 * c->ngh[d] : 1);
     if (nel == 0 || nel >= (1ull << 31)) { set_error("fused induction: element count"); return 0; }
-    return v->ws((int)nel);
+    return v->ws[mixed]((int)nel);
 }
 
-int spdk_induction_step_fused(spdk_ctx* c, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz,
-                              spdk_sd_shape sz, void* ws, double dt) {
+int step(spdk_ctx* c, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz, spdk_sd_shape sz, void* ws,
+         double dt, int mixed) {
     if (!c || !Bx || !By || !Bz || !ws) { set_error("fused induction: null argument"); return SPDK_E_ARG; }
     const IVariant* v = find_ivariant(c);
     if (!v || !bc_supported(c)) { set_error("fused induction: unsupported configuration"); return SPDK_E_CONFIG; }
@@ -553,7 +588,25 @@ int spdk_induction_step_fused(spdk_ctx* c, double* Bx, spdk_sd_shape sx, double*
     if (nel >= (1ull << 31)) { set_error("fused induction: too many elements"); return SPDK_E_SHAPE; }
     a.nel = (int)nel;
     std::strncpy(c->kernel_name, "k_isweep", sizeof(c->kernel_name) - 1);
-    return v->step(c, a, ws);
+    return v->step[mixed](c, a, ws);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t spdk_induction_fused_workspace_bytes(spdk_ctx* c, const int N[3]) { return workspace_bytes(c, N, 0); }
+
+int spdk_induction_step_fused(spdk_ctx* c, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz,
+                              spdk_sd_shape sz, void* ws, double dt) {
+    return step(c, Bx, sx, By, sy, Bz, sz, ws, dt, 0);
+}
+
+size_t spdk_induction_mixed_workspace_bytes(spdk_ctx* c, const int N[3]) { return workspace_bytes(c, N, 1); }
+
+int spdk_induction_step_mixed(spdk_ctx* c, double* Bx, spdk_sd_shape sx, double* By, spdk_sd_shape sy, double* Bz,
+                              spdk_sd_shape sz, void* ws, double dt) {
+    return step(c, Bx, sx, By, sy, Bz, sz, ws, dt, 1);
 }
 
 }  // extern "C"

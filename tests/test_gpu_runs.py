@@ -326,3 +326,56 @@ def test_mixed_residual_converges_to_fixed_point(tmp_path):
     mean, mx = _rel(a, b, i, 0)
     assert mean < 1e-5, (mean, mx)
     assert op.mass_drift(b) < 1e-13
+
+
+# ---- induction in mixed precision (FP32 edge field and predictor, FP64 corrector)
+def _b2(d, i):
+    g = op.grid(d)
+    return np.stack([op.load_sd(d, g, i, v, "B2_cv", 4) for v in range(3)])
+
+
+@pytest.mark.parametrize("name,args", [
+    ("loop2d_p3", LOOP2D + ["mesh/nx1=24", "mesh/nx2=20", "time/tlim=0.2", "output/dt=0.1"]),
+    ("loop2d_p4_gradfree", LOOP2D + ["mesh/p=4", "mesh/nx1=12", "mesh/nx2=12", "mesh/x1_bc=gradfree",
+                                     "induction/cx=0.25", "time/tlim=0.2", "output/dt=0.1"]),
+    ("loop3d_p3", ["-i", IND, "job/fallback=false", "mesh/nx1=8", "mesh/nx2=8", "mesh/nx3=6", "induction/v3=0.3",
+                   "time/tlim=0.1", "output/dt=0.05"]),
+])
+def test_induction_mixed_vs_fp64(tmp_path, name, args):
+    """The mixed form against the FP64 fused form on the same run: B cell
+    averages within 2e-5 of the largest |B| and magnetic energy within 2e-5
+    relative (FP32 edge values; the p=4 loop cut by a gradfree wall is the
+    worst case, ~5e-6), div B at FP64 round-off (every update is a discrete
+    curl of one set of edge values, accumulated in FP64)."""
+    a, b = str(tmp_path / "fp64"), str(tmp_path / "mixed")
+    ta = op.run_driver(args, a)
+    tb = op.run_driver(args + ["job/precision=mixed"], b)
+    assert "induction path: fused mixed" in tb
+    i = op.output_indices(a, "B2_cv")[-1]
+    x, y = _b2(a, i), _b2(b, i)
+    rel = float(np.abs(x - y).max() / np.abs(x).max())
+    da, db = _diag(ta), _diag(tb)
+    print(f"{name}: max |dB| / max |B| = {rel:.3e}; div B fp64 {da[-1][1]:.3e} mixed {db[-1][1]:.3e}; "
+          f"energy rel diff {abs(da[-1][2] - db[-1][2]) / da[-1][2]:.3e}")
+    assert rel < 2e-5
+    bmax, h = float(np.abs(x).max()), 2.0 / 24
+    for (_, m1, e1), (_, m2, e2) in zip(da, db):
+        assert m2 < 1e-11 * bmax / h * 100 and m2 < 10 * max(m1, 1e-300) + 1e-13
+        assert abs(e1 - e2) <= 2e-5 * e1
+
+
+def test_induction_mixed_matched_l1_one_period(tmp_path):
+    """Matched error on the field loop: after one period of the advection
+    (v = (1, 0.5) on [0,2]^2, t = 4) the exact solution is the initial
+    field. The mixed form's L1 error of the B cell averages against it is
+    within 1.1x of the FP64 form's; div B stays at round-off in both."""
+    args = LOOP2D + ["mesh/nx1=48", "mesh/nx2=48", "induction/nu=0", "time/tlim=4", "output/dt=4"]
+    res = {}
+    for form, extra in (("fp64", []), ("mixed", ["job/precision=mixed"])):
+        d = str(tmp_path / form)
+        text = op.run_driver(args + extra, d)
+        b0, b1 = _b2(d, 0)[:2], _b2(d, 1)[:2]
+        res[form] = (float(np.abs(b1 - b0).mean()), _diag(text)[-1])
+        print(form, res[form], re.findall(r"evolution: .*", text))
+    assert res["mixed"][0] <= 1.1 * res["fp64"][0], res
+    assert res["mixed"][1][1] < 1e-10 and res["fp64"][1][1] < 1e-10, res

@@ -45,7 +45,7 @@ namespace {
 
 constexpr int BS = 256;
 constexpr int kMaxT = 5;
-constexpr int kLdsBudget = 32 * 1024;  // bytes of element tiles per workgroup
+constexpr int kLdsBudget = 40 * 1024;  // bytes of element tiles per workgroup
 
 // Point counts of one configuration: N solution points per active axis; D = 3,
 // or D = 2 with z inactive (one point along z).
@@ -69,8 +69,9 @@ struct IG {
 template <int N, int D, int T, class R>
 struct Tile {
     using G = IG<N, D>;
-    // per element: one slice of raw and of face-solved E, one predictor slice
-    static constexpr int PER = 2 * G::EP + G::BP;
+    // per element: raw and face-solved E of every slice (one predictor slice
+    // reuses the raw buffer)
+    static constexpr int PER = 2 * T * G::EP;
     // LDS bound, and at most two face-B points per thread (their dB/dt of
     // every slice stay in registers)
     static constexpr int EPB_LDS = kLdsBudget / (int)(PER * sizeof(R));
@@ -284,19 +285,21 @@ __global__ void __launch_bounds__(BS) k_istart(IArgs a) {
 }
 
 // One Picard sweep (CORR = false: predictor slices and their raw E; CORR =
-// true: the corrector into B). Slice by slice: the tile's raw E of the slice
-// is staged in LDS (face partners inside the tile are read there, others from
-// HBM), face-solved into a second LDS buffer and differentiated into the
-// dB/dt of the thread's face-B points, kept in registers for all slices.
+// true: the corrector into B). The tile's raw E of every slice is staged in
+// LDS (face partners inside the tile are read there, others from HBM),
+// face-solved into a second LDS buffer and differentiated into the dB/dt of
+// the thread's face-B points, kept in registers for all slices; the
+// predictor slices then go through the raw buffer one at a time.
 template <int N, int D, int T, bool CORR, class R>
 __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
     using G = IG<N, D>;
     using TL = Tile<N, D, T, R>;
     constexpr bool PLAIN = std::is_same<R, double>::value;
     using Acc = typename std::conditional<PLAIN || CORR, double, float>::type;  // dB/dt
-    constexpr int EPB = TL::EPB, M = G::M, NB = TL::NB;
-    __shared__ R sE[EPB * G::EP], sS[EPB * G::EP];
-    __shared__ R sB[CORR ? 1 : EPB * G::BP];
+    constexpr int EPB = TL::EPB, M = G::M, NB = TL::NB, EPT = T * G::EP;
+    static_assert(G::BP <= EPT, "a predictor slice fits in the raw-E buffer");
+    __shared__ R sE[EPB * EPT], sS[EPB * EPT];  // (element, slice, point)
+    R* const sB = sE;                           // predictor slice, after the face solves
     __shared__ R sS2F[M * N];
     __shared__ double sDm[N * M], sInv[T * T], sW[T];
     __shared__ Acc sDmH[PLAIN ? 1 : D][PLAIN ? 1 : N * M];  // dfp_to_sp / h_axis
@@ -323,57 +326,60 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
         sEl[threadIdx.x][1] = (lin / a.n[0]) % a.n[1];
         sEl[threadIdx.x][2] = lin / (a.n[0] * a.n[1]);
     }
-    __syncthreads();
     const R* Ein = (const R*)a.Ein;
 
-    Acc dBdt[NB][T];
-    for (int t = 0; t < T; t++) {
-        // A1. the tile's raw E of slice t (contiguous per component)
+    // A1. the tile's raw E of every slice (contiguous per component and slice)
 #pragma unroll
-        for (int e = 0; e < 3; e++) {
+    for (int e = 0; e < 3; e++)
+        for (int t = 0; t < T; t++) {
             const R* src = Ein + eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + e0) * G::epts(e);
             for (int i = threadIdx.x; i < ne * G::epts(e); i += BS) {
                 int q = i / G::epts(e), p = i % G::epts(e);
-                sE[q * G::EP + G::eoff(e) + p] = src[i];
+                sE[(q * T + t) * G::EP + G::eoff(e) + p] = src[i];
             }
         }
-        __syncthreads();
-        // A2. face solves
-        auto raw = [&](int e, const int el[3], const int pt[3]) -> R {
-            int r[3] = {pt[0], pt[1], pt[2]};
-            int lin = resolve<N, D>(a, e, el, r);
-            int p = (r[2] * G::ecnt(e, 1) + r[1]) * G::ecnt(e, 0) + r[0];
-            int q = lin - e0;
-            if (q >= 0 && q < ne) return sE[q * G::EP + G::eoff(e) + p];
-            return Ein[eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + lin) * G::epts(e) + p];
+    __syncthreads();
+    // A2. face solves of every slice; items run point-major over the tile's
+    // elements, points in class order within a slice
+    for (int idx = threadIdx.x; idx < ne * EPT; idx += BS) {
+        int rank = idx / ne, q = idx - rank * ne;
+        int t = rank / G::EP;
+        int p = sPerm[rank - t * G::EP];
+        int e = p < G::eoff(1) ? 0 : (p < G::eoff(2) ? 1 : 2);
+        int pt[3];
+        split_pt(p - G::eoff(e), G::ecnt(e, 0), G::ecnt(e, 1), pt);
+        auto raw = [&](int ec, const int el[3], const int pc[3]) -> R {
+            int r[3] = {pc[0], pc[1], pc[2]};
+            int lin = resolve<N, D>(a, ec, el, r);
+            int pp = (r[2] * G::ecnt(ec, 1) + r[1]) * G::ecnt(ec, 0) + r[0];
+            int qq = lin - e0;
+            if (qq >= 0 && qq < ne) return sE[(qq * T + t) * G::EP + G::eoff(ec) + pp];
+            return Ein[eslab<N, D, T>(ec, a.nel) + ((size_t)t * a.nel + lin) * G::epts(ec) + pp];
         };
-        // items run point-major over the tile's elements, points in class order
-        for (int idx = threadIdx.x; idx < ne * G::EP; idx += BS) {
-            int rank = idx / ne, q = idx - rank * ne;
-            int p = sPerm[rank];
-            int e = p < G::eoff(1) ? 0 : (p < G::eoff(2) ? 1 : 2);
-            int pt[3];
-            split_pt(p - G::eoff(e), G::ecnt(e, 0), G::ecnt(e, 1), pt);
-            sS[q * G::EP + p] = solved<N, D, R>(a, e, sEl[q], pt, raw);
-        }
-        __syncthreads();
-        // B. dB/dt of slice t at the thread's face-B points
+        sS[(q * T + t) * G::EP + p] = solved<N, D, R>(a, e, sEl[q], pt, raw);
+    }
+    __syncthreads();
+    // B. dB/dt of every slice at the thread's face-B points
+    Acc dBdt[NB][T];
 #pragma unroll
-        for (int j = 0; j < NB; j++) {
-            int idx = threadIdx.x + j * BS;
-            Acc dB = 0;
-            if (idx < ne * G::BP) {
-                // point-major over the tile's elements: a wave's lanes share
-                // the component
-                int p = idx / ne, q = idx - p * ne;
-                int d = p < G::boff(1) ? 0 : (p < G::boff(2) ? 1 : 2);
-                int pt[3];
-                split_pt(p - G::boff(d), G::bcnt(d, 0), G::bcnt(d, 1), pt);
-                const int a1 = d == 2 ? 0 : d + 1, a2 = a1 == 2 ? 0 : a1 + 1;
+    for (int j = 0; j < NB; j++) {
+        int idx = threadIdx.x + j * BS;
+#pragma unroll
+        for (int t = 0; t < T; t++) dBdt[j][t] = 0;
+        if (idx < ne * G::BP) {
+            // point-major over the tile's elements: a wave's lanes share the
+            // component
+            int p = idx / ne, q = idx - p * ne;
+            int d = p < G::boff(1) ? 0 : (p < G::boff(2) ? 1 : 2);
+            int pt[3];
+            split_pt(p - G::boff(d), G::bcnt(d, 0), G::bcnt(d, 1), pt);
+            const int a1 = d == 2 ? 0 : d + 1, a2 = a1 == 2 ? 0 : a1 + 1;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
                 // E1 = E_{d+1} differentiated along d+2, E2 = E_{d+2} along d+1
-                const R* E1 = sS + q * G::EP + G::eoff(a1);
-                const R* E2 = sS + q * G::EP + G::eoff(a2);
-                Acc E;
+                const R* E1 = sS + (q * T + t) * G::EP + G::eoff(a1);
+                const R* E2 = sS + (q * T + t) * G::EP + G::eoff(a2);
+                Acc dB = 0, E;
                 for (int l = 0; l < M; l++) {
                     if (a2 < D) {
                         int r[3] = {pt[0], pt[1], pt[2]};
@@ -390,10 +396,9 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
                         else dB -= E * sDmH[PLAIN ? 0 : a1][pt[a1] * M + l];
                     }
                 }
+                dBdt[j][t] = dB;
             }
-            dBdt[j][t] = dB;
         }
-        __syncthreads();
     }
 
     // the thread's face-B points: old values, then the corrector or the
@@ -425,7 +430,9 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
         return;
     }
     R* Eout = (R*)a.Eout;
+    const R vel[3] = {(R)a.vel[0], (R)a.vel[1], (R)a.vel[2]};
     for (int t = 0; t < T; t++) {
+        __syncthreads();  // the raw buffer is free (face solves done / previous slice's E written)
         // C1. predictor slice t
 #pragma unroll
         for (int j = 0; j < NB; j++) {
@@ -444,7 +451,6 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
         }
         __syncthreads();
         // C2. its raw E
-        const R vel[3] = {(R)a.vel[0], (R)a.vel[1], (R)a.vel[2]};
 #pragma unroll
         for (int e = 0; e < 3; e++) {
             R* dst = Eout + eslab<N, D, T>(e, a.nel) + ((size_t)t * a.nel + e0) * G::epts(e);
@@ -459,7 +465,6 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
                 dst[i] = edge_e<N, D, R>(e, pt, sS2F, vel, bget);
             }
         }
-        __syncthreads();
     }
 }
 

@@ -25,6 +25,10 @@ Matched L1 error, measured in the same run for every form:
 A form matches when both errors are within 1.1x of the plain FP64 form's.
 `value` is the fastest matching form (named in config.form).
 
+Also measured (the "induction" object): the 2-D field-loop induction
+configuration, 1024^2 elements p=3, stage sequence vs fused FP64 vs mixed,
+with its one-period matched-error check.
+
 Prints one JSON line with the headline throughput, the per-form figures, the
 HBM roofline of the dominant kernel (the mixed Picard sweep; device time from
 HIP events on its stream, HBM traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -182,6 +186,43 @@ def accuracy(forms):
             "forms": out}
 
 
+def induction_loop(n=1024, steps=6, warmup=2):
+    """The 2-D field-loop induction configuration: n^2 elements, p=3,
+    v = (1, 0.5), A_z = 1e-3 (0.3 - r) inside r < 0.3, periodic [0,2]^2 (the
+    [-1,1]^2 box shifted by (1,1)). DOF-updates/s of the stage sequence, the
+    fused FP64 step and the mixed step; and the matched-error check: after
+    one period (t = 4) the exact field is the initial one; L1 of the B cell
+    averages against it at 48^2 for the FP64 and the mixed form."""
+    import numpy as np
+    import spd_k_amd
+    from spd_k_amd.outputs import grid, load_sd
+    base = ["-i", os.path.join(ROOT, "tests", "inputs", "induction_loop.athinput"), "job/fallback=false",
+            "mesh/nx3=1", "mesh/x1len=2", "mesh/x2len=2", "induction/v1=1.0", "induction/v2=0.5", "induction/v3=0.0",
+            "induction/cx=1.0", "induction/cy=1.0", "induction/nu=0"]
+    forms = {"stages_fp64": ["job/path=stages"], "fused_fp64": [], "fused_mixed": ["job/precision=mixed"]}
+    out = {"workload": f"2-D field-loop induction {n}^2 elements p=3, v=(1,0.5), ADER step", "forms": {}}
+    with tempfile.TemporaryDirectory() as d:
+        for name, extra in forms.items():
+            r = spd_k_amd.run(base + [f"mesh/nx1={n}", f"mesh/nx2={n}", "time/tlim=8", f"time/nlim={steps}",
+                                      f"job/warmup={warmup}", "output/dt=-1"] + extra, os.path.join(d, name))
+            out["forms"][name] = {"value": r["dofs"] * r["steps"] / r["evolution_seconds"],
+                                  "ms_per_step": r["evolution_seconds"] / r["steps"] * 1e3}
+        l1 = {}
+        for name in ("fused_fp64", "fused_mixed"):
+            od = os.path.join(d, "period_" + name)
+            spd_k_amd.run(base + ["mesh/nx1=48", "mesh/nx2=48", "time/tlim=4", "output/dt=4"] + forms[name], od)
+            g = grid(od)
+            b0 = np.stack([load_sd(od, g, 0, v, "B2_cv", 4) for v in (0, 1)])
+            b1 = np.stack([load_sd(od, g, 1, v, "B2_cv", 4) for v in (0, 1)])
+            l1[name] = float(np.abs(b1 - b0).mean())
+    ref = out["forms"]["fused_fp64"]["value"]
+    for f in out["forms"].values():
+        f["vs_fused_fp64"] = f["value"] / ref
+    out["one_period_l1_48"] = l1
+    out["mixed_l1_vs_fp64"] = l1["fused_mixed"] / l1["fused_fp64"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--no-induction", action="store_true", help="skip the 2-D induction configuration")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the SD update here runs on a single GPU (multi-GPU halo exchange is a "
@@ -265,6 +307,7 @@ def main():
         "accuracy": acc,
         "roofline": roof,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(),
+        "induction": None if a.no_induction else induction_loop(),
     }
     sys.stdout.flush()
     os.dup2(json_fd, 1)

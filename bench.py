@@ -223,6 +223,59 @@ def induction_loop(n=1024, steps=6, warmup=2):
     return out
 
 
+def all_configs(steps=6, warmup=2):
+    """DOF-updates/s of every synthetic configuration on one GPU, each form
+    the driver offers for it (`--all-configs`; the default run measures the
+    3-D KH workload and the 2-D induction one):
+      cfg1 2-D sine advection 32^2 p=3, ADER, one period (t = 1), L1 of
+           density against the exact solution;
+      cfg2 Sod (256^2, x gradfree, y periodic) and the central-cells Sedov
+           blast (256^2, P = 1e-5 background), p=3, NAD + MUSCL fallback (the
+           stage path), `steps` steps;
+      cfg4 2-D KH 512^2 p=4 (seeded perturbation, CFL 0.5 without the
+           fallback), FP64 / mixed / residual (1e-6 and 1e-10) forms.
+    cfg3 and cfg5 are the default run's workloads."""
+    import spd_k_amd
+    from spd_k_amd.outputs import l1_vs_exact_sine, output_indices
+    inp = lambda f: os.path.join(ROOT, "tests", "inputs", f)
+    out = {}
+
+    def timed(args, d, full=False):
+        extra = [] if full else [f"time/nlim={steps}", f"job/warmup={warmup}", "output/dt=-1"]
+        r = spd_k_amd.run(args + extra, d)
+        rec = {"value": r["dofs"] * r["steps"] / r["evolution_seconds"],
+               "ms_per_step": r["evolution_seconds"] / r["steps"] * 1e3, "steps": r["steps"], "dofs": r["dofs"]}
+        if r.get("picard_steps"):
+            rec["sweeps_per_step"] = r["picard_sweeps"] / r["picard_steps"]
+        return rec
+
+    with tempfile.TemporaryDirectory() as d:
+        sine = ["-i", inp("sine_wave.athinput"), "mesh/nx1=32", "mesh/nx2=32", "mesh/nx3=1", "problem/amp=0.1",
+                "time/tlim=1.0", "output/dt=1.0"]
+        c1 = {}
+        for name, extra in (("plain_fp64", FORMS["plain_fp64"]), ("mixed_fp32", FORMS["mixed_fp32"]),
+                            ("residual_fp32", FORMS["residual_fp32"])):
+            od = os.path.join(d, "c1_" + name)
+            c1[name] = timed(sine + extra, od, full=True)
+            c1[name]["l1_vs_exact"] = l1_vs_exact_sine(od, output_indices(od)[-1], 1.0, amp=0.1)
+        out["cfg1_sine2d_32_p3"] = c1
+        sod = ["-i", inp("sod.athinput"), "mesh/nx1=256", "mesh/nx2=256", "mesh/x2_bc=periodic", "time/tlim=0.2"]
+        sedov = ["-i", inp("sine_wave.athinput"), "problem/problem=sedov_cells", "job/fallback=true", "mesh/nx1=256",
+                 "mesh/nx2=256", "mesh/nx3=1", "hydro/gamma=1.4", "time/tlim=0.05"]
+        out["cfg2_sod_256_p3_fallback"] = {"stages_fp64": timed(sod, os.path.join(d, "c2a"))}
+        out["cfg2_sedov_256_p3_fallback"] = {"stages_fp64": timed(sedov, os.path.join(d, "c2b"))}
+        kh2 = ["-i", inp("kelvin_helmholtz.athinput"), "job/fallback=false", "mesh/nx1=512", "mesh/nx2=512",
+               "mesh/nx3=1", "mesh/p=4", "problem/perturb=random", "problem/amp=0.01", "time/cfl=0.5", "time/tlim=1"]
+        c4 = {}
+        for name, extra in (("plain_fp64", FORMS["plain_fp64"]), ("mixed_fp32", FORMS["mixed_fp32"]),
+                            ("residual_fp32", FORMS["residual_fp32"]),
+                            ("residual_fp32_tol1e-10", ["job/precision=mixed", "job/picard=residual",
+                                                        "job/picard_tol=1e-10"])):
+            c4[name] = timed(kh2 + extra, os.path.join(d, "c4_" + name))
+        out["cfg4_kh2d_512_p4"] = c4
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +287,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-induction", action="store_true", help="skip the 2-D induction configuration")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="print one JSON object with the other synthetic configurations instead")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the SD update here runs on a single GPU (multi-GPU halo exchange is a "
@@ -251,6 +306,12 @@ def main():
     if "learned_fp32" in forms and not os.path.exists(WEIGHTS):
         raise SystemExit(f"{WEIGHTS} is missing: train it with the driver's job/mode=train_guess")
     torch.cuda.synchronize()
+    if a.all_configs:
+        res = all_configs(a.steps, a.warmup)
+        sys.stdout.flush()
+        os.dup2(json_fd, 1)
+        print(json.dumps(res), flush=True)
+        return
     res = {}
     for name in forms:
         with tempfile.TemporaryDirectory() as d:

@@ -321,6 +321,12 @@ int spdk_mixed_attach_mlp(spdk_ctx* ctx, spdk_mlp* net);
  * rows [e0, e0 + nrows) of features (X, nrows x din) and of targets from the
  * current predictor buffer (Y, nrows x dout); X or Y may be NULL. */
 int spdk_mixed_iterate(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, int nsweeps);
+/* The predictor buffer (after spdk_mixed_iterate) as FP64 space-time slices in
+ * the stage layout of U_ader (T, >= 5 variables, elements, points; the
+ * reference's U_ader_sp, src/hydro_ader.hpp:291-305): U = mean + deviation,
+ * further variable slots 0. With it the FV update with trouble detection and
+ * the MUSCL fallback (spdk_fv_* entries) runs on a mixed-precision predictor. */
+int spdk_mixed_export_predictor(spdk_ctx* ctx, void* workspace, double* U_ader, spdk_sd_shape s_ader);
 int spdk_mixed_guess_data(spdk_ctx* ctx, void* workspace, const int N[3], double dt, int e0, int nrows, float* X,
                           long ldx, float* Y, long ldy);
 

@@ -222,7 +222,8 @@ struct HydroRun {
                 CK(spdk_mixed_attach_mlp(ctx, net));
                 pc.guess = SPDK_GUESS_MLP;
             }
-        } else if (!fused) {
+        }
+        if (!fused && (!mixed || cfg.fallback)) {  // the stage arrays (with fallback, also behind the mixed predictor)
             U_cv = mem.alloc(ctx, nsp);
             U_ader = mem.alloc(ctx, sd_count(s_ader));
             if (cfg.integrator == INT_RK) U0 = mem.alloc(ctx, nsp);
@@ -231,7 +232,7 @@ struct HydroRun {
                 U_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
                 if (d) F_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
             }
-        } else {
+        } else if (fused) {
             int N[3] = {X.n_total, Y.n_total, Z.n_total};
             size_t b = spdk_fused_workspace_bytes(ctx, N);
             if (b == 0) throw std::runtime_error(std::string("fused workspace: ") + spdk_last_error());
@@ -353,6 +354,20 @@ struct HydroRun {
         return next;
     }
     double step_raw(bool output_due) {
+        if (mixed && cfg.fallback) {
+            // the p predictor updates in mixed precision, then the last flux
+            // evaluation and the FV update with trouble detection and the
+            // MUSCL fallback in FP64 (src/hydro_ader.hpp:291-305, :500-586)
+            CK(spdk_mixed_iterate(ctx, U_sp, s_sp, ws, dt, T - 1));
+            CK(spdk_mixed_export_predictor(ctx, ws, U_ader, s_ader));
+            solve_fluxes();
+            fv_update();
+            CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
+            CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
+            double next = 0;
+            CK(spdk_min_dt(ctx, W_cv, s_sp, &next));
+            return next;
+        }
         if (mixed) {
             double next = 0;
             CK(spdk_hydro_step_mixed(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr,
@@ -579,7 +594,7 @@ struct HydroRun {
         res->t_final = t;
         std::printf("\nevolution: %d steps, %g s (%g ms/step), %g zone-cycles/s\n", res->steps, sec,
                     res->steps ? sec / res->steps * 1e3 : 0.0, res->dof_updates_per_s);
-        if (mixed) {
+        if (mixed && !cfg.fallback) {
             res->picard_steps = pst.steps;
             res->picard_sweeps = pst.sweeps;
             res->picard_fallbacks = pst.fallbacks;

@@ -157,6 +157,12 @@ std::string configure(Params& pin, RunConfig& c) {
             return "job/precision=mixed covers the induction ADER step without the FV update (fixed Picard count)";
         return "";
     }
+    if (c.system == "hydro" && c.precision == "mixed" && c.fallback) {
+        // mixed predictor, FP64 flux evaluation and FV update with the fallback
+        if (c.integrator != INT_ADER || c.picard != "fixed" || c.guess != "zero")
+            return "job/precision=mixed with job/fallback=true covers the ADER step with the fixed Picard count";
+        return "";
+    }
     if (c.precision == "mixed" || c.picard == "residual" || c.guess == "mlp") {
         if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
             return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step without fallback";

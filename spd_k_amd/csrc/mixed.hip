@@ -983,6 +983,18 @@ __global__ void __launch_bounds__(256) k_mrestart(const float* __restrict__ d_ol
     if (go && s == 0) ver[(size_t)wb * E + e] = cur;
 }
 
+// The predictor buffer as FP64 slices in the stage layout (t, v, element,
+// point): U = mu + d for the five physical variables, 0 in any further slot.
+__global__ void k_mexport(const float* __restrict__ D, const double* __restrict__ mu, double* __restrict__ U, int nt,
+                          int nv, size_t slabE, size_t tpe, size_t total) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i % slabE, tv = i / slabE;
+        const int v = (int)(tv % nv), t = (int)(tv / nv);
+        const size_t E = slabE / tpe;
+        U[i] = v < NVP ? mu[(size_t)v * E + r / tpe] + (double)D[((size_t)t * NVP + v) * slabE + r] : 0.0;
+    }
+}
+
 __global__ void k_fill_u8(unsigned char* p, size_t n, unsigned char v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1690,6 +1702,24 @@ int spdk_mixed_predictor_buffer(spdk_ctx* c, void* ws, const int N[3], float** D
 }
 
 static int check_mixed(spdk_ctx* c, const void* U_sp, const spdk_sd_shape& s, const void* ws, const MVariant** out);
+
+int spdk_mixed_export_predictor(spdk_ctx* c, void* ws, double* U_ader, spdk_sd_shape s) {
+    const MVariant* v = c ? find_mvariant(c) : nullptr;
+    if (!v || !ws || !U_ader) { set_error("mixed export: no variant, workspace or output"); return SPDK_E_ARG; }
+    const int tpe = v->dim == 1 ? v->n : (v->dim == 2 ? v->n * v->n : v->n * v->n * v->n);
+    if (s.nt != c->T || s.nv < NVP || (size_t)s.nz * s.ny * s.nx != (size_t)tpe) {
+        set_error("mixed export: U_ader shape (T, >= 5 variables, solution points)");
+        return SPDK_E_SHAPE;
+    }
+    const size_t E = (size_t)s.Nx * s.Ny * s.Nz;
+    MLayout L = v->layout(E);
+    const size_t total = (size_t)s.nt * s.nv * E * tpe;
+    hipLaunchKernelGGL(k_mexport, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65536)), dim3(256), 0, c->stream,
+                       (const float*)((char*)ws + L.D), (const double*)((char*)ws + L.mu), U_ader, s.nt, s.nv,
+                       E * tpe, (size_t)tpe, total);
+    SPDK_LAUNCH_CHECK("k_mexport");
+    return SPDK_OK;
+}
 
 int spdk_mixed_guess_dims(spdk_ctx* c, int* din, int* dout) {
     const MVariant* v = c ? find_mvariant(c) : nullptr;

@@ -385,15 +385,20 @@ template <int D>
 __device__ __forceinline__ void fb_face(const double* __restrict__ W, double* __restrict__ F, FV sF,
                                         const double* __restrict__ th, FV s, const Geo& g, int k, int j, int i, double dt,
                                         int ay, int az, double gm) {
+    size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    size_t c1 = ((size_t)(k - (D == 2)) * s.Ny + (j - (D == 1))) * s.Nx + (i - (D == 0));
+    double t = smax(th[c0], th[c1]);
+    // Blend weight 0: F + 0 * (F_HLLC - F) is F (up to the sign of a zero
+    // flux) for any finite HLLC flux, so the face keeps F and the MUSCL-HLLC
+    // evaluation is skipped. Troubled cells and their ring are a small
+    // fraction of the mesh, and cluster, so most waves skip as a whole.
+    if (t == 0.0) return;
     double wL0[NVP], wR0[NVP], wL1[NVP], wR1[NVP], uL[NVP], uR[NVP], fL[NVP];
     slopes_d<D>(W, s, g, k - (D == 2), j - (D == 1), i - (D == 0), wL0, wR0, dt, ay, az, gm);
     slopes_d<D>(W, s, g, k, j, i, wL1, wR1, dt, ay, az, gm);
     cons(wL0, uL, gm);
     cons(wR1, uR, gm);
     riemann_hllc<D>(uL, uR, fL, gm);
-    size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
-    size_t c1 = ((size_t)(k - (D == 2)) * s.Ny + (j - (D == 1))) * s.Nx + (i - (D == 0));
-    double t = smax(th[c0], th[c1]);
 #pragma unroll
     for (int v = 0; v < NVP; v++) {
         size_t o = sF.off(v, k, j, i);

@@ -23,28 +23,49 @@ int hip_fail(hipError_t e, const char* what) {
 static constexpr int BS = 256;
 
 // ---------------------------------------------------------------------------
-// transform_a_to_b_1d / _slice: src/transforms.cpp:120-177
+// transform_a_to_b_1d / _slice: src/transforms.cpp:120-177. One thread per
+// pencil (time slice, variable, element, line of points along `dim`): it
+// reads the line's q inputs once and writes all of its outputs, each the
+// sum over ll of A[ll] M(id, ll) in ascending ll, the reference's order.
+// Arrays are rows of points, row = (t, v, element); consecutive threads take
+// consecutive lines of one element, so loads and stores stay within the
+// element's contiguous block (coalesced along y and z, 32-byte strided along
+// x with every line of the block read by the same wave).
+template <int Q>
 __global__ void __launch_bounds__(BS) k_transform_1d(const double* __restrict__ A, SD sa, double* __restrict__ B, SD sb,
                                                      const double* __restrict__ M, int ldm, int dim, int t_src,
-                                                     uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+                                                     size_t total) {
+    size_t idx = (size_t)blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
-    int k, j, i, kk, jj, ii;
-    split6(idx, sb.Nz, sb.Ny, sb.Nx, sb.nz, sb.ny, sb.nx, k, j, i, kk, jj, ii);
-    int q = dim == 0 ? sa.nx : (dim == 1 ? sa.ny : sa.nz);
-    int id = dim == 0 ? ii : (dim == 1 ? jj : kk);
-    int nt = t_src < 0 ? sb.nt : 1;
-    const double* Mrow = M + (size_t)id * ldm;
-    // source point stride along `dim` inside an element
-    size_t step = dim == 0 ? 1 : (dim == 1 ? (size_t)sa.nx : (size_t)sa.nx * sa.ny);
-    for (int tt = 0; tt < nt; tt++) {
-        int ts = t_src < 0 ? tt : t_src;
-        for (int v = 0; v < sb.nv; v++) {
-            size_t base = sa.off(ts, v, k, j, i, dim == 2 ? 0 : kk, dim == 1 ? 0 : jj, dim == 0 ? 0 : ii);
-            double u = 0;
-            for (int ll = 0; ll < q; ll++) u += A[base + ll * step] * Mrow[ll];
-            B[sb.off(tt, v, k, j, i, kk, jj, ii)] = u;
-        }
+    const int nxa = sa.nx, nya = sa.ny, nxb = sb.nx, nyb = sb.ny;
+    const int nout = dim == 0 ? sb.nx : (dim == 1 ? sb.ny : sb.nz);
+    // transverse extent: lines per element
+    const int c0 = dim == 0 ? sb.ny : sb.nx, c1 = dim == 2 ? sb.ny : sb.nz;
+    const int P = c0 * c1;
+    const size_t rb = idx / P;  // output row: (t, v, element) of B
+    const int line = (int)(idx % P);
+    const int a0 = line % c0, a1 = line / c0;
+    size_t ra = rb;
+    if (t_src >= 0) {  // B holds one slice; A's row is that of slice t_src
+        const size_t E = (size_t)sb.Nz * sb.Ny * sb.Nx;
+        ra = ((size_t)t_src * sa.nv) * E + rb;
+    }
+    const double* Ar = A + ra * sa.pts();
+    double* Br = B + rb * sb.pts();
+    // offsets of point ll along the line in A and of output f in B
+    int abase, astep, bbase, bstep;
+    if (dim == 0) { abase = (a1 * nya + a0) * nxa; astep = 1; bbase = (a1 * nyb + a0) * nxb; bstep = 1; }
+    else if (dim == 1) { abase = a1 * nya * nxa + a0; astep = nxa; bbase = a1 * nyb * nxb + a0; bstep = nxb; }
+    else { abase = a1 * nxa + a0; astep = nxa * nya; bbase = a1 * nxb + a0; bstep = nxb * nyb; }
+    double in[Q];
+#pragma unroll
+    for (int ll = 0; ll < Q; ll++) in[ll] = Ar[abase + ll * astep];
+    for (int f = 0; f < nout; f++) {
+        const double* Mrow = M + (size_t)f * ldm;
+        double u = 0;
+#pragma unroll
+        for (int ll = 0; ll < Q; ll++) u += in[ll] * Mrow[ll];
+        Br[bbase + f * bstep] = u;
     }
 }
 
@@ -471,9 +492,22 @@ int spdk_transform_1d(spdk_ctx* c, const double* A, spdk_sd_shape sa, double* B,
         int pa = d == 0 ? sa.nx : (d == 1 ? sa.ny : sa.nz), pb = d == 0 ? sb.nx : (d == 1 ? sb.ny : sb.nz);
         if (d != dim && pa != pb) { set_error("transform transverse mismatch"); return SPDK_E_SHAPE; }
     }
-    uint32_t total = (uint32_t)((size_t)sb.Nz * sb.Ny * sb.Nx * sb.nz * sb.ny * sb.nx);
-    hipLaunchKernelGGL(k_transform_1d, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat],
-                       c->mcols[mat], dim, t_src, total);
+    // one thread per (slice, variable, element, line along dim)
+    const int lines = (dim == 0 ? sb.ny * sb.nz : (dim == 1 ? sb.nx * sb.nz : sb.nx * sb.ny));
+    const size_t rows = (size_t)(t_src < 0 ? sb.nt : 1) * sb.nv * sb.Nz * sb.Ny * sb.Nx;
+    const size_t total = rows * lines;
+    if (total == 0) return SPDK_OK;
+    dim3 g((unsigned)((total + BS - 1) / BS));
+    switch (q) {
+        case 1: hipLaunchKernelGGL(k_transform_1d<1>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 2: hipLaunchKernelGGL(k_transform_1d<2>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 3: hipLaunchKernelGGL(k_transform_1d<3>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 4: hipLaunchKernelGGL(k_transform_1d<4>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 5: hipLaunchKernelGGL(k_transform_1d<5>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 6: hipLaunchKernelGGL(k_transform_1d<6>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 7: hipLaunchKernelGGL(k_transform_1d<7>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        default: set_error("transform: more than 7 points along the axis"); return SPDK_E_SHAPE;
+    }
     SPDK_LAUNCH_CHECK("transform_1d");
     return SPDK_OK;
 }

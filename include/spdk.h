@@ -256,6 +256,14 @@ size_t spdk_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
  * Bit-identical to the stage sequence. */
 int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
                           double* F_x, double* dt_next);
+/* The predictor part of ADER_step only (src/hydro_ader.hpp:291-303: copy_ader
+ * and the p Solve_fluxes + update_prediction pairs) with the fused tile
+ * kernel, bit-identical to the stage sequence, its T slices written to
+ * U_ader in the stage layout (T, >= 5 variables, elements, points). The
+ * caller then runs the last flux evaluation and the FV update with the
+ * fallback (src/hydro_ader.hpp:500-586) through the stage entries. */
+int spdk_fused_predict(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* U_ader,
+                       spdk_sd_shape s_ader);
 
 /* ---- mixed-precision ADER path (hydro) ------------------------------------ */
 /* The ADER step of src/hydro_ader.hpp:291-305 with the space-time state, its

@@ -13,6 +13,7 @@
 // transform, dt) is performed in the same order as the stage kernels and the
 // reference, so this path is bit-identical to the stage path.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstring>
 #include "ctx.hpp"
 #include "physics.hpp"
@@ -462,9 +463,23 @@ Layout layout_for(size_t E, int T) {
     return L;
 }
 
+// The predictor slices (T, NVP, E, TPE) in the stage layout of U_ader (T, nv,
+// E, TPE); slots past the physical variables are zero, as the stage
+// predictor leaves them.
+__global__ void k_export_ader(const double* __restrict__ Ua, double* __restrict__ out, int nv, size_t slabE,
+                              size_t total) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i % slabE, tv = i / slabE;
+        const int v = (int)(tv % nv), t = (int)(tv / nv);
+        out[i] = v < NVP ? Ua[((size_t)t * NVP + v) * slabE + r] : 0.0;
+    }
+}
+
+// One step (U_export null), or with U_export the predictor only: the T-1
+// predictor sweeps from the zero-order guess, written to U_export.
 template <int N, int DIM, int T>
 int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, double* W_cv, double* F_x,
-             double* dt_next) {
+             double* dt_next, double* U_export = nullptr, int nv_export = 0) {
     using G = Tile<N, DIM>;
     size_t E = (size_t)Ne[0] * Ne[1] * Ne[2];
     Layout L = layout_for<N, DIM>(E, T);
@@ -501,7 +516,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     dim3 grid(L.blocks), block(G::BLOCK);
     // timing: one event pair per sweep launch, harvested after the step's own sync
     auto launch_sweep = [&](const SweepArgs& args) -> int {
-        bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
+        bool rec = c->timing && !U_export && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
         hipLaunchKernelGGL((k_sweep<N, DIM, T>), grid, block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_sweep");
@@ -516,7 +531,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     int rc;
     if (T > 1) {  // ADER: K = T sweeps, the last one with the corrector
         if ((rc = traces0())) return rc;
-        for (int k = 0; k < T; k++) {
+        for (int k = 0; k < (U_export ? T - 1 : T); k++) {
             SweepArgs s = a;
             bool last = k == T - 1;
             double** tin = (k % 2 == 0) ? trA : trB;
@@ -531,6 +546,13 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
             s.W_cv = last ? W_cv : nullptr;
             s.F_x = last ? F_x : nullptr;
             if ((rc = launch_sweep(s))) return rc;
+        }
+        if (U_export) {
+            const size_t slabE = E * G::TPE, total = (size_t)T * nv_export * slabE;
+            hipLaunchKernelGGL(k_export_ader, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65536)), dim3(256), 0,
+                               c->stream, W + L.off_uader, U_export, nv_export, slabE, total);
+            SPDK_LAUNCH_CHECK("k_export_ader");
+            return SPDK_OK;
         }
     } else {  // SSP-RK stages (src/hydro_ader.hpp:313-325)
         int order = c->prm.rk_order;
@@ -568,7 +590,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     return SPDK_OK;
 }
 
-typedef int (*step_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, double*);
+typedef int (*step_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, double*, double*, int);
 typedef Layout (*layout_fn)(size_t, int);
 
 struct Variant {
@@ -629,8 +651,33 @@ int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
     if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
-    if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next);
-    return v->rk(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next);
+    if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr, 0);
+    return v->rk(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr, 0);
+}
+
+int spdk_fused_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* U_ader,
+                       spdk_sd_shape s_ader) {
+    if (!c || !U_sp || !ws || !U_ader) { set_error("fused predict: null argument"); return SPDK_E_ARG; }
+    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0 || c->T < 2) {
+        set_error("fused predict covers the inviscid, source-free ADER predictor");
+        return SPDK_E_CONFIG;
+    }
+    const Variant* v = find_variant(c);
+    if (!v) { set_error("fused path: no tile variant for this degree/dimensionality"); return SPDK_E_CONFIG; }
+    if (s.nv != NV || s.nt != 1 || s.nx != c->n || s.ny != (c->prm.active[1] ? c->n : 1) ||
+        s.nz != (c->prm.active[2] ? c->n : 1) || s_ader.nt != c->T || s_ader.nv < NVP || s_ader.Nx != s.Nx ||
+        s_ader.Ny != s.Ny || s_ader.Nz != s.Nz || s_ader.nx != s.nx || s_ader.ny != s.ny || s_ader.nz != s.nz) {
+        set_error("fused predict: U_sp / U_ader shape");
+        return SPDK_E_SHAPE;
+    }
+    for (int d = 0; d < 3; d++) {
+        int Nd = d == 0 ? s.Nx : (d == 1 ? s.Ny : s.Nz);
+        if (c->prm.active[d] && Nd < 3) { set_error("fused: axis needs ghost elements"); return SPDK_E_SHAPE; }
+    }
+    if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
+    return v->ader(c, U_sp, Ne, ws, dt, nullptr, nullptr, nullptr, U_ader, s_ader.nv);
 }
 
 }  // extern "C"

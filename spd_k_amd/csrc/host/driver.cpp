@@ -135,6 +135,7 @@ struct HydroRun {
     double rk_a[3] = {0, 0, 0};
     double t = 0, dt = 0, Dt = 0;
     bool fused = true;
+    bool fused_pred = false;  // fused predictor ahead of the stage FV update (fallback runs)
     spdk_sd_shape s_sp{}, s_ader{}, s_fp[3]{};
     spdk_fv_shape s_fv{}, s_th{}, s_fvd[3]{};
     double *U_sp = nullptr, *W_sp = nullptr, *W_cv = nullptr, *U_cv = nullptr, *T_sweep = nullptr, *U_ader = nullptr,
@@ -180,10 +181,16 @@ struct HydroRun {
         ctx = make_context(cfg, op, ax);
         fused = false;
         mixed = cfg.precision == "mixed";
-        if (!mixed && cfg.path == "fused" && !cfg.fallback) {
+        if (!mixed && cfg.path == "fused" && cfg.integrator == INT_ADER) {
             int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
             fused = spdk_fused_workspace_bytes(ctx, Nq) > 0;  // else the stage sequence runs
+        } else if (!mixed && cfg.path == "fused" && !cfg.fallback) {
+            int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
+            fused = spdk_fused_workspace_bytes(ctx, Nq) > 0;
         }
+        // with the fallback, the fused kernel runs the predictor and the stage
+        // kernels the last flux evaluation and the FV update
+        fused_pred = fused && cfg.fallback;
         spdk_timing_enable(ctx, timing ? 1 : 0);
 
         s_sp = spdk_sd_shape{1, kNVAR, Z.n_total, Y.n_total, X.n_total, Z.n_sp, Y.n_sp, X.n_sp};
@@ -223,7 +230,8 @@ struct HydroRun {
                 pc.guess = SPDK_GUESS_MLP;
             }
         }
-        if (!fused && (!mixed || cfg.fallback)) {  // the stage arrays (with fallback, also behind the mixed predictor)
+        if ((!fused && (!mixed || cfg.fallback)) || fused_pred) {  // the stage arrays (with fallback, also behind
+                                                                     // the mixed or fused predictor)
             U_cv = mem.alloc(ctx, nsp);
             U_ader = mem.alloc(ctx, sd_count(s_ader));
             if (cfg.integrator == INT_RK) U0 = mem.alloc(ctx, nsp);
@@ -232,7 +240,8 @@ struct HydroRun {
                 U_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
                 if (d) F_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
             }
-        } else if (fused) {
+        }
+        if (fused) {
             int N[3] = {X.n_total, Y.n_total, Z.n_total};
             size_t b = spdk_fused_workspace_bytes(ctx, N);
             if (b == 0) throw std::runtime_error(std::string("fused workspace: ") + spdk_last_error());
@@ -360,6 +369,16 @@ struct HydroRun {
             // MUSCL fallback in FP64 (src/hydro_ader.hpp:291-305, :500-586)
             CK(spdk_mixed_iterate(ctx, U_sp, s_sp, ws, dt, T - 1));
             CK(spdk_mixed_export_predictor(ctx, ws, U_ader, s_ader));
+            solve_fluxes();
+            fv_update();
+            CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
+            CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
+            double next = 0;
+            CK(spdk_min_dt(ctx, W_cv, s_sp, &next));
+            return next;
+        }
+        if (fused_pred) {
+            CK(spdk_fused_predict(ctx, U_sp, s_sp, ws, dt, U_ader, s_ader));
             solve_fluxes();
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));

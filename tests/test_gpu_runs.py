@@ -83,6 +83,13 @@ CASES = {
 }
 
 
+# With the fallback on, job/path=fused runs the fused predictor ahead of the
+# stage flux evaluation and FV update: the same cases through that path.
+for _k in ("kh2d_p4_fallback", "blast2d_fallback", "blast3d_fallback", "sod1d_gradfree", "shu_osher1d",
+           "implosion2d_reflective", "square2d_knobs", "sedov_cells2d", "user2d"):
+    CASES[_k + "_fusedpred"] = (CASES[_k][0], "fused")
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_parity_with_oracle(tmp_path, name):
     args, path = CASES[name]
@@ -101,9 +108,10 @@ CFG2 = {
 }
 
 
+@pytest.mark.parametrize("path", ["stages", "fused"])
 @pytest.mark.parametrize("name", sorted(CFG2))
-def test_fallback_full_size_masks(tmp_path, name):
-    a, b = both(tmp_path, CFG2[name], "stages")
+def test_fallback_full_size_masks(tmp_path, name, path):
+    a, b = both(tmp_path, CFG2[name], path)
     assert_bit_exact(a, b)
     g = op.grid(a)
     flagged = [np.fromfile(os.path.join(b, f"troubles_N{4 * g['N'][0] + 4}_{i}_0.dat")).max()

@@ -47,10 +47,16 @@ __global__ void __launch_bounds__(BS) k_face_int_ref(const double* __restrict__ 
     }
 }
 
-// face_integral, second sweep: src/transforms.cpp:466-480 (T holds the dim1 sweep)
-__global__ void __launch_bounds__(BS) k_face_int_scatter(const double* __restrict__ Tm, SD s, double* __restrict__ F,
-                                                         FV sf, const double* __restrict__ S, int lds, int dim, int dim2,
-                                                         int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
+// face_integral with both transverse axes active, the two sweeps of
+// src/transforms.cpp:435-480 in one pass: for each output, the dim1 sweep
+// value at every dim2 position (the sum over ll of F S(id1, ll)) and then the
+// dim2 sum of those times S(id2, nn), each in the reference's order, so the
+// result equals the two-pass one bit for bit without the intermediate array.
+template <int Q>
+__global__ void __launch_bounds__(BS) k_face_int_fused(const double* __restrict__ Ffp, SD s, double* __restrict__ F,
+                                                       FV sf, const double* __restrict__ S, int lds, int t_id, int dim,
+                                                       int dim1, int dim2, int gx, int gy, int gz, int fx, int fy, int fz,
+                                                       uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
     int Nx = s.Nx + (dim == 0), Ny = s.Ny + (dim == 1), Nz = s.Nz + (dim == 2);
@@ -60,13 +66,27 @@ __global__ void __launch_bounds__(BS) k_face_int_scatter(const double* __restric
     k += gz; j += gy; i += gx;
     int I = ii + fx + (i - gx) * px, J = jj + fy + (j - gy) * py, K = kk + fz + (k - gz) * pz;
     if (!(K < sf.Nz && J < sf.Ny && I < sf.Nx)) return;
-    int q2 = dim2 == 0 ? px : (dim2 == 1 ? py : pz);
-    int id2 = dim2 == 0 ? ii : (dim2 == 1 ? jj : kk);
+    const int id1 = dim1 == 0 ? ii : (dim1 == 1 ? jj : kk), id2 = dim2 == 0 ? ii : (dim2 == 1 ? jj : kk);
+    double S1[Q], S2[Q];
+#pragma unroll
+    for (int l = 0; l < Q; l++) {
+        S1[l] = S[id1 * lds + l];
+        S2[l] = S[id2 * lds + l];
+    }
+    // point strides of dim1 and dim2 inside an element
+    const int st1 = dim1 == 0 ? 1 : (dim1 == 1 ? s.nx : s.nx * s.ny), st2 = dim2 == 0 ? 1 : (dim2 == 1 ? s.nx : s.nx * s.ny);
+    const int a0 = dim1 == 0 || dim2 == 0 ? 0 : ii, b0 = dim1 == 1 || dim2 == 1 ? 0 : jj, c0 = dim1 == 2 || dim2 == 2 ? 0 : kk;
+    const size_t base0 = s.off(t_id, 0, k, j, i, c0, b0, a0);
+    const size_t slab = s.slab();
     for (int v = 0; v < s.nv; v++) {
+        const double* Fv = Ffp + base0 + (size_t)v * slab;
         double f = 0;
-        for (int nn = 0; nn < q2; nn++) {
-            int a = dim2 == 0 ? nn : ii, b = dim2 == 1 ? nn : jj, c = dim2 == 2 ? nn : kk;
-            f += Tm[s.off(0, v, k, j, i, c, b, a)] * S[id2 * lds + nn];
+#pragma unroll
+        for (int nn = 0; nn < Q; nn++) {
+            double u = 0;
+#pragma unroll
+            for (int ll = 0; ll < Q; ll++) u += Fv[nn * st2 + ll * st1] * S1[ll];
+            f += u * S2[nn];
         }
         F[sf.off(v, K, J, I)] = f;
     }
@@ -465,12 +485,19 @@ int spdk_fv_face_integral(spdk_ctx* c, const double* Ffp, spdk_sd_shape s_fp, do
         SPDK_LAUNCH_CHECK("face_integral_ref");
         return SPDK_OK;
     }
-    spdk_sd_shape st = s_fp;
-    st.nt = 1;
-    int rc = spdk_transform_1d(c, Ffp, s_fp, T, st, SPDK_M_SP_TO_CV, dim1, t_id);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_face_int_scatter, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, T, SD(st), F, FV(sf), S, lds,
-                       dim, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
+    (void)T;  // the two sweeps run fused; no intermediate slice
+    int q1 = dim1 == 0 ? s_fp.nx : (dim1 == 1 ? s_fp.ny : s_fp.nz), q2 = dim2 == 0 ? s_fp.nx : (dim2 == 1 ? s_fp.ny : s_fp.nz);
+    if (q1 != q2 || q1 != lds) { set_error("face_integral: transverse point counts"); return SPDK_E_SHAPE; }
+    if (t_id < 0 || t_id >= s_fp.nt) { set_error("face_integral: slice"); return SPDK_E_ARG; }
+    dim3 g(nblocks(total, BS));
+    switch (q1) {
+        case 2: hipLaunchKernelGGL(k_face_int_fused<2>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
+        case 3: hipLaunchKernelGGL(k_face_int_fused<3>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
+        case 4: hipLaunchKernelGGL(k_face_int_fused<4>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
+        case 5: hipLaunchKernelGGL(k_face_int_fused<5>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
+        case 6: hipLaunchKernelGGL(k_face_int_fused<6>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
+        default: set_error("face_integral: degree"); return SPDK_E_SHAPE;
+    }
     SPDK_LAUNCH_CHECK("face_integral");
     return SPDK_OK;
 }

@@ -106,9 +106,12 @@ __global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, doubl
                                                   FvUpd a, int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
-    int k, j, i, kk, jj, ii;
-    split6(idx, s.Nz - 2 * gz, s.Ny - 2 * gy, s.Nx - 2 * gx, s.nz, s.ny, s.nx, k, j, i, kk, jj, ii);
-    k += gz; j += gy; i += gx;
+    // one thread per active FV cell, x fastest: the FV arrays (fluxes, old
+    // and new state) are read and written in contiguous runs; the SD-layout
+    // state in runs of one element row
+    const int Mx = (s.Nx - 2 * gx) * s.nx, My = (s.Ny - 2 * gy) * s.ny;
+    const int ic = (int)(idx % Mx), jc = (int)((idx / Mx) % My), kc = (int)(idx / ((uint32_t)Mx * My));
+    const int i = gx + ic / s.nx, ii = ic % s.nx, j = gy + jc / s.ny, jj = jc % s.ny, k = gz + kc / s.nz, kk = kc % s.nz;
     int I = ii + fx + (i - gx) * s.nx, J = jj + fy + (j - gy) * s.ny, K = kk + fz + (k - gz) * s.nz;
     size_t slab = s.slab();
     size_t o = s.off(0, 0, k, j, i, kk, jj, ii);

@@ -48,47 +48,87 @@ __global__ void __launch_bounds__(BS) k_face_int_ref(const double* __restrict__ 
 }
 
 // face_integral with both transverse axes active, the two sweeps of
-// src/transforms.cpp:435-480 in one pass: for each output, the dim1 sweep
-// value at every dim2 position (the sum over ll of F S(id1, ll)) and then the
-// dim2 sum of those times S(id2, nn), each in the reference's order, so the
-// result equals the two-pass one bit for bit without the intermediate array.
+// src/transforms.cpp:435-480 on element tiles: a workgroup owns EX consecutive
+// elements of the (k, j, i) element range; it stages their flux-point panels
+// of one variable in LDS with contiguous loads, forms the dim1 sweep for every
+// (face, dim1 output, dim2 point) once, and writes the dim2 sweep with x
+// fastest, so the FV face array is written in runs of EX*N values. Each sum
+// runs in the reference's order (the dim1 sum over ll of F S(o1, ll), then
+// the dim2 sum over nn of those times S(o2, nn)), so the result equals the
+// two-pass one bit for bit without the intermediate array.
 template <int Q>
-__global__ void __launch_bounds__(BS) k_face_int_fused(const double* __restrict__ Ffp, SD s, double* __restrict__ F,
-                                                       FV sf, const double* __restrict__ S, int lds, int t_id, int dim,
-                                                       int dim1, int dim2, int gx, int gy, int gz, int fx, int fy, int fz,
-                                                       uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
-    if (idx >= total) return;
-    int Nx = s.Nx + (dim == 0), Ny = s.Ny + (dim == 1), Nz = s.Nz + (dim == 2);
-    int px = s.nx - (dim == 0), py = s.ny - (dim == 1), pz = s.nz - (dim == 2);
-    int k, j, i, kk, jj, ii;
-    split6(idx, Nz - 2 * gz, Ny - 2 * gy, Nx - 2 * gx, pz, py, px, k, j, i, kk, jj, ii);
-    k += gz; j += gy; i += gx;
-    int I = ii + fx + (i - gx) * px, J = jj + fy + (j - gy) * py, K = kk + fz + (k - gz) * pz;
-    if (!(K < sf.Nz && J < sf.Ny && I < sf.Nx)) return;
-    const int id1 = dim1 == 0 ? ii : (dim1 == 1 ? jj : kk), id2 = dim2 == 0 ? ii : (dim2 == 1 ? jj : kk);
-    double S1[Q], S2[Q];
-#pragma unroll
-    for (int l = 0; l < Q; l++) {
-        S1[l] = S[id1 * lds + l];
-        S2[l] = S[id2 * lds + l];
+struct FaceTile {
+    static constexpr int EX = Q <= 4 ? 16 : 8;    // elements per workgroup
+    static constexpr int P = (Q + 1) * Q * Q;     // flux points of one element, one variable
+    static constexpr int U = Q * Q * Q;           // dim1-swept values (= FV faces) per element
+};
+template <int Q>
+__global__ void __launch_bounds__(BS) k_face_int_tile(const double* __restrict__ Ffp, SD s, double* __restrict__ F,
+                                                      FV sf, const double* __restrict__ S, int lds, int t_id, int dim,
+                                                      int gx, int gy, int gz, int fx, int fy, int fz, int ex, int ey,
+                                                      uint32_t n_el) {
+    using TL = FaceTile<Q>;
+    __shared__ double sF[TL::EX * TL::P];
+    __shared__ double sU[TL::EX * TL::U];
+    __shared__ double sS[Q * Q];
+    __shared__ size_t sbase[TL::EX];
+    __shared__ int sIJK[TL::EX][3];
+    const int tid = threadIdx.x;
+    const uint32_t e0 = blockIdx.x * TL::EX;
+    const int ne = (int)min((uint32_t)TL::EX, n_el - e0);
+    if (tid < Q * Q) sS[tid] = S[(tid / Q) * lds + tid % Q];
+    if (tid < TL::EX) {
+        int i = -1, j = 0, k = 0;
+        if (tid < ne) {
+            const uint32_t e = e0 + tid;
+            i = gx + (int)(e % ex); j = gy + (int)((e / ex) % ey); k = gz + (int)(e / ((uint32_t)ex * ey));
+            // an element past the stored range (possible only along dim) has no FV face in range
+            if (i >= s.Nx || j >= s.Ny || k >= s.Nz) i = -1;
+        }
+        sIJK[tid][0] = i; sIJK[tid][1] = j; sIJK[tid][2] = k;
+        sbase[tid] = i < 0 ? 0 : s.off(t_id, 0, k, j, i, 0, 0, 0);
     }
-    // point strides of dim1 and dim2 inside an element
-    const int st1 = dim1 == 0 ? 1 : (dim1 == 1 ? s.nx : s.nx * s.ny), st2 = dim2 == 0 ? 1 : (dim2 == 1 ? s.nx : s.nx * s.ny);
-    const int a0 = dim1 == 0 || dim2 == 0 ? 0 : ii, b0 = dim1 == 1 || dim2 == 1 ? 0 : jj, c0 = dim1 == 2 || dim2 == 2 ? 0 : kk;
-    const size_t base0 = s.off(t_id, 0, k, j, i, c0, b0, a0);
+    // panel extents: Q + 1 flux points along dim, Q along the others
+    const int Lx = dim == 0 ? Q + 1 : Q, Ly = dim == 1 ? Q + 1 : Q;
+    const int dim1 = dim == 0 ? 1 : (dim == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
+    const int st[3] = {1, Lx, Lx * Ly};
+    const int sd = st[dim], s1 = st[dim1], s2 = st[dim2];
     const size_t slab = s.slab();
+    __syncthreads();
     for (int v = 0; v < s.nv; v++) {
-        const double* Fv = Ffp + base0 + (size_t)v * slab;
-        double f = 0;
-#pragma unroll
-        for (int nn = 0; nn < Q; nn++) {
+        for (int q = tid; q < ne * TL::P; q += BS) {
+            const int e = q / TL::P, p = q - e * TL::P;
+            if (sIJK[e][0] >= 0) sF[q] = Ffp[sbase[e] + (size_t)v * slab + p];
+        }
+        __syncthreads();
+        // dim1 sweep: sU[e][f][o1][nn] = sum_ll F(f, ll, nn) S(o1, ll)
+        for (int q = tid; q < ne * TL::U; q += BS) {
+            const int e = q / TL::U, r = q - e * TL::U;
+            const int nn = r % Q, o1 = (r / Q) % Q, f = r / (Q * Q);
+            const double* Fe = sF + e * TL::P + f * sd + nn * s2;
             double u = 0;
 #pragma unroll
-            for (int ll = 0; ll < Q; ll++) u += Fv[nn * st2 + ll * st1] * S1[ll];
-            f += u * S2[nn];
+            for (int ll = 0; ll < Q; ll++) u += Fe[ll * s1] * sS[o1 * Q + ll];
+            sU[q] = u;
         }
-        F[sf.off(v, K, J, I)] = f;
+        __syncthreads();
+        // dim2 sweep and the write, x fastest: q -> (kk, jj, e, ii)
+        for (int q = tid; q < ne * TL::U; q += BS) {
+            const int ii = q % Q, e = (q / Q) % ne, jj = (q / (Q * ne)) % Q, kk = q / (Q * Q * ne);
+            const int i = sIJK[e][0];
+            if (i < 0) continue;
+            const int j = sIJK[e][1], k = sIJK[e][2];
+            const int I = ii + fx + (i - gx) * Q, J = jj + fy + (j - gy) * Q, K = kk + fz + (k - gz) * Q;
+            if (!(K < sf.Nz && J < sf.Ny && I < sf.Nx)) continue;
+            const int f = dim == 0 ? ii : (dim == 1 ? jj : kk);
+            const int o1 = dim1 == 0 ? ii : (dim1 == 1 ? jj : kk), o2 = dim2 == 0 ? ii : (dim2 == 1 ? jj : kk);
+            const double* Ue = sU + e * TL::U + (f * Q + o1) * Q;
+            double acc = 0;
+#pragma unroll
+            for (int nn = 0; nn < Q; nn++) acc += Ue[nn] * sS[o2 * Q + nn];
+            F[sf.off(v, K, J, I)] = acc;
+        }
+        __syncthreads();
     }
 }
 
@@ -492,15 +532,23 @@ int spdk_fv_face_integral(spdk_ctx* c, const double* Ffp, spdk_sd_shape s_fp, do
     int q1 = dim1 == 0 ? s_fp.nx : (dim1 == 1 ? s_fp.ny : s_fp.nz), q2 = dim2 == 0 ? s_fp.nx : (dim2 == 1 ? s_fp.ny : s_fp.nz);
     if (q1 != q2 || q1 != lds) { set_error("face_integral: transverse point counts"); return SPDK_E_SHAPE; }
     if (t_id < 0 || t_id >= s_fp.nt) { set_error("face_integral: slice"); return SPDK_E_ARG; }
-    dim3 g(nblocks(total, BS));
+    if (s_fp.nt < 1 || px != q1 || py != q1 || pz != q1) { set_error("face_integral: face counts"); return SPDK_E_SHAPE; }
+    const int ex = Nx - 2 * gx, ey = Ny - 2 * gy;
+    const uint32_t n_el = (uint32_t)((size_t)ex * ey * (Nz - 2 * gz));
+#define SPDK_FACE_TILE(QQ)                                                                                          \
+    hipLaunchKernelGGL(k_face_int_tile<QQ>, dim3((n_el + FaceTile<QQ>::EX - 1) / FaceTile<QQ>::EX), dim3(BS), 0,       \
+                       c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, gx, gy, gz, c->fgh[0], c->fgh[1],     \
+                       c->fgh[2], ex, ey, n_el)
+    if (n_el == 0) return SPDK_OK;
     switch (q1) {
-        case 2: hipLaunchKernelGGL(k_face_int_fused<2>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
-        case 3: hipLaunchKernelGGL(k_face_int_fused<3>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
-        case 4: hipLaunchKernelGGL(k_face_int_fused<4>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
-        case 5: hipLaunchKernelGGL(k_face_int_fused<5>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
-        case 6: hipLaunchKernelGGL(k_face_int_fused<6>, g, dim3(BS), 0, c->stream, Ffp, SD(s_fp), F, FV(sf), S, lds, t_id, dim, dim1, dim2, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total); break;
+        case 2: SPDK_FACE_TILE(2); break;
+        case 3: SPDK_FACE_TILE(3); break;
+        case 4: SPDK_FACE_TILE(4); break;
+        case 5: SPDK_FACE_TILE(5); break;
+        case 6: SPDK_FACE_TILE(6); break;
         default: set_error("face_integral: degree"); return SPDK_E_SHAPE;
     }
+#undef SPDK_FACE_TILE
     SPDK_LAUNCH_CHECK("face_integral");
     return SPDK_OK;
 }

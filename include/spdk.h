@@ -264,6 +264,24 @@ int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
  * fallback (src/hydro_ader.hpp:500-586) through the stage entries. */
 int spdk_fused_predict(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* U_ader,
                        spdk_sd_shape s_ader);
+/* The predictor part of ADER_step and the last Solve_fluxes
+ * (src/hydro_ader.hpp:278-303) with the fused tile kernel: the final sweep
+ * writes, instead of the corrector, the flux-point fluxes of every slice and
+ * active axis (Riemann fluxes at element faces included) to F_x / F_y / F_z in
+ * the stage flux-point layout of s_ader with p + 2 points along the axis
+ * (slots past the physical variables zero). Bit-identical to the stage
+ * sequence; the caller then runs the FV update with the fallback
+ * (src/hydro_ader.hpp:500-586). Uses the fused workspace. */
+int spdk_fused_predict_fluxes(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* F_x,
+                              double* F_y, double* F_z, spdk_sd_shape s_ader);
+/* Solve_fluxes of given ADER slices (src/hydro_ader.hpp:278-289: sp->fp,
+ * fluxes, trace fill, LLF) with the fused tile kernel: U_ader in the stage
+ * layout (T, >= 5 variables, elements, points), fluxes written as by
+ * spdk_fused_predict_fluxes. The workspace holds the slices' face traces:
+ * spdk_fused_fluxes_workspace_bytes(N) bytes (0: no tile variant). */
+size_t spdk_fused_fluxes_workspace_bytes(spdk_ctx* ctx, const int N[3]);
+int spdk_fused_fluxes(spdk_ctx* ctx, const double* U_ader, spdk_sd_shape s_ader, void* workspace, double* F_x,
+                      double* F_y, double* F_z);
 
 /* ---- mixed-precision ADER path (hydro) ------------------------------------ */
 /* The ADER step of src/hydro_ader.hpp:291-305 with the space-time state, its

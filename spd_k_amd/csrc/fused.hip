@@ -50,6 +50,7 @@ struct SweepArgs {
     double* U0;           // (5,E,TPE) RK step-start state
     double* W_cv;         // optional (1,6,E,TPE)
     double* F_x;          // optional (T,6,E,NT,M) x flux-point fluxes
+    double* F_out[3];     // flux-only sweep: per axis (T,out_nv,E,FSZ) flux-point fluxes
     double* part;         // per-block dt minima
     const double* Dm;     // dfp_to_sp (N x M)
     const double* S2F;    // sp_to_fp  (M x N)
@@ -62,6 +63,9 @@ struct SweepArgs {
     double h[3];
     double dt, gm, cfl, rk_a;
     int tin_slices;       // slices present in tr_in (1 or T)
+    int in_nv;            // variables per slice of U_in
+    int out_nv;           // variables per slice of F_out (slots past the physical ones get zero)
+    int flux_only;        // write the flux-point fluxes of every slice and axis; no divergence
     int in_ader;          // state from U_in (else U_sp broadcast to every slice)
     int out_pred;         // write predictor U_out + tr_out, else corrector
     int rk_first;         // RK: write U0 = U_sp
@@ -119,7 +123,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     }
     double u_old[NVP];
 #pragma unroll
-    for (int v = 0; v < NVP; v++) u_old[v] = live ? a.U_sp[(size_t)v * slabE + (size_t)e * TPE + s] : 0.0;
+    for (int v = 0; v < NVP; v++) u_old[v] = live && !a.flux_only ? a.U_sp[(size_t)v * slabE + (size_t)e * TPE + s] : 0.0;
 
     double dudt[T][NVP];
     const int nslices = a.in_ader ? T : 1;  // a broadcast state has identical slices
@@ -131,7 +135,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
         if (live) {
 #pragma unroll
             for (int v = 0; v < NVP; v++)
-                us[v * TPE + s] = a.in_ader ? a.U_in[((size_t)t * NVP + v) * slabE + (size_t)e * TPE + s] : u_old[v];
+                us[v * TPE + s] = a.in_ader ? a.U_in[((size_t)t * a.in_nv + v) * slabE + (size_t)e * TPE + s] : u_old[v];
         }
         __syncthreads();
         // ---- pencils: interpolation to flux points, fluxes, face Riemann solves
@@ -216,6 +220,20 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
             }
         }
         __syncthreads();
+        if (a.flux_only) {  // the fluxes of slice t in the stage flux-point layout (src/hydro_ader.hpp:278-289)
+            if (live) {
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    for (int idx = s; idx < G::FSZ; idx += TPE) {
+                        double* o = a.F_out[d] + ((size_t)t * a.out_nv * E + e) * G::FSZ + idx;
+#pragma unroll
+                        for (int v = 0; v < NVP; v++) o[(size_t)v * E * G::FSZ] = Fs[(d * NVP + v) * G::FSZ + idx];
+                        for (int v = NVP; v < a.out_nv; v++) o[(size_t)v * E * G::FSZ] = 0.0;
+                    }
+            }
+            __syncthreads();
+            continue;
+        }
         // ---- flux divergence at the solution points (src/transforms.cpp:301-307)
         if (live) {
             const int qx = DIM == 3 ? kk * N + jj : jj;
@@ -246,6 +264,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
         }
         __syncthreads();
     }
+    if (a.flux_only) return;
     if (!a.in_ader) {
 #pragma unroll
         for (int t = 1; t < T; t++)
@@ -380,10 +399,11 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     if (tid == 0) a.part[blockIdx.x] = red[0];
 }
 
-// Face traces of the solution U_sp (slice-independent), the input of a
-// step's first sweep.
+// Face traces of slice blockIdx.y of U (in_nv variables per slice): of the
+// solution U_sp (one slice), the input of a step's first sweep, or of given
+// predictor slices.
 template <int N, int DIM>
-__global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U_sp, double* tr0,
+__global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U, int in_nv, double* tr0,
                                                                double* tr1, double* tr2, const double* __restrict__ S2F,
                                                                int Eint) {
     using G = Tile<N, DIM>;
@@ -392,10 +412,10 @@ __global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U_sp,
     const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
     const int e = blockIdx.x * EPB + el;
     const bool live = el < EPB && e < Eint;
-    const size_t E = (size_t)Eint, slabE = E * TPE;
+    const size_t E = (size_t)Eint, slabE = E * TPE, t = blockIdx.y;
     if (live)
 #pragma unroll
-        for (int v = 0; v < NVP; v++) us[(el * NVP + v) * TPE + s] = U_sp[(size_t)v * slabE + (size_t)e * TPE + s];
+        for (int v = 0; v < NVP; v++) us[(el * NVP + v) * TPE + s] = U[(t * in_nv + v) * slabE + (size_t)e * TPE + s];
     __syncthreads();
     for (int task = s; live && task < DIM * NT; task += TPE) {
         const int d = task / NT, q = task % NT;
@@ -413,7 +433,7 @@ __global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U_sp,
                 double p = us[(el * NVP + v) * TPE + spidx<N, DIM>(d, l, q)];
                 x1 += p * S2F[(M - 1) * N + l];
             }
-            double* to = trd + (((size_t)v * E + e) * 2) * NT + q;
+            double* to = trd + (((t * NVP + v) * E + e) * 2) * NT + q;
             to[0] = x0;
             to[NT] = x1;
         }
@@ -475,11 +495,24 @@ __global__ void k_export_ader(const double* __restrict__ Ua, double* __restrict_
     }
 }
 
-// One step (U_export null), or with U_export the predictor only: the T-1
-// predictor sweeps from the zero-order guess, written to U_export.
+// What a call produces instead of a whole step (all null: one step).
+struct Extra {
+    double* U_export = nullptr;  // the T-1 predictor sweeps from the zero-order guess, slices written here
+    int nv_export = 0;
+    double* F_out[3] = {nullptr, nullptr, nullptr};  // the predictor sweeps, then the flux evaluation of the
+    int out_nv = 0;                                  // final slices written here (stage flux-point layout)
+    const double* U_slices = nullptr;  // with F_out: the flux evaluation of these slices (stage layout), no
+    int slices_nv = 0;                 // predictor; the workspace then holds only their traces
+};
+
+template <int N, int DIM>
+size_t slice_trace_doubles(size_t E, int T) {
+    return (size_t)DIM * T * NVP * E * 2 * Tile<N, DIM>::NT;
+}
+
 template <int N, int DIM, int T>
 int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, double* W_cv, double* F_x,
-             double* dt_next, double* U_export = nullptr, int nv_export = 0) {
+             double* dt_next, const Extra* x) {
     using G = Tile<N, DIM>;
     size_t E = (size_t)Ne[0] * Ne[1] * Ne[2];
     Layout L = layout_for<N, DIM>(E, T);
@@ -508,6 +541,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
         a.h[d] = c->prm.h[d];
     }
     a.E = (int)E;
+    a.in_nv = NVP;
     a.dt = dt;
     a.gm = c->prm.gamma;
     a.cfl = c->prm.cfl;
@@ -516,7 +550,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     dim3 grid(L.blocks), block(G::BLOCK);
     // timing: one event pair per sweep launch, harvested after the step's own sync
     auto launch_sweep = [&](const SweepArgs& args) -> int {
-        bool rec = c->timing && !U_export && c->ev_used < spdk_ctx::kEvPool;
+        bool rec = c->timing && !x && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
         hipLaunchKernelGGL((k_sweep<N, DIM, T>), grid, block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_sweep");
@@ -524,14 +558,34 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
         return SPDK_OK;
     };
     auto traces0 = [&]() -> int {
-        hipLaunchKernelGGL((k_trace0<N, DIM>), grid, block, 0, c->stream, U_sp, trA[0], trA[1], trA[2], a.S2F, (int)E);
+        hipLaunchKernelGGL((k_trace0<N, DIM>), grid, block, 0, c->stream, U_sp, NV, trA[0], trA[1], trA[2], a.S2F, (int)E);
         SPDK_LAUNCH_CHECK("k_trace0");
         return SPDK_OK;
     };
     int rc;
-    if (T > 1) {  // ADER: K = T sweeps, the last one with the corrector
+    const bool flux_out = x && x->F_out[0];
+    if (T > 1 && x && x->U_slices) {  // flux evaluation of given slices: their traces, then one flux sweep
+        double* trS[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < DIM; d++) trS[d] = W + (size_t)d * L.tr_axis;
+        hipLaunchKernelGGL((k_trace0<N, DIM>), dim3(L.blocks, T), block, 0, c->stream, x->U_slices, x->slices_nv, trS[0],
+                           trS[1], trS[2], a.S2F, (int)E);
+        SPDK_LAUNCH_CHECK("k_trace0");
+        SweepArgs s = a;
+        s.U0 = nullptr;
+        s.part = nullptr;
+        for (int d = 0; d < 3; d++) { s.tr_in[d] = trS[d]; s.tr_out[d] = nullptr; s.F_out[d] = x->F_out[d]; }
+        s.tin_slices = T;
+        s.in_ader = 1;
+        s.U_in = x->U_slices;
+        s.in_nv = x->slices_nv;
+        s.flux_only = 1;
+        s.out_nv = x->out_nv;
+        return launch_sweep(s);
+    }
+    if (T > 1) {  // ADER: K = T sweeps, the last one with the corrector (or the flux evaluation)
         if ((rc = traces0())) return rc;
-        for (int k = 0; k < (U_export ? T - 1 : T); k++) {
+        const bool pred_only = x && x->U_export;
+        for (int k = 0; k < (pred_only ? T - 1 : T); k++) {
             SweepArgs s = a;
             bool last = k == T - 1;
             double** tin = (k % 2 == 0) ? trA : trB;
@@ -545,12 +599,21 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
             s.want_dt = last;
             s.W_cv = last ? W_cv : nullptr;
             s.F_x = last ? F_x : nullptr;
+            if (last && flux_out) {
+                s.want_dt = 0;
+                s.W_cv = nullptr;
+                s.F_x = nullptr;
+                s.flux_only = 1;
+                s.out_nv = x->out_nv;
+                for (int d = 0; d < 3; d++) s.F_out[d] = x->F_out[d];
+            }
             if ((rc = launch_sweep(s))) return rc;
         }
-        if (U_export) {
-            const size_t slabE = E * G::TPE, total = (size_t)T * nv_export * slabE;
+        if (flux_out) return SPDK_OK;
+        if (pred_only) {
+            const size_t slabE = E * G::TPE, total = (size_t)T * x->nv_export * slabE;
             hipLaunchKernelGGL(k_export_ader, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65536)), dim3(256), 0,
-                               c->stream, W + L.off_uader, U_export, nv_export, slabE, total);
+                               c->stream, W + L.off_uader, x->U_export, x->nv_export, slabE, total);
             SPDK_LAUNCH_CHECK("k_export_ader");
             return SPDK_OK;
         }
@@ -590,15 +653,17 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     return SPDK_OK;
 }
 
-typedef int (*step_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, double*, double*, int);
+typedef int (*step_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, double*, const Extra*);
 typedef Layout (*layout_fn)(size_t, int);
+typedef size_t (*trace_fn)(size_t, int);
 
 struct Variant {
     int n, dim;
     step_fn ader, rk;
     layout_fn layout;
+    trace_fn slice_traces;
 };
-#define SPDK_VARIANT(Nn, D) {Nn, D, run_step<Nn, D, Nn>, run_step<Nn, D, 1>, layout_for<Nn, D>}
+#define SPDK_VARIANT(Nn, D) {Nn, D, run_step<Nn, D, Nn>, run_step<Nn, D, 1>, layout_for<Nn, D>, slice_trace_doubles<Nn, D>}
 const Variant kVariants[] = {SPDK_VARIANT(2, 1), SPDK_VARIANT(3, 1), SPDK_VARIANT(4, 1), SPDK_VARIANT(5, 1),
                              SPDK_VARIANT(2, 2), SPDK_VARIANT(3, 2), SPDK_VARIANT(4, 2), SPDK_VARIANT(5, 2),
                              SPDK_VARIANT(2, 3), SPDK_VARIANT(3, 3), SPDK_VARIANT(4, 3)};
@@ -651,8 +716,8 @@ int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
     if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
-    if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr, 0);
-    return v->rk(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr, 0);
+    if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr);
+    return v->rk(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr);
 }
 
 int spdk_fused_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* U_ader,
@@ -677,7 +742,80 @@ int spdk_fused_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, dou
     if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
-    return v->ader(c, U_sp, Ne, ws, dt, nullptr, nullptr, nullptr, U_ader, s_ader.nv);
+    Extra x;
+    x.U_export = U_ader;
+    x.nv_export = s_ader.nv;
+    return v->ader(c, U_sp, Ne, ws, dt, nullptr, nullptr, nullptr, &x);
+}
+
+// Checks shared by the flux-evaluation entries: the ADER slices shape and the
+// flux-point arrays of the active axes.
+static int check_flux_args(spdk_ctx* c, spdk_sd_shape s_ader, double* const F[3], const Variant** vout) {
+    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0 || c->T < 2) {
+        set_error("fused flux evaluation covers the inviscid, source-free ADER step");
+        return SPDK_E_CONFIG;
+    }
+    const Variant* v = find_variant(c);
+    if (!v) { set_error("fused path: no tile variant for this degree/dimensionality"); return SPDK_E_CONFIG; }
+    if (s_ader.nt != c->T || s_ader.nv < NVP || s_ader.nx != c->n || s_ader.ny != (c->prm.active[1] ? c->n : 1) ||
+        s_ader.nz != (c->prm.active[2] ? c->n : 1)) {
+        set_error("fused fluxes: slice shape");
+        return SPDK_E_SHAPE;
+    }
+    for (int d = 0; d < 3; d++) {
+        int Nd = d == 0 ? s_ader.Nx : (d == 1 ? s_ader.Ny : s_ader.Nz);
+        if (c->prm.active[d] && Nd < 3) { set_error("fused: axis needs ghost elements"); return SPDK_E_SHAPE; }
+        if (c->prm.active[d] && !F[d]) { set_error("fused fluxes: null flux array of an active axis"); return SPDK_E_ARG; }
+    }
+    if ((uint64_t)s_ader.Nx * s_ader.Ny * s_ader.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
+    *vout = v;
+    return SPDK_OK;
+}
+
+int spdk_fused_predict_fluxes(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* F_x,
+                              double* F_y, double* F_z, spdk_sd_shape s_ader) {
+    if (!c || !U_sp || !ws) { set_error("fused predict fluxes: null argument"); return SPDK_E_ARG; }
+    double* F[3] = {F_x, F_y, F_z};
+    const Variant* v = nullptr;
+    int rc = check_flux_args(c, s_ader, F, &v);
+    if (rc) return rc;
+    if (s.nv != NV || s.nt != 1 || s.nx != s_ader.nx || s.ny != s_ader.ny || s.nz != s_ader.nz || s.Nx != s_ader.Nx ||
+        s.Ny != s_ader.Ny || s.Nz != s_ader.Nz) {
+        set_error("fused predict fluxes: U_sp shape");
+        return SPDK_E_SHAPE;
+    }
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    Extra x;
+    for (int d = 0; d < 3; d++) x.F_out[d] = F[d];
+    x.out_nv = s_ader.nv;
+    std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
+    return v->ader(c, U_sp, Ne, ws, dt, nullptr, nullptr, nullptr, &x);
+}
+
+size_t spdk_fused_fluxes_workspace_bytes(spdk_ctx* c, const int N[3]) {
+    const Variant* v = c ? find_variant(c) : nullptr;
+    if (!v || c->T < 2) {
+        set_error("fused fluxes: no tile variant for this degree/dimensionality, or not ADER");
+        return 0;
+    }
+    return v->slice_traces((size_t)N[0] * N[1] * N[2], c->T) * sizeof(double);
+}
+
+int spdk_fused_fluxes(spdk_ctx* c, const double* U_ader, spdk_sd_shape s_ader, void* ws, double* F_x, double* F_y,
+                      double* F_z) {
+    if (!c || !U_ader || !ws) { set_error("fused fluxes: null argument"); return SPDK_E_ARG; }
+    double* F[3] = {F_x, F_y, F_z};
+    const Variant* v = nullptr;
+    int rc = check_flux_args(c, s_ader, F, &v);
+    if (rc) return rc;
+    int Ne[3] = {s_ader.Nx, s_ader.Ny, s_ader.Nz};
+    Extra x;
+    for (int d = 0; d < 3; d++) x.F_out[d] = F[d];
+    x.out_nv = s_ader.nv;
+    x.U_slices = U_ader;
+    x.slices_nv = s_ader.nv;
+    std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
+    return v->ader(c, nullptr, Ne, ws, 0.0, nullptr, nullptr, nullptr, &x);
 }
 
 }  // extern "C"

@@ -135,7 +135,8 @@ struct HydroRun {
     double rk_a[3] = {0, 0, 0};
     double t = 0, dt = 0, Dt = 0;
     bool fused = true;
-    bool fused_pred = false;  // fused predictor ahead of the stage FV update (fallback runs)
+    bool fused_pred = false;  // fused predictor and flux evaluation ahead of the stage FV update (fallback runs)
+    void* ws_fl = nullptr;    // mixed predictor with fallback: face traces for the fused flux evaluation
     spdk_sd_shape s_sp{}, s_ader{}, s_fp[3]{};
     spdk_fv_shape s_fv{}, s_th{}, s_fvd[3]{};
     double *U_sp = nullptr, *W_sp = nullptr, *W_cv = nullptr, *U_cv = nullptr, *T_sweep = nullptr, *U_ader = nullptr,
@@ -188,9 +189,14 @@ struct HydroRun {
             int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
             fused = spdk_fused_workspace_bytes(ctx, Nq) > 0;
         }
-        // with the fallback, the fused kernel runs the predictor and the stage
-        // kernels the last flux evaluation and the FV update
+        // with the fallback, the fused kernel runs the predictor and the last
+        // flux evaluation, the stage kernels the FV update
         fused_pred = fused && cfg.fallback;
+        size_t fl_bytes = 0;  // mixed with fallback: the flux evaluation in the fused kernel where a variant exists
+        if (mixed && cfg.fallback) {
+            int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
+            fl_bytes = spdk_fused_fluxes_workspace_bytes(ctx, Nq);
+        }
         spdk_timing_enable(ctx, timing ? 1 : 0);
 
         s_sp = spdk_sd_shape{1, kNVAR, Z.n_total, Y.n_total, X.n_total, Z.n_sp, Y.n_sp, X.n_sp};
@@ -233,12 +239,17 @@ struct HydroRun {
         if ((!fused && (!mixed || cfg.fallback)) || fused_pred) {  // the stage arrays (with fallback, also behind
                                                                      // the mixed or fused predictor)
             U_cv = mem.alloc(ctx, nsp);
-            U_ader = mem.alloc(ctx, sd_count(s_ader));
+            if (!fused_pred) U_ader = mem.alloc(ctx, sd_count(s_ader));
             if (cfg.integrator == INT_RK) U0 = mem.alloc(ctx, nsp);
+            const bool stage_fluxes = !fused_pred && fl_bytes == 0;
             for (int d = 0; d < 3; d++) {
                 if (!cfg.act[d]) continue;
-                U_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
+                if (stage_fluxes) U_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
                 if (d) F_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
+            }
+            if (fl_bytes) {
+                CK(spdk_malloc(&ws_fl, fl_bytes));
+                mem.ptrs.push_back(ws_fl);
             }
         }
         if (fused) {
@@ -369,7 +380,8 @@ struct HydroRun {
             // MUSCL fallback in FP64 (src/hydro_ader.hpp:291-305, :500-586)
             CK(spdk_mixed_iterate(ctx, U_sp, s_sp, ws, dt, T - 1));
             CK(spdk_mixed_export_predictor(ctx, ws, U_ader, s_ader));
-            solve_fluxes();
+            if (ws_fl) CK(spdk_fused_fluxes(ctx, U_ader, s_ader, ws_fl, F_fp[0], F_fp[1], F_fp[2]));
+            else solve_fluxes();
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
             CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
@@ -378,8 +390,7 @@ struct HydroRun {
             return next;
         }
         if (fused_pred) {
-            CK(spdk_fused_predict(ctx, U_sp, s_sp, ws, dt, U_ader, s_ader));
-            solve_fluxes();
+            CK(spdk_fused_predict_fluxes(ctx, U_sp, s_sp, ws, dt, F_fp[0], F_fp[1], F_fp[2], s_ader));
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
             CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));

@@ -76,6 +76,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.spdk_last_error.restype = ctypes.c_char_p
         L.spdk_fused_workspace_bytes.restype = ctypes.c_size_t
+        L.spdk_fused_fluxes_workspace_bytes.restype = ctypes.c_size_t
         L.spdk_ctx_stream.restype = ctypes.c_void_p
         L.spdk_mlp_param_count.restype = ctypes.c_size_t
         for f in ("spdk_mlp_set_params", "spdk_mlp_get_params", "spdk_mlp_destroy", "spdk_mlp_dims"):
@@ -258,6 +259,21 @@ class Context:
     def blend_theta(self, troubles, theta_tmp, theta):
         check(lib().spdk_blend_theta(self.ctx, _ptr(troubles), _ptr(theta_tmp), _ptr(theta), FVShape.of(troubles.shape)),
               "blend_theta")
+
+    # --- Solve_fluxes (src/hydro_ader.hpp:278-289) with the fused tile kernel
+    def solve_fluxes_fused(self, U_ader, Fx, Fy, Fz):
+        """Flux-point fluxes of every slice of U_ader (sp->fp, fluxes, trace
+        fill, LLF) into Fx/Fy/Fz (None for an inactive axis)."""
+        import torch
+        L = lib()
+        N = (ctypes.c_int * 3)(U_ader.shape[4], U_ader.shape[3], U_ader.shape[2])
+        nb = L.spdk_fused_fluxes_workspace_bytes(self.ctx, N)
+        if nb == 0:
+            raise RuntimeError(f"fused_fluxes: {last_error()}")
+        ws = torch.empty(nb, dtype=torch.uint8, device=U_ader.device)
+        check(L.spdk_fused_fluxes(self.ctx, _ptr(U_ader), SDShape.of(U_ader.shape), _ptr(ws), _ptr(Fx), _ptr(Fy),
+                                  _ptr(Fz)), "fused_fluxes")
+        self.sync()
 
     def fallback_fluxes(self, W, theta, Fx, Fy, Fz, dt_sub):
         check(lib().spdk_muscl_hllc_blend(self.ctx, _ptr(W), _ptr(theta), _ptr(Fx), _ptr(Fy), _ptr(Fz),

@@ -109,6 +109,42 @@ def test_transform_and_flux_riemann(act, N, p):
 
 
 @pytest.mark.parametrize("act,N", GEOMS)
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("bc", [0, 1, 2])
+def test_solve_fluxes_fused(act, N, p, bc):
+    """The fused flux evaluation of given ADER slices against the oracle's
+    Solve_fluxes sequence (transform sp->fp, fluxes, trace fill, LLF) per
+    axis, every slice, element and point, bit for bit."""
+    rng = np.random.default_rng(1000 + 10 * p + bc + sum(act))
+    c, m, _, _ = ctx_for(p, act, bc=(bc, bc, bc), N=N)
+    n, T = p + 1, p + 1
+    s_a = sd_shape(T, 6, N, n, act)
+    Ua = state(s_a, rng)
+    cfg = op.OrcCfg.make(act=act, bc=(bc, bc, bc))
+    F_o, F_g = [], []
+    for d in range(3):
+        if not act[d]:
+            F_o.append(None)
+            F_g.append(None)
+            continue
+        s_fp = sd_shape(T, 6, N, n, act, stag=d)
+        Ufp = np.zeros(s_fp)
+        op.orc().orc_transform_1d(op.dp(Ua), op.ip(s_a), op.dp(Ufp), op.ip(s_fp), op.dp(m["sp_to_fp"]), n + 1, n, d, -1)
+        F = np.zeros(s_fp)
+        op.orc().orc_fluxes(ctypes.byref(cfg), op.dp(Ufp), op.ip(s_fp), op.dp(F), d)
+        op.orc().orc_sd_bc(ctypes.byref(cfg), op.dp(Ufp), op.ip(s_fp), d)
+        op.orc().orc_riemann(ctypes.byref(cfg), op.dp(Ufp), op.dp(F), op.ip(s_fp), d)
+        F_o.append(F)
+        F_g.append(torch.full(s_fp, float("nan"), dtype=torch.float64, device="cuda"))
+    c.solve_fluxes_fused(dev(Ua), F_g[0], F_g[1], F_g[2])
+    for d in range(3):
+        if act[d]:
+            g = host(F_g[d])
+            assert np.array_equal(g, F_o[d]), f"axis {d}: {int((g != F_o[d]).sum())} of {g.size} differ"
+    c.close()
+
+
+@pytest.mark.parametrize("act,N", GEOMS)
 @pytest.mark.parametrize("p", [2, 3])
 def test_predict_correct_dt(act, N, p):
     rng = np.random.default_rng(100 + p)

@@ -126,7 +126,9 @@ int spdk_memcpy_d2d(spdk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int spdk_transform_1d(spdk_ctx* ctx, const double* A, spdk_sd_shape sa, double* B, spdk_sd_shape sb, int mat, int dim,
                       int t_src);
 /* transform_a_to_b (src/transforms.cpp:99-117): x, y, z sweeps over the active
- * axes through scratch T (shape of B). */
+ * axes through scratch T (shape of B). When the active axes are x, x-y or
+ * x-y-z with the matrix's point count along each (up to 6), all sweeps of an
+ * element run in one pass and T is not touched; the result is the same bits. */
 int spdk_transform(spdk_ctx* ctx, const double* A, double* B, double* T, spdk_sd_shape s, int mat);
 /* compute_conservatives / compute_primitives, SD (src/hydro.cpp:47-72, :108-132) */
 int spdk_conservatives(spdk_ctx* ctx, const double* W, double* U, spdk_sd_shape s);
@@ -159,7 +161,9 @@ int spdk_min_dt(spdk_ctx* ctx, const double* W_cv, spdk_sd_shape s, double* dt_h
  * shape with nt = 1 */
 int spdk_fv_face_integral(spdk_ctx* ctx, const double* F_fp, spdk_sd_shape s_fp, double* F, spdk_fv_shape sf,
                           double* T, int t_id, int dim);
-/* fv_update_solution (src/transforms.cpp:483-543) */
+/* fv_update_solution (src/transforms.cpp:483-543). update = 2 is update = 1
+ * for a caller whose U_old already holds U_cv's FV values (the update = 0 call
+ * of the same slice, U_cv unchanged since): U_old is then not written again. */
 int spdk_fv_update(spdk_ctx* ctx, double* U_new, double* U_old, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
                    const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id, double dt, int update);
 /* boundaries, FV cell-centred fields (src/boundary.cpp:123-190), all active axes x->y->z */

@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, doubl
     for (int v = 0; v < s.nv; v++) {
         double h, F = 0;
         double uo = Ucv[v * slab + o];
-        Uo[sc.off(v, K, J, I)] = uo;
+        if (a.update != 2) Uo[sc.off(v, K, J, I)] = uo;
         if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
         if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
         if (a.act[2]) { h = geo.f[2][K + 1] - geo.f[2][K]; F += (Fz[sz.off(v, K + 1, J, I)] - Fz[sz.off(v, K, J, I)]) / h; }
@@ -560,6 +560,7 @@ int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_sha
     a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
     if (t_id < 0 || t_id >= c->T) { set_error("fv_update: slice"); return SPDK_E_ARG; }
     a.wdt = c->wt_host[t_id] * dt;
+    if (update < 0 || update > 2) { set_error("fv_update: update is 0, 1 or 2"); return SPDK_E_ARG; }
     a.update = update;
     int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
     uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);

@@ -70,6 +70,82 @@ __global__ void __launch_bounds__(BS) k_transform_1d(const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// transform_a_to_b (src/transforms.cpp:99-117) with the sweeps of every active
+// axis (x, x-y or x-y-z) of an element in one pass: a workgroup owns EPB whole
+// elements; per (slice, variable) plane their values go through LDS and the
+// x, y, z sweeps run there, each output the sum over ll of in[ll] M(id, ll)
+// in ascending ll as in k_transform_1d. The result equals the ping-pong
+// sequence bit for bit, with one read of A and one write of B.
+template <int Q, int D>
+__global__ void __launch_bounds__(BS) k_transform_el(const double* __restrict__ A, double* __restrict__ B,
+                                                     const double* __restrict__ M, int ldm, uint32_t E, int planes) {
+    constexpr int TPE = D == 1 ? Q : (D == 2 ? Q * Q : Q * Q * Q);
+    constexpr int EPB = TPE >= BS ? 1 : BS / TPE;
+    static_assert(TPE <= BS, "one thread per point of an element");
+    __shared__ double buf[2][EPB * TPE];
+    __shared__ double sM[Q * Q];
+    const int tid = threadIdx.x, el = tid / TPE, s = tid % TPE;
+    const uint32_t e = blockIdx.x * EPB + el;
+    const bool live = el < EPB && e < E;
+    if (tid < Q * Q) sM[tid] = M[(tid / Q) * ldm + tid % Q];
+    const int ii = s % Q, jj = (s / Q) % Q, kk = s / (Q * Q);
+    double* b0 = buf[0] + (live ? el : 0) * TPE;
+    double* b1 = buf[1] + (live ? el : 0) * TPE;
+    const size_t slab = (size_t)E * TPE;
+    __syncthreads();
+    for (int pl = 0; pl < planes; pl++) {
+        const size_t o = (size_t)pl * slab + (size_t)e * TPE + s;
+        if (live) b0[s] = A[o];
+        __syncthreads();
+        double u = 0;
+        if (live) {
+#pragma unroll
+            for (int l = 0; l < Q; l++) u += b0[s - ii + l] * sM[ii * Q + l];
+        }
+        if (D >= 2) {
+            if (live) b1[s] = u;
+            __syncthreads();
+            u = 0;
+            if (live) {
+#pragma unroll
+                for (int l = 0; l < Q; l++) u += b1[s + (l - jj) * Q] * sM[jj * Q + l];
+            }
+            if (D == 3) {
+                if (live) b0[s] = u;
+                __syncthreads();
+                u = 0;
+                if (live) {
+#pragma unroll
+                    for (int l = 0; l < Q; l++) u += b0[s + (l - kk) * Q * Q] * sM[kk * Q + l];
+                }
+            }
+        }
+        if (live) B[o] = u;
+        __syncthreads();
+    }
+}
+
+template <int D>
+static bool launch_transform_el(spdk_ctx* c, const double* A, double* B, const spdk_sd_shape& s, int mat) {
+    const uint32_t E = (uint32_t)((size_t)s.Nz * s.Ny * s.Nx);
+    const int planes = s.nt * s.nv;
+    const int q = s.nx;
+#define SPDK_TEL(QQ)                                                                                                  \
+    case QQ: {                                                                                                        \
+        constexpr int TPE = D == 1 ? QQ : (D == 2 ? QQ * QQ : QQ * QQ * QQ);                                          \
+        constexpr int EPB = TPE >= BS ? 1 : BS / TPE;                                                                 \
+        hipLaunchKernelGGL((k_transform_el<QQ, D>), dim3((E + EPB - 1) / EPB), dim3(BS), 0, c->stream, A, B, c->mat[mat], \
+                           c->mcols[mat], E, planes);                                                                 \
+        return true;                                                                                                  \
+    }
+    switch (q) {
+        SPDK_TEL(1) SPDK_TEL(2) SPDK_TEL(3) SPDK_TEL(4) SPDK_TEL(5) SPDK_TEL(6)
+        default: return false;
+    }
+#undef SPDK_TEL
+}
+
+// ---------------------------------------------------------------------------
 // compute_conservatives / compute_primitives: src/hydro.cpp:47-72, :108-132
 __global__ void __launch_bounds__(BS) k_cons_prim(const double* __restrict__ A, double* __restrict__ B, SD s, double gm,
                                                   int to_prim, uint32_t total) {
@@ -516,6 +592,20 @@ int spdk_transform(spdk_ctx* c, const double* A, double* B, double* T, spdk_sd_s
     int dims[3], nd = 0;
     for (int d = 0; d < 3; d++)
         if (c->prm.active[d]) dims[nd++] = d;
+    // every active-axis sweep of an element in one pass: active axes x, x-y or
+    // x-y-z, square matrix, the same point count along every active axis
+    REQUIRE_SHAPE(s);
+    if (mat < 0 || mat >= SPDK_M_COUNT || !A || !B) { set_error("bad transform args"); return SPDK_E_ARG; }
+    const int q = s.nx, act_ok = nd >= 1 && dims[0] == 0 && (nd < 2 || dims[1] == 1) && (nd < 3 || dims[2] == 2);
+    const bool pts_ok = (s.ny == (nd >= 2 ? q : 1)) && (s.nz == (nd >= 3 ? q : 1));
+    if (act_ok && pts_ok && c->mcols[mat] == q && c->mrows[mat] == q && A != B && (size_t)s.Nz * s.Ny * s.Nx < (1ull << 31)) {
+        bool launched = nd == 1 ? launch_transform_el<1>(c, A, B, s, mat)
+                                : (nd == 2 ? launch_transform_el<2>(c, A, B, s, mat) : launch_transform_el<3>(c, A, B, s, mat));
+        if (launched) {
+            SPDK_LAUNCH_CHECK("transform_el");
+            return SPDK_OK;
+        }
+    }
     const double* src = A;
     for (int q = 0; q < nd; q++) {
         double* dst = ((nd - 1 - q) % 2 == 0) ? B : T;

@@ -109,6 +109,36 @@ def test_transform_and_flux_riemann(act, N, p):
 
 
 @pytest.mark.parametrize("act,N", GEOMS)
+@pytest.mark.parametrize("p", [1, 3, 4, 5, 6])
+@pytest.mark.parametrize("mat", ["sp_to_cv", "cv_to_sp"])
+@pytest.mark.parametrize("nt", [1, 3])
+def test_transform_a_to_b(act, N, p, mat, nt):
+    """transform_a_to_b (x, y, z sweeps over the active axes) against the
+    oracle's one-axis transforms applied in that order, bit for bit; 3-D p=6
+    (343 points per element) takes the ping-pong sweeps, the rest the
+    one-pass element kernel."""
+    rng = np.random.default_rng(2000 + 10 * p + nt + sum(act))
+    c, m, _, _ = ctx_for(p, act, N=N)
+    n = p + 1
+    shape = sd_shape(nt, 6, N, n, act)
+    A = np.ascontiguousarray(rng.normal(size=shape))
+    M = m[mat]
+    cur = A
+    for d in range(3):
+        if not act[d]:
+            continue
+        nxt = np.zeros(shape)
+        op.orc().orc_transform_1d(op.dp(cur), op.ip(shape), op.dp(nxt), op.ip(shape), op.dp(M), n, n, d, -1)
+        cur = nxt
+    B = torch.full(shape, float("nan"), dtype=torch.float64, device="cuda")
+    T = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    c.transform_a_to_b(dev(A), B, T, getattr(sk, "SPDK_M_" + mat.upper()))
+    got = host(B)
+    assert np.array_equal(got, cur), f"{int((got != cur).sum())} of {got.size} differ"
+    c.close()
+
+
+@pytest.mark.parametrize("act,N", GEOMS)
 @pytest.mark.parametrize("p", [1, 2, 3])
 @pytest.mark.parametrize("bc", [0, 1, 2])
 def test_solve_fluxes_fused(act, N, p, bc):

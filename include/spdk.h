@@ -161,11 +161,22 @@ int spdk_min_dt(spdk_ctx* ctx, const double* W_cv, spdk_sd_shape s, double* dt_h
  * shape with nt = 1 */
 int spdk_fv_face_integral(spdk_ctx* ctx, const double* F_fp, spdk_sd_shape s_fp, double* F, spdk_fv_shape sf,
                           double* T, int t_id, int dim);
-/* fv_update_solution (src/transforms.cpp:483-543). update = 2 is update = 1
- * for a caller whose U_old already holds U_cv's FV values (the update = 0 call
- * of the same slice, U_cv unchanged since): U_old is then not written again. */
+/* fv_update_solution (src/transforms.cpp:483-543) */
 int spdk_fv_update(spdk_ctx* ctx, double* U_new, double* U_old, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
                    const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id, double dt, int update);
+/* fv_update_solution's update = 1 call as FV_Update_solution makes it
+ * (src/hydro_ader.hpp:500-586): after the update = 0 call of the same slice
+ * and the blend of the fallback fluxes with weights theta (FV shape, one
+ * variable), with U_cv unchanged since. Cells with a nonzero theta in
+ * themselves or a face neighbour are recomputed; for the others U_new from
+ * the update = 0 call is the result (their face fluxes are unchanged) and
+ * only U_cv = U_new is written (with gravity every cell is recomputed: the
+ * update = 1 form takes the source from the density it has just written).
+ * U_old, which already holds U_cv, is not written. Same bits as the
+ * update = 1 call. */
+int spdk_fv_update_blended(spdk_ctx* ctx, double* U_new, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
+                           const double* Fy, const double* Fz, const double* theta, spdk_fv_shape sc, int t_id,
+                           double dt);
 /* boundaries, FV cell-centred fields (src/boundary.cpp:123-190), all active axes x->y->z */
 int spdk_fill_fv_ghosts(spdk_ctx* ctx, double* U, spdk_fv_shape s);
 /* compute_primitives, FV (src/hydro.cpp:134-153) */

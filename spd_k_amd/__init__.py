@@ -249,6 +249,11 @@ class Context:
                                    _ptr(Fy), _ptr(Fz), FVShape.of(U_new.shape), t_id, ctypes.c_double(dt), int(update)),
               "fv_update")
 
+    def fv_update_blended(self, U_new, U_cv, Fx, Fy, Fz, theta, t_id, dt):
+        check(lib().spdk_fv_update_blended(self.ctx, _ptr(U_new), _ptr(U_cv), SDShape.of(U_cv.shape), _ptr(Fx), _ptr(Fy),
+                                           _ptr(Fz), _ptr(theta), FVShape.of(U_new.shape), t_id, ctypes.c_double(dt)),
+              "fv_update_blended")
+
     def compute_primitives_fv(self, U, W):
         check(lib().spdk_fv_primitives(self.ctx, _ptr(U), _ptr(W), FVShape.of(U.shape)), "fv_primitives")
 

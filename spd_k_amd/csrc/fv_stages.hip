@@ -140,25 +140,18 @@ struct FvUpd {
     double wdt;  // w(t_id)*dt
     int update;
 };
-__global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, double* __restrict__ Uo, double* __restrict__ Ucv,
-                                                  SD s, const double* __restrict__ Fx, const double* __restrict__ Fy,
-                                                  const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz, Geo geo,
-                                                  FvUpd a, int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
-    if (idx >= total) return;
-    // one thread per active FV cell, x fastest: the FV arrays (fluxes, old
-    // and new state) are read and written in contiguous runs; the SD-layout
-    // state in runs of one element row
-    const int Mx = (s.Nx - 2 * gx) * s.nx, My = (s.Ny - 2 * gy) * s.ny;
-    const int ic = (int)(idx % Mx), jc = (int)((idx / Mx) % My), kc = (int)(idx / ((uint32_t)Mx * My));
-    const int i = gx + ic / s.nx, ii = ic % s.nx, j = gy + jc / s.ny, jj = jc % s.ny, k = gz + kc / s.nz, kk = kc % s.nz;
-    int I = ii + fx + (i - gx) * s.nx, J = jj + fy + (j - gy) * s.ny, K = kk + fz + (k - gz) * s.nz;
-    size_t slab = s.slab();
-    size_t o = s.off(0, 0, k, j, i, kk, jj, ii);
+// One FV cell (K, J, I), SD position o: U_old = U_cv (write_old), U_new =
+// U_old - w dt (sum_d dF_d / h_d - S), and with update U_cv = U_new.
+__device__ __forceinline__ void fv_cell(double* __restrict__ Un, double* __restrict__ Uo, double* __restrict__ Ucv,
+                                        const SD& s, const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                        const double* __restrict__ Fz, const FV& sc, const FV& sx, const FV& sy,
+                                        const FV& sz, const Geo& geo, const FvUpd& a, int K, int J, int I, size_t o,
+                                        bool write_old, bool update) {
+    const size_t slab = s.slab();
     for (int v = 0; v < s.nv; v++) {
         double h, F = 0;
         double uo = Ucv[v * slab + o];
-        if (a.update != 2) Uo[sc.off(v, K, J, I)] = uo;
+        if (write_old) Uo[sc.off(v, K, J, I)] = uo;
         if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
         if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
         if (a.act[2]) { h = geo.f[2][K + 1] - geo.f[2][K]; F += (Fz[sz.off(v, K + 1, J, I)] - Fz[sz.off(v, K, J, I)]) / h; }
@@ -172,8 +165,66 @@ __global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, doubl
         }
         double un = uo - a.wdt * F;
         Un[sc.off(v, K, J, I)] = un;
-        if (a.update) Ucv[v * slab + o] = un;
+        if (update) Ucv[v * slab + o] = un;
     }
+}
+
+// One thread per active FV cell, x fastest: the FV arrays (fluxes, old and
+// new state) are read and written in contiguous runs; the SD-layout state in
+// runs of one element row.
+struct FvIdx {
+    int K, J, I;
+    size_t o;
+};
+__device__ __forceinline__ FvIdx fv_idx(uint32_t idx, const SD& s, int gx, int gy, int gz, int fx, int fy, int fz) {
+    const int Mx = (s.Nx - 2 * gx) * s.nx, My = (s.Ny - 2 * gy) * s.ny;
+    const int ic = (int)(idx % Mx), jc = (int)((idx / Mx) % My), kc = (int)(idx / ((uint32_t)Mx * My));
+    const int i = gx + ic / s.nx, ii = ic % s.nx, j = gy + jc / s.ny, jj = jc % s.ny, k = gz + kc / s.nz, kk = kc % s.nz;
+    FvIdx r;
+    r.I = ii + fx + (i - gx) * s.nx;
+    r.J = jj + fy + (j - gy) * s.ny;
+    r.K = kk + fz + (k - gz) * s.nz;
+    r.o = s.off(0, 0, k, j, i, kk, jj, ii);
+    return r;
+}
+
+__global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, double* __restrict__ Uo, double* __restrict__ Ucv,
+                                                  SD s, const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                  const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz, Geo geo,
+                                                  FvUpd a, int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    const FvIdx c = fv_idx(idx, s, gx, gy, gz, fx, fy, fz);
+    fv_cell(Un, Uo, Ucv, s, Fx, Fy, Fz, sc, sx, sy, sz, geo, a, c.K, c.J, c.I, c.o, true, a.update != 0);
+}
+
+// The update = 1 call after the update = 0 call of the same slice and the
+// blend: a face flux changed only where its blend weight max(theta of the two
+// cells) is nonzero, so a cell whose theta and face neighbours' theta are all
+// zero keeps the update = 0 result, already in U_new, and only U_cv = U_new
+// is written; the other cells are recomputed. U_old already holds U_cv. With
+// gravity every cell is recomputed: the update = 1 form takes the source of
+// the momenta and energy from the density it has just written to U_cv.
+__global__ void __launch_bounds__(BS) k_fv_update_blended(double* __restrict__ Un, double* __restrict__ Ucv, SD s,
+                                                          const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                          const double* __restrict__ Fz, const double* __restrict__ th,
+                                                          FV sc, FV sx, FV sy, FV sz, Geo geo, FvUpd a, int gx, int gy,
+                                                          int gz, int fx, int fy, int fz, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    const FvIdx c = fv_idx(idx, s, gx, gy, gz, fx, fy, fz);
+    const size_t sy1 = (size_t)sc.Nx, sz1 = (size_t)sc.Nx * sc.Ny;
+    const size_t t0 = ((size_t)c.K * sc.Ny + c.J) * sc.Nx + c.I;
+    bool touched = a.grav || th[t0] != 0.0;
+    if (a.act[0]) touched = touched || th[t0 - 1] != 0.0 || th[t0 + 1] != 0.0;
+    if (a.act[1]) touched = touched || th[t0 - sy1] != 0.0 || th[t0 + sy1] != 0.0;
+    if (a.act[2]) touched = touched || th[t0 - sz1] != 0.0 || th[t0 + sz1] != 0.0;
+    if (touched) {
+        fv_cell(Un, nullptr, Ucv, s, Fx, Fy, Fz, sc, sx, sy, sz, geo, a, c.K, c.J, c.I, c.o, false, true);
+        return;
+    }
+    const size_t slab = s.slab();
+    for (int v = 0; v < s.nv; v++) Ucv[v * slab + c.o] = Un[sc.off(v, c.K, c.J, c.I)];
 }
 
 // ---------------------------------------------------------------------------
@@ -560,7 +611,6 @@ int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_sha
     a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
     if (t_id < 0 || t_id >= c->T) { set_error("fv_update: slice"); return SPDK_E_ARG; }
     a.wdt = c->wt_host[t_id] * dt;
-    if (update < 0 || update > 2) { set_error("fv_update: update is 0, 1 or 2"); return SPDK_E_ARG; }
     a.update = update;
     int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
     uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);
@@ -568,6 +618,24 @@ int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_sha
                        Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]), fv_stag(sc, 2, a.act[2]),
                        geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
     SPDK_LAUNCH_CHECK("fv_update");
+    return SPDK_OK;
+}
+
+int spdk_fv_update_blended(spdk_ctx* c, double* Un, double* Ucv, spdk_sd_shape s_cv, const double* Fx, const double* Fy,
+                           const double* Fz, const double* theta, spdk_fv_shape sc, int t_id, double dt) {
+    if (!theta || !Un || !Ucv) { set_error("fv_update_blended: null argument"); return SPDK_E_ARG; }
+    FvUpd a;
+    for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
+    if (t_id < 0 || t_id >= c->T) { set_error("fv_update_blended: slice"); return SPDK_E_ARG; }
+    a.wdt = c->wt_host[t_id] * dt;
+    a.update = 1;
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);
+    hipLaunchKernelGGL(k_fv_update_blended, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Un, Ucv, SD(s_cv), Fx,
+                       Fy ? Fy : Fx, Fz ? Fz : Fx, theta, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
+                       fv_stag(sc, 2, a.act[2]), geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
+    SPDK_LAUNCH_CHECK("fv_update_blended");
     return SPDK_OK;
 }
 

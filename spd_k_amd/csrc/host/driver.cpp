@@ -358,7 +358,7 @@ struct HydroRun {
             CK(spdk_blend_theta(ctx, troubles, theta_tmp, theta, s_fv));
             CK(spdk_fill_fv_ghosts(ctx, theta, s_th));
             CK(spdk_muscl_hllc_blend(ctx, W_old, theta, F_fv[0], F_fv[1], F_fv[2], s_fv, op.wt[a] * dt));
-            CK(spdk_fv_update(ctx, U_new, U_old, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], s_fv, a, dt, 2));  // U_old kept
+            CK(spdk_fv_update_blended(ctx, U_new, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], theta, s_fv, a, dt));
         }
         CK(spdk_transform(ctx, U_cv, U_sp, T_sweep, s_sp, SPDK_M_CV_TO_SP));
     }

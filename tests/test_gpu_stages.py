@@ -271,4 +271,36 @@ def test_fv_fallback_stages(act, N, opts):
     for d in range(3):
         if act[d]:
             assert np.array_equal(host(Fg[d]), Fo[d]), d
+    # the FV update pair of one slice: update = 0 with the SD face fluxes, then
+    # the blended update (recomputing only cells next to a nonzero theta)
+    # against update = 1 with the blended fluxes; theta nonzero on a few
+    # cells, the fluxes changed on exactly their faces
+    s_cv = sd_shape(1, 6, N, n, act)
+    Ucv = state(s_cv, rng)
+    th_s = np.zeros(s_th)
+    lo = [2 if act[d] else 0 for d in range(3)]
+    cells = [tuple(int(rng.integers(lo[d], s_fv[3 - d] - lo[d])) for d in (2, 1, 0)) for _ in range(4)]
+    Fb = [f.copy() for f in Fs]
+    for K, J, I in cells:
+        th_s[0, K, J, I] = 0.5
+        for d in range(3):
+            if act[d]:
+                for off in (0, 1):
+                    KK, JJ, II = K + off * (d == 2), J + off * (d == 1), I + off * (d == 0)
+                    Fb[d][:, KK, JJ, II] += rng.normal(size=6)
+    assert (th_s == 0).sum() > th_s.size // 2
+    Fsd = [dev(f) if act[d] else None for d, f in enumerate(Fs)]
+    Fbl = [dev(f) if act[d] else None for d, f in enumerate(Fb)]
+    th_g = dev(th_s)
+    t_id, dt = 1, 0.0137
+    Un_a, Uo_a, Ucv_a = (torch.zeros(s_fv, dtype=torch.float64, device="cuda"),
+                         torch.zeros(s_fv, dtype=torch.float64, device="cuda"), dev(Ucv))
+    c.fv_update_solution(Un_a, Uo_a, Ucv_a, Fsd[0], Fsd[1], Fsd[2], t_id, dt, 0)
+    c.fv_update_blended(Un_a, Ucv_a, Fbl[0], Fbl[1], Fbl[2], th_g, t_id, dt)
+    Un_b, Uo_b, Ucv_b = (torch.zeros(s_fv, dtype=torch.float64, device="cuda"),
+                         torch.zeros(s_fv, dtype=torch.float64, device="cuda"), dev(Ucv))
+    c.fv_update_solution(Un_b, Uo_b, Ucv_b, Fbl[0], Fbl[1], Fbl[2], t_id, dt, 1)
+    assert np.array_equal(host(Un_a), host(Un_b))
+    assert np.array_equal(host(Ucv_a), host(Ucv_b))
+    assert np.array_equal(host(Uo_a), host(Uo_b))
     c.close()

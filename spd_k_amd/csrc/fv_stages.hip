@@ -9,6 +9,15 @@
 using namespace spdk;
 static constexpr int BS = 256;
 
+// Workgroups are dispatched to the 8 XCDs in turn. The stencil kernels give
+// XCD x the x-th contiguous eighth of their blocks, so the rows and planes
+// that neighbouring cells share are read through one L2; the grid is padded
+// to a multiple of 8 (xcd_grid) and blocks past the range exit on the index
+// check.
+constexpr unsigned kXcd = 8;
+static inline unsigned xcd_grid(size_t total) { return (nblocks(total, BS) + kXcd - 1) / kXcd * kXcd; }
+__device__ __forceinline__ uint32_t xcd_block() { return (blockIdx.x % kXcd) * (gridDim.x / kXcd) + blockIdx.x / kXcd; }
+
 struct Geo {  // FV sub-grid coordinates per axis (device pointers)
     const double* f[3];
     const double* c[3];
@@ -192,7 +201,7 @@ __global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, doubl
                                                   SD s, const double* __restrict__ Fx, const double* __restrict__ Fy,
                                                   const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz, Geo geo,
                                                   FvUpd a, int gx, int gy, int gz, int fx, int fy, int fz, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     const FvIdx c = fv_idx(idx, s, gx, gy, gz, fx, fy, fz);
     fv_cell(Un, Uo, Ucv, s, Fx, Fy, Fz, sc, sx, sy, sz, geo, a, c.K, c.J, c.I, c.o, true, a.update != 0);
@@ -210,7 +219,7 @@ __global__ void __launch_bounds__(BS) k_fv_update_blended(double* __restrict__ U
                                                           const double* __restrict__ Fz, const double* __restrict__ th,
                                                           FV sc, FV sx, FV sy, FV sz, Geo geo, FvUpd a, int gx, int gy,
                                                           int gz, int fx, int fy, int fz, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     const FvIdx c = fv_idx(idx, s, gx, gy, gz, fx, fy, fz);
     const size_t sy1 = (size_t)sc.Nx, sz1 = (size_t)sc.Nx * sc.Ny;
@@ -296,7 +305,7 @@ struct Det {
 __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, const double* __restrict__ Wo,
                                                 double* __restrict__ tr, double* __restrict__ alx, double* __restrict__ aly,
                                                 double* __restrict__ alz, FV s, Det a, Geo geo, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
     split3(idx, s.Ny, s.Nx, k, j, i);
@@ -360,7 +369,7 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
 __global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn, double* __restrict__ tr,
                                                   const double* __restrict__ alx, const double* __restrict__ aly,
                                                   const double* __restrict__ alz, FV s, Det a, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int ax = a.act[0], ay = a.act[1], az = a.act[2];
     int Mx = s.Nx - 2 * ax, My = s.Ny - 2 * ay;
@@ -391,7 +400,7 @@ __global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn,
 // apply_blending (src/trouble_detection.cpp:191-210), cells 1..N-2
 __global__ void __launch_bounds__(BS) k_blend(const double* __restrict__ tr, double* __restrict__ th, FV s, int ax, int ay,
                                               int az, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
     split3(idx, s.Ny - 2 * ay, s.Nx - 2 * ax, k, j, i);
@@ -410,7 +419,7 @@ __global__ void __launch_bounds__(BS) k_blend(const double* __restrict__ tr, dou
 // blending_ring (src/trouble_detection.cpp:226-242), cells 1..N-2
 __global__ void __launch_bounds__(BS) k_ring(const double* __restrict__ ti, double* __restrict__ to, FV s, int ax, int ay,
                                              int az, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
     split3(idx, s.Ny - 2 * ay, s.Nx - 2 * ax, k, j, i);
@@ -531,7 +540,7 @@ __global__ void __launch_bounds__(BS) k_fallback(const double* __restrict__ W, c
                                                  double* __restrict__ Fx, double* __restrict__ Fy, double* __restrict__ Fz,
                                                  FV s, FV sx, FV sy, FV sz, Geo g, double dt, int ax, int ay, int az,
                                                  double gm, uint32_t total) {
-    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int Mx = s.Nx - 4 * ax + ax, My = s.Ny - 4 * ay + ay;
     int k, j, i;
@@ -614,7 +623,7 @@ int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_sha
     a.update = update;
     int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
     uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);
-    hipLaunchKernelGGL(k_fv_update, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Un, Uo, Ucv, SD(s_cv), Fx, Fy ? Fy : Fx,
+    hipLaunchKernelGGL(k_fv_update, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Un, Uo, Ucv, SD(s_cv), Fx, Fy ? Fy : Fx,
                        Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]), fv_stag(sc, 2, a.act[2]),
                        geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
     SPDK_LAUNCH_CHECK("fv_update");
@@ -632,7 +641,7 @@ int spdk_fv_update_blended(spdk_ctx* c, double* Un, double* Ucv, spdk_sd_shape s
     a.update = 1;
     int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
     uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);
-    hipLaunchKernelGGL(k_fv_update_blended, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Un, Ucv, SD(s_cv), Fx,
+    hipLaunchKernelGGL(k_fv_update_blended, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Un, Ucv, SD(s_cv), Fx,
                        Fy ? Fy : Fx, Fz ? Fz : Fx, theta, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
                        fv_stag(sc, 2, a.act[2]), geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
     SPDK_LAUNCH_CHECK("fv_update_blended");
@@ -667,11 +676,11 @@ int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double
     a.mask = mask;
     a.tol = c->prm.nad_tol;
     uint32_t total = (uint32_t)FV(s).plane();
-    hipLaunchKernelGGL(k_nad_sed, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, Wn, Wo, tr, alx, aly ? aly : alx,
+    hipLaunchKernelGGL(k_nad_sed, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, alx, aly ? aly : alx,
                        alz ? alz : alx, FV(s), a, geo_of(c), total);
     SPDK_LAUNCH_CHECK("nad_sed");
     uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * a.act[2]) * (s.Ny - 2 * a.act[1]) * (s.Nx - 2 * a.act[0]));
-    hipLaunchKernelGGL(k_relax_pad, dim3(nblocks(t2, BS)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx,
+    hipLaunchKernelGGL(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx,
                        alz ? alz : alx, FV(s), a, t2);
     SPDK_LAUNCH_CHECK("relax_pad");
     return SPDK_OK;
@@ -681,9 +690,9 @@ int spdk_blend_theta(spdk_ctx* c, const double* tr, double* th_tmp, double* th, 
     int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
     if (c->prm.blending) {
         uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * az) * (s.Ny - 2 * ay) * (s.Nx - 2 * ax));
-        hipLaunchKernelGGL(k_blend, dim3(nblocks(t2, BS)), dim3(BS), 0, c->stream, tr, th_tmp, FV(s), ax, ay, az, t2);
+        hipLaunchKernelGGL(k_blend, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, tr, th_tmp, FV(s), ax, ay, az, t2);
         SPDK_LAUNCH_CHECK("blend");
-        hipLaunchKernelGGL(k_ring, dim3(nblocks(t2, BS)), dim3(BS), 0, c->stream, th_tmp, th, FV(s), ax, ay, az, t2);
+        hipLaunchKernelGGL(k_ring, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, th_tmp, th, FV(s), ax, ay, az, t2);
         SPDK_LAUNCH_CHECK("ring");
     } else {
         uint32_t t = (uint32_t)FV(s).plane();
@@ -697,7 +706,7 @@ int spdk_muscl_hllc_blend(spdk_ctx* c, const double* W, const double* th, double
                           spdk_fv_shape s, double dt_sub) {
     int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
     uint32_t total = (uint32_t)((size_t)(s.Nz - 4 * az + az) * (s.Ny - 4 * ay + ay) * (s.Nx - 4 * ax + ax));
-    hipLaunchKernelGGL(k_fallback, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, W, th, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx,
+    hipLaunchKernelGGL(k_fallback, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, W, th, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx,
                        FV(s), fv_stag(s, 0, ax), fv_stag(s, 1, ay), fv_stag(s, 2, az), geo_of(c), dt_sub, ax, ay, az,
                        c->prm.gamma, total);
     SPDK_LAUNCH_CHECK("muscl_hllc");

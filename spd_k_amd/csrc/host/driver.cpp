@@ -294,20 +294,36 @@ struct HydroRun {
         if (cfg.outputs) write_outputs();
     }
 
+    // One '.' per step as the reference's loop prints (src/hydro_ader.hpp:246),
+    // flushed at most every 2 s so that a long run shows its progress.
+    std::chrono::steady_clock::time_point last_flush = std::chrono::steady_clock::now();
+    void progress_dot() {
+        std::fputc('.', stdout);
+        auto now = std::chrono::steady_clock::now();
+        if (std::chrono::duration<double>(now - last_flush).count() > 2.0) {
+            std::fflush(stdout);
+            last_flush = now;
+        }
+    }
+
     void write_outputs() {  // src/hydro_ader.hpp:419-426
         std::string f = output_folder(outdir);
         std::printf("\nOUTPUT %d\n", n_output);
-        if (cfg.fallback) {
+        if (cfg.fallback && cfg.out_troubles) {
             host.resize(fv_count(s_fv));
             CK(spdk_memcpy_d2h(ctx, host.data(), troubles, host.size() * sizeof(double)));
             write_raw(f + fv_name("troubles", s_fv.Nx, n_output), host.data(), host.size());
         }
-        host.resize(sd_count(s_fp[0]));
-        CK(spdk_memcpy_d2h(ctx, host.data(), F_fp[0], host.size() * sizeof(double)));
-        write_raw(f + sd_name("F_ader_fp_x", s_fp[0].Nx, s_fp[0].nx, n_output), host.data(), host.size());
-        host.resize(sd_count(s_sp));
-        CK(spdk_memcpy_d2h(ctx, host.data(), W_cv, host.size() * sizeof(double)));
-        write_raw(f + sd_name("W_cv", s_sp.Nx, s_sp.nx, n_output), host.data(), host.size());
+        if (cfg.out_flux) {
+            host.resize(sd_count(s_fp[0]));
+            CK(spdk_memcpy_d2h(ctx, host.data(), F_fp[0], host.size() * sizeof(double)));
+            write_raw(f + sd_name("F_ader_fp_x", s_fp[0].Nx, s_fp[0].nx, n_output), host.data(), host.size());
+        }
+        if (cfg.out_wcv) {
+            host.resize(sd_count(s_sp));
+            CK(spdk_memcpy_d2h(ctx, host.data(), W_cv, host.size() * sizeof(double)));
+            write_raw(f + sd_name("W_cv", s_sp.Nx, s_sp.nx, n_output), host.data(), host.size());
+        }
         n_output++;
     }
 
@@ -591,6 +607,7 @@ struct HydroRun {
             t += dt;
             n_step++;
             dt = next;
+            progress_dot();
             if (cfg.outputs) {
                 if (t >= t_output) {
                     t_output = t + cfg.dt_output;

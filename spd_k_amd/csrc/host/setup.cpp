@@ -97,6 +97,17 @@ std::string configure(Params& pin, RunConfig& c) {
     c.tlim = pin.real_or("time", "tlim", 0.1);
     c.outputs = pin.has("output", "dt") && pin.get_real("output", "dt") > 0.0;
     c.dt_output = c.outputs ? pin.get_real("output", "dt") : c.tlim;
+    if (pin.has("output", "fields")) {
+        c.out_troubles = c.out_flux = c.out_wcv = false;
+        std::string list = pin.get_string("output", "fields") + ",";
+        for (size_t a = 0, b; (b = list.find(',', a)) != std::string::npos; a = b + 1) {
+            std::string f = list.substr(a, b - a);
+            if (f == "troubles") c.out_troubles = true;
+            else if (f == "F_ader_fp_x") c.out_flux = true;
+            else if (f == "W_cv") c.out_wcv = true;
+            else if (!f.empty()) return "output/fields: unknown field '" + f + "' (troubles, F_ader_fp_x, W_cv)";
+        }
+    }
     std::string integ = pin.string_or("time", "integrator", "ader");
     if (integ == "ader") c.integrator = INT_ADER;
     else if (integ.size() == 3 && integ.compare(0, 2, "rk") == 0 && integ[2] >= '1' && integ[2] <= '3') {

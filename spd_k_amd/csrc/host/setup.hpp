@@ -60,6 +60,9 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     std::string mlp_weights;         // job/mlp_weights: trained network file (job/guess=mlp)
     std::string mode = "run";        // job/mode: run | train_guess
     bool final_output = false;       // job/final_output: write the state after the last step
+    // output/fields (hydro): comma-separated subset of troubles, F_ader_fp_x,
+    // W_cv; the writer's default is all of them
+    bool out_troubles = true, out_flux = true, out_wcv = true;
     // <mlp> block (job/mode=train_guess): fixed-point data from this run, then training
     std::string mlp_out;             // mlp/out: weights file written
     int mlp_snapshots = 8, mlp_every = 4, mlp_sweeps = 20, mlp_epochs = 6, mlp_batch = 4096, mlp_hidden = 256;

@@ -469,58 +469,66 @@ __device__ __forceinline__ void corrector(const double* W, double* dWt, const do
     dWt[3] -= W[1] * dWx[3];
     dWt[3] -= W[2] * dWy[3];
 }
-// slopes_d<D> (src/hydro.cpp:622-681): reconstructed primitive states of
-// cell (k,j,i) at its two faces along D.
-template <int D>
-__device__ __forceinline__ void slopes_d(const double* __restrict__ W, FV s, const Geo& g, int k, int j, int i, double* WL,
-                                         double* WR, double dt, int ay, int az, double gm) {
-    double w[NVP], dwx[NVP], dwy[NVP], dwz[NVP], dwt[NVP];
+// slopes_d (src/hydro.cpp:622-681), split in two: the cell's reconstruction
+// (its primitive state, minmod slopes along the active axes and the Hancock
+// time derivative), which is the same for every face of the cell, and the
+// state at one of its faces along D. A cell's reconstruction is computed once
+// per thread and used for every face that needs it.
+struct Recon {
+    double w[NVP], dw[3][NVP], dwt[NVP];
+};
+__device__ __forceinline__ void recon(const double* __restrict__ W, const FV& s, const Geo& g, int k, int j, int i,
+                                      int ay, int az, double gm, Recon& r) {
     const double *xc = g.c[0], *xf = g.f[0], *yc = g.c[1], *yf = g.f[1], *zc = g.c[2], *zf = g.f[2];
 #pragma unroll
     for (int v = 0; v < NVP; v++) {
-        w[v] = W[s.off(v, k, j, i)];
+        const double w = W[s.off(v, k, j, i)];
+        r.w[v] = w;
         double wL = W[s.off(v, k, j, i - 1)], wR = W[s.off(v, k, j, i + 1)];
-        dwx[v] = minmod((wR - w[v]) / (xc[i + 1] - xc[i]), (w[v] - wL) / (xc[i] - xc[i - 1]), xf[i], xf[i + 1]);
-        dwy[v] = 0;
-        dwz[v] = 0;
+        r.dw[0][v] = minmod((wR - w) / (xc[i + 1] - xc[i]), (w - wL) / (xc[i] - xc[i - 1]), xf[i], xf[i + 1]);
+        r.dw[1][v] = 0;
+        r.dw[2][v] = 0;
         if (ay) {
             wL = W[s.off(v, k, j - 1, i)];
             wR = W[s.off(v, k, j + 1, i)];
-            dwy[v] = minmod((wR - w[v]) / (yc[j + 1] - yc[j]), (w[v] - wL) / (yc[j] - yc[j - 1]), yf[j], yf[j + 1]);
+            r.dw[1][v] = minmod((wR - w) / (yc[j + 1] - yc[j]), (w - wL) / (yc[j] - yc[j - 1]), yf[j], yf[j + 1]);
         }
         if (az) {
             wL = W[s.off(v, k - 1, j, i)];
             wR = W[s.off(v, k + 1, j, i)];
-            dwz[v] = minmod((wR - w[v]) / (zc[k + 1] - zc[k]), (w[v] - wL) / (zc[k] - zc[k - 1]), zf[k], zf[k + 1]);
+            r.dw[2][v] = minmod((wR - w) / (zc[k + 1] - zc[k]), (w - wL) / (zc[k] - zc[k - 1]), zf[k], zf[k + 1]);
         }
     }
-    corrector(w, dwt, dwx, dwy, dwz, gm);
-    double h = (D == 0 ? xf[i + 1] - xf[i] : (D == 1 ? yf[j + 1] - yf[j] : zf[k + 1] - zf[k]));
+    corrector(r.w, r.dwt, r.dw[0], r.dw[1], r.dw[2], gm);
+}
+// The reconstructed state at the cell's upper (WL of the reference) or lower
+// (WR) face along D.
+template <int D>
+__device__ __forceinline__ void face_state(const Recon& r, const Geo& g, int k, int j, int i, double dt, bool upper,
+                                           double* out) {
+    const double h = (D == 0 ? g.f[0][i + 1] - g.f[0][i] : (D == 1 ? g.f[1][j + 1] - g.f[1][j] : g.f[2][k + 1] - g.f[2][k]));
 #pragma unroll
     for (int v = 0; v < NVP; v++) {
-        double dw = (D == 0 ? dwx[v] : (D == 1 ? dwy[v] : dwz[v]));
-        WR[v] = w[v] - dw + dwt[v] * dt / h;
-        WL[v] = w[v] + dw + dwt[v] * dt / h;
+        const double dw = r.dw[D][v];
+        out[v] = upper ? r.w[v] + dw + r.dwt[v] * dt / h : r.w[v] - dw + r.dwt[v] * dt / h;
     }
 }
-// compute_fluxes<D> (src/hydro.cpp:683-751)
+// compute_fluxes<D> (src/hydro.cpp:683-751) at the lower D face of cell
+// (k,j,i), given that cell's lower-face state wR: the lower neighbour's
+// reconstruction, HLLC, and the theta blend of F.
 template <int D>
-__device__ __forceinline__ void fb_face(const double* __restrict__ W, double* __restrict__ F, FV sF,
-                                        const double* __restrict__ th, FV s, const Geo& g, int k, int j, int i, double dt,
-                                        int ay, int az, double gm) {
-    size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
-    size_t c1 = ((size_t)(k - (D == 2)) * s.Ny + (j - (D == 1))) * s.Nx + (i - (D == 0));
-    double t = smax(th[c0], th[c1]);
-    // Blend weight 0: F + 0 * (F_HLLC - F) is F (up to the sign of a zero
-    // flux) for any finite HLLC flux, so the face keeps F and the MUSCL-HLLC
-    // evaluation is skipped. Troubled cells and their ring are a small
-    // fraction of the mesh, and cluster, so most waves skip as a whole.
-    if (t == 0.0) return;
-    double wL0[NVP], wR0[NVP], wL1[NVP], wR1[NVP], uL[NVP], uR[NVP], fL[NVP];
-    slopes_d<D>(W, s, g, k - (D == 2), j - (D == 1), i - (D == 0), wL0, wR0, dt, ay, az, gm);
-    slopes_d<D>(W, s, g, k, j, i, wL1, wR1, dt, ay, az, gm);
-    cons(wL0, uL, gm);
-    cons(wR1, uR, gm);
+__device__ __forceinline__ void fb_face(const double* __restrict__ W, double* __restrict__ F, const FV& sF, const FV& s,
+                                        const Geo& g, int k, int j, int i, double t, const double* wR, double dt, int ay,
+                                        int az, double gm) {
+    const int kn = k - (D == 2), jn = j - (D == 1), in = i - (D == 0);
+    double wL[NVP], uL[NVP], uR[NVP], fL[NVP];
+    {
+        Recon rn;
+        recon(W, s, g, kn, jn, in, ay, az, gm, rn);
+        face_state<D>(rn, g, kn, jn, in, dt, true, wL);
+    }
+    cons(wL, uL, gm);
+    cons(wR, uR, gm);
     riemann_hllc<D>(uL, uR, fL, gm);
 #pragma unroll
     for (int v = 0; v < NVP; v++) {
@@ -536,6 +544,11 @@ __device__ __forceinline__ void fb_face(const double* __restrict__ W, double* __
         F[o] = f;
     }
 }
+// One thread per cell, its three lower faces. Blend weight 0: F + 0 * (F_HLLC
+// - F) is F (up to the sign of a zero flux) for any finite HLLC flux, so such
+// a face keeps F and its MUSCL-HLLC evaluation is skipped. Troubled cells and
+// their ring are a small fraction of the mesh, and cluster, so most waves skip
+// as a whole.
 __global__ void __launch_bounds__(BS) k_fallback(const double* __restrict__ W, const double* __restrict__ th,
                                                  double* __restrict__ Fx, double* __restrict__ Fy, double* __restrict__ Fz,
                                                  FV s, FV sx, FV sy, FV sz, Geo g, double dt, int ax, int ay, int az,
@@ -546,9 +559,23 @@ __global__ void __launch_bounds__(BS) k_fallback(const double* __restrict__ W, c
     int k, j, i;
     split3(idx, My, Mx, k, j, i);
     k += 2 * az; j += 2 * ay; i += 2 * ax;
-    fb_face<0>(W, Fx, sx, th, s, g, k, j, i, dt, ay, az, gm);
-    if (ay) fb_face<1>(W, Fy, sy, th, s, g, k, j, i, dt, ay, az, gm);
-    if (az) fb_face<2>(W, Fz, sz, th, s, g, k, j, i, dt, ay, az, gm);
+    const size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    const double tc = th[c0];
+    const double t0 = smax(tc, th[c0 - 1]);
+    const double t1 = ay ? smax(tc, th[c0 - s.Nx]) : 0.0;
+    const double t2 = az ? smax(tc, th[c0 - (size_t)s.Nx * s.Ny]) : 0.0;
+    if (t0 == 0.0 && t1 == 0.0 && t2 == 0.0) return;
+    double wR[3][NVP];  // this cell's lower-face states
+    {
+        Recon rc;
+        recon(W, s, g, k, j, i, ay, az, gm, rc);
+        if (t0 != 0.0) face_state<0>(rc, g, k, j, i, dt, false, wR[0]);
+        if (t1 != 0.0) face_state<1>(rc, g, k, j, i, dt, false, wR[1]);
+        if (t2 != 0.0) face_state<2>(rc, g, k, j, i, dt, false, wR[2]);
+    }
+    if (t0 != 0.0) fb_face<0>(W, Fx, sx, s, g, k, j, i, t0, wR[0], dt, ay, az, gm);
+    if (t1 != 0.0) fb_face<1>(W, Fy, sy, s, g, k, j, i, t1, wR[1], dt, ay, az, gm);
+    if (t2 != 0.0) fb_face<2>(W, Fz, sz, s, g, k, j, i, t2, wR[2], dt, ay, az, gm);
 }
 
 // ===========================================================================

@@ -449,3 +449,25 @@ def test_kh3d_to_t1_mixed_with_fallback(tmp_path):
     print(f"KH 24^3 t=1: density L1 {l1:.3e}, Linf {linf:.3e} relative to the FP64 run")
     assert op.mass_drift(b) < 1e-12 and op.mass_drift(a) < 1e-12
     assert l1 < 1e-2
+
+
+def test_output_fields_and_progress(tmp_path):
+    """output/fields limits the written fields (the written ones equal a full
+    run's bit for bit), and the time loop prints one '.' per step as the
+    reference's does."""
+    args = ["-i", op.INPUTS + "/sine_wave.athinput", "mesh/nx1=6", "mesh/nx2=5", "mesh/nx3=4", "job/fallback=true",
+            "time/tlim=0.02", "output/dt=0.01"]
+    full, part = str(tmp_path / "full"), str(tmp_path / "part")
+    out = op.run_driver(args, full)
+    op.run_driver(args + ["output/fields=W_cv,troubles"], part)
+    fa = sorted(os.path.basename(f) for f in os.listdir(full) if f.endswith(".dat"))
+    fb = sorted(os.path.basename(f) for f in os.listdir(part) if f.endswith(".dat"))
+    assert any(f.startswith("F_ader_fp_x") for f in fa)
+    assert fb == [f for f in fa if f.startswith(("W_cv", "troubles"))] and fb
+    for f in fb:
+        assert np.array_equal(np.fromfile(os.path.join(full, f)), np.fromfile(os.path.join(part, f))), f
+    steps = int(re.search(r"evolution: (\d+) steps", out).group(1))
+    assert sum(line.count(".") for line in out.splitlines() if set(line) == {"."}) == steps
+    with pytest.raises(RuntimeError):
+        op.run_driver(args + ["output/fields=W_cv,bogus"], str(tmp_path / "bad"))
+

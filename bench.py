@@ -161,8 +161,8 @@ def accuracy(forms):
     import spd_k_amd
     from spd_k_amd.outputs import element_means, l1_vs_exact_sine, output_indices
     sine = ["-i", os.path.join(ROOT, "tests", "inputs", "sine_wave.athinput"), "mesh/nx1=24", "mesh/nx2=24",
-            "mesh/nx3=24", "mesh/p=3", "problem/amp=0.1", "time/tlim=0.2", "output/dt=0.2"]
-    kh = lambda n: kh_args(n, perturb="sin")[:-2] + ["time/tlim=0.03", "output/dt=0.03"]
+            "mesh/nx3=24", "mesh/p=3", "problem/amp=0.1", "time/tlim=0.2", "output/dt=0.2", "output/fields=W_cv"]
+    kh = lambda n: kh_args(n, perturb="sin")[:-2] + ["time/tlim=0.03", "output/dt=0.03", "output/fields=W_cv"]
     out = {}
     with tempfile.TemporaryDirectory() as d:
         fine = os.path.join(d, "kh_fine")

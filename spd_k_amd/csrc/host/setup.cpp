@@ -552,6 +552,8 @@ void write_raw(const std::string& path, const double* a, size_t n) {
     std::ofstream o(path, std::ios::binary);
     if (!o) throw std::runtime_error("cannot write " + path);
     o.write(reinterpret_cast<const char*>(a), (std::streamsize)(n * sizeof(double)));
+    o.close();
+    if (!o) throw std::runtime_error("short write to " + path + " (file system full?)");
 }
 std::string sd_name(const std::string& label, int Nx_total, int nx, int n_out) {
     return label + "_N" + std::to_string(Nx_total - 2 * kNGH) + "p" + std::to_string(nx - 1) + "_" + std::to_string(n_out) + "_0.dat";

@@ -49,7 +49,7 @@ def test_train_then_guess(tmp_path):
     assert os.path.getsize(w) > 0
     assert after < before, (before, after)  # unseen rows of the training distribution
 
-    run = BASE + ["time/tlim=0.02", "output/dt=0.02"]
+    run = BASE + ["time/tlim=0.02", "output/dt=0.02", "output/fields=W_cv"]
     ref, zero, learned = (str(tmp_path / k) for k in ("fp64", "zero", "learned"))
     op.run_driver(run, ref)
     res = ["job/precision=mixed", "job/picard=residual", "job/picard_tol=1e-6"]
@@ -94,7 +94,7 @@ def test_kh2d_p4_learned_histogram(tmp_path):
     after = float(re.findall(r"held-out loss ([0-9.eE+-]+), last snapshot", out)[-1])
     assert after < before, (before, after)
 
-    run = base + ["time/tlim=0.002", "output/dt=0.002"]
+    run = base + ["time/tlim=0.002", "output/dt=0.002", "output/fields=W_cv"]
     ref = str(tmp_path / "fp64")
     op.run_driver(run, ref)
     i = op.output_indices(ref)[-1]

@@ -230,8 +230,9 @@ def all_configs(steps=6, warmup=2):
       cfg1 2-D sine advection 32^2 p=3, ADER, one period (t = 1), L1 of
            density against the exact solution;
       cfg2 Sod (256^2, x gradfree, y periodic) and the central-cells Sedov
-           blast (256^2, P = 1e-5 background), p=3, NAD + MUSCL fallback (the
-           stage path), `steps` steps;
+           blast (256^2, P = 1e-5 background), p=3, NAD + MUSCL fallback: the
+           stage sequence and the default (fused predictor and flux sweep,
+           stage FV update), `steps` steps;
       cfg4 2-D KH 512^2 p=4 (seeded perturbation, CFL 0.5 without the
            fallback), FP64 / mixed / residual (1e-6 and 1e-10) forms.
     cfg3 and cfg5 are the default run's workloads."""
@@ -262,8 +263,9 @@ def all_configs(steps=6, warmup=2):
         sod = ["-i", inp("sod.athinput"), "mesh/nx1=256", "mesh/nx2=256", "mesh/x2_bc=periodic", "time/tlim=0.2"]
         sedov = ["-i", inp("sine_wave.athinput"), "problem/problem=sedov_cells", "job/fallback=true", "mesh/nx1=256",
                  "mesh/nx2=256", "mesh/nx3=1", "hydro/gamma=1.4", "time/tlim=0.05"]
-        out["cfg2_sod_256_p3_fallback"] = {"stages_fp64": timed(sod, os.path.join(d, "c2a"))}
-        out["cfg2_sedov_256_p3_fallback"] = {"stages_fp64": timed(sedov, os.path.join(d, "c2b"))}
+        for key, args in (("cfg2_sod_256_p3_fallback", sod), ("cfg2_sedov_256_p3_fallback", sedov)):
+            out[key] = {"fp64_stages": timed(args + ["job/path=stages"], os.path.join(d, key + "_s")),
+                        "fp64_fused": timed(args, os.path.join(d, key + "_f"))}
         kh2 = ["-i", inp("kelvin_helmholtz.athinput"), "job/fallback=false", "mesh/nx1=512", "mesh/nx2=512",
                "mesh/nx3=1", "mesh/p=4", "problem/perturb=random", "problem/amp=0.01", "time/cfl=0.5", "time/tlim=1"]
         c4 = {}

@@ -302,6 +302,10 @@ struct Det {
     int moore, delta, use_sed, pad, mask;
     double tol;
 };
+// The active axes are template arguments, so the Moore box and the SED
+// stencils unroll with their neighbour offsets formed once per thread; the
+// box is visited in the same (z, y, x) order as the reference's loops.
+template <int AX, int AY, int AZ>
 __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, const double* __restrict__ Wo,
                                                 double* __restrict__ tr, double* __restrict__ alx, double* __restrict__ aly,
                                                 double* __restrict__ alz, FV s, Det a, Geo geo, uint32_t total) {
@@ -309,28 +313,36 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
     if (idx >= total) return;
     int k, j, i;
     split3(idx, s.Ny, s.Nx, k, j, i);
-    int ax = a.act[0], ay = a.act[1], az = a.act[2];
-    int nvar = s.nv - 1;
-    bool in1 = (!ax || (i >= 1 && i <= s.Nx - 2)) && (!ay || (j >= 1 && j <= s.Ny - 2)) && (!az || (k >= 1 && k <= s.Nz - 2));
-    bool in2 = (!ax || (i >= 2 && i <= s.Nx - 3)) && (!ay || (j >= 2 && j <= s.Ny - 3)) && (!az || (k >= 2 && k <= s.Nz - 3));
+    const int nvar = s.nv - 1;
+    const bool in1 = (!AX || (i >= 1 && i <= s.Nx - 2)) && (!AY || (j >= 1 && j <= s.Ny - 2)) &&
+                     (!AZ || (k >= 1 && k <= s.Nz - 2));
+    const bool in2 = (!AX || (i >= 2 && i <= s.Nx - 3)) && (!AY || (j >= 2 && j <= s.Ny - 3)) &&
+                     (!AZ || (k >= 2 && k <= s.Nz - 3));
+    const size_t pl = s.plane(), c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
     for (int v = 0; v < nvar; v++) {
         if (!((a.mask >> v) & 1)) continue;
+        const double* wo = Wo + v * pl + c0;
+        const double* wn = Wn + v * pl + c0;
         if (in1) {
-            double un = Wn[s.off(v, k, j, i)];
-            double mx = Wo[s.off(v, k, j, i)], mn = mx;
+            double un = wn[0];
+            double mx = wo[0], mn = mx;
             if (a.moore) {
-                for (int dk = -az; dk <= az; dk++)
-                    for (int dj = -ay; dj <= ay; dj++)
-                        for (int di = -ax; di <= ax; di++) {
-                            double u = Wo[s.off(v, k + dk, j + dj, i + di)];
+#pragma unroll
+                for (int dk = -AZ; dk <= AZ; dk++)
+#pragma unroll
+                    for (int dj = -AY; dj <= AY; dj++)
+#pragma unroll
+                        for (int di = -AX; di <= AX; di++) {
+                            double u = wo[dk * sz + dj * sy + di];
                             mx = smax(mx, u);
                             mn = smin(mn, u);
                         }
             } else {
                 double uL, uR;
-                if (ax) { uL = Wo[s.off(v, k, j, i - 1)]; uR = Wo[s.off(v, k, j, i + 1)]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
-                if (ay) { uL = Wo[s.off(v, k, j - 1, i)]; uR = Wo[s.off(v, k, j + 1, i)]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
-                if (az) { uL = Wo[s.off(v, k - 1, j, i)]; uR = Wo[s.off(v, k + 1, j, i)]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (AX) { uL = wo[-1]; uR = wo[1]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (AY) { uL = wo[-sy]; uR = wo[sy]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
+                if (AZ) { uL = wo[-sz]; uR = wo[sz]; mx = smax(smax(uL, mx), uR); mn = smin(smin(uL, mn), uR); }
             }
             if (a.delta) {
                 double eps = a.tol * (mx - mn);
@@ -340,26 +352,27 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
                 mn -= fabs(mn) * a.tol;
                 mx += fabs(mx) * a.tol;
             }
-            tr[s.off(v, k, j, i)] = (un > mx || un < mn) ? 1.0 : 0.0;
+            tr[v * pl + c0] = (un > mx || un < mn) ? 1.0 : 0.0;
         }
         if (a.use_sed && in2) {
-            double u = Wn[s.off(v, k, j, i)];
+            double u = wn[0];
+#pragma unroll
             for (int d = 0; d < 3; d++) {
-                if (!a.act[d]) continue;
+                if (!(d == 0 ? AX : (d == 1 ? AY : AZ))) continue;
                 const double* c = geo.c[d];
                 const double* f = geo.f[d];
-                int l = d == 0 ? i : (d == 1 ? j : k);
-                int di = d == 0, dj = d == 1, dk = d == 2;
+                const int l = d == 0 ? i : (d == 1 ? j : k);
+                const long st = d == 0 ? 1 : (d == 1 ? sy : sz);
                 double h = (c[l + 1] - c[l - 1]);
                 double hp = (c[l + 2] - c[l]);
                 double hm = (c[l] - c[l - 2]);
-                double du = (Wn[s.off(v, k + dk, j + dj, i + di)] - Wn[s.off(v, k - dk, j - dj, i - di)]) / h;
-                double dup = (Wn[s.off(v, k + 2 * dk, j + 2 * dj, i + 2 * di)] - u) / hp;
-                double dum = (u - Wn[s.off(v, k - 2 * dk, j - 2 * dj, i - 2 * di)]) / hm;
+                double du = (wn[st] - wn[-st]) / h;
+                double dup = (wn[2 * st] - u) / hp;
+                double dum = (u - wn[-2 * st]) / hm;
                 double d2u = (dup - dum) / (c[l + 1] - c[l - 1]);
                 double dv = 0.5 * d2u * (f[l + 1] - f[l]);
                 double* al = d == 0 ? alx : (d == 1 ? aly : alz);
-                al[s.off(v, k, j, i)] = alpha_fn(dv, dum, du, dup);
+                al[v * pl + c0] = alpha_fn(dv, dum, du, dup);
             }
         }
     }
@@ -703,8 +716,20 @@ int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double
     a.mask = mask;
     a.tol = c->prm.nad_tol;
     uint32_t total = (uint32_t)FV(s).plane();
-    hipLaunchKernelGGL(k_nad_sed, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, alx, aly ? aly : alx,
-                       alz ? alz : alx, FV(s), a, geo_of(c), total);
+#define SPDK_NAD(X, Y, Z)                                                                                       \
+    hipLaunchKernelGGL((k_nad_sed<X, Y, Z>), dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, alx,        \
+                       aly ? aly : alx, alz ? alz : alx, FV(s), a, geo_of(c), total)
+    switch ((a.act[0] ? 4 : 0) | (a.act[1] ? 2 : 0) | (a.act[2] ? 1 : 0)) {
+        case 0: SPDK_NAD(0, 0, 0); break;
+        case 1: SPDK_NAD(0, 0, 1); break;
+        case 2: SPDK_NAD(0, 1, 0); break;
+        case 3: SPDK_NAD(0, 1, 1); break;
+        case 4: SPDK_NAD(1, 0, 0); break;
+        case 5: SPDK_NAD(1, 0, 1); break;
+        case 6: SPDK_NAD(1, 1, 0); break;
+        default: SPDK_NAD(1, 1, 1); break;
+    }
+#undef SPDK_NAD
     SPDK_LAUNCH_CHECK("nad_sed");
     uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * a.act[2]) * (s.Ny - 2 * a.act[1]) * (s.Nx - 2 * a.act[0]));
     hipLaunchKernelGGL(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx,

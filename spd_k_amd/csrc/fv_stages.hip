@@ -379,71 +379,85 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
 }
 
 // relax_NAD (src/trouble_detection.cpp:152-183) then PAD_criteria (:244-259), cells 1..N-2
+template <int AX, int AY, int AZ>
 __global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn, double* __restrict__ tr,
                                                   const double* __restrict__ alx, const double* __restrict__ aly,
                                                   const double* __restrict__ alz, FV s, Det a, uint32_t total) {
     uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
-    int ax = a.act[0], ay = a.act[1], az = a.act[2];
-    int Mx = s.Nx - 2 * ax, My = s.Ny - 2 * ay;
     int k, j, i;
-    split3(idx, My, Mx, k, j, i);
-    k += az; j += ay; i += ax;
-    int nvar = s.nv - 1;
+    split3(idx, s.Ny - 2 * AY, s.Nx - 2 * AX, k, j, i);
+    k += AZ; j += AY; i += AX;
+    const int nvar = s.nv - 1;
+    const size_t pl = s.plane(), c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
     for (int v = 0; v < nvar; v++) {
         if (!((a.mask >> v) & 1)) continue;
+        const size_t o = v * pl + c0;
         double alpha = 1;
-        if (ax) alpha = smin(smin(alx[s.off(v, k, j, i - 1)], alx[s.off(v, k, j, i)]), alx[s.off(v, k, j, i + 1)]);
-        if (ay) alpha = smin(alpha, smin(smin(aly[s.off(v, k, j - 1, i)], aly[s.off(v, k, j, i)]), aly[s.off(v, k, j + 1, i)]));
-        if (az) alpha = smin(alpha, smin(smin(alz[s.off(v, k - 1, j, i)], alz[s.off(v, k, j, i)]), alz[s.off(v, k + 1, j, i)]));
-        double t = tr[s.off(v, k, j, i)];
-        if (a.use_sed) tr[s.off(v, k, j, i)] = alpha < 1 ? t : 0;
+        if (AX) alpha = smin(smin(alx[o - 1], alx[o]), alx[o + 1]);
+        if (AY) alpha = smin(alpha, smin(smin(aly[o - sy], aly[o]), aly[o + sy]));
+        if (AZ) alpha = smin(alpha, smin(smin(alz[o - sz], alz[o]), alz[o + sz]));
+        double t = tr[o];
+        if (a.use_sed) tr[o] = alpha < 1 ? t : 0;
     }
     double t = 0;
     for (int v = 0; v < nvar; v++)
-        if ((a.mask >> v) & 1) t = smax(t, tr[s.off(v, k, j, i)]);
-    tr[s.off(nvar, k, j, i)] = t;
+        if ((a.mask >> v) & 1) t = smax(t, tr[v * pl + c0]);
+    tr[nvar * pl + c0] = t;
     if (a.pad) {
-        double d = Wn[s.off(0, k, j, i)], p = Wn[s.off(4, k, j, i)];
-        if (d < kRhoMin || d > kRhoMax) tr[s.off(nvar, k, j, i)] = 1;
-        if (p < kPMin || p > kPMax) tr[s.off(nvar, k, j, i)] = 1;
+        double d = Wn[c0], p = Wn[4 * pl + c0];
+        if (d < kRhoMin || d > kRhoMax) tr[nvar * pl + c0] = 1;
+        if (p < kPMin || p > kPMax) tr[nvar * pl + c0] = 1;
     }
 }
 
 // apply_blending (src/trouble_detection.cpp:191-210), cells 1..N-2
-__global__ void __launch_bounds__(BS) k_blend(const double* __restrict__ tr, double* __restrict__ th, FV s, int ax, int ay,
-                                              int az, uint32_t total) {
+template <int AX, int AY, int AZ>
+__global__ void __launch_bounds__(BS) k_blend(const double* __restrict__ tr, double* __restrict__ th, FV s,
+                                              uint32_t total) {
     uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
-    split3(idx, s.Ny - 2 * ay, s.Nx - 2 * ax, k, j, i);
-    k += az; j += ay; i += ax;
+    split3(idx, s.Ny - 2 * AY, s.Nx - 2 * AX, k, j, i);
+    k += AZ; j += AY; i += AX;
     const double w[4] = {1.0, 0.75, 0.5, 0.375};
-    int nvar = s.nv - 1;
+    const size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
+    const double* tv = tr + (size_t)(s.nv - 1) * s.plane() + c0;
     double t = 0;
-    for (int dk = -az; dk <= az; dk++)
-        for (int dj = -ay; dj <= ay; dj++)
-            for (int di = -ax; di <= ax; di++) {
-                int n0 = (di != 0) + (dj != 0) + (dk != 0);
-                t = smax(t, w[n0] * tr[s.off(nvar, k + dk, j + dj, i + di)]);
+#pragma unroll
+    for (int dk = -AZ; dk <= AZ; dk++)
+#pragma unroll
+        for (int dj = -AY; dj <= AY; dj++)
+#pragma unroll
+            for (int di = -AX; di <= AX; di++) {
+                const int n0 = (di != 0) + (dj != 0) + (dk != 0);
+                t = smax(t, w[n0] * tv[dk * sz + dj * sy + di]);
             }
-    th[((size_t)k * s.Ny + j) * s.Nx + i] = t;
+    th[c0] = t;
 }
 // blending_ring (src/trouble_detection.cpp:226-242), cells 1..N-2
-__global__ void __launch_bounds__(BS) k_ring(const double* __restrict__ ti, double* __restrict__ to, FV s, int ax, int ay,
-                                             int az, uint32_t total) {
+template <int AX, int AY, int AZ>
+__global__ void __launch_bounds__(BS) k_ring(const double* __restrict__ ti, double* __restrict__ to, FV s,
+                                             uint32_t total) {
     uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
-    split3(idx, s.Ny - 2 * ay, s.Nx - 2 * ax, k, j, i);
-    k += az; j += ay; i += ax;
-    auto at = [&](int kk, int jj, int ii) { return ((size_t)kk * s.Ny + jj) * s.Nx + ii; };
-    double t = ti[at(k, j, i)], ring = 0;
-    for (int dk = -az; dk <= az; dk++)
-        for (int dj = -ay; dj <= ay; dj++)
-            for (int di = -ax; di <= ax; di++)
-                if (ti[at(k + dk, j + dj, i + di)] > 0) ring = 0.25;
-    to[at(k, j, i)] = smax(t, ring);
+    split3(idx, s.Ny - 2 * AY, s.Nx - 2 * AX, k, j, i);
+    k += AZ; j += AY; i += AX;
+    const size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
+    const double* tp = ti + c0;
+    double t = tp[0], ring = 0;
+#pragma unroll
+    for (int dk = -AZ; dk <= AZ; dk++)
+#pragma unroll
+        for (int dj = -AY; dj <= AY; dj++)
+#pragma unroll
+            for (int di = -AX; di <= AX; di++)
+                if (tp[dk * sz + dj * sy + di] > 0) ring = 0.25;
+    to[c0] = smax(t, ring);
 }
 // theta_from_troubles (src/trouble_detection.cpp:213-221)
 __global__ void __launch_bounds__(BS) k_theta_raw(const double* __restrict__ tr, double* __restrict__ th, FV s, uint32_t total) {
@@ -732,8 +746,22 @@ int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double
 #undef SPDK_NAD
     SPDK_LAUNCH_CHECK("nad_sed");
     uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * a.act[2]) * (s.Ny - 2 * a.act[1]) * (s.Nx - 2 * a.act[0]));
-    hipLaunchKernelGGL(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx,
-                       alz ? alz : alx, FV(s), a, t2);
+#define SPDK_AXES(KER, ...)                                                                                     \
+    switch ((a3[0] ? 4 : 0) | (a3[1] ? 2 : 0) | (a3[2] ? 1 : 0)) {                                              \
+        case 0: hipLaunchKernelGGL((KER<0, 0, 0>), __VA_ARGS__); break;                                         \
+        case 1: hipLaunchKernelGGL((KER<0, 0, 1>), __VA_ARGS__); break;                                         \
+        case 2: hipLaunchKernelGGL((KER<0, 1, 0>), __VA_ARGS__); break;                                         \
+        case 3: hipLaunchKernelGGL((KER<0, 1, 1>), __VA_ARGS__); break;                                         \
+        case 4: hipLaunchKernelGGL((KER<1, 0, 0>), __VA_ARGS__); break;                                         \
+        case 5: hipLaunchKernelGGL((KER<1, 0, 1>), __VA_ARGS__); break;                                         \
+        case 6: hipLaunchKernelGGL((KER<1, 1, 0>), __VA_ARGS__); break;                                         \
+        default: hipLaunchKernelGGL((KER<1, 1, 1>), __VA_ARGS__); break;                                        \
+    }
+    {
+        const int* a3 = a.act;
+        SPDK_AXES(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx, alz ? alz : alx,
+                  FV(s), a, t2)
+    }
     SPDK_LAUNCH_CHECK("relax_pad");
     return SPDK_OK;
 }
@@ -742,9 +770,10 @@ int spdk_blend_theta(spdk_ctx* c, const double* tr, double* th_tmp, double* th, 
     int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
     if (c->prm.blending) {
         uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * az) * (s.Ny - 2 * ay) * (s.Nx - 2 * ax));
-        hipLaunchKernelGGL(k_blend, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, tr, th_tmp, FV(s), ax, ay, az, t2);
+        const int a3[3] = {ax, ay, az};
+        SPDK_AXES(k_blend, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, tr, th_tmp, FV(s), t2)
         SPDK_LAUNCH_CHECK("blend");
-        hipLaunchKernelGGL(k_ring, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, th_tmp, th, FV(s), ax, ay, az, t2);
+        SPDK_AXES(k_ring, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, th_tmp, th, FV(s), t2)
         SPDK_LAUNCH_CHECK("ring");
     } else {
         uint32_t t = (uint32_t)FV(s).plane();

@@ -164,6 +164,17 @@ int spdk_fv_face_integral(spdk_ctx* ctx, const double* F_fp, spdk_sd_shape s_fp,
 /* fv_update_solution (src/transforms.cpp:483-543) */
 int spdk_fv_update(spdk_ctx* ctx, double* U_new, double* U_old, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
                    const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id, double dt, int update);
+/* fv_update_solution's update = 0 call as FV_Update_solution makes it
+ * (src/hydro_ader.hpp:500-586), followed by compute_primitives of U_new and
+ * U_old: U_new is written, and on the active cells W_new = prim(U_new) and
+ * W_old = prim(U_cv); U_old is not written. Fill the ghosts of W_new and
+ * W_old (spdk_fill_fv_ghosts) where the reference fills those of U_new and
+ * U_old before converting: the result is the same bits (a ghost is a copy of
+ * an active cell, the reflective wall negating the normal momentum, which the
+ * conversion maps to the exactly negated velocity). Six variables. */
+int spdk_fv_update_prims(spdk_ctx* ctx, double* U_new, double* W_new, double* W_old, const double* U_cv,
+                         spdk_sd_shape s_cv, const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc,
+                         int t_id, double dt);
 /* fv_update_solution's update = 1 call as FV_Update_solution makes it
  * (src/hydro_ader.hpp:500-586): after the update = 0 call of the same slice
  * and the blend of the fallback fluxes with weights theta (FV shape, one

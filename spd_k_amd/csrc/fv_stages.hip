@@ -207,6 +207,57 @@ __global__ void __launch_bounds__(BS) k_fv_update(double* __restrict__ Un, doubl
     fv_cell(Un, Uo, Ucv, s, Fx, Fy, Fz, sc, sx, sy, sz, geo, a, c.K, c.J, c.I, c.o, true, a.update != 0);
 }
 
+// The update = 0 call followed by compute_primitives of U_new and U_old on the
+// active cells: W_new = prim(U_new), W_old = prim(U_cv), U_new written, U_old
+// not (its only reader is that conversion). With ghosts filled on W instead
+// of U the result is the same: a ghost is a copy of an active cell, with the
+// normal momentum negated for reflective walls, and the conversion maps the
+// negated momentum to the exactly negated velocity and the same pressure.
+__global__ void __launch_bounds__(BS) k_fv_update_prims(double* __restrict__ Un, double* __restrict__ Wn,
+                                                        double* __restrict__ Wo, const double* __restrict__ Ucv, SD s,
+                                                        const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                        const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz,
+                                                        Geo geo, FvUpd a, double gm, int gx, int gy, int gz, int fx,
+                                                        int fy, int fz, uint32_t total) {
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
+    if (idx >= total) return;
+    const FvIdx c = fv_idx(idx, s, gx, gy, gz, fx, fy, fz);
+    const int K = c.K, J = c.J, I = c.I;
+    const size_t o = c.o, slab = s.slab();
+    double uo[NV], un[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        double h, F = 0;
+        uo[v] = Ucv[v * slab + o];
+        if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
+        if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
+        if (a.act[2]) { h = geo.f[2][K + 1] - geo.f[2][K]; F += (Fz[sz.off(v, K + 1, J, I)] - Fz[sz.off(v, K, J, I)]) / h; }
+        if (a.grav) {
+            double S = 0.0;
+            if (v == 1) S = Ucv[o] * a.g[0];
+            else if (v == 2) S = Ucv[o] * a.g[1];
+            else if (v == 3) S = Ucv[o] * a.g[2];
+            else if (v == 4) S = Ucv[slab + o] * a.g[0] + Ucv[2 * slab + o] * a.g[1] + Ucv[3 * slab + o] * a.g[2];
+            F -= S;
+        }
+        un[v] = uo[v] - a.wdt * F;
+        Un[sc.off(v, K, J, I)] = un[v];
+    }
+    double wn[NVP], wo[NVP];
+    prim(un, wn, gm);
+    prim(uo, wo, gm);
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        Wn[sc.off(v, K, J, I)] = wn[v];
+        Wo[sc.off(v, K, J, I)] = wo[v];
+    }
+#pragma unroll
+    for (int v = NVP; v < NV; v++) {
+        Wn[sc.off(v, K, J, I)] = 0.0;
+        Wo[sc.off(v, K, J, I)] = 0.0;
+    }
+}
+
 // The update = 1 call after the update = 0 call of the same slice and the
 // blend: a face flux changed only where its blend weight max(theta of the two
 // cells) is nonzero, so a cell whose theta and face neighbours' theta are all
@@ -681,6 +732,26 @@ int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_sha
                        Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]), fv_stag(sc, 2, a.act[2]),
                        geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
     SPDK_LAUNCH_CHECK("fv_update");
+    return SPDK_OK;
+}
+
+int spdk_fv_update_prims(spdk_ctx* c, double* Un, double* Wn, double* Wo, const double* Ucv, spdk_sd_shape s_cv,
+                         const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id, double dt) {
+    if (!Un || !Wn || !Wo || !Ucv) { set_error("fv_update_prims: null argument"); return SPDK_E_ARG; }
+    if (sc.nv != NV || s_cv.nv != NV) { set_error("fv_update_prims: 6 variables"); return SPDK_E_SHAPE; }
+    FvUpd a;
+    for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
+    if (t_id < 0 || t_id >= c->T) { set_error("fv_update_prims: slice"); return SPDK_E_ARG; }
+    a.wdt = c->wt_host[t_id] * dt;
+    a.update = 0;
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    uint32_t total = (uint32_t)((size_t)(s_cv.Nz - 2 * gz) * (s_cv.Ny - 2 * gy) * (s_cv.Nx - 2 * gx) * s_cv.nz * s_cv.ny * s_cv.nx);
+    hipLaunchKernelGGL(k_fv_update_prims, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Un, Wn, Wo, Ucv, SD(s_cv), Fx,
+                       Fy ? Fy : Fx, Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
+                       fv_stag(sc, 2, a.act[2]), geo_of(c), a, c->prm.gamma, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2],
+                       total);
+    SPDK_LAUNCH_CHECK("fv_update_prims");
     return SPDK_OK;
 }
 

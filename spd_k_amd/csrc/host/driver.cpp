@@ -142,7 +142,7 @@ struct HydroRun {
     double *U_sp = nullptr, *W_sp = nullptr, *W_cv = nullptr, *U_cv = nullptr, *T_sweep = nullptr, *U_ader = nullptr,
            *U0 = nullptr;
     double *U_fp[3] = {}, *F_fp[3] = {};
-    double *U_new = nullptr, *W_new = nullptr, *U_old = nullptr, *W_old = nullptr, *troubles = nullptr, *theta = nullptr,
+    double *U_new = nullptr, *W_new = nullptr, *W_old = nullptr, *troubles = nullptr, *theta = nullptr,
            *theta_tmp = nullptr;
     double *F_fv[3] = {}, *alpha[3] = {}, *T_fp[3] = {};
     void* ws = nullptr;
@@ -268,7 +268,6 @@ struct HydroRun {
             size_t nfv = fv_count(s_fv);
             U_new = mem.alloc(ctx, nfv);
             W_new = mem.alloc(ctx, nfv);
-            U_old = mem.alloc(ctx, nfv);
             W_old = mem.alloc(ctx, nfv);
             troubles = mem.alloc(ctx, nfv);
             theta = mem.alloc(ctx, fv_count(s_th));
@@ -364,11 +363,10 @@ struct HydroRun {
         for (int a = 0; a < T; a++) {
             for (int d = 0; d < 3; d++)
                 if (cfg.act[d]) CK(spdk_fv_face_integral(ctx, F_fp[d], s_fp[d], F_fv[d], s_fvd[d], T_fp[d], a, d));
-            CK(spdk_fv_update(ctx, U_new, U_old, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], s_fv, a, dt, 0));
-            CK(spdk_fill_fv_ghosts(ctx, U_old, s_fv));
-            CK(spdk_fill_fv_ghosts(ctx, U_new, s_fv));
-            CK(spdk_fv_primitives(ctx, U_new, W_new, s_fv));
-            CK(spdk_fv_primitives(ctx, U_old, W_old, s_fv));
+            // update = 0 and the primitives of U_new and U_old in one pass, ghosts filled on W
+            CK(spdk_fv_update_prims(ctx, U_new, W_new, W_old, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], s_fv, a, dt));
+            CK(spdk_fill_fv_ghosts(ctx, W_old, s_fv));
+            CK(spdk_fill_fv_ghosts(ctx, W_new, s_fv));
             CK(spdk_detect_troubles(ctx, W_new, W_old, troubles, alpha[0], alpha[1], alpha[2], s_fv, 1, mask));
             CK(spdk_fill_fv_ghosts(ctx, troubles, s_fv));
             CK(spdk_blend_theta(ctx, troubles, theta_tmp, theta, s_fv));

@@ -249,6 +249,11 @@ class Context:
                                    _ptr(Fy), _ptr(Fz), FVShape.of(U_new.shape), t_id, ctypes.c_double(dt), int(update)),
               "fv_update")
 
+    def fv_update_prims(self, U_new, W_new, W_old, U_cv, Fx, Fy, Fz, t_id, dt):
+        check(lib().spdk_fv_update_prims(self.ctx, _ptr(U_new), _ptr(W_new), _ptr(W_old), _ptr(U_cv),
+                                         SDShape.of(U_cv.shape), _ptr(Fx), _ptr(Fy), _ptr(Fz), FVShape.of(U_new.shape),
+                                         t_id, ctypes.c_double(dt)), "fv_update_prims")
+
     def fv_update_blended(self, U_new, U_cv, Fx, Fy, Fz, theta, t_id, dt):
         check(lib().spdk_fv_update_blended(self.ctx, _ptr(U_new), _ptr(U_cv), SDShape.of(U_cv.shape), _ptr(Fx), _ptr(Fy),
                                            _ptr(Fz), _ptr(theta), FVShape.of(U_new.shape), t_id, ctypes.c_double(dt)),

@@ -304,3 +304,36 @@ def test_fv_fallback_stages(act, N, opts):
     assert np.array_equal(host(Ucv_a), host(Ucv_b))
     assert np.array_equal(host(Uo_a), host(Uo_b))
     c.close()
+
+
+@pytest.mark.parametrize("act,N", GEOMS)
+@pytest.mark.parametrize("bc", [0, 1, 2])
+def test_fv_update_prims(act, N, bc):
+    """The update = 0 FV update fused with the primitives of U_new and U_old,
+    ghosts filled on W, against the reference's order (update = 0, ghosts on
+    U, then compute_primitives), every cell bit for bit, all boundary types."""
+    p, n = 3, 4
+    rng = np.random.default_rng(300 + bc + sum(act))
+    c, _, _, _ = ctx_for(p, act, bc=(bc, bc, bc), N=N)
+    s_fv = fv_shape(6, N, n, act)
+    s_cv = sd_shape(1, 6, N, n, act)
+    Ucv = dev(state(s_cv, rng))
+    F = [dev(0.01 * rng.normal(size=fv_shape(6, N, n, act, stag=d))) if act[d] else None for d in range(3)]
+    z = lambda: torch.zeros(s_fv, dtype=torch.float64, device="cuda")
+    Un_a, Uo_a, Wn_a, Wo_a = z(), z(), z(), z()
+    c.fv_update_solution(Un_a, Uo_a, Ucv, F[0], F[1], F[2], 2, 0.01, 0)
+    c.fv_boundaries(Uo_a)
+    c.fv_boundaries(Un_a)
+    c.compute_primitives_fv(Un_a, Wn_a)
+    c.compute_primitives_fv(Uo_a, Wo_a)
+    Un_b, Wn_b, Wo_b = z(), z(), z()
+    c.fv_update_prims(Un_b, Wn_b, Wo_b, Ucv, F[0], F[1], F[2], 2, 0.01)
+    c.fv_boundaries(Wo_b)
+    c.fv_boundaries(Wn_b)
+    c.fv_boundaries(Un_b)
+    for name, x, y in (("W_new", Wn_a, Wn_b), ("W_old", Wo_a, Wo_b), ("U_new", Un_a, Un_b)):
+        a, b = host(x), host(y)
+        assert np.array_equal(a, b), f"{name}: {int((a != b).sum())} of {a.size} differ"
+        assert np.array_equal(np.signbit(a), np.signbit(b)), name
+    c.close()
+

@@ -714,7 +714,7 @@ extern "C" int spdk_run_argv(int argc, const char* const* argv, const char* outd
             std::ofstream dump(output_folder(od) + "parameters.txt");
             pin.dump(dump);
         }
-        int warmup = pin.has("job", "warmup") ? pin.get_int("job", "warmup") : 0;
+        int warmup = pin.has("job/warmup") ? pin.get<int>("job/warmup") : 0;
         if (cfg.system == "hydro") return run_hydro(cfg, od, warmup, res);
         if (cfg.system == "induction") return run_induction(cfg, od, warmup, res);
         std::fprintf(stderr, "ERROR: unknown system '%s'\n", cfg.system.c_str());

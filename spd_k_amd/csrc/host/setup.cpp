@@ -47,59 +47,59 @@ static void problem_defaults(int prob, ProblemParams& pp) {
 }
 
 std::string configure(Params& pin, RunConfig& c) {
-    c.p = pin.int_or("mesh", "p", 3);
-    c.nx[0] = pin.int_or("mesh", "nx1", 8);
-    c.nx[1] = pin.int_or("mesh", "nx2", 8);
-    c.nx[2] = pin.int_or("mesh", "nx3", 8);
-    c.len[0] = pin.real_or("mesh", "x1len", 1.0);
-    c.len[1] = pin.real_or("mesh", "x2len", 1.0);
-    c.len[2] = pin.real_or("mesh", "x3len", 1.0);
+    c.p = pin.get_or<int>("mesh/p", 3);
+    c.nx[0] = pin.get_or<int>("mesh/nx1", 8);
+    c.nx[1] = pin.get_or<int>("mesh/nx2", 8);
+    c.nx[2] = pin.get_or<int>("mesh/nx3", 8);
+    c.len[0] = pin.get_or<double>("mesh/x1len", 1.0);
+    c.len[1] = pin.get_or<double>("mesh/x2len", 1.0);
+    c.len[2] = pin.get_or<double>("mesh/x3len", 1.0);
     for (int d = 0; d < 3; d++) c.act[d] = c.nx[d] > 1;
     if (!c.act[0]) return "the x-direction must be active (nx1>1)";
     if (c.p < 1 || c.p > 8) return "mesh/p must be in 1..8";
     c.ndim = (int)c.act[0] + (int)c.act[1] + (int)c.act[2];
-    c.cfl = pin.real_or("time", "cfl", 0.8);
-    c.gamma = pin.real_or("hydro", "gamma", 1.4);
-    c.g[0] = pin.real_or("hydro", "g1", 0.0);
-    c.g[1] = pin.real_or("hydro", "g2", 0.0);
-    c.g[2] = pin.real_or("hydro", "g3", 0.0);
-    c.fallback = pin.bool_or("job", "fallback", true);
-    c.nad_tol = pin.real_or("fallback", "tolerance", 1e-5);
-    c.nad_delta = pin.string_or("fallback", "NAD", "relative") == "delta";
-    c.nad_moore = pin.string_or("fallback", "NAD_neighbors", "2nd") == "2nd";
-    c.sed = pin.bool_or("fallback", "SED", true);
-    c.blending = pin.bool_or("fallback", "blending", true);
-    c.problem = problem_from(pin.string_or("problem", "problem", "sine_wave"));
-    if (c.problem < 0) return "unknown problem '" + pin.get_string("problem", "problem") + "'";
+    c.cfl = pin.get_or<double>("time/cfl", 0.8);
+    c.gamma = pin.get_or<double>("hydro/gamma", 1.4);
+    c.g[0] = pin.get_or<double>("hydro/g1", 0.0);
+    c.g[1] = pin.get_or<double>("hydro/g2", 0.0);
+    c.g[2] = pin.get_or<double>("hydro/g3", 0.0);
+    c.fallback = pin.get_or<bool>("job/fallback", true);
+    c.nad_tol = pin.get_or<double>("fallback/tolerance", 1e-5);
+    c.nad_delta = pin.get_or("fallback/NAD", "relative") == "delta";
+    c.nad_moore = pin.get_or("fallback/NAD_neighbors", "2nd") == "2nd";
+    c.sed = pin.get_or<bool>("fallback/SED", true);
+    c.blending = pin.get_or<bool>("fallback/blending", true);
+    c.problem = problem_from(pin.get_or("problem/problem", "sine_wave"));
+    if (c.problem < 0) return "unknown problem '" + pin.get<std::string>("problem/problem") + "'";
     problem_defaults(c.problem, c.pp);
     ProblemParams& pp = c.pp;
-    pp.amp = pin.real_or("problem", "amp", pp.amp);
-    pp.v1 = pin.real_or("problem", "v1", pp.v1);
-    pp.v2 = pin.real_or("problem", "v2", pp.v2);
-    pp.v3 = pin.real_or("problem", "v3", pp.v3);
-    pp.d0 = pin.real_or("problem", "d0", pp.d0);
-    pp.d1 = pin.real_or("problem", "d1", pp.d1);
-    pp.p0 = pin.real_or("problem", "p0", pp.p0);
-    pp.p1 = pin.real_or("problem", "p1", pp.p1);
-    pp.radius = pin.real_or("problem", "radius", pp.radius);
-    pp.sigma = pin.real_or("problem", "sigma", pp.sigma);
-    pp.dir = pin.int_or("problem", "dir", pp.dir);
-    if (pin.has("problem", "perturb")) pp.random_perturb = pin.get_string("problem", "perturb") == "random";
-    if (pin.has("problem", "seed")) pp.seed = (uint64_t)std::stoull(pin.get_string("problem", "seed"));
+    pp.amp = pin.get_or<double>("problem/amp", pp.amp);
+    pp.v1 = pin.get_or<double>("problem/v1", pp.v1);
+    pp.v2 = pin.get_or<double>("problem/v2", pp.v2);
+    pp.v3 = pin.get_or<double>("problem/v3", pp.v3);
+    pp.d0 = pin.get_or<double>("problem/d0", pp.d0);
+    pp.d1 = pin.get_or<double>("problem/d1", pp.d1);
+    pp.p0 = pin.get_or<double>("problem/p0", pp.p0);
+    pp.p1 = pin.get_or<double>("problem/p1", pp.p1);
+    pp.radius = pin.get_or<double>("problem/radius", pp.radius);
+    pp.sigma = pin.get_or<double>("problem/sigma", pp.sigma);
+    pp.dir = pin.get_or<int>("problem/dir", pp.dir);
+    if (pin.has("problem/perturb")) pp.random_perturb = pin.get<std::string>("problem/perturb") == "random";
+    if (pin.has("problem/seed")) pp.seed = (uint64_t)std::stoull(pin.get<std::string>("problem/seed"));
     pp.cx = 0.5 * c.len[0];
     pp.cy = 0.5 * c.len[1];
     pp.cz = 0.5 * c.len[2];
     const char* keys[3] = {"x1_bc", "x2_bc", "x3_bc"};
     for (int d = 0; d < 3; d++) {
-        c.bc[d] = bc_from(pin.string_or("mesh", keys[d], "periodic"));
+        c.bc[d] = bc_from(pin.get_or(std::string("mesh/") + keys[d], "periodic"));
         if (c.bc[d] < 0) return std::string("unknown boundary type for mesh/") + keys[d];
     }
-    c.tlim = pin.real_or("time", "tlim", 0.1);
-    c.outputs = pin.has("output", "dt") && pin.get_real("output", "dt") > 0.0;
-    c.dt_output = c.outputs ? pin.get_real("output", "dt") : c.tlim;
-    if (pin.has("output", "fields")) {
+    c.tlim = pin.get_or<double>("time/tlim", 0.1);
+    c.outputs = pin.has("output/dt") && pin.get<double>("output/dt") > 0.0;
+    c.dt_output = c.outputs ? pin.get<double>("output/dt") : c.tlim;
+    if (pin.has("output/fields")) {
         c.out_troubles = c.out_flux = c.out_wcv = false;
-        std::string list = pin.get_string("output", "fields") + ",";
+        std::string list = pin.get<std::string>("output/fields") + ",";
         for (size_t a = 0, b; (b = list.find(',', a)) != std::string::npos; a = b + 1) {
             std::string f = list.substr(a, b - a);
             if (f == "troubles") c.out_troubles = true;
@@ -108,54 +108,54 @@ std::string configure(Params& pin, RunConfig& c) {
             else if (!f.empty()) return "output/fields: unknown field '" + f + "' (troubles, F_ader_fp_x, W_cv)";
         }
     }
-    std::string integ = pin.string_or("time", "integrator", "ader");
+    std::string integ = pin.get_or("time/integrator", "ader");
     if (integ == "ader") c.integrator = INT_ADER;
     else if (integ.size() == 3 && integ.compare(0, 2, "rk") == 0 && integ[2] >= '1' && integ[2] <= '3') {
         c.integrator = INT_RK;
         c.rk_order = integ[2] - '0';
     } else
         return "unknown integrator '" + integ + "' (ader, rk1, rk2, rk3)";
-    c.system = pin.string_or("job", "system", "hydro");
-    c.nu = pin.real_or("hydro", "nu", 0.0);
-    c.beta = pin.real_or("hydro", "beta", -2. / 3 * c.nu);
+    c.system = pin.get_or("job/system", "hydro");
+    c.nu = pin.get_or<double>("hydro/nu", 0.0);
+    c.beta = pin.get_or<double>("hydro/beta", -2. / 3 * c.nu);
     if (c.system == "induction") {
-        c.eta = pin.real_or("induction", "nu", 0.0025);
-        c.vel[0] = pin.has("induction", "v1") ? pin.get_real("induction", "v1") : 1.0;
-        c.vel[1] = pin.has("induction", "v2") ? pin.get_real("induction", "v2") : 1.0;
-        c.vel[2] = pin.has("induction", "v3") ? pin.get_real("induction", "v3") : 0.0;
-        c.loop_c[0] = pin.has("induction", "cx") ? pin.get_real("induction", "cx") : 0.5;
-        c.loop_c[1] = pin.has("induction", "cy") ? pin.get_real("induction", "cy") : 0.5;
-        c.loop_R = pin.has("induction", "radius") ? pin.get_real("induction", "radius") : 0.3;
-        c.loop_A0 = pin.has("induction", "amp") ? pin.get_real("induction", "amp") : 0.001;
+        c.eta = pin.get_or<double>("induction/nu", 0.0025);
+        c.vel[0] = pin.has("induction/v1") ? pin.get<double>("induction/v1") : 1.0;
+        c.vel[1] = pin.has("induction/v2") ? pin.get<double>("induction/v2") : 1.0;
+        c.vel[2] = pin.has("induction/v3") ? pin.get<double>("induction/v3") : 0.0;
+        c.loop_c[0] = pin.has("induction/cx") ? pin.get<double>("induction/cx") : 0.5;
+        c.loop_c[1] = pin.has("induction/cy") ? pin.get<double>("induction/cy") : 0.5;
+        c.loop_R = pin.has("induction/radius") ? pin.get<double>("induction/radius") : 0.3;
+        c.loop_A0 = pin.has("induction/amp") ? pin.get<double>("induction/amp") : 0.001;
     }
-    c.max_steps = pin.has("time", "nlim") ? pin.get_int("time", "nlim") : -1;
-    c.path = pin.has("job", "path") ? pin.get_string("job", "path") : std::string("fused");
+    c.max_steps = pin.has("time/nlim") ? pin.get<int>("time/nlim") : -1;
+    c.path = pin.has("job/path") ? pin.get<std::string>("job/path") : std::string("fused");
     if (c.path != "fused" && c.path != "stages") return "job/path must be fused or stages";
-    c.timing = pin.has("job", "timing") && pin.get_bool("job", "timing");
-    if (pin.has("job", "precision")) c.precision = pin.get_string("job", "precision");
+    c.timing = pin.has("job/timing") && pin.get<bool>("job/timing");
+    if (pin.has("job/precision")) c.precision = pin.get<std::string>("job/precision");
     if (c.precision != "fp64" && c.precision != "mixed") return "job/precision must be fp64 or mixed";
-    if (pin.has("job", "picard")) c.picard = pin.get_string("job", "picard");
+    if (pin.has("job/picard")) c.picard = pin.get<std::string>("job/picard");
     if (c.picard != "fixed" && c.picard != "residual") return "job/picard must be fixed or residual";
-    if (pin.has("job", "picard_tol")) c.picard_tol = pin.get_real("job", "picard_tol");
-    if (pin.has("job", "picard_max")) c.picard_max = pin.get_int("job", "picard_max");
-    if (pin.has("job", "guess")) c.guess = pin.get_string("job", "guess");
+    if (pin.has("job/picard_tol")) c.picard_tol = pin.get<double>("job/picard_tol");
+    if (pin.has("job/picard_max")) c.picard_max = pin.get<int>("job/picard_max");
+    if (pin.has("job/guess")) c.guess = pin.get<std::string>("job/guess");
     if (c.guess != "zero" && c.guess != "mlp") return "job/guess must be zero or mlp";
-    if (pin.has("job", "mlp_weights")) c.mlp_weights = pin.get_string("job", "mlp_weights");
-    if (pin.has("job", "mode")) c.mode = pin.get_string("job", "mode");
-    c.final_output = pin.has("job", "final_output") && pin.get_bool("job", "final_output");
+    if (pin.has("job/mlp_weights")) c.mlp_weights = pin.get<std::string>("job/mlp_weights");
+    if (pin.has("job/mode")) c.mode = pin.get<std::string>("job/mode");
+    c.final_output = pin.has("job/final_output") && pin.get<bool>("job/final_output");
     if (c.mode != "run" && c.mode != "train_guess") return "job/mode must be run or train_guess";
     if (c.mode == "train_guess") {
-        if (!pin.has("mlp", "out")) return "job/mode=train_guess needs mlp/out";
-        c.mlp_out = pin.get_string("mlp", "out");
-        if (pin.has("mlp", "snapshots")) c.mlp_snapshots = pin.get_int("mlp", "snapshots");
-        if (pin.has("mlp", "every")) c.mlp_every = pin.get_int("mlp", "every");
-        if (pin.has("mlp", "sweeps")) c.mlp_sweeps = pin.get_int("mlp", "sweeps");
-        if (pin.has("mlp", "epochs")) c.mlp_epochs = pin.get_int("mlp", "epochs");
-        if (pin.has("mlp", "batch")) c.mlp_batch = pin.get_int("mlp", "batch");
-        if (pin.has("mlp", "hidden")) c.mlp_hidden = pin.get_int("mlp", "hidden");
-        if (pin.has("mlp", "lr")) c.mlp_lr = pin.get_real("mlp", "lr");
-        if (pin.has("mlp", "max_target")) c.mlp_max_target = pin.get_real("mlp", "max_target");
-        if (pin.has("mlp", "seed")) c.mlp_seed = (uint64_t)std::stoull(pin.get_string("mlp", "seed"));
+        if (!pin.has("mlp/out")) return "job/mode=train_guess needs mlp/out";
+        c.mlp_out = pin.get<std::string>("mlp/out");
+        if (pin.has("mlp/snapshots")) c.mlp_snapshots = pin.get<int>("mlp/snapshots");
+        if (pin.has("mlp/every")) c.mlp_every = pin.get<int>("mlp/every");
+        if (pin.has("mlp/sweeps")) c.mlp_sweeps = pin.get<int>("mlp/sweeps");
+        if (pin.has("mlp/epochs")) c.mlp_epochs = pin.get<int>("mlp/epochs");
+        if (pin.has("mlp/batch")) c.mlp_batch = pin.get<int>("mlp/batch");
+        if (pin.has("mlp/hidden")) c.mlp_hidden = pin.get<int>("mlp/hidden");
+        if (pin.has("mlp/lr")) c.mlp_lr = pin.get<double>("mlp/lr");
+        if (pin.has("mlp/max_target")) c.mlp_max_target = pin.get<double>("mlp/max_target");
+        if (pin.has("mlp/seed")) c.mlp_seed = (uint64_t)std::stoull(pin.get<std::string>("mlp/seed"));
         if (c.mlp_snapshots < 2 || c.mlp_every < 1 || c.mlp_sweeps < 1 || c.mlp_epochs < 1 || c.mlp_batch < 64 ||
             c.mlp_hidden < 64)
             return "mlp/: snapshots >= 2, every >= 1, sweeps >= 1, epochs >= 1, batch >= 64, hidden >= 64";

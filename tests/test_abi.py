@@ -62,3 +62,55 @@ def test_no_gpu_is_reported_not_faked():
     with pytest.raises(RuntimeError):
         spd_k_amd.Context(3, (1, 1, 1), (0, 0, 0), 4, ops)
     assert _run_argv(["mesh/nx1=4", "mesh/nx2=4", "mesh/nx3=4", "time/tlim=0.001"]) in (-4, -5)
+
+
+def _param_table(path):
+    """block/name -> value of an input file (comments and blank lines skipped)."""
+    table, block = {}, None
+    for raw in open(path):
+        s = raw.split("#", 1)[0].strip()
+        if not s:
+            continue
+        if s.startswith("<"):
+            block = s[1:s.index(">")].strip()
+        else:
+            k, v = s.split("=", 1)
+            table[f"{block}/{k.strip()}"] = v.strip()
+    return table
+
+
+def test_parameter_dump_is_a_loadable_input_with_last_assignment_winning(tmp_path):
+    """The effective parameters written with outputs on (src/main.cpp:248-253) reload to the same
+    values; a later file line beats an earlier one and a command-line assignment beats the file
+    (src/parameter_input.cpp:24-56)."""
+    src = tmp_path / "in.athinput"
+    src.write_text("<mesh>\nnx1 = 6   # first\nnx2 = 4\n<time>\ntlim = 0.5\n<mesh>\nnx1 = 7\n"
+                   "<output>\ndt = 0.1\n")
+    out1, out2 = tmp_path / "o1", tmp_path / "o2"
+    args = ["-i", str(src), "mesh/nx2=5", "time/cfl = 0.25", "time/nlim=1"]
+    for out, a in ((out1, args), (out2, None)):
+        if a is None:
+            a = ["-i", str(out1 / "parameters.txt")]
+        L = spd_k_amd.lib()
+        argv = (ctypes.c_char_p * len(a))(*[x.encode() for x in a])
+        rc = L.spdk_run_argv(len(a), argv, str(out).encode(), None)
+        assert rc in (0, -4, -5), rc  # without a GPU the run stops after the dump, before any step
+    t1 = _param_table(out1 / "parameters.txt")
+    assert t1["mesh/nx1"] == "7" and t1["mesh/nx2"] == "5" and t1["time/tlim"] == "0.5"
+    assert float(t1["time/cfl"]) == 0.25 and float(t1["hydro/gamma"]) == 1.4  # a default is recorded
+    assert _param_table(out2 / "parameters.txt") == t1
+
+
+@pytest.mark.parametrize("text", [
+    "<mesh\nnx1 = 4\n",          # header without '>'
+    "<  >\nnx1 = 4\n",           # header naming no block
+    "nx1 = 4\n",                 # assignment before any block
+    "<mesh>\nnx1\n",             # neither header nor assignment
+    "<mesh>\nnx1 =   # none\n",  # empty value
+    "<mesh>\n = 4\n",            # empty name
+    "<mesh>\nnx1 = four\n",      # not an integer where one is read
+])
+def test_input_file_syntax_errors_are_config_errors(tmp_path, text):
+    f = tmp_path / "bad.athinput"
+    f.write_text(text)
+    assert _run_argv(["-i", str(f)]) == -5

@@ -184,7 +184,10 @@ int spdk_fv_update_prims(spdk_ctx* ctx, double* U_new, double* W_new, double* W_
  * only U_cv = U_new is written (with gravity every cell is recomputed: the
  * update = 1 form takes the source from the density it has just written).
  * U_old, which already holds U_cv, is not written. Same bits as the
- * update = 1 call. */
+ * update = 1 call. theta has the FV cell shape sc (its first variable is
+ * read). The three FV updates check that s_cv and sc agree: six variables,
+ * and per active axis sc's cells = active elements x points + 2 FV ghosts
+ * (SPDK_E_SHAPE otherwise). */
 int spdk_fv_update_blended(spdk_ctx* ctx, double* U_new, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
                            const double* Fy, const double* Fz, const double* theta, spdk_fv_shape sc, int t_id,
                            double dt);

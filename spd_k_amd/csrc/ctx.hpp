@@ -97,6 +97,13 @@ __device__ __forceinline__ void split3(uint32_t idx, int My, int Mx, int& k, int
     k = idx / My;
 }
 
+// Reductions that keep a NaN: once either operand is NaN the result is NaN,
+// so a diverged state reaches the host (dt, residual, diagnostics) instead of
+// being folded away. For ordered operands they equal min / max.
+__host__ __device__ __forceinline__ double nan_min(double a, double b) { return (b < a || b != b) ? b : a; }
+__host__ __device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+__host__ __device__ __forceinline__ float nan_maxf(float a, float b) { return (b > a || b != b) ? b : a; }
+
 inline unsigned nblocks(size_t total, int bs) { return (unsigned)((total + bs - 1) / bs); }
 
 }  // namespace spdk

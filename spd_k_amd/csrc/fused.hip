@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     red[tid] = r;
     __syncthreads();
     for (int wdt = G::BLOCK / 2; wdt > 0; wdt >>= 1) {
-        if (tid < wdt) red[tid] = red[tid] < red[tid + wdt] ? red[tid] : red[tid + wdt];
+        if (tid < wdt) red[tid] = nan_min(red[tid], red[tid + wdt]);
         __syncthreads();
     }
     if (tid == 0) a.part[blockIdx.x] = red[0];
@@ -443,11 +443,11 @@ __global__ void __launch_bounds__(256) k_trace0(const double* __restrict__ U, in
 __global__ void k_partial_min(double* part, int n) {
     __shared__ double red[1024];
     double r = __builtin_inf();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) r = r < part[i] ? r : part[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) r = nan_min(r, part[i]);
     red[threadIdx.x] = r;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        if ((int)threadIdx.x < w) red[threadIdx.x] = nan_min(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[0] = red[0];

@@ -718,8 +718,34 @@ int spdk_fv_face_integral(spdk_ctx* c, const double* Ffp, spdk_sd_shape s_fp, do
     return SPDK_OK;
 }
 
+// Shape agreement of an SD cell-value array and the FV arrays of one update
+// (state, flux faces derived from it, theta): the same variable count (6),
+// and along every axis the FV cells are the active elements' solution
+// points plus the FV ghost layers; an inactive axis is one element, one
+// point and one cell.
+static int fv_shapes_ok(const spdk_ctx* c, const spdk_sd_shape& s_cv, const spdk_fv_shape& sc, const char* who) {
+    if (s_cv.nt < 1 || s_cv.nv != NV || sc.nv != NV) {
+        set_error(std::string(who) + ": SD and FV arrays need 6 variables");
+        return SPDK_E_SHAPE;
+    }
+    const int Ne[3] = {s_cv.Nx, s_cv.Ny, s_cv.Nz}, np[3] = {s_cv.nx, s_cv.ny, s_cv.nz}, Nf[3] = {sc.Nx, sc.Ny, sc.Nz};
+    for (int d = 0; d < 3; d++) {
+        const bool act = c->prm.active[d] != 0;
+        const bool ok = act ? (np[d] == c->n && Ne[d] > 2 * c->ngh[d] &&
+                               Nf[d] == (Ne[d] - 2 * c->ngh[d]) * c->n + 2 * c->fgh[d])
+                            : (np[d] == 1 && Ne[d] == 1 && Nf[d] == 1);
+        if (!ok) {
+            set_error(std::string(who) + ": SD cell-value and FV shapes disagree along axis " + std::to_string(d));
+            return SPDK_E_SHAPE;
+        }
+    }
+    return SPDK_OK;
+}
+
 int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_shape s_cv, const double* Fx, const double* Fy,
                    const double* Fz, spdk_fv_shape sc, int t_id, double dt, int update) {
+    if (!Un || !Uo || !Ucv || !Fx) { set_error("fv_update: null argument"); return SPDK_E_ARG; }
+    if (int rc = fv_shapes_ok(c, s_cv, sc, "fv_update")) return rc;
     FvUpd a;
     for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
     a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
@@ -737,8 +763,8 @@ int spdk_fv_update(spdk_ctx* c, double* Un, double* Uo, double* Ucv, spdk_sd_sha
 
 int spdk_fv_update_prims(spdk_ctx* c, double* Un, double* Wn, double* Wo, const double* Ucv, spdk_sd_shape s_cv,
                          const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id, double dt) {
-    if (!Un || !Wn || !Wo || !Ucv) { set_error("fv_update_prims: null argument"); return SPDK_E_ARG; }
-    if (sc.nv != NV || s_cv.nv != NV) { set_error("fv_update_prims: 6 variables"); return SPDK_E_SHAPE; }
+    if (!Un || !Wn || !Wo || !Ucv || !Fx) { set_error("fv_update_prims: null argument"); return SPDK_E_ARG; }
+    if (int rc = fv_shapes_ok(c, s_cv, sc, "fv_update_prims")) return rc;
     FvUpd a;
     for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
     a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
@@ -757,7 +783,9 @@ int spdk_fv_update_prims(spdk_ctx* c, double* Un, double* Wn, double* Wo, const 
 
 int spdk_fv_update_blended(spdk_ctx* c, double* Un, double* Ucv, spdk_sd_shape s_cv, const double* Fx, const double* Fy,
                            const double* Fz, const double* theta, spdk_fv_shape sc, int t_id, double dt) {
-    if (!theta || !Un || !Ucv) { set_error("fv_update_blended: null argument"); return SPDK_E_ARG; }
+    // theta has the FV cell shape sc (one variable is read)
+    if (!theta || !Un || !Ucv || !Fx) { set_error("fv_update_blended: null argument"); return SPDK_E_ARG; }
+    if (int rc = fv_shapes_ok(c, s_cv, sc, "fv_update_blended")) return rc;
     FvUpd a;
     for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
     a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);

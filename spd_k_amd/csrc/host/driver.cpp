@@ -18,6 +18,10 @@
 #include "setup.hpp"
 #include "driver.hpp"
 
+namespace spdk {
+void set_error(const std::string& msg);  // spdk_last_error() returns it
+}
+
 namespace spdk_host {
 
 void check(int rc, const char* what) {
@@ -290,6 +294,9 @@ struct HydroRun {
         CK(spdk_conservatives(ctx, W_sp, U_sp, s_sp));
         CK(spdk_min_dt(ctx, W_cv, s_sp, &Dt));
         std::printf("dx = %.17g dt = %.17g\n", X.h, Dt);
+        if (!(Dt > 0.0) || !std::isfinite(Dt))
+            throw std::runtime_error("the initial time step is " + std::to_string(Dt) +
+                                     ": the initial state is outside the physical range");
         if (cfg.outputs) write_outputs();
     }
 
@@ -721,6 +728,7 @@ extern "C" int spdk_run_argv(int argc, const char* const* argv, const char* outd
         return SPDK_E_CONFIG;
     } catch (std::exception& e) {
         std::fprintf(stderr, "ERROR: %s\n", e.what());
+        spdk::set_error(e.what());
         return SPDK_E_CONFIG;
     }
 }

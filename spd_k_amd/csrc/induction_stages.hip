@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(BS) k_ind_diag(const double* __restrict__ Bx, 
             }
             div += sum / hd[d];
         }
-        m = fmax(m, fabs(div));
+        m = nan_max(m, fabs(div));
         double vol = 1;
         const int el[3] = {i, j, k}, pt[3] = {ii, jj, kk}, np[3] = {s2.nx, s2.ny, s2.nz};
         for (int d = 0; d < 3; d++) {
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(BS) k_ind_diag(const double* __restrict__ Bx, 
     __syncthreads();
     for (int w = BS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
-            smax_[threadIdx.x] = fmax(smax_[threadIdx.x], smax_[threadIdx.x + w]);
+            smax_[threadIdx.x] = nan_max(smax_[threadIdx.x], smax_[threadIdx.x + w]);
             ssum_[threadIdx.x] += ssum_[threadIdx.x + w];
         }
         __syncthreads();
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(BS) k_ind_diag_final(double* __restrict__ part
     __shared__ double smax_[BS], ssum_[BS];
     double m = 0, e = 0;
     for (int b = threadIdx.x; b < nb; b += BS) {
-        m = fmax(m, part[b]);
+        m = nan_max(m, part[b]);
         e += part[nb + b];
     }
     smax_[threadIdx.x] = m;
@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(BS) k_ind_diag_final(double* __restrict__ part
     __syncthreads();
     for (int w = BS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
-            smax_[threadIdx.x] = fmax(smax_[threadIdx.x], smax_[threadIdx.x + w]);
+            smax_[threadIdx.x] = nan_max(smax_[threadIdx.x], smax_[threadIdx.x + w]);
             ssum_[threadIdx.x] += ssum_[threadIdx.x + w];
         }
         __syncthreads();

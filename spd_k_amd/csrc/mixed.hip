@@ -510,7 +510,7 @@ __device__ __forceinline__ void block_min(double r, double* red, double* part) {
     red[threadIdx.x] = r;
     __syncthreads();
     for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        if ((int)threadIdx.x < w) red[threadIdx.x] = nan_min(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         __syncthreads();
         if (act_e && s == 0) {
             float re = 0.f;
-            for (int j = 0; j < TPE; j++) re = Fs[j] > re ? Fs[j] : re;
+            for (int j = 0; j < TPE; j++) re = nan_maxf(re, Fs[j]);
             double sc = 0.0;
 #pragma unroll
             for (int v = 0; v < NVP; v++) sc = fmax(sc, fabs(X[v]));
@@ -775,7 +775,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             double rr = ghost ? 0.0 : (double)re / (sc > 0.0 ? sc : 1.0);
             X[G::XRES] = rr;
             if (a.conv && rr < a.tol && a.conv[e] == 0xff) a.conv[e] = (unsigned char)a.sweep;
-            if (a.big_out) a.big_out[e] = rr >= a.tol ? 1 : 0;
+            if (a.big_out) a.big_out[e] = rr < a.tol ? 0 : 1;  // NaN: not converged
             a.ver[(size_t)wb * E + e] = a.cur;
         }
         if (live && !act_e && s == 0) {  // kept state and traces
@@ -788,7 +788,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             for (int j = 0; j < EPB; j++)
                 if (blk * EPB + j < a.E) {
                     const double* Xj = (const double*)(lds_raw + (size_t)j * G::ELEM_BYTES);
-                    rb = Xj[G::XRES] > rb ? Xj[G::XRES] : rb;
+                    rb = nan_max(rb, Xj[G::XRES]);
                     cnt += Xj[G::XACT];
                 }
             a.res_part[blockIdx.x] = rb;
@@ -1007,8 +1007,8 @@ __global__ void k_mreduce(const double* a, const double* b, const double* c, int
     if (counter && threadIdx.x == 0) *counter = 0;
     double x = __builtin_inf(), y = 0.0, z = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        if (a) x = x < a[i] ? x : a[i];
-        if (b) y = y > b[i] ? y : b[i];
+        if (a) x = nan_min(x, a[i]);
+        if (b) y = nan_max(y, b[i]);
         if (c) z += c[i];
     }
     ra[threadIdx.x] = x;
@@ -1017,8 +1017,8 @@ __global__ void k_mreduce(const double* a, const double* b, const double* c, int
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
-            ra[threadIdx.x] = ra[threadIdx.x] < ra[threadIdx.x + w] ? ra[threadIdx.x] : ra[threadIdx.x + w];
-            rb[threadIdx.x] = rb[threadIdx.x] > rb[threadIdx.x + w] ? rb[threadIdx.x] : rb[threadIdx.x + w];
+            ra[threadIdx.x] = nan_min(ra[threadIdx.x], ra[threadIdx.x + w]);
+            rb[threadIdx.x] = nan_max(rb[threadIdx.x], rb[threadIdx.x + w]);
             rc[threadIdx.x] += rc[threadIdx.x + w];
         }
         __syncthreads();

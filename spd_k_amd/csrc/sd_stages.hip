@@ -384,14 +384,14 @@ __global__ void __launch_bounds__(BS) k_dt_partial(const double* __restrict__ W,
         if (nu > 0.0) {
             double dx_sub = dx_min / s.nx;
             double dt_visc = 0.25 * cfl * dx_sub * dx_sub / nu;
-            dt_min = dt_min < dt_visc ? dt_min : dt_visc;
+            dt_min = nan_min(dt_min, dt_visc);
         }
-        r = r < dt_min ? r : dt_min;
+        r = nan_min(r, dt_min);
     }
     red[threadIdx.x] = r;
     __syncthreads();
     for (int w = BS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        if (threadIdx.x < w) red[threadIdx.x] = nan_min(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
@@ -399,11 +399,11 @@ __global__ void __launch_bounds__(BS) k_dt_partial(const double* __restrict__ W,
 __global__ void __launch_bounds__(BS) k_min_final(double* __restrict__ part, int n) {
     __shared__ double red[BS];
     double r = __builtin_inf();
-    for (int i = threadIdx.x; i < n; i += BS) r = r < part[i] ? r : part[i];
+    for (int i = threadIdx.x; i < n; i += BS) r = nan_min(r, part[i]);
     red[threadIdx.x] = r;
     __syncthreads();
     for (int w = BS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        if (threadIdx.x < w) red[threadIdx.x] = nan_min(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[0] = red[0];

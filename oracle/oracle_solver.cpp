@@ -7,19 +7,34 @@
 //
 // Pinning:
 //  - SD ADER path (transforms, fluxes, periodic traces, LLF, predictor,
-//    corrector, dt): the reference golden tests/golden/hydro_sd (rtol 1e-6,
-//    whole file), the analytic L1 limits of tests/run_tests.py:52,63,72,109,
-//    and exact invariance of a uniform state.
-//  - RK path: analytic L1 limit (run_tests.py:109) only.
-//  - FV fallback, NAD/SED/relax/PAD, blending, MUSCL/HLLC: exact mass
-//    conservation (< 1e-12, run_tests.py:260) plus hand-built closed-form
-//    cases in tests/test_oracle_fallback.py (NAD band, SED alpha, PAD,
-//    blending weights, HLLC on a contact, minmod). Value-level parity of the
-//    blended fallback flux against the reference is "parity unpinned"
-//    (no reference fixture covers it).
-//  - gradfree / reflective traces: mass conservation with reflective walls
-//    (run_tests.py:123-132) and mirror-symmetry tests; values are otherwise
-//    parity unpinned.
+//    corrector, dt): the reference golden tests/golden/hydro_sd (bit-exact;
+//    the reference's own bar is rtol 1e-6), the analytic L1 limits of
+//    tests/run_tests.py:52,63,72,109, and exact invariance of a uniform state
+//    (tests/test_oracle_pins.py).
+//  - RK path: analytic L1 limit (run_tests.py:109).
+//  - FV fallback (tests/test_oracle_fallback.py, each check derived by hand):
+//    NAD -- unchanged fields never flagged, a planted spike flagged at exactly
+//    its cell, a diagonal-only extremum seen by the Moore box only, relative
+//    vs delta band edges, the written range 1..N-2; SED -- alpha = 1 on a
+//    quadratic (uniform grid), alpha < 1 at a step, written on 2..N-3 only;
+//    relax -- smooth flags cleared except next to the never-written alpha;
+//    PAD -- inclusive bounds; theta -- weights 1/.75/.5/.375 by nonzero
+//    offsets plus the 0.25 ring; MUSCL-Hancock + HLLC -- theta = 0 keeps the
+//    SD flux, the written face range 2..N-2, physical flux of a uniform
+//    state, exact reconstruction of a linear profile on the sub-grid, the
+//    Hancock shift -dt/2 (v . grad rho) with its transverse terms, the face
+//    weight max(theta_i, theta_i-1); the scheme alone converges to the exact
+//    Sod solution; exact mass conservation (run_tests.py:260).
+//  - Boundaries: reflective wall fluxes carry no mass, transverse momentum
+//    or energy; gradfree wall flux = physical flux of the interior trace;
+//    periodic walls share one face flux; literal 1-D FV ghost layers and 2-D
+//    corners; a centred blast gives the same result with reflective and
+//    periodic walls (fields within 1e-10, identical troubled cells).
+// oracle/mutation_check.py plants one plausible mistake at a time in a copy
+// of this file (permuted theta weights, ring radius, face range, Hancock
+// sign and scale, minmod factor, ghost and trace indices, NAD box and band,
+// SED range and factor, relax stencil, PAD bound, HLLC star pressure) and
+// shows each caught by one of those tests (18 of 18).
 #include <algorithm>
 #include <chrono>
 #include <fstream>

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_DIR = os.path.join(ROOT, "oracle", "_build")
+# SPDK_ORACLE_BUILD points the checker at another build of the oracle (oracle/mutation_check.py)
+ORACLE_DIR = os.environ.get("SPDK_ORACLE_BUILD", os.path.join(ROOT, "oracle", "_build"))
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 ORACLE_EXE = os.path.join(ORACLE_DIR, "spdk_oracle_omp")
 ORACLE_EXE_SERIAL = os.path.join(ORACLE_DIR, "spdk_oracle")

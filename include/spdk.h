@@ -359,6 +359,16 @@ int spdk_mixed_predictor_buffer(spdk_ctx* ctx, void* workspace, const int N[3], 
  * once per sweep to read the residual). */
 int spdk_hydro_step_mixed(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
                           double* F_x, const spdk_picard* pc, spdk_picard_stats* stats, double* dt_next);
+/* The predictor part of that step only (the Picard iteration of
+ * src/hydro_ader.hpp:291-305 up to, not including, the last flux evaluation
+ * and update_solution): the guess (zero-order or learned) and the fixed
+ * count or the residual test as pc says; the predictor buffer then holds the
+ * space-time predictor for spdk_mixed_export_predictor, ahead of the FP64
+ * flux evaluation and the FV update with trouble detection and the MUSCL
+ * fallback (FV_Update_solution, src/hydro_ader.hpp:500-586). U_sp is read,
+ * not written. Stats count the sweeps plus 1 for that flux evaluation. */
+int spdk_mixed_predict(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, const spdk_picard* pc,
+                       spdk_picard_stats* stats);
 /* Histogram of the predictor sweeps each element needed to pass the residual
  * test, accumulated over residual-form steps (bin k = k sweeps counted from
  * the step's first sweep, a restart included; bin 31 = never). Copies nbins (<= 32) counts; reset clears. */
@@ -376,7 +386,7 @@ int spdk_mixed_attach_mlp(spdk_ctx* ctx, spdk_mlp* net);
  * rows [e0, e0 + nrows) of features (X, nrows x din) and of targets from the
  * current predictor buffer (Y, nrows x dout); X or Y may be NULL. */
 int spdk_mixed_iterate(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, int nsweeps);
-/* The predictor buffer (after spdk_mixed_iterate) as FP64 space-time slices in
+/* The predictor buffer (after spdk_mixed_iterate or spdk_mixed_predict) as FP64 space-time slices in
  * the stage layout of U_ader (T, >= 5 variables, elements, points; the
  * reference's U_ader_sp, src/hydro_ader.hpp:291-305): U = mean + deviation,
  * further variable slots 0. With it the FV update with trouble detection and

@@ -53,6 +53,7 @@ struct Problem {  // src/global.hpp:22-37 plus the random-perturbation knobs
     double cx = 0.5, cy = 0.5, cz = 0.5;
     int perturb = 0;            // 0: reference sine perturbation, 1: seeded Gaussian noise
     uint64_t seed = 20251202ull;
+    int noise_cells = 0;        // 0: one sample per FV sub-cell; M: per cell of an M^d lattice
     double hx = 0, hy = 0, hz = 0;  // element widths (for sedov_cells)
 };
 

@@ -148,6 +148,7 @@ int parse_config(int argc, char** argv, Config& cfg, std::string& err) {
         cfg.pp.dir = g.i("problem", "dir", cfg.pp.dir);
         cfg.pp.perturb = g.s("problem", "perturb", "sine") == "random" ? 1 : 0;
         cfg.pp.seed = (uint64_t)std::stoull(g.s("problem", "seed", "20251202"));
+        cfg.pp.noise_cells = g.i("problem", "noise_cells", 0);
         cfg.pp.cx = 0.5 * cfg.L[0];
         cfg.pp.cy = 0.5 * cfg.L[1];
         cfg.pp.cz = 0.5 * cfg.L[2];

@@ -957,7 +957,16 @@ static void Initialize(const Config& cfg, SDA& W, const Dim& X, const Dim& Y, co
             long long gx = wrap(i, cfg.ngh[0], X.N, X.nsp, ii);
             long long gy = wrap(j, cfg.ngh[1], Y.N, Y.nsp, jj);
             long long gz = wrap(k, cfg.ngh[2], Z.N, Z.nsp, kk);
-            noise = normal_sample(cfg.pp.seed, (uint64_t)((gz * NYc + gy) * NXc + gx));
+            long long sx = NXc, sy = NYc;
+            if (cfg.pp.noise_cells > 0) {  // one sample per cell of an M^d lattice
+                const long long M = cfg.pp.noise_cells;
+                gx = gx / (NXc / M);
+                gy = cfg.act[1] ? gy / (NYc / M) : 0;
+                gz = cfg.act[2] ? gz / ((long long)Z.N * Z.nsp / M) : 0;
+                sx = M;
+                sy = cfg.act[1] ? M : 1;
+            }
+            noise = normal_sample(cfg.pp.seed, (uint64_t)((gz * sy + gy) * sx + gx));
         }
         for (int t = 0; t < W.nt; t++)
             for (int var = 0; var < W.nv; var++) {

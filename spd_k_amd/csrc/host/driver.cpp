@@ -396,10 +396,11 @@ struct HydroRun {
     }
     double step_raw(bool output_due) {
         if (mixed && cfg.fallback) {
-            // the p predictor updates in mixed precision, then the last flux
+            // the predictor in mixed precision (fixed count or residual
+            // test, zero-order or learned guess), then the last flux
             // evaluation and the FV update with trouble detection and the
             // MUSCL fallback in FP64 (src/hydro_ader.hpp:291-305, :500-586)
-            CK(spdk_mixed_iterate(ctx, U_sp, s_sp, ws, dt, T - 1));
+            CK(spdk_mixed_predict(ctx, U_sp, s_sp, ws, dt, &pc, &pst));
             CK(spdk_mixed_export_predictor(ctx, ws, U_ader, s_ader));
             if (ws_fl) CK(spdk_fused_fluxes(ctx, U_ader, s_ader, ws_fl, F_fp[0], F_fp[1], F_fp[2]));
             else solve_fluxes();
@@ -646,7 +647,7 @@ struct HydroRun {
         res->t_final = t;
         std::printf("\nevolution: %d steps, %g s (%g ms/step), %g zone-cycles/s\n", res->steps, sec,
                     res->steps ? sec / res->steps * 1e3 : 0.0, res->dof_updates_per_s);
-        if (mixed && !cfg.fallback) {
+        if (mixed) {
             res->picard_steps = pst.steps;
             res->picard_sweeps = pst.sweeps;
             res->picard_fallbacks = pst.fallbacks;

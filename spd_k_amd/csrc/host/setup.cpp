@@ -86,6 +86,12 @@ std::string configure(Params& pin, RunConfig& c) {
     pp.dir = pin.get_or<int>("problem/dir", pp.dir);
     if (pin.has("problem/perturb")) pp.random_perturb = pin.get<std::string>("problem/perturb") == "random";
     if (pin.has("problem/seed")) pp.seed = (uint64_t)std::stoull(pin.get<std::string>("problem/seed"));
+    // noise lattice: one sample per cell of an M per axis lattice (M divides
+    // every active nx), so runs at different resolutions share one field
+    pp.noise_cells = pin.get_or<int>("problem/noise_cells", 0);
+    for (int d = 0; d < 3; d++)
+        if (pp.noise_cells < 0 || (pp.noise_cells > 0 && c.act[d] && c.nx[d] % pp.noise_cells != 0))
+            return "problem/noise_cells must divide mesh/nx1..3";
     pp.cx = 0.5 * c.len[0];
     pp.cy = 0.5 * c.len[1];
     pp.cz = 0.5 * c.len[2];
@@ -159,8 +165,8 @@ std::string configure(Params& pin, RunConfig& c) {
         if (c.mlp_snapshots < 2 || c.mlp_every < 1 || c.mlp_sweeps < 1 || c.mlp_epochs < 1 || c.mlp_batch < 64 ||
             c.mlp_hidden < 64)
             return "mlp/: snapshots >= 2, every >= 1, sweeps >= 1, epochs >= 1, batch >= 64, hidden >= 64";
-        if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
-            return "job/mode=train_guess covers the ADER hydro step without fallback";
+        if (c.system != "hydro" || c.integrator != INT_ADER)
+            return "job/mode=train_guess covers the ADER hydro step";
         c.precision = "mixed";
     }
     if (c.system == "induction" && c.precision == "mixed") {
@@ -168,15 +174,11 @@ std::string configure(Params& pin, RunConfig& c) {
             return "job/precision=mixed covers the induction ADER step without the FV update (fixed Picard count)";
         return "";
     }
-    if (c.system == "hydro" && c.precision == "mixed" && c.fallback) {
-        // mixed predictor, FP64 flux evaluation and FV update with the fallback
-        if (c.integrator != INT_ADER || c.picard != "fixed" || c.guess != "zero")
-            return "job/precision=mixed with job/fallback=true covers the ADER step with the fixed Picard count";
-        return "";
-    }
+    // with the fallback, the mixed predictor is followed by the FP64 flux
+    // evaluation and FV update with trouble detection
     if (c.precision == "mixed" || c.picard == "residual" || c.guess == "mlp") {
-        if (c.system != "hydro" || c.integrator != INT_ADER || c.fallback)
-            return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step without fallback";
+        if (c.system != "hydro" || c.integrator != INT_ADER)
+            return "job/precision=mixed, job/picard=residual and job/guess=mlp cover the ADER hydro step";
         if (c.precision != "mixed") return "job/picard=residual and job/guess=mlp run on job/precision=mixed";
         if (c.picard_tol <= 0 || c.picard_max < 0 || c.picard_max > 30) return "job/picard_tol > 0, 0 <= job/picard_max <= 30";
         if (c.guess == "mlp" && (c.picard != "residual" || c.mlp_weights.empty()))
@@ -509,7 +511,16 @@ void initialize_hydro(const RunConfig& c, const Axis& X, const Axis& Y, const Ax
                                 long long gx = wrap(i, c.act[0] ? kNGH : 0, X.n, X.n_sp, ii);
                                 long long gy = wrap(j, c.act[1] ? kNGH : 0, Y.n, Y.n_sp, jj);
                                 long long gz = wrap(k, c.act[2] ? kNGH : 0, Z.n, Z.n_sp, kk);
-                                noise = normal_sample(c.pp.seed, (uint64_t)((gz * ncy + gy) * ncx + gx));
+                                long long sx = ncx, sy = ncy;
+                                if (c.pp.noise_cells > 0) {  // one sample per cell of an M^d lattice
+                                    const long long M = c.pp.noise_cells;
+                                    gx = gx / (ncx / M);
+                                    gy = c.act[1] ? gy / (ncy / M) : 0;
+                                    gz = c.act[2] ? gz / ((long long)Z.n * Z.n_sp / M) : 0;
+                                    sx = M;
+                                    sy = c.act[1] ? M : 1;
+                                }
+                                noise = normal_sample(c.pp.seed, (uint64_t)((gz * sy + gy) * sx + gx));
                             }
                             size_t cell = (((((size_t)k * Y.n_total + j) * X.n_total + i) * pz + kk) * py + jj) * px + ii;
                             for (int var = 0; var < kNVAR; var++) {

@@ -26,6 +26,7 @@ struct ProblemParams {  // src/global.hpp:22-37, plus the seeded-noise knobs
     int dir = 0;
     double cx = 0.5, cy = 0.5, cz = 0.5;
     bool random_perturb = false;
+    int noise_cells = 0;  // 0: one noise sample per FV sub-cell; M: per cell of an M^d lattice
     uint64_t seed = 20251202ull;
     double hx = 0, hy = 0, hz = 0;  // element widths (sedov_cells deposit region)
 };

@@ -1345,7 +1345,7 @@ int mlp_guess(spdk_ctx* c, void* ws, size_t E, double dt) {
 
 template <int N, int DIM>
 int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, double* W_cv, double* F_x,
-              const spdk_picard* pc, spdk_picard_stats* st, double* dt_next) {
+              const spdk_picard* pc, spdk_picard_stats* st, double* dt_next, bool predict_only) {
     using G = MT<N, DIM>;
     constexpr int T = G::T;
     if (c->T != T) { set_error("mixed path: ADER with T = p+1 time nodes only"); return SPDK_E_CONFIG; }
@@ -1443,11 +1443,12 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         nlist = flags ? (unsigned)n : 0;
         return SPDK_OK;
     };
-    // fixed count: p predictor sweeps from the zero-order guess, then the corrector
+    // fixed count: p predictor sweeps from the zero-order guess, then the
+    // corrector (left to the caller when predict_only)
     auto fixed_count = [&]() -> int {
-        for (int k = 0; k < T; k++) {
+        for (int k = 0; k < (predict_only ? T - 1 : T); k++) {
             MArgs s = a;
-            const bool last = k == T - 1;
+            const bool last = !predict_only && k == T - 1;
             s.cur = k + 1;
             s.path = k > 0;
             s.tin_slices = k == 0 ? 1 : T;
@@ -1538,23 +1539,27 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         hipLaunchKernelGGL(k_mhist, dim3(256), dim3(256), 0, c->stream, (const unsigned char*)conv, (int)E,
                            kHistBins, (unsigned long long*)(W + L.hist));
         SPDK_LAUNCH_CHECK("k_mhist");
-        MArgs sa = a;  // the corrector over every element
-        sa.cur = ++cur;
-        sa.path = 1;
-        sa.tin_slices = T;
-        sa.out_pred = 0;
-        sa.U_new = U_sp;
-        sa.W_cv = W_cv;
-        sa.F_x = F_x;
-        sa.sweep = ksw;
-        if ((rc = sweep(sa, L.blocks))) return rc;
-        sweeps += 1.0;
+        if (!predict_only) {
+            MArgs sa = a;  // the corrector over every element
+            sa.cur = ++cur;
+            sa.path = 1;
+            sa.tin_slices = T;
+            sa.out_pred = 0;
+            sa.U_new = U_sp;
+            sa.W_cv = W_cv;
+            sa.F_x = F_x;
+            sa.sweep = ksw;
+            if ((rc = sweep(sa, L.blocks))) return rc;
+        }
+        sweeps += 1.0;  // the corrector, or the caller's flux evaluation of the predictor
     } else {
         if ((rc = fixed_count())) return rc;
         sweeps += T;
     }
-    if ((rc = reduce(a.part, nullptr, nullptr, L.blocks, nullptr))) return rc;
-    if (dt_next) *dt_next = c->red_host[0];
+    if (!predict_only) {
+        if ((rc = reduce(a.part, nullptr, nullptr, L.blocks, nullptr))) return rc;
+        if (dt_next) *dt_next = c->red_host[0];
+    }
     if (st) {
         st->steps++;
         st->sweeps += sweeps;
@@ -1646,7 +1651,7 @@ int run_iterate(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt,
 }
 
 typedef int (*mstep_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, const spdk_picard*,
-                        spdk_picard_stats*, double*);
+                        spdk_picard_stats*, double*, bool);
 typedef MLayout (*mlayout_fn)(size_t);
 typedef int (*mprep_fn)(spdk_ctx*, const double*, const int*, void*);
 typedef int (*miter_fn)(spdk_ctx*, double*, const int*, void*, double, int);
@@ -1813,11 +1818,7 @@ int spdk_mixed_prepare(spdk_ctx* c, const double* U_sp, spdk_sd_shape s, void* w
     return v->prepare(c, U_sp, Ne, ws);
 }
 
-int spdk_hydro_step_mixed(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
-                          const spdk_picard* pc, spdk_picard_stats* stats, double* dt_next) {
-    const MVariant* v = nullptr;
-    int rc0 = check_mixed(c, U_sp, s, ws, &v);
-    if (rc0) return rc0;
+static int check_picard(spdk_ctx* c, const spdk_picard* pc) {
     if (pc && pc->residual && !(pc->tol > 0.0)) { set_error("mixed: residual tolerance must be positive"); return SPDK_E_ARG; }
     if (pc && pc->guess != SPDK_GUESS_ZERO && pc->guess != SPDK_GUESS_EXTERNAL && pc->guess != SPDK_GUESS_MLP) {
         set_error("mixed: unknown guess");
@@ -1831,9 +1832,27 @@ int spdk_hydro_step_mixed(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
         set_error("mixed: no network attached (spdk_mixed_attach_mlp)");
         return SPDK_E_CONFIG;
     }
+    return SPDK_OK;
+}
+
+int spdk_hydro_step_mixed(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
+                          const spdk_picard* pc, spdk_picard_stats* stats, double* dt_next) {
+    const MVariant* v = nullptr;
+    int rc0 = check_mixed(c, U_sp, s, ws, &v);
+    if (rc0 || (rc0 = check_picard(c, pc))) return rc0;
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);
-    return v->step(c, U_sp, Ne, ws, dt, W_cv, F_x, pc, stats, dt_next);
+    return v->step(c, U_sp, Ne, ws, dt, W_cv, F_x, pc, stats, dt_next, false);
+}
+
+int spdk_mixed_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, const spdk_picard* pc,
+                       spdk_picard_stats* stats) {
+    const MVariant* v = nullptr;
+    int rc0 = check_mixed(c, U_sp, s, ws, &v);
+    if (rc0 || (rc0 = check_picard(c, pc))) return rc0;
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);
+    return v->step(c, U_sp, Ne, ws, dt, nullptr, nullptr, pc, stats, nullptr, true);
 }
 
 }  // extern "C"

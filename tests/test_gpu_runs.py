@@ -49,6 +49,9 @@ CASES = {
     "sine2d_p1": (["-i", S, "mesh/p=1", "mesh/nx3=1", "time/tlim=0.02", "output/dt=0.01"], "fused"),
     "kh3d_random": (["-i", KH, "job/fallback=false", "mesh/nx1=6", "mesh/nx2=8", "mesh/nx3=5", "problem/perturb=random",
                      "problem/amp=0.01", "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    "kh3d_noise_lattice_fallback": (["-i", KH, "mesh/nx1=8", "mesh/nx2=8", "mesh/nx3=4", "problem/perturb=random",
+                                     "problem/noise_cells=4", "problem/amp=0.01", "time/tlim=0.006",
+                                     "output/dt=0.003"], "fused"),
     "kh2d_p4_fallback": (["-i", KH, "mesh/p=4", "mesh/nx1=10", "mesh/nx2=10", "time/tlim=0.01", "output/dt=0.005"], "stages"),
     "blast2d_fallback": (["-i", S, "job/fallback=true", "mesh/nx3=1", "problem/problem=spherical_blast",
                           "hydro/gamma=1.6666666666667", "time/tlim=0.02", "output/dt=0.01"], "stages"),
@@ -408,16 +411,33 @@ FB_CASES = {
 }
 
 
+WEIGHTS = os.path.join(op.ROOT, "spd_k_amd", "weights", "kh3d_p3.mlp")
+PICARD_FORMS = {
+    "fixed": [],
+    "residual": ["job/picard=residual", "job/picard_tol=1e-6"],
+    "learned": ["job/picard=residual", "job/picard_tol=1e-6", "job/guess=mlp", f"job/mlp_weights={WEIGHTS}"],
+}
+
+
+@pytest.mark.parametrize("form", sorted(PICARD_FORMS))
 @pytest.mark.parametrize("name", sorted(FB_CASES))
-def test_mixed_predictor_with_fallback(tmp_path, name):
-    """The mixed predictor (FP32 deviations, FP64 fluxes) feeding the FP64
+def test_mixed_predictor_with_fallback(tmp_path, name, form):
+    """The mixed predictor (FP32 deviations, FP64 fluxes; fixed count,
+    residual test, or residual test from the learned guess) feeding the FP64
     flux evaluation and FV update with NAD + MUSCL/HLLC blending: mass
     conserved to round-off, every variable within 1e-4 of the largest value of
     the FP64 run, troubled-cell masks differing in under 1% of the cells (a
     cell at the NAD threshold may flip)."""
+    if form == "learned" and name != "blast3d":
+        pytest.skip("the shipped network is the 3-D p=3 one")
     a, b = str(tmp_path / "fp64"), str(tmp_path / "mixed")
     op.run_driver(FB_CASES[name], a)
-    op.run_driver(FB_CASES[name] + ["job/precision=mixed"], b)
+    out = op.run_driver(FB_CASES[name] + ["job/precision=mixed"] + PICARD_FORMS[form], b)
+    if form != "fixed":
+        steps, sweeps, _ = _picard(out)
+        # at most the fixed count (3 predictor sweeps) + 1 flux evaluation; from
+        # the guess, up to 3 more sweeps for elements restarted from the zero-order guess
+        assert steps > 0 and sweeps <= (4.0 if form == "residual" else 7.0) + 1e-9, sweeps
     assert op.mass_drift(b) < 1e-12
     g = op.grid(a)
     i = op.output_indices(a)[-1]

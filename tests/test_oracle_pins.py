@@ -234,3 +234,25 @@ def test_induction_2d_period(tmp_path):
         errs.append(np.abs(b1 - b0).mean() / np.abs(b0).mean())
     assert errs[1] < 0.75 * errs[0], errs
     assert errs[1] < 0.5, errs
+
+
+def test_noise_lattice_is_resolution_independent(tmp_path):
+    """problem/noise_cells=M draws one N(0,1) sample per cell of an M^d
+    lattice: the seeded KH initial state at N and 2N elements then has the
+    same coarse-element means (the smooth factors differ only by quadrature
+    error, 1.2e-3 relative at 16^2 vs 32^2), while per-sub-cell noise gives unrelated fields."""
+    def means(n, extra):
+        d = tmp_path / f"n{n}{len(extra)}"
+        op.run_oracle(["-i", os.path.join(op.INPUTS, "kelvin_helmholtz.athinput"), "job/fallback=false",
+                       f"mesh/nx1={n}", f"mesh/nx2={n}", "mesh/nx3=1", "problem/perturb=random", "problem/amp=0.01",
+                       "time/tlim=1e-9", "output/dt=1"] + extra, str(d))
+        g = op.grid(str(d))
+        vy = op.load_sd(str(d), g, 0, 2)  # v_y of the initial state (output 0)
+        vy = vy.reshape(vy.shape[-2], vy.shape[-1])
+        return vy.reshape(4, vy.shape[0] // 4, 4, vy.shape[1] // 4).mean(axis=(1, 3))  # 4 x 4 lattice cells
+    lat = ["problem/noise_cells=4"]
+    a, b = means(16, lat), means(32, lat)
+    assert np.abs(a).max() > 1e-4
+    assert np.abs(a - b).max() < 1e-2 * np.abs(a).max()  # quadrature of the narrow Gaussian envelope
+    c = means(32, [])
+    assert np.abs(a - c).max() > 0.1 * np.abs(a).max()

@@ -109,9 +109,19 @@ def fv_geometry(N, p, L=1.0):
     return faces, centers
 
 
+def _oracle_env():
+    """OpenMP threads for one oracle process: the host's share per pytest
+    worker (spinning OpenMP barriers on oversubscribed cores make small runs
+    take minutes), capped by OMP_NUM_THREADS; idle threads sleep."""
+    share = max(1, (os.cpu_count() or 1) // int(os.environ.get("PYTEST_XDIST_WORKER_COUNT", "1")))
+    cap = int(os.environ.get("OMP_NUM_THREADS", share))
+    return dict(os.environ, OMP_NUM_THREADS=str(max(1, min(share, cap))), OMP_WAIT_POLICY="PASSIVE")
+
+
 def run_oracle(args, outdir, serial=False):
     exe = ORACLE_EXE_SERIAL if serial else ORACLE_EXE
-    r = subprocess.run([exe] + list(args) + ["-o", outdir], capture_output=True, text=True, timeout=600)
+    r = subprocess.run([exe] + list(args) + ["-o", outdir], capture_output=True, text=True, timeout=600,
+                       env=_oracle_env())
     if r.returncode != 0:
         raise RuntimeError(f"oracle failed: {r.stdout[-2000:]}{r.stderr[-2000:]}")
     return r.stdout

@@ -1,39 +1,46 @@
 """Benchmark: DOF-updates/s of the spectral-difference step on one MI355X.
 
-Workload (one "step" = one full ADER step, all Picard sweeps included):
-3-D Kelvin-Helmholtz, 96^3 elements, p=3 (56.6 M active solution points),
-seeded velocity perturbation, periodic box, gamma 1.4, inputs resident in
-HBM. DOF-updates/s = active solution points x steps / evolution seconds (the
-reference's zone-cycles/s, src/hydro_ader.hpp:263-271).
+Workload (one "step" = one full ADER step, all Picard sweeps and the FV
+update included): 3-D Kelvin-Helmholtz, 96^3 elements, p=3 (56.6 M active
+solution points), seeded velocity perturbation, periodic box, gamma 1.4,
+with trouble detection (NAD + SED + PAD) and the blended MUSCL-Hancock +
+HLLC fallback on, as the reference's kelvin_helmholtz input has it; inputs
+resident in HBM. DOF-updates/s = active solution points x steps / evolution
+seconds (the reference's zone-cycles/s, src/hydro_ader.hpp:263-271).
 
-Forms timed on the same workload:
-  plain_fp64     FP64 fused element-tile path (bit-identical to the reference
-                 arithmetic), p predictor sweeps + corrector;
-  mixed_fp32     FP32 state and contractions, FP64 fluxes / Riemann /
-                 accumulation, the same sweep count;
+Forms timed on the same workload (the flux evaluation, FV update, NAD and
+fallback are FP64 in every form):
+  plain_fp64     FP64 throughout (bit-identical to the reference arithmetic):
+                 p predictor sweeps and the flux evaluation in the element-tile
+                 kernel, then the FV update with trouble detection and blending;
+  mixed_fp32     the predictor with FP32 state, transforms and contractions
+                 and FP64 fluxes / Riemann solves, the same sweep count;
   residual_fp32  mixed; each element sweeps until its residual passes 1e-6,
                  at most the fixed count (zero-order guess);
   learned_fp32   residual_fp32 started from the trained per-element MLP guess;
                  elements unconverged at the cap restart from the zero-order
                  guess with the fixed count.
+`value` is the learned_fp32 form, the form the metric names; the other forms
+are reported beside it.
+
 Matched L1 error, measured in the same run for every form:
+  kh3d    the timed recipe (seeded perturbation on a 48^3 noise lattice, so
+          both resolutions start from one field; fallback on) at 48^3, p=3,
+          to t=0.03: L1 of the element means of density against the FP64 run
+          at 96^3 (averaged onto the 48^3 elements);
   sine3d  3-D sine advection, 24^3, p=3, t=0.2: L1 of density against the
-          exact solution;
-  kh3d    3-D KH (the reference's sinusoidal perturbation), 24^3, p=3,
-          t=0.03: L1 of the element means of density against the FP64 run
-          at 48^3 (averaged onto the 24^3 elements).
+          exact solution.
 A form matches when both errors are within 1.1x of the plain FP64 form's.
-`value` is the fastest matching form (named in config.form).
 
 Also measured (the "induction" object): the 2-D field-loop induction
 configuration, 1024^2 elements p=3, stage sequence vs fused FP64 vs mixed,
 with its one-period matched-error check.
 
 Prints one JSON line with the headline throughput, the per-form figures, the
-HBM roofline of the dominant kernel (the mixed Picard sweep; device time from
-HIP events on its stream, HBM traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE
-scaled by the same counters on a device copy of known size) and the CPU
-oracle timed on the host's cores on a bounded sample of the same workload.
+HBM roofline of the dominant kernel (device time from HIP events on its
+stream, HBM traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the
+same counters on a device copy of known size) and the CPU oracle timed on the
+host's cores on a bounded sample of the same workload.
 """
 import argparse
 import csv
@@ -58,12 +65,18 @@ WEIGHTS = os.path.join(ROOT, "spd_k_amd", "weights", "kh3d_p3.mlp")
 DRIVER = os.path.join(ROOT, "spd_k_amd", "_build", "spdk")
 
 
-def kh_args(n, p=P, perturb="random"):
-    a = ["-i", os.path.join(ROOT, "tests", "inputs", "kelvin_helmholtz.athinput"), "job/fallback=false",
+HEADLINE = "learned_fp32"
+NOISE_LATTICE = 48  # accuracy runs: one noise sample per cell of a 48^3 lattice (divides 48 and 96)
+
+
+def kh_args(n, p=P, noise_cells=0):
+    """The timed recipe: 3-D KH with the seeded perturbation and the FV
+    fallback (the input file's job/fallback=true)."""
+    a = ["-i", os.path.join(ROOT, "tests", "inputs", "kelvin_helmholtz.athinput"), "job/fallback=true",
          f"mesh/p={p}", f"mesh/nx1={n}", f"mesh/nx2={n}", f"mesh/nx3={n}", "problem/amp=0.01", "hydro/gamma=1.4",
-         "time/tlim=1.0", "output/dt=-1"]
-    if perturb == "random":
-        a.append("problem/perturb=random")
+         "problem/perturb=random", "time/tlim=1.0", "output/dt=-1"]
+    if noise_cells:
+        a.append(f"problem/noise_cells={noise_cells}")
     return a
 
 
@@ -76,9 +89,10 @@ FORMS = {
 }
 
 
-def mixed_sweep_bytes(p, dim, E):
-    """Compulsory HBM bytes of the mixed sweeps of one fixed-count ADER step
-    (each array read or written once per sweep; FP32 = 4 B, FP64 = 8 B), with
+def mixed_sweep_bytes(p, dim, E, corrector=False):
+    """Compulsory HBM bytes of the mixed predictor sweeps of one fixed-count
+    ADER step, and of the corrector when it runs in the same kernel (each
+    array read or written once per sweep; FP32 = 4 B, FP64 = 8 B), with
     n = T = p+1, V = 5, TPE = n^dim points, NT = n^(dim-1) face points:
       first sweep  : d_old (V TPE 4) + neighbour traces (2 dim NT V 4)
                      + D out (T V TPE 4) + traces out (2 dim NT T V 4)
@@ -94,11 +108,13 @@ def mixed_sweep_bytes(p, dim, E):
     first = d_old + tr1 + D + trT + mu
     middle = D + d_old + trT + D + trT + mu
     corr = D + d_old + trT + 2 * V * TPE * 8 + mu
-    return float(first + (T - 2) * middle + corr) * E, T
+    if corrector:
+        return float(first + (T - 2) * middle + corr) * E, T
+    return float(first + (T - 2) * middle) * E, T - 1
 
 
 def hbm_traffic(n, steps=1):
-    """HBM bytes per sweep launch of the fixed-count mixed step, from two
+    """HBM bytes per predictor-sweep launch of the fixed-count mixed step, from two
     rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch). Each
     counter is scaled by its reading on the warm-up's device copy of U_sp, a
     plain copy of known size. None when rocprofv3 is absent or fails."""
@@ -126,7 +142,7 @@ def hbm_traffic(n, steps=1):
                 v = float(row["Counter_Value"])
                 if "copyBuffer" in row["Kernel_Name"]:
                     copy.append(v)
-                elif "k_msweep" in row["Kernel_Name"]:
+                elif "k_msweep<4, 3, 0>" in row["Kernel_Name"]:  # the predictor sweeps
                     sweeps.append(v)
             if not copy or not sweeps:
                 return None
@@ -155,24 +171,26 @@ def cpu_baseline(sample_n=24, steps=3):
                       f"steps, evolution loop {float(m.group(2)):.2f} s"}
 
 
-def accuracy(forms):
+def accuracy(forms, coarse=48, t_end=0.03):
     """Matched-L1 check of every form (module docstring)."""
     import numpy as np
     import spd_k_amd
     from spd_k_amd.outputs import element_means, l1_vs_exact_sine, output_indices
     sine = ["-i", os.path.join(ROOT, "tests", "inputs", "sine_wave.athinput"), "mesh/nx1=24", "mesh/nx2=24",
             "mesh/nx3=24", "mesh/p=3", "problem/amp=0.1", "time/tlim=0.2", "output/dt=0.2", "output/fields=W_cv"]
-    kh = lambda n: kh_args(n, perturb="sin")[:-2] + ["time/tlim=0.03", "output/dt=0.03", "output/fields=W_cv"]
+    kh = lambda n: kh_args(n, noise_cells=NOISE_LATTICE)[:-2] + [f"time/tlim={t_end}", f"output/dt={t_end}",
+                                                                 "output/fields=W_cv"]
     out = {}
     with tempfile.TemporaryDirectory() as d:
         fine = os.path.join(d, "kh_fine")
-        spd_k_amd.run(kh(48) + FORMS["plain_fp64"], fine)
+        spd_k_amd.run(kh(2 * coarse) + FORMS["plain_fp64"], fine)
         ref = element_means(fine)
+        shutil.rmtree(fine, ignore_errors=True)
         ref = ref.reshape(ref.shape[0] // 2, 2, ref.shape[1] // 2, 2, ref.shape[2] // 2, 2).mean(axis=(1, 3, 5))
         for name in forms:
             sd, kd = os.path.join(d, "s_" + name), os.path.join(d, "k_" + name)
             spd_k_amd.run(sine + FORMS[name], sd)
-            spd_k_amd.run(kh(24) + FORMS[name], kd)
+            spd_k_amd.run(kh(coarse) + FORMS[name], kd)
             out[name] = {"sine3d_l1": l1_vs_exact_sine(sd, output_indices(sd)[-1], 0.2, amp=0.1),
                          "kh3d_l1": float(np.abs(element_means(kd) - ref).mean() / np.abs(ref).mean())}
     base = out.get("plain_fp64")
@@ -182,8 +200,9 @@ def accuracy(forms):
             r["kh3d_vs_fp64"] = r["kh3d_l1"] / base["kh3d_l1"]
             r["matched"] = r["sine3d_vs_fp64"] <= MATCH and r["kh3d_vs_fp64"] <= MATCH
     return {"sine3d": "3-D sine advection 24^3 p=3 to t=0.2, density L1 vs exact",
-            "kh3d": "3-D KH 24^3 p=3 to t=0.03, element-mean density L1 vs FP64 at 48^3", "match_factor": MATCH,
-            "forms": out}
+            "kh3d": f"3-D KH (timed recipe, fallback on, {NOISE_LATTICE}^3 noise lattice) {coarse}^3 p=3 to "
+                    f"t={t_end}, element-mean density L1 vs FP64 at {2 * coarse}^3",
+            "match_factor": MATCH, "forms": out}
 
 
 def induction_loop(n=1024, steps=6, warmup=2):
@@ -278,6 +297,35 @@ def all_configs(steps=6, warmup=2):
     return out
 
 
+def zone_cycles(integrator="rk3", R_list=(32, 64, 128, 256), p_list=(3, 7), sims=("sd", "sdfb")):
+    """The reference's zone-cycles/s sweep (tests/benchmark.py:32-76): 3-D
+    advected sine wave, RK3, without (sd) and with (sdfb) the FV fallback,
+    p in {3, 7}, R = N (p+1) cells per axis, tlim = 1.4 / R (about 10 steps),
+    outputs off, the driver's evolution clock. One row per configuration, in
+    the reference's row format, with the form this driver runs (the fused
+    tile kernel where a variant exists, else the stage sequence)."""
+    import spd_k_amd
+    rows = []
+    with tempfile.TemporaryDirectory() as d:
+        for sim in sims:
+            for p in p_list:
+                for R in R_list:
+                    N = R // (p + 1)
+                    args = ["-i", os.path.join(ROOT, "tests", "inputs", "sine_wave.athinput"), f"mesh/p={p}",
+                            f"mesh/nx1={N}", f"mesh/nx2={N}", f"mesh/nx3={N}",
+                            f"job/fallback={'true' if sim == 'sdfb' else 'false'}", f"time/integrator={integrator}",
+                            f"time/tlim={1.4 / R}", "output/dt=-1"]
+                    try:
+                        r = spd_k_amd.run(args, os.path.join(d, f"{sim}_{p}_{R}"))
+                        rows.append({"repo": "spd_k_amd", "sim": sim, "p": p, "R": R, "steps": r["steps"],
+                                     "ms_per_step": r["evolution_seconds"] / max(r["steps"], 1) * 1e3,
+                                     "zcps": r["dof_updates_per_s"]})
+                    except RuntimeError as e:
+                        rows.append({"repo": "spd_k_amd", "sim": sim, "p": p, "R": R, "error": str(e)[-300:]})
+    return {"zone_cycles": rows, "protocol": "tests/benchmark.py of the reference: 3-D sine, " + integrator +
+            ", sd and sdfb, p in {3,7}, R = N (p+1) in {32,64,128,256}, tlim = 1.4/R, outputs off"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +339,8 @@ def main():
     ap.add_argument("--no-induction", action="store_true", help="skip the 2-D induction configuration")
     ap.add_argument("--all-configs", action="store_true",
                     help="print one JSON object with the other synthetic configurations instead")
+    ap.add_argument("--zone-cycles", action="store_true",
+                    help="print the reference's zone-cycles/s sweep (3-D sine, RK3, sd/sdfb, p 3 and 7) instead")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the SD update here runs on a single GPU (multi-GPU halo exchange is a "
@@ -308,8 +358,8 @@ def main():
     if "learned_fp32" in forms and not os.path.exists(WEIGHTS):
         raise SystemExit(f"{WEIGHTS} is missing: train it with the driver's job/mode=train_guess")
     torch.cuda.synchronize()
-    if a.all_configs:
-        res = all_configs(a.steps, a.warmup)
+    if a.all_configs or a.zone_cycles:
+        res = all_configs(a.steps, a.warmup) if a.all_configs else zone_cycles()
         sys.stdout.flush()
         os.dup2(json_fd, 1)
         print(json.dumps(res), flush=True)
@@ -333,10 +383,8 @@ def main():
             per_form[name]["steps_with_capped_elements"] = r["picard_fallbacks"]
         if acc and name in acc["forms"]:
             per_form[name]["matched_l1"] = acc["forms"][name].get("matched")
-    # headline: the fastest form at matched L1 error (without the check: the
-    # residual form, whose tolerance was chosen by it)
-    ok = [n for n in forms if n != "plain_fp64" and (acc["forms"][n].get("matched") if acc else n == "residual_fp32")]
-    head = max(ok, key=lambda n: res[n]["value"]) if ok else forms[-1]
+    # headline: the form the metric names (learned guess + FP32 contractions)
+    head = HEADLINE if HEADLINE in res else forms[-1]
     h = res[head]
     roof = None
     m = res.get("mixed_fp32")
@@ -348,7 +396,7 @@ def main():
         tr = None if a.no_traffic else hbm_traffic(a.n)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(tr["bytes_per_launch"]) if tr else None,
-                "kernel": "k_msweep (mixed Picard sweep, fixed-count step)", "avg_launch_ms": round(avg_ms, 3),
+                "kernel": "k_msweep (mixed Picard predictor sweep, fixed-count step)", "avg_launch_ms": round(avg_ms, 3),
                 "bytes_per_launch": int(bytes_step / launches), "traffic_detail": tr}
     out = {
         "metric": "DOF-updates/s on 1 MI355X, learned+FP32 vs plain FP64 at matched L1 error",
@@ -363,8 +411,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"3-D Kelvin-Helmholtz {a.n}^3 elements p=3 (seeded perturbation), ADER step",
-                   "dofs": h["dofs"], "form": head, "picard_tol": TOL},
+        "config": {"workload": f"3-D Kelvin-Helmholtz {a.n}^3 elements p=3 (seeded perturbation), ADER step with "
+                               "NAD + MUSCL-HLLC fallback on (FP64 FV update)",
+                   "dofs": h["dofs"], "form": head, "picard_tol": TOL,
+                   "matched_l1": per_form[head].get("matched_l1")},
         "vs_plain_fp64": per_form[head]["vs_plain_fp64"],
         "forms": per_form,
         "accuracy": acc,

@@ -67,7 +67,7 @@ typedef struct spdk_params {
     int rk_order;       /* SSP-RK stages (1..3) when n_ader == 1               */
     double gamma, cfl;
     double g[3];        /* constant gravity (source term)                      */
-    double nu;          /* kinematic viscosity (dt clause only)               */
+    double nu;          /* kinematic viscosity (viscous terms when > 0)        */
     double h[3];        /* element widths dx, dy, dz                           */
     double nad_tol;     /* NAD band width                                      */
     int nad_delta;      /* band scaled by local range                          */
@@ -75,6 +75,7 @@ typedef struct spdk_params {
     int sed;            /* smooth-extrema relaxation (applied when p>1)        */
     int blending;       /* fractional theta blending                           */
     double vel[3];      /* induction advection velocity                        */
+    double beta;        /* bulk viscosity coefficient (hydro/beta, -2/3 nu)    */
 } spdk_params;
 
 /* Host-side operator data copied to the device at creation. Matrices are
@@ -141,6 +142,28 @@ int spdk_fill_sd_traces(spdk_ctx* ctx, double* U, spdk_sd_shape s, int dim);
 /* sd_riemann_solver (src/hydro.cpp:368-418): LLF at element faces along `dim`,
  * written in place over F on both sides */
 int spdk_riemann_llf(spdk_ctx* ctx, const double* U, double* F, spdk_sd_shape s, int dim);
+/* sd_riemann_solver with its viscous flag (src/hydro.cpp:368-418): the LLF
+ * face fluxes as spdk_riemann_llf; with viscous != 0 both copies of each
+ * face state in U are then replaced by riemann_wind(u_L, u_R, 0.5)
+ * (src/hydro.cpp:265-268, :402-409), the interface state the viscous
+ * gradients are taken from. */
+int spdk_sd_riemann(spdk_ctx* ctx, double* U, double* F, spdk_sd_shape s, int dim, int viscous);
+/* compute_gradient (src/hydro.cpp:452-482): dU (solution points, n_ader
+ * slices, all variables) of the active elements from the flux-point array
+ * U_fp along dim, du = sum_ll U(ll) dfp_to_sp(id, ll) / h_dim. Ghost elements
+ * of dU are not written. */
+int spdk_sd_gradient(spdk_ctx* ctx, const double* U_fp, spdk_sd_shape s_fp, double* dU, spdk_sd_shape s_sp, int dim);
+/* compute_viscous_flux + viscous_fluxes (src/hydro.cpp:484-555): at every
+ * flux point along dim (ghost elements included) the gradients dU1 (along
+ * dim), dU2 (along dim+1), dU3 (along dim+2, cyclic) interpolated from the
+ * solution points along dim, the primitives of U_fp there, and the viscous
+ * flux (prm.nu, prm.beta) written over U_fp. */
+int spdk_viscous_flux(spdk_ctx* ctx, double* U_fp, spdk_sd_shape s_fp, const double* dU1, const double* dU2,
+                      const double* dU3, spdk_sd_shape s_sp, int dim);
+/* sd_rusanov_solver (src/hydro.cpp:420-450): at every element face along
+ * dim, riemann_wind(-0.5) of (the right element's face value, the left
+ * element's) added to F on both sides. */
+int spdk_sd_rusanov(spdk_ctx* ctx, const double* U_fp, double* F, spdk_sd_shape s, int dim);
 /* copy_ader (src/hydro_ader.hpp:428-444): zero-order Picard guess */
 int spdk_copy_ader(spdk_ctx* ctx, const double* U, double* U_ader, spdk_sd_shape s_ader);
 /* update_prediction (src/transforms.cpp:263-325); Fy/Fz may be NULL on
@@ -199,6 +222,14 @@ int spdk_fv_primitives(spdk_ctx* ctx, const double* U, double* W, spdk_fv_shape 
  * relax, PAD. alpha_* are persistent per-axis scratch (zero-initialised). */
 int spdk_detect_troubles(spdk_ctx* ctx, const double* W_new, const double* W_old, double* troubles, double* alpha_x,
                          double* alpha_y, double* alpha_z, spdk_fv_shape s, int pad, int limit_mask);
+/* detect_troubles with the SED coefficients kept as one 16-bit word per FV
+ * cell (bit v*3+d: alpha_d of variable v >= 1) instead of three FP64 arrays:
+ * relax_NAD only tests whether the smallest alpha around a cell is below 1,
+ * so the troubles are the same as spdk_detect_troubles' for finite states.
+ * abits (FV cell count words) must be zero before the first call; mask covers
+ * variables 0..4. */
+int spdk_detect_troubles_bits(spdk_ctx* ctx, const double* W_new, const double* W_old, double* troubles,
+                              uint16_t* abits, spdk_fv_shape s, int pad, int mask);
 /* apply_blending + blending_ring, or theta_from_troubles when blending is off
  * (src/trouble_detection.cpp:191-242) */
 int spdk_blend_theta(spdk_ctx* ctx, const double* troubles, double* theta_tmp, double* theta, spdk_fv_shape s_tr);
@@ -303,6 +334,18 @@ int spdk_fused_predict(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* works
  * (src/hydro_ader.hpp:500-586). Uses the fused workspace. */
 int spdk_fused_predict_fluxes(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* F_x,
                               double* F_y, double* F_z, spdk_sd_shape s_ader);
+/* As spdk_fused_predict_fluxes, with the face_integral of every slice and
+ * active axis (src/transforms.cpp:378-481, called per ADER node by
+ * src/hydro_ader.hpp:500-510) formed in the same sweep: F_fv[t * 3 + d]
+ * receives the sub-cell face fluxes of slice t along axis d in the FV
+ * layout of the cell shape sc (6 variables, ghost layers included) with one
+ * more face along d; only the faces face_integral writes are written
+ * (variable 5 is left as it is). F_x (optional, may be NULL) receives the x
+ * flux-point fluxes in the stage layout of s_ader (the F_ader_fp_x output).
+ * Bit-identical to the stage flux evaluation followed by
+ * spdk_fv_face_integral. */
+int spdk_fused_predict_fv_fluxes(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt,
+                                 double* const* F_fv, spdk_fv_shape sc, double* F_x, spdk_sd_shape s_ader);
 /* Solve_fluxes of given ADER slices (src/hydro_ader.hpp:278-289: sp->fp,
  * fluxes, trace fill, LLF) with the fused tile kernel: U_ader in the stage
  * layout (T, >= 5 variables, elements, points), fluxes written as by
@@ -369,6 +412,21 @@ int spdk_hydro_step_mixed(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* wo
  * not written. Stats count the sweeps plus 1 for that flux evaluation. */
 int spdk_mixed_predict(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, const spdk_picard* pc,
                        spdk_picard_stats* stats);
+/* spdk_mixed_predict followed, in the same call, by the last flux evaluation
+ * (src/hydro_ader.hpp:278-289: face Riemann fluxes in FP64, interior
+ * flux-point fluxes as FP32 deviations from the element mean's flux) and the
+ * face_integral of every slice and active axis (src/transforms.cpp:378-481;
+ * transverse contraction in FP32, on MFMA at 3-D p=3 unless valu != 0, the
+ * mean flux added in FP64): F_fv[t * 3 + d] receives the sub-cell face fluxes
+ * of slice t along axis d in the FV layout of the cell shape sc (6
+ * variables, ghost layers included, one more face along d); each face is
+ * written once, so the FV update on them conserves exactly. F_x (optional)
+ * receives the x flux-point fluxes (F_ader_fp_x output). The FV update with
+ * trouble detection and the fallback (src/hydro_ader.hpp:510-586) follows
+ * through the stage entries. No counterpart for the mixed arithmetic. */
+int spdk_mixed_predict_fv_fluxes(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt,
+                                 const spdk_picard* pc, spdk_picard_stats* stats, double* const* F_fv,
+                                 spdk_fv_shape sc, double* F_x, int valu);
 /* Histogram of the predictor sweeps each element needed to pass the residual
  * test, accumulated over residual-form steps (bin k = k sweeps counted from
  * the step's first sweep, a restart included; bin 31 = never). Copies nbins (<= 32) counts; reset clears. */

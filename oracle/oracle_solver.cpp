@@ -245,8 +245,15 @@ static inline void riemann_llf(double* F, const double* U_L, const double* U_R, 
     for (int v = 0; v < NVAR; v++) F[v] = 0.5 * (F_R[v] + F_L[v]) - 0.5 * c_max * (U_R[v] - U_L[v]);
 }
 
-// src/hydro.cpp:368-412: one thread per element face, writes both sides.
-static void sd_riemann(const Config& cfg, const SDA& U, SDA& F, int D) {
+// src/hydro.cpp:265-268
+static inline void riemann_wind(double* F, const double* U_L, const double* U_R, double wind) {
+    for (int v = 0; v < NVAR; v++) F[v] = 0.5 * (U_R[v] + U_L[v]) + wind * (U_R[v] - U_L[v]);
+}
+
+// src/hydro.cpp:368-412: one thread per element face, writes both sides;
+// with viscosity the two face states are then replaced by riemann_wind(0.5)
+// of them (the interface state the gradients use).
+static void sd_riemann(const Config& cfg, SDA& U, SDA& F, int D, bool viscous = false) {
     double gm = cfg.gamma;
     int v1, v2, v3;
     vel_idx(D, v1, v2, v3);
@@ -270,6 +277,107 @@ static void sd_riemann(const Config& cfg, const SDA& U, SDA& F, int D) {
                 F(t, v, kL, jL, iL, kkL, jjL, iiL) = f[v];
                 F(t, v, kR, jR, iR, kkR, jjR, iiR) = f[v];
             }
+            if (viscous) {
+                riemann_wind(f, uL, uR, 0.5);
+                for (int v = 0; v < NVAR; v++) {
+                    U(t, v, kL, jL, iL, kkL, jjL, iiL) = f[v];
+                    U(t, v, kR, jR, iR, kkR, jjR, iiR) = f[v];
+                }
+            }
+        }
+    });
+}
+
+// src/hydro.cpp:420-450 (sd_rusanov_solver): at every element face the
+// left state is the right element's value and the right state the left
+// element's; riemann_wind(-0.5) of them is added to F on both sides.
+static void sd_rusanov(const SDA& U, SDA& F, int D) {
+    int Nx = U.Nx - (D == 0), Ny = U.Ny - (D == 1), Nz = U.Nz - (D == 2);
+    int px = D == 0 ? 1 : U.nx, py = D == 1 ? 1 : U.ny, pz = D == 2 ? 1 : U.nz;
+    int n = choose(D, U.nx, U.ny, U.nz);
+    for6(Nz, Ny, Nx, pz, py, px, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        int kL = k, jL = j, iL = i, kkL = kk, jjL = jj, iiL = ii;
+        int kR = k, jR = j, iR = i, kkR = kk, jjR = jj, iiR = ii;
+        if (D == 0) { iR = i + 1; iiL = n - 1; iiR = 0; }
+        if (D == 1) { jR = j + 1; jjL = n - 1; jjR = 0; }
+        if (D == 2) { kR = k + 1; kkL = n - 1; kkR = 0; }
+        for (int t = 0; t < U.nt; t++) {
+            double uL[NVAR] = {0}, uR[NVAR] = {0}, f[NVAR];
+            for (int v = 0; v < U.nv; v++) {
+                uL[v] = U(t, v, kR, jR, iR, kkR, jjR, iiR);
+                uR[v] = U(t, v, kL, jL, iL, kkL, jjL, iiL);
+            }
+            riemann_wind(f, uL, uR, -0.5);
+            for (int v = 0; v < U.nv; v++) {
+                F(t, v, kL, jL, iL, kkL, jjL, iiL) += f[v];
+                F(t, v, kR, jR, iR, kkR, jjR, iiR) += f[v];
+            }
+        }
+    });
+}
+
+// src/hydro.cpp:452-482 (compute_gradient): dU at the solution points of
+// the active elements from the flux-point values along `dim`,
+// du = sum_ll U(ll) D(id, ll) / h, ascending ll.
+static void compute_gradient(const Config& cfg, const SDA& U, SDA& dU, double h, const Mat& Dm, int dim) {
+    int q = choose(dim, dU.nx, dU.ny, dU.nz);
+    for6(dU.Nz - 2 * cfg.ngh[2], dU.Ny - 2 * cfg.ngh[1], dU.Nx - 2 * cfg.ngh[0], dU.nz, dU.ny, dU.nx, cfg.ngh[2],
+         cfg.ngh[1], cfg.ngh[0], [&](int k, int j, int i, int kk, int jj, int ii) {
+             for (int t = 0; t < dU.nt; t++)
+                 for (int var = 0; var < dU.nv; var++) {
+                     double du = 0;
+                     for (int ll = 0; ll < q + 1; ll++) {
+                         if (dim == 0) du += U(t, var, k, j, i, kk, jj, ll) * Dm(ii, ll) / h;
+                         else if (dim == 1) du += U(t, var, k, j, i, kk, ll, ii) * Dm(jj, ll) / h;
+                         else du += U(t, var, k, j, i, ll, jj, ii) * Dm(kk, ll) / h;
+                     }
+                     dU(t, var, k, j, i, kk, jj, ii) = du;
+                 }
+         });
+}
+
+// src/hydro.cpp:484-506
+static inline void viscous_fluxes(double* f, const double* w, const double* du1, int v1, const double* du2, int v2,
+                                  const double* du3, int v3, double nu, double beta) {
+    f[0] = 0;
+    f[P_] = 0;
+    f[v1] = 2 * nu * du1[v1] + beta * du1[v1];
+    f[v1] += beta * du2[v2];
+    f[v2] = nu * (du1[v2] + du2[v1]);
+    f[P_] += w[v2] * f[v2];
+    f[v1] += beta * du3[v3];
+    f[v3] = nu * (du1[v3] + du3[v1]);
+    f[P_] += w[v3] * f[v3];
+    f[P_] += w[v1] * f[v1];
+}
+
+// src/hydro.cpp:508-555 (compute_viscous_flux): at every flux point along
+// `dim` (ghost elements included) the three gradients interpolated from the
+// solution points along `dim`, the primitive state of U there, and the
+// viscous flux written over U.
+static void compute_viscous_flux(const Config& cfg, SDA& U, const SDA& dU1, int v1, const SDA& dU2, int v2,
+                                 const SDA& dU3, int v3, const Mat& S, int dim) {
+    int q = choose(dim, U.nx, U.ny, U.nz);
+    double gm = cfg.gamma;
+    for6(U.Nz, U.Ny, U.Nx, U.nz, U.ny, U.nx, 0, 0, 0, [&](int k, int j, int i, int kk, int jj, int ii) {
+        for (int t = 0; t < U.nt; t++) {
+            double f[NVAR] = {0}, u[NVAR], w[NVAR] = {0}, du1[NVAR], du2[NVAR], du3[NVAR];
+            int id = choose(dim, ii, jj, kk);
+            for (int var = 0; var < U.nv; var++) {
+                du1[var] = 0;
+                du2[var] = 0;
+                du3[var] = 0;
+                u[var] = U(t, var, k, j, i, kk, jj, ii);
+                for (int ll = 0; ll < q - 1; ll++) {
+                    int a = dim == 0 ? ll : ii, b = dim == 1 ? ll : jj, c = dim == 2 ? ll : kk;
+                    du1[var] += dU1(t, var, k, j, i, c, b, a) * S(id, ll);
+                    du2[var] += dU2(t, var, k, j, i, c, b, a) * S(id, ll);
+                    du3[var] += dU3(t, var, k, j, i, c, b, a) * S(id, ll);
+                }
+            }
+            primitives(u, w, gm);
+            viscous_fluxes(f, w, du1, v1, du2, v2, du3, v3, cfg.nu, cfg.beta);
+            for (int var = 0; var < U.nv; var++) U(t, var, k, j, i, kk, jj, ii) = f[var];
         }
     });
 }
@@ -1031,7 +1139,7 @@ struct Hydro {
     int nvar = NVAR, n_ader = 1, n_stages = 1, n_output = 0, n_step = 0;
     double rk_a[3] = {0, 0, 0};
     double t = 0, dt = 0, Dt = 0;
-    SDA W_sp, U_sp, W_cv, U_cv, T_sweep, U0_sp, U_ader_sp, U_fp[3], F_fp[3];
+    SDA W_sp, U_sp, W_cv, U_cv, T_sweep, U0_sp, U_ader_sp, U_fp[3], F_fp[3], dU[3];
     FVA U_new, W_new, U_old, W_old, troubles, theta, theta_tmp, Fv[3], alpha[3];
     SDA T_fp[3];
 
@@ -1062,6 +1170,10 @@ struct Hydro {
         T_sweep.init("T_sweep", 1, nvar, Z, Y, X, 0, 0, 0);
         U_ader_sp.init("U_ader_sp", n_ader, nvar, Z, Y, X, 0, 0, 0);
         if (cfg.integrator == INT_RK) U0_sp.init("U0_sp", 1, nvar, Z, Y, X, 0, 0, 0);
+        if (cfg.nu > 0.0) {  // hydro_ader.hpp: dUx_sp, dUy_sp, dUz_sp
+            const char* gn[3] = {"dUx_sp", "dUy_sp", "dUz_sp"};
+            for (int d = 0; d < 3; d++) dU[d].init(gn[d], n_ader, nvar, Z, Y, X, 0, 0, 0);
+        }
         const char* un[3] = {"U_ader_fp_x", "U_ader_fp_y", "U_ader_fp_z"};
         const char* fn[3] = {"F_ader_fp_x", "F_ader_fp_y", "F_ader_fp_z"};
         for (int d = 0; d < 3; d++) {
@@ -1100,7 +1212,8 @@ struct Hydro {
         write_sd(outdir, W_cv, n_output++);
     }
 
-    void solve_fluxes() {  // hydro_ader.hpp:278-289 (inviscid)
+    void solve_fluxes() {  // hydro_ader.hpp:278-289 (Viscosity :380-405 when nu > 0)
+        const bool visc = cfg.nu > 0.0;
         for (int d = 0; d < 3; d++)
             if (cfg.act[d]) transform_1d(U_ader_sp, U_fp[d], op.sp_to_fp, d, -1);
         for (int d = 0; d < 3; d++)
@@ -1108,7 +1221,19 @@ struct Hydro {
         for (int d = 0; d < 3; d++)
             if (cfg.act[d]) sd_boundaries(cfg, U_fp[d], d);
         for (int d = 0; d < 3; d++)
-            if (cfg.act[d]) sd_riemann(cfg, U_fp[d], F_fp[d], d);
+            if (cfg.act[d]) sd_riemann(cfg, U_fp[d], F_fp[d], d, visc);
+        if (!visc) return;
+        const double h[3] = {X.h, Y.h, Z.h};
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) compute_gradient(cfg, U_fp[d], dU[d], h[d], op.dfp_to_sp, d);
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d])
+                compute_viscous_flux(cfg, U_fp[d], dU[d], 1 + d, dU[(d + 1) % 3], 1 + (d + 1) % 3, dU[(d + 2) % 3],
+                                     1 + (d + 2) % 3, op.sp_to_fp, d);
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) sd_boundaries(cfg, U_fp[d], d);
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) sd_rusanov(U_fp[d], F_fp[d], d);
     }
 
     void ader_step() {  // hydro_ader.hpp:291-305
@@ -1861,6 +1986,40 @@ int orc_riemann(const OrcCfg* c, const double* U, double* F, const int* s, int d
     orc::SDA u = sda(s, U), f = sda(s, F);
     orc::sd_riemann(k, u, f, dim);
     out(f.a, F);
+    return 0;
+}
+// sd_riemann_solver with viscous = true: F, and U's face states replaced
+int orc_riemann_viscous(const OrcCfg* c, double* U, double* F, const int* s, int dim) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s, U), f = sda(s, F);
+    orc::sd_riemann(k, u, f, dim, true);
+    out(f.a, F);
+    out(u.a, U);
+    return 0;
+}
+int orc_rusanov(const double* U, double* F, const int* s, int dim) {
+    orc::SDA u = sda(s, U), f = sda(s, F);
+    orc::sd_rusanov(u, f, dim);
+    out(f.a, F);
+    return 0;
+}
+int orc_gradient(const OrcCfg* c, const double* U, const int* s_fp, double* dU, const int* s_sp, double h,
+                 const double* D, int dr, int dc, int dim) {
+    orc::Config k = to_cfg(*c);
+    orc::SDA u = sda(s_fp, U), g = sda(s_sp, dU);
+    orc::compute_gradient(k, u, g, h, mat(D, dr, dc), dim);
+    out(g.a, dU);
+    return 0;
+}
+int orc_viscous_flux(const OrcCfg* c, double* U, const int* s_fp, const double* dU1, const double* dU2,
+                     const double* dU3, const int* s_sp, const double* S, int sr, int sc, double nu, double beta,
+                     int dim) {
+    orc::Config k = to_cfg(*c);
+    k.nu = nu;
+    k.beta = beta;
+    orc::SDA u = sda(s_fp, U), g1 = sda(s_sp, dU1), g2 = sda(s_sp, dU2), g3 = sda(s_sp, dU3);
+    orc::compute_viscous_flux(k, u, g1, 1 + dim, g2, 1 + (dim + 1) % 3, g3, 1 + (dim + 2) % 3, mat(S, sr, sc), dim);
+    out(u.a, U);
     return 0;
 }
 int orc_sd_bc(const OrcCfg* c, double* U, const int* s, int dim) {

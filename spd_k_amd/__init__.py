@@ -47,7 +47,7 @@ class Params(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double), ("g", ctypes.c_double * 3),
                 ("nu", ctypes.c_double), ("h", ctypes.c_double * 3), ("nad_tol", ctypes.c_double),
                 ("nad_delta", ctypes.c_int), ("nad_moore", ctypes.c_int), ("sed", ctypes.c_int),
-                ("blending", ctypes.c_int), ("vel", ctypes.c_double * 3)]
+                ("blending", ctypes.c_int), ("vel", ctypes.c_double * 3), ("beta", ctypes.c_double)]
 
 
 class Operators(ctypes.Structure):
@@ -137,7 +137,7 @@ class Context:
 
     def __init__(self, p, active, bc, n_ader, ops, gamma=1.4, cfl=0.8, h=(1.0, 1.0, 1.0), g=(0.0, 0.0, 0.0), nu=0.0,
                  nad_tol=1e-5, nad_delta=False, nad_moore=True, sed=True, blending=True, rk_order=0,
-                 vel=(1.0, 1.0, 0.0)):
+                 vel=(1.0, 1.0, 0.0), beta=None):
         import numpy as np
         import torch
         if not torch.cuda.is_available():
@@ -153,6 +153,7 @@ class Context:
             prm.vel[d] = vel[d]
         prm.n_ader, prm.rk_order = n_ader, rk_order
         prm.gamma, prm.cfl, prm.nu, prm.nad_tol = gamma, cfl, nu, nad_tol
+        prm.beta = -2.0 / 3.0 * nu if beta is None else beta  # hydro/beta default (src/main.cpp:236-237)
         prm.nad_delta, prm.nad_moore, prm.sed, prm.blending = int(nad_delta), int(nad_moore), int(sed), int(blending)
         o = Operators()
 
@@ -224,8 +225,20 @@ class Context:
     def compute_fluxes(self, U, F, dim):
         check(lib().spdk_hydro_flux(self.ctx, _ptr(U), _ptr(F), SDShape.of(U.shape), dim), "hydro_flux")
 
-    def sd_riemann_solver(self, U, F, dim):
-        check(lib().spdk_riemann_llf(self.ctx, _ptr(U), _ptr(F), SDShape.of(U.shape), dim), "riemann_llf")
+    def sd_riemann_solver(self, U, F, dim, viscous=False):
+        check(lib().spdk_sd_riemann(self.ctx, _ptr(U), _ptr(F), SDShape.of(U.shape), dim, int(viscous)), "sd_riemann")
+
+    # --- viscous terms (src/hydro.cpp:420-555)
+    def compute_gradient(self, U_fp, dU, dim):
+        check(lib().spdk_sd_gradient(self.ctx, _ptr(U_fp), SDShape.of(U_fp.shape), _ptr(dU), SDShape.of(dU.shape), dim),
+              "sd_gradient")
+
+    def compute_viscous_flux(self, U_fp, dU1, dU2, dU3, dim):
+        check(lib().spdk_viscous_flux(self.ctx, _ptr(U_fp), SDShape.of(U_fp.shape), _ptr(dU1), _ptr(dU2), _ptr(dU3),
+                                      SDShape.of(dU1.shape), dim), "viscous_flux")
+
+    def sd_rusanov_solver(self, U_fp, F, dim):
+        check(lib().spdk_sd_rusanov(self.ctx, _ptr(U_fp), _ptr(F), SDShape.of(U_fp.shape), dim), "sd_rusanov")
 
     def compute_dt(self, W_cv):
         out = ctypes.c_double()
@@ -266,6 +279,12 @@ class Context:
         check(lib().spdk_detect_troubles(self.ctx, _ptr(W_new), _ptr(W_old), _ptr(troubles), _ptr(ax), _ptr(ay), _ptr(az),
                                          FVShape.of(W_new.shape), int(pad), int(mask)), "detect_troubles")
 
+    def detect_troubles_bits(self, W_new, W_old, troubles, abits, pad, mask):
+        """detect_troubles with the SED alphas as 16-bit words per cell
+        (abits: int16 tensor of the FV cell count, zero before the first call)."""
+        check(lib().spdk_detect_troubles_bits(self.ctx, _ptr(W_new), _ptr(W_old), _ptr(troubles), _ptr(abits),
+                                              FVShape.of(W_new.shape), int(pad), int(mask)), "detect_troubles_bits")
+
     def blend_theta(self, troubles, theta_tmp, theta):
         check(lib().spdk_blend_theta(self.ctx, _ptr(troubles), _ptr(theta_tmp), _ptr(theta), FVShape.of(troubles.shape)),
               "blend_theta")
@@ -288,3 +307,19 @@ class Context:
     def fallback_fluxes(self, W, theta, Fx, Fy, Fz, dt_sub):
         check(lib().spdk_muscl_hllc_blend(self.ctx, _ptr(W), _ptr(theta), _ptr(Fx), _ptr(Fy), _ptr(Fz),
                                           FVShape.of(W.shape), ctypes.c_double(dt_sub)), "muscl_hllc_blend")
+
+    # --- one whole step (src/hydro_ader.hpp:291-325 + compute_dt) with the fused tile kernel
+    def hydro_step_fused(self, U_sp, dt, W_cv=None):
+        """One plain FP64 step on U_sp with a workspace allocated here; returns
+        the next dt (the call blocks for it)."""
+        import torch
+        L = lib()
+        N = (ctypes.c_int * 3)(U_sp.shape[4], U_sp.shape[3], U_sp.shape[2])
+        nb = L.spdk_fused_workspace_bytes(self.ctx, N)
+        if nb == 0:
+            raise RuntimeError(f"fused step: {last_error()}")
+        ws = torch.empty(nb, dtype=torch.uint8, device=U_sp.device)
+        out = ctypes.c_double()
+        check(L.spdk_hydro_step_fused(self.ctx, _ptr(U_sp), SDShape.of(U_sp.shape), _ptr(ws), ctypes.c_double(dt),
+                                      _ptr(W_cv), None, ctypes.byref(out)), "hydro_step_fused")
+        return out.value

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include "ctx.hpp"
 #include "physics.hpp"
 
@@ -65,6 +66,9 @@ struct SweepArgs {
     int tin_slices;       // slices present in tr_in (1 or T)
     int in_nv;            // variables per slice of U_in
     int out_nv;           // variables per slice of F_out (slots past the physical ones get zero)
+    double* Ffv[kMaxT][3];  // flux-only sweep: sub-cell face fluxes of slice t along axis d (FV layout)
+    int fvn[3];           // FV cells per axis incl. ghost layers (an axis' face array has one more along it)
+    int fgh[3];           // FV ghost layers per axis
     int flux_only;        // write the flux-point fluxes of every slice and axis; no divergence
     int in_ader;          // state from U_in (else U_sp broadcast to every slice)
     int out_pred;         // write predictor U_out + tr_out, else corrector
@@ -89,7 +93,96 @@ __device__ __forceinline__ int spidx(int d, int ll, int q) {
     return ll * N * N + q;                                  // q = jj*N + ii
 }
 
-template <int N, int DIM, int T>
+// face_integral of slice t for every active axis (src/transforms.cpp:378-481)
+// from the element's flux-point fluxes in LDS (Fs, the fidx layout of each
+// axis): each element writes the sub-cell faces of its cells' lower sides
+// along d (the right ghost along d writes the domain's last face), so each
+// FV face is written once, by the element whose copy face_integral reads.
+// With both transverse axes active, the dim1 sweep u(f, o1, nn) = sum_ll
+// F(f, ll, nn) S(o1, ll) goes through `scr` (the element's state area, free
+// after the pencils) and the dim2 sum over nn of u S(o2, nn) is written;
+// otherwise the single contraction of face_integral_ref. Sums in the
+// reference's order, so the faces equal the stage kernels' bit for bit.
+template <int N, int DIM>
+__device__ __forceinline__ void fv_faces(const SweepArgs& a, double* scr, const double* Fs, int t, int s, bool live,
+                                         int ei, int ej, int ek) {
+    using G = Tile<N, DIM>;
+    constexpr int M = G::M, NT = G::NT, TPE = G::TPE, FSZ = G::FSZ;
+    const double* S = a.S2C;
+    const int ec[3] = {ei, ej, ek};
+    const int o[3] = {s % N, DIM >= 2 ? (s / N) % N : 0, DIM == 3 ? s / (N * N) : 0};  // x fastest
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        bool own = live;
+#pragma unroll
+        for (int x = 0; x < DIM; x++) own = own && ec[x] >= 1 && ec[x] <= (x == d ? a.Ne[x] - 1 : a.Ne[x] - 2);
+        const int f = o[d];
+        const bool put = own && (ec[d] < a.Ne[d] - 1 || f == 0);
+        const int dim1 = d == 0 ? 1 : (d == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
+        const int nx = a.fvn[0] + (d == 0), ny = a.fvn[1] + (d == 1), nz = a.fvn[2] + (d == 2);
+        const int I = o[0] + a.fgh[0] + (ei - 1) * N;
+        const int J = DIM >= 2 ? o[1] + a.fgh[1] + (ej - 1) * N : 0;
+        const int K = DIM == 3 ? o[2] + a.fgh[2] + (ek - 1) * N : 0;
+        double* out = a.Ffv[t][d] + ((size_t)K * ny + J) * nx + I;
+        const size_t vstride = (size_t)nz * ny * nx;
+        if (DIM == 3) {
+            {  // dim1 sweep: thread s -> (f, o1, nn)
+                const int f1 = s / (N * N), o1 = (s / N) % N, nn = s % N;
+                if (own)
+#pragma unroll
+                    for (int v = 0; v < NVP; v++) {
+                        double u = 0;
+#pragma unroll
+                        for (int ll = 0; ll < N; ll++) {
+                            const int q = d == 1 ? ll * N + nn : nn * N + ll;
+                            u += Fs[(d * NVP + v) * FSZ + fidx(d, f1, q, N, NT, M)] * S[o1 * N + ll];
+                        }
+                        scr[v * TPE + s] = u;
+                    }
+            }
+            __syncthreads();
+            if (put) {
+                const int o1 = o[dim1], o2 = o[dim2];
+#pragma unroll
+                for (int v = 0; v < NVP; v++) {
+                    double acc = 0;
+#pragma unroll
+                    for (int nn = 0; nn < N; nn++) acc += scr[v * TPE + (f * N + o1) * N + nn] * S[o2 * N + nn];
+                    out[(size_t)v * vstride] = acc;
+                }
+            }
+            __syncthreads();
+        } else if (put) {  // at most one transverse axis active: face_integral_ref
+#pragma unroll
+            for (int v = 0; v < NVP; v++) {
+                double acc = 0;
+                if (DIM == 1) {
+                    acc += Fs[v * FSZ + fidx(0, f, 0, N, NT, M)];
+                } else if (d == 0) {  // dim1 = y active, dim2 = z inactive
+#pragma unroll
+                    for (int ll = 0; ll < N; ll++) {
+                        double x = Fs[v * FSZ + fidx(0, f, ll, N, NT, M)];
+                        x *= S[o[1] * N + ll];
+                        acc += x;
+                    }
+                } else {  // d = 1: dim1 = z inactive, dim2 = x active
+#pragma unroll
+                    for (int nn = 0; nn < N; nn++) {
+                        double x = Fs[(NVP + v) * FSZ + fidx(1, f, nn, N, NT, M)];
+                        x *= S[o[0] * N + nn];
+                        acc += x;
+                    }
+                }
+                out[(size_t)v * vstride] = acc;
+            }
+        }
+    }
+}
+
+// FVF: the flux-only sweep that also forms the face integrals (fv_faces),
+// compiled apart so that the predictor and corrector sweeps keep their
+// register budget (2 waves per SIMD at 3-D p=3).
+template <int N, int DIM, int T, bool FVF>
 __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     using G = Tile<N, DIM>;
     constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB;
@@ -191,7 +284,10 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
                     br[v] = (ed == Nd - 2 || ed == Nd - 1) ? sg * trace(Nd - 2, 1, v) : 0.0;
                 }
             }
-            for (int side = 0; side < 2; side++) {
+            // the face integrals read faces 0..M-2 only: the upper face's
+            // flux is formed when the flux-point array is written as well
+            const int nsides = (FVF && !a.F_out[d]) ? 1 : 2;
+            for (int side = 0; side < nsides; side++) {
                 double fl[NVP];
                 const int f = side ? M - 1 : 0;
                 bool raw = side ? (ed == Nd - 1) : (ed == 0);
@@ -220,11 +316,12 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
             }
         }
         __syncthreads();
-        if (a.flux_only) {  // the fluxes of slice t in the stage flux-point layout (src/hydro_ader.hpp:278-289)
+        if (FVF || a.flux_only) {  // the fluxes of slice t in the stage flux-point layout (src/hydro_ader.hpp:278-289)
+            if (FVF) fv_faces<N, DIM>(a, us, Fs, t, s, live, ei, ej, ek);
             if (live) {
 #pragma unroll
                 for (int d = 0; d < DIM; d++)
-                    for (int idx = s; idx < G::FSZ; idx += TPE) {
+                    for (int idx = s; a.F_out[d] && idx < G::FSZ; idx += TPE) {
                         double* o = a.F_out[d] + ((size_t)t * a.out_nv * E + e) * G::FSZ + idx;
 #pragma unroll
                         for (int v = 0; v < NVP; v++) o[(size_t)v * E * G::FSZ] = Fs[(d * NVP + v) * G::FSZ + idx];
@@ -264,7 +361,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
         }
         __syncthreads();
     }
-    if (a.flux_only) return;
+    if (FVF || a.flux_only) return;
     if (!a.in_ader) {
 #pragma unroll
         for (int t = 1; t < T; t++)
@@ -503,6 +600,8 @@ struct Extra {
     int out_nv = 0;                                  // final slices written here (stage flux-point layout)
     const double* U_slices = nullptr;  // with F_out: the flux evaluation of these slices (stage layout), no
     int slices_nv = 0;                 // predictor; the workspace then holds only their traces
+    double* const* Ffv = nullptr;      // the flux evaluation's sub-cell face fluxes, [t * 3 + d] (FV layout)
+    int fvn[3] = {1, 1, 1}, fgh[3] = {0, 0, 0};
 };
 
 template <int N, int DIM>
@@ -552,7 +651,10 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     auto launch_sweep = [&](const SweepArgs& args) -> int {
         bool rec = c->timing && !x && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
-        hipLaunchKernelGGL((k_sweep<N, DIM, T>), grid, block, lds, c->stream, args);
+        if (args.Ffv[0][0])
+            hipLaunchKernelGGL((k_sweep<N, DIM, T, true>), grid, block, lds, c->stream, args);
+        else
+            hipLaunchKernelGGL((k_sweep<N, DIM, T, false>), grid, block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_sweep");
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
         return SPDK_OK;
@@ -563,7 +665,13 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
         return SPDK_OK;
     };
     int rc;
-    const bool flux_out = x && x->F_out[0];
+    const bool flux_out = x && (x->F_out[0] || x->Ffv);
+    auto set_fv = [&](SweepArgs& sa) {
+        if (!x || !x->Ffv) return;
+        for (int t = 0; t < T; t++)
+            for (int d = 0; d < 3; d++) sa.Ffv[t][d] = x->Ffv[t * 3 + d];
+        for (int d = 0; d < 3; d++) { sa.fvn[d] = x->fvn[d]; sa.fgh[d] = x->fgh[d]; }
+    };
     if (T > 1 && x && x->U_slices) {  // flux evaluation of given slices: their traces, then one flux sweep
         double* trS[3] = {nullptr, nullptr, nullptr};
         for (int d = 0; d < DIM; d++) trS[d] = W + (size_t)d * L.tr_axis;
@@ -580,6 +688,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
         s.in_nv = x->slices_nv;
         s.flux_only = 1;
         s.out_nv = x->out_nv;
+        set_fv(s);
         return launch_sweep(s);
     }
     if (T > 1) {  // ADER: K = T sweeps, the last one with the corrector (or the flux evaluation)
@@ -606,6 +715,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
                 s.flux_only = 1;
                 s.out_nv = x->out_nv;
                 for (int d = 0; d < 3; d++) s.F_out[d] = x->F_out[d];
+                set_fv(s);
             }
             if ((rc = launch_sweep(s))) return rc;
         }
@@ -695,9 +805,8 @@ size_t spdk_fused_workspace_bytes(spdk_ctx* c, const int N[3]) {
     return L.total * sizeof(double);
 }
 
-int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
-                          double* dt_next) {
-    if (!c || !U_sp || !ws) { set_error("fused: null argument"); return SPDK_E_ARG; }
+// Checks of a whole fused step on U_sp of shape s.
+static int check_step_fused(spdk_ctx* c, const spdk_sd_shape& s, const Variant** vout) {
     if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
         set_error("fused path covers the inviscid, source-free update");
         return SPDK_E_CONFIG;
@@ -714,6 +823,15 @@ int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
         if (c->prm.active[d] && Nd < 3) { set_error("fused: axis needs ghost elements"); return SPDK_E_SHAPE; }
     }
     if ((uint64_t)s.Nx * s.Ny * s.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
+    *vout = v;
+    return SPDK_OK;
+}
+
+int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* W_cv, double* F_x,
+                          double* dt_next) {
+    if (!c || !U_sp || !ws) { set_error("fused: null argument"); return SPDK_E_ARG; }
+    const Variant* v = nullptr;
+    if (int rc = check_step_fused(c, s, &v)) return rc;
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
     if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr);
@@ -750,7 +868,8 @@ int spdk_fused_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, dou
 
 // Checks shared by the flux-evaluation entries: the ADER slices shape and the
 // flux-point arrays of the active axes.
-static int check_flux_args(spdk_ctx* c, spdk_sd_shape s_ader, double* const F[3], const Variant** vout) {
+static int check_flux_args(spdk_ctx* c, spdk_sd_shape s_ader, double* const F[3], const Variant** vout,
+                           bool need_F = true) {
     if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0 || c->T < 2) {
         set_error("fused flux evaluation covers the inviscid, source-free ADER step");
         return SPDK_E_CONFIG;
@@ -765,7 +884,10 @@ static int check_flux_args(spdk_ctx* c, spdk_sd_shape s_ader, double* const F[3]
     for (int d = 0; d < 3; d++) {
         int Nd = d == 0 ? s_ader.Nx : (d == 1 ? s_ader.Ny : s_ader.Nz);
         if (c->prm.active[d] && Nd < 3) { set_error("fused: axis needs ghost elements"); return SPDK_E_SHAPE; }
-        if (c->prm.active[d] && !F[d]) { set_error("fused fluxes: null flux array of an active axis"); return SPDK_E_ARG; }
+        if (need_F && c->prm.active[d] && !F[d]) {
+            set_error("fused fluxes: null flux array of an active axis");
+            return SPDK_E_ARG;
+        }
     }
     if ((uint64_t)s_ader.Nx * s_ader.Ny * s_ader.Nz >= (1ull << 31)) { set_error("fused: too many elements"); return SPDK_E_SHAPE; }
     *vout = v;
@@ -788,6 +910,40 @@ int spdk_fused_predict_fluxes(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* 
     Extra x;
     for (int d = 0; d < 3; d++) x.F_out[d] = F[d];
     x.out_nv = s_ader.nv;
+    std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
+    return v->ader(c, U_sp, Ne, ws, dt, nullptr, nullptr, nullptr, &x);
+}
+
+int spdk_fused_predict_fv_fluxes(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* const* F_fv,
+                                 spdk_fv_shape sc, double* F_x, spdk_sd_shape s_ader) {
+    if (!c || !U_sp || !ws || !F_fv) { set_error("fused predict fv fluxes: null argument"); return SPDK_E_ARG; }
+    double* F[3] = {F_x, nullptr, nullptr};  // flux-point arrays are optional here
+    const Variant* v = nullptr;
+    int rc = check_flux_args(c, s_ader, F, &v, false);
+    if (rc) return rc;
+    if (s.nv != NV || s.nt != 1 || s.nx != s_ader.nx || s.ny != s_ader.ny || s.nz != s_ader.nz || s.Nx != s_ader.Nx ||
+        s.Ny != s_ader.Ny || s.Nz != s_ader.Nz) {
+        set_error("fused predict fv fluxes: U_sp shape");
+        return SPDK_E_SHAPE;
+    }
+    const int Ne[3] = {s.Nx, s.Ny, s.Nz}, nc[3] = {sc.Nx, sc.Ny, sc.Nz};
+    for (int d = 0; d < 3; d++) {
+        const bool act = c->prm.active[d] != 0;
+        const bool ok = act ? (Ne[d] > 2 * c->ngh[d] && nc[d] == (Ne[d] - 2 * c->ngh[d]) * c->n + 2 * c->fgh[d] &&
+                               c->ngh[d] == 1)
+                            : (Ne[d] == 1 && nc[d] == 1);
+        if (!ok || sc.nv != NV) {
+            set_error("fused predict fv fluxes: FV cell shape disagrees with U_sp along axis " + std::to_string(d));
+            return SPDK_E_SHAPE;
+        }
+        for (int t = 0; t < c->T; t++)
+            if (act && !F_fv[t * 3 + d]) { set_error("fused predict fv fluxes: null face array"); return SPDK_E_ARG; }
+    }
+    Extra x;
+    x.F_out[0] = F_x;  // optional: the x flux-point fluxes (outputs)
+    x.out_nv = s_ader.nv;
+    x.Ffv = F_fv;
+    for (int d = 0; d < 3; d++) { x.fvn[d] = nc[d]; x.fgh[d] = c->fgh[d]; }
     std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
     return v->ader(c, U_sp, Ne, ws, dt, nullptr, nullptr, nullptr, &x);
 }

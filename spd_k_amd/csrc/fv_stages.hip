@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(BS) k_face_int_ref(const double* __restrict__ 
 // two-pass one bit for bit without the intermediate array.
 template <int Q>
 struct FaceTile {
-    static constexpr int EX = Q <= 4 ? 16 : 8;    // elements per workgroup
+    static constexpr int EX = Q <= 4 ? 16 : (Q <= 6 ? 8 : 4);  // elements per workgroup (LDS below 64 KB)
     static constexpr int P = (Q + 1) * Q * Q;     // flux points of one element, one variable
     static constexpr int U = Q * Q * Q;           // dim1-swept values (= FV faces) per element
 };
@@ -356,10 +356,16 @@ struct Det {
 // The active axes are template arguments, so the Moore box and the SED
 // stencils unroll with their neighbour offsets formed once per thread; the
 // box is visited in the same (z, y, x) order as the reference's loops.
-template <int AX, int AY, int AZ>
+// BITS: instead of the alpha arrays, one 16-bit word per cell whose bit
+// v * 3 + d says alpha_d(v) >= 1; relax_NAD only asks whether the smallest
+// alpha around a cell is below 1, so the words give the same flags (for
+// finite states) with 2 bytes per cell instead of 48. Cells outside 2..N-3
+// are never written and read as 0, the reference's never-written alpha = 0.
+template <int AX, int AY, int AZ, bool BITS = false>
 __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, const double* __restrict__ Wo,
                                                 double* __restrict__ tr, double* __restrict__ alx, double* __restrict__ aly,
-                                                double* __restrict__ alz, FV s, Det a, Geo geo, uint32_t total) {
+                                                double* __restrict__ alz, uint16_t* __restrict__ abits, FV s, Det a,
+                                                Geo geo, uint32_t total) {
     uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
@@ -371,6 +377,7 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
                      (!AZ || (k >= 2 && k <= s.Nz - 3));
     const size_t pl = s.plane(), c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
     const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
+    uint32_t bits = 0;
     for (int v = 0; v < nvar; v++) {
         if (!((a.mask >> v) & 1)) continue;
         const double* wo = Wo + v * pl + c0;
@@ -422,18 +429,25 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
                 double dum = (u - wn[-2 * st]) / hm;
                 double d2u = (dup - dum) / (c[l + 1] - c[l - 1]);
                 double dv = 0.5 * d2u * (f[l + 1] - f[l]);
-                double* al = d == 0 ? alx : (d == 1 ? aly : alz);
-                al[v * pl + c0] = alpha_fn(dv, dum, du, dup);
+                const double al_v = alpha_fn(dv, dum, du, dup);
+                if (BITS) {
+                    if (al_v >= 1.0) bits |= 1u << (v * 3 + d);
+                } else {
+                    double* al = d == 0 ? alx : (d == 1 ? aly : alz);
+                    al[v * pl + c0] = al_v;
+                }
             }
         }
     }
+    if (BITS && a.use_sed && in2) abits[c0] = (uint16_t)bits;
 }
 
 // relax_NAD (src/trouble_detection.cpp:152-183) then PAD_criteria (:244-259), cells 1..N-2
-template <int AX, int AY, int AZ>
+template <int AX, int AY, int AZ, bool BITS = false>
 __global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn, double* __restrict__ tr,
                                                   const double* __restrict__ alx, const double* __restrict__ aly,
-                                                  const double* __restrict__ alz, FV s, Det a, uint32_t total) {
+                                                  const double* __restrict__ alz, const uint16_t* __restrict__ abits,
+                                                  FV s, Det a, uint32_t total) {
     uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     int k, j, i;
@@ -442,15 +456,34 @@ __global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn,
     const int nvar = s.nv - 1;
     const size_t pl = s.plane(), c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
     const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
+    // BITS: bit v*3+d of all_ge1 is set when alpha_d(v) >= 1 at the cell and
+    // both of its neighbours along d (each axis' bits from its own neighbours)
+    uint32_t all_ge1 = 0x7fffu;
+    if (BITS && a.use_sed) {
+        constexpr uint32_t kX = 0x1249u, kY = 0x2492u, kZ = 0x4924u;  // bits v*3+0, v*3+1, v*3+2
+        const uint32_t m0 = abits[c0];
+        uint32_t ax = kX, ay = kY, az = kZ;
+        if (AX) ax = (uint32_t)abits[c0 - 1] & m0 & (uint32_t)abits[c0 + 1] & kX;
+        if (AY) ay = (uint32_t)abits[c0 - sy] & m0 & (uint32_t)abits[c0 + sy] & kY;
+        if (AZ) az = (uint32_t)abits[c0 - sz] & m0 & (uint32_t)abits[c0 + sz] & kZ;
+        all_ge1 = ax | ay | az;
+    }
     for (int v = 0; v < nvar; v++) {
         if (!((a.mask >> v) & 1)) continue;
         const size_t o = v * pl + c0;
-        double alpha = 1;
-        if (AX) alpha = smin(smin(alx[o - 1], alx[o]), alx[o + 1]);
-        if (AY) alpha = smin(alpha, smin(smin(aly[o - sy], aly[o]), aly[o + sy]));
-        if (AZ) alpha = smin(alpha, smin(smin(alz[o - sz], alz[o]), alz[o + sz]));
+        bool small;  // the smallest alpha around the cell is below 1
+        if (BITS) {
+            const uint32_t need = (AX ? 1u << (v * 3) : 0u) | (AY ? 1u << (v * 3 + 1) : 0u) | (AZ ? 1u << (v * 3 + 2) : 0u);
+            small = (all_ge1 & need) != need;
+        } else {
+            double alpha = 1;
+            if (AX) alpha = smin(smin(alx[o - 1], alx[o]), alx[o + 1]);
+            if (AY) alpha = smin(alpha, smin(smin(aly[o - sy], aly[o]), aly[o + sy]));
+            if (AZ) alpha = smin(alpha, smin(smin(alz[o - sz], alz[o]), alz[o + sz]));
+            small = alpha < 1;
+        }
         double t = tr[o];
-        if (a.use_sed) tr[o] = alpha < 1 ? t : 0;
+        if (a.use_sed) tr[o] = small ? t : 0;
     }
     double t = 0;
     for (int v = 0; v < nvar; v++)
@@ -656,6 +689,150 @@ __global__ void __launch_bounds__(BS) k_fallback(const double* __restrict__ W, c
     if (t2 != 0.0) fb_face<2>(W, Fz, sz, s, g, k, j, i, t2, wR[2], dt, ay, az, gm);
 }
 
+// The same fallback on tiles of TX x TY x TZ cells (x active; AY, AZ the other
+// active axes): every cell that has a blended face, and the cells of the
+// low-side halo layer whose upper face a tile cell needs, is reconstructed
+// once; the upper-face states go through LDS to the cell above, so each
+// face's two states come from one reconstruction per cell instead of one per
+// face (the per-cell kernel above reconstructs a cell up to four times).
+// The arithmetic of each state and flux is the per-cell kernel's, so the
+// faces are the same bits. A tile with no nonzero theta in reach exits.
+template <int AY, int AZ>
+struct FbTile {
+    static constexpr int TX = AZ ? 16 : (AY ? 16 : 256), TY = AZ ? 4 : (AY ? 16 : 1), TZ = AZ ? 4 : 1;
+    static constexpr int NC = TX * TY * TZ;          // = BS
+    static constexpr int HX = TY * TZ, HY = AY ? TX * TZ : 0, HZ = AZ ? TX * TY : 0;  // halo cells per layer
+    static_assert(NC == BS, "one cell per thread");
+    static_assert(HX + HY + HZ <= BS, "one halo cell per thread");
+};
+template <int AY, int AZ>
+__global__ void __launch_bounds__(BS) k_fallback_tile(const double* __restrict__ W, const double* __restrict__ th,
+                                                      double* __restrict__ Fx, double* __restrict__ Fy,
+                                                      double* __restrict__ Fz, FV s, FV fx, FV fy, FV fz, Geo g,
+                                                      double dt, double gm, int bx, int by, int bz) {
+    using TL = FbTile<AY, AZ>;
+    constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ;
+    __shared__ double up[3][NVP][TL::NC];                    // upper-face states of the tile cells, per axis
+    __shared__ double uph[NVP][TL::HX + TL::HY + TL::HZ];    // upper-face state of each halo cell (its one axis)
+    const int tid = threadIdx.x;
+    const uint32_t b = xcd_block();
+    if (b >= (uint32_t)bx * by * bz) return;  // padding blocks of the XCD grid (uniform per block)
+    const int bi = b % bx, bj = (b / bx) % by, bk = b / (bx * by);
+    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+    const int i0 = 2 + bi * TX, j0 = AY ? 2 + bj * TY : 0, k0 = AZ ? 2 + bk * TZ : 0;
+    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+    // processed cells: 2..N-2 along each active axis
+    const bool valid = i <= s.Nx - 2 && (!AY || j <= s.Ny - 2) && (!AZ || k <= s.Nz - 2);
+    const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
+    const size_t c0 = ((size_t)k * s.Ny + j) * s.Nx + i;
+    double tc = 0, tlo[3] = {0, 0, 0}, tf[3] = {0, 0, 0};
+    if (valid) {
+        tc = th[c0];
+        tlo[0] = th[c0 - 1];
+        if (AY) tlo[1] = th[c0 - sy];
+        if (AZ) tlo[2] = th[c0 - sz];
+        tf[0] = smax(tc, tlo[0]);
+        if (AY) tf[1] = smax(tc, tlo[1]);
+        if (AZ) tf[2] = smax(tc, tlo[2]);
+    }
+    const bool lower = tf[0] != 0.0 || tf[1] != 0.0 || tf[2] != 0.0;
+    // the upper face along d feeds the tile cell above if that one is processed and blends it
+    bool upper = false;
+    double thi[3] = {0, 0, 0};
+    if (valid) {
+        if (tx + 1 < TX && i + 1 <= s.Nx - 2) { thi[0] = th[c0 + 1]; upper = upper || smax(thi[0], tc) != 0.0; }
+        if (AY && ty + 1 < TY && j + 1 <= s.Ny - 2) { thi[1] = th[c0 + sy]; upper = upper || smax(thi[1], tc) != 0.0; }
+        if (AZ && tz + 1 < TZ && k + 1 <= s.Nz - 2) { thi[2] = th[c0 + sz]; upper = upper || smax(thi[2], tc) != 0.0; }
+    }
+    if (!__syncthreads_or(lower)) return;  // no blended face in the tile
+    // halo cell of this thread: layer x (i0-1, j0+.., k0+..), then y, then z
+    int hd = -1, hi = 0, hj = 0, hk = 0, hslot = tid;
+    if (tid < TL::HX) {
+        hd = 0; hi = i0 - 1; hj = j0 + tid % TY; hk = k0 + tid / TY;
+    } else if (AY && tid < TL::HX + TL::HY) {
+        const int r = tid - TL::HX;
+        hd = 1; hi = i0 + r % TX; hj = j0 - 1; hk = k0 + r / TX;
+    } else if (AZ && tid < TL::HX + TL::HY + TL::HZ) {
+        const int r = tid - TL::HX - TL::HY;
+        hd = 2; hi = i0 + r % TX; hj = j0 + r / TX; hk = k0 - 1;
+    }
+    bool hneed = false;
+    if (hd >= 0) {  // the tile cell above the halo cell is processed and blends that face
+        const int ui = hi + (hd == 0), uj = hj + (hd == 1), uk = hk + (hd == 2);
+        if (ui <= s.Nx - 2 && (!AY || uj <= s.Ny - 2) && (!AZ || uk <= s.Nz - 2)) {
+            const size_t cu = ((size_t)uk * s.Ny + uj) * s.Nx + ui;
+            const size_t ch = ((size_t)hk * s.Ny + hj) * s.Nx + hi;
+            hneed = smax(th[cu], th[ch]) != 0.0;
+        }
+    }
+    double wR[3][NVP];  // this cell's lower-face states
+    if (valid && (lower || upper)) {
+        Recon rc;
+        recon(W, s, g, k, j, i, AY, AZ, gm, rc);
+        if (tf[0] != 0.0) face_state<0>(rc, g, k, j, i, dt, false, wR[0]);
+        if (AY && tf[1] != 0.0) face_state<1>(rc, g, k, j, i, dt, false, wR[1]);
+        if (AZ && tf[2] != 0.0) face_state<2>(rc, g, k, j, i, dt, false, wR[2]);
+        if (upper) {
+            double wu[NVP];
+            face_state<0>(rc, g, k, j, i, dt, true, wu);
+#pragma unroll
+            for (int v = 0; v < NVP; v++) up[0][v][tid] = wu[v];
+            if (AY) {
+                face_state<1>(rc, g, k, j, i, dt, true, wu);
+#pragma unroll
+                for (int v = 0; v < NVP; v++) up[1][v][tid] = wu[v];
+            }
+            if (AZ) {
+                face_state<2>(rc, g, k, j, i, dt, true, wu);
+#pragma unroll
+                for (int v = 0; v < NVP; v++) up[2][v][tid] = wu[v];
+            }
+        }
+    }
+    if (hneed) {
+        Recon rh;
+        recon(W, s, g, hk, hj, hi, AY, AZ, gm, rh);
+        double wu[NVP];
+        if (hd == 0) face_state<0>(rh, g, hk, hj, hi, dt, true, wu);
+        else if (hd == 1) face_state<1>(rh, g, hk, hj, hi, dt, true, wu);
+        else face_state<2>(rh, g, hk, hj, hi, dt, true, wu);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) uph[v][hslot] = wu[v];
+    }
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if ((d == 1 && !AY) || (d == 2 && !AZ) || tf[d] == 0.0) continue;
+        const int td = d == 0 ? tx : (d == 1 ? ty : tz);
+        const int stride = d == 0 ? 1 : (d == 1 ? TX : TX * TY);
+        const int hs = d == 0 ? ty + TY * tz : (d == 1 ? TL::HX + tx + TX * tz : TL::HX + TL::HY + tx + TX * ty);
+        double wL[NVP], uL[NVP], uR[NVP], fL[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) wL[v] = td > 0 ? up[d][v][tid - stride] : uph[v][hs];
+        cons(wL, uL, gm);
+        cons(wR[d], uR, gm);
+        if (d == 0) riemann_hllc<0>(uL, uR, fL, gm);
+        else if (d == 1) riemann_hllc<1>(uL, uR, fL, gm);
+        else riemann_hllc<2>(uL, uR, fL, gm);
+        double* F = d == 0 ? Fx : (d == 1 ? Fy : Fz);
+        const FV& sF = d == 0 ? fx : (d == 1 ? fy : fz);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            size_t o = sF.off(v, k, j, i);
+            double f = F[o];
+            f = f + tf[d] * (fL[v] - f);
+            F[o] = f;
+        }
+        for (int v = NVP; v < sF.nv; v++) {  // slot: HLLC slot taken as zero
+            size_t o = sF.off(v, k, j, i);
+            double f = F[o];
+            f = f + tf[d] * (0.0 - f);
+            F[o] = f;
+        }
+    }
+}
+
 // ===========================================================================
 static Geo geo_of(spdk_ctx* c) {
     Geo g;
@@ -711,6 +888,9 @@ int spdk_fv_face_integral(spdk_ctx* c, const double* Ffp, spdk_sd_shape s_fp, do
         case 4: SPDK_FACE_TILE(4); break;
         case 5: SPDK_FACE_TILE(5); break;
         case 6: SPDK_FACE_TILE(6); break;
+        case 7: SPDK_FACE_TILE(7); break;
+        case 8: SPDK_FACE_TILE(8); break;
+        case 9: SPDK_FACE_TILE(9); break;
         default: set_error("face_integral: degree"); return SPDK_E_SHAPE;
     }
 #undef SPDK_FACE_TILE
@@ -818,8 +998,9 @@ int spdk_fv_primitives(spdk_ctx* c, const double* U, double* W, spdk_fv_shape s)
     return SPDK_OK;
 }
 
-int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double* tr, double* alx, double* aly, double* alz,
-                         spdk_fv_shape s, int pad, int mask) {
+static int detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double* tr, double* alx, double* aly,
+                           double* alz, uint16_t* abits, spdk_fv_shape s, int pad, int mask) {
+    const bool bits = abits != nullptr;
     Det a;
     for (int d = 0; d < 3; d++) a.act[d] = c->prm.active[d];
     a.moore = c->prm.nad_moore;
@@ -830,8 +1011,14 @@ int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double
     a.tol = c->prm.nad_tol;
     uint32_t total = (uint32_t)FV(s).plane();
 #define SPDK_NAD(X, Y, Z)                                                                                       \
-    hipLaunchKernelGGL((k_nad_sed<X, Y, Z>), dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, alx,        \
-                       aly ? aly : alx, alz ? alz : alx, FV(s), a, geo_of(c), total)
+    do {                                                                                                        \
+        if (bits)                                                                                               \
+            hipLaunchKernelGGL((k_nad_sed<X, Y, Z, true>), dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, \
+                               nullptr, nullptr, nullptr, abits, FV(s), a, geo_of(c), total);                     \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_nad_sed<X, Y, Z, false>), dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, \
+                               alx, aly ? aly : alx, alz ? alz : alx, nullptr, FV(s), a, geo_of(c), total);       \
+    } while (0)
     switch ((a.act[0] ? 4 : 0) | (a.act[1] ? 2 : 0) | (a.act[2] ? 1 : 0)) {
         case 0: SPDK_NAD(0, 0, 0); break;
         case 1: SPDK_NAD(0, 0, 1); break;
@@ -856,13 +1043,43 @@ int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double
         case 6: hipLaunchKernelGGL((KER<1, 1, 0>), __VA_ARGS__); break;                                         \
         default: hipLaunchKernelGGL((KER<1, 1, 1>), __VA_ARGS__); break;                                        \
     }
+#define SPDK_AXES_B(KER, ...)                                                                                   \
+    switch ((a3[0] ? 4 : 0) | (a3[1] ? 2 : 0) | (a3[2] ? 1 : 0)) {                                              \
+        case 0: hipLaunchKernelGGL((KER<0, 0, 0, true>), __VA_ARGS__); break;                                   \
+        case 1: hipLaunchKernelGGL((KER<0, 0, 1, true>), __VA_ARGS__); break;                                   \
+        case 2: hipLaunchKernelGGL((KER<0, 1, 0, true>), __VA_ARGS__); break;                                   \
+        case 3: hipLaunchKernelGGL((KER<0, 1, 1, true>), __VA_ARGS__); break;                                   \
+        case 4: hipLaunchKernelGGL((KER<1, 0, 0, true>), __VA_ARGS__); break;                                   \
+        case 5: hipLaunchKernelGGL((KER<1, 0, 1, true>), __VA_ARGS__); break;                                   \
+        case 6: hipLaunchKernelGGL((KER<1, 1, 0, true>), __VA_ARGS__); break;                                   \
+        default: hipLaunchKernelGGL((KER<1, 1, 1, true>), __VA_ARGS__); break;                                  \
+    }
     {
         const int* a3 = a.act;
-        SPDK_AXES(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx, alz ? alz : alx,
-                  FV(s), a, t2)
+        if (bits) {
+            SPDK_AXES_B(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, nullptr, nullptr, nullptr,
+                        (const uint16_t*)abits, FV(s), a, t2)
+        } else {
+            SPDK_AXES(k_relax_pad, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, Wn, tr, alx, aly ? aly : alx,
+                      alz ? alz : alx, (const uint16_t*)nullptr, FV(s), a, t2)
+        }
     }
+#undef SPDK_AXES_B
     SPDK_LAUNCH_CHECK("relax_pad");
     return SPDK_OK;
+}
+
+int spdk_detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, double* tr, double* alx, double* aly, double* alz,
+                         spdk_fv_shape s, int pad, int mask) {
+    if (!c || !Wn || !Wo || !tr || !alx) { set_error("detect_troubles: null argument"); return SPDK_E_ARG; }
+    return detect_troubles(c, Wn, Wo, tr, alx, aly, alz, nullptr, s, pad, mask);
+}
+
+int spdk_detect_troubles_bits(spdk_ctx* c, const double* Wn, const double* Wo, double* tr, uint16_t* abits,
+                              spdk_fv_shape s, int pad, int mask) {
+    if (!c || !Wn || !Wo || !tr || !abits) { set_error("detect_troubles_bits: null argument"); return SPDK_E_ARG; }
+    if ((mask & ~0x1f) != 0) { set_error("detect_troubles_bits: mask covers variables 0..4"); return SPDK_E_ARG; }
+    return detect_troubles(c, Wn, Wo, tr, nullptr, nullptr, nullptr, abits, s, pad, mask);
 }
 
 int spdk_blend_theta(spdk_ctx* c, const double* tr, double* th_tmp, double* th, spdk_fv_shape s) {
@@ -885,6 +1102,25 @@ int spdk_blend_theta(spdk_ctx* c, const double* tr, double* th_tmp, double* th, 
 int spdk_muscl_hllc_blend(spdk_ctx* c, const double* W, const double* th, double* Fx, double* Fy, double* Fz,
                           spdk_fv_shape s, double dt_sub) {
     int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
+    if (ax && (ay || !az) && c->fb_tile) {  // active axes x, x-y or x-y-z: the tiled kernel
+        const int Mx = s.Nx - 3, My = ay ? s.Ny - 3 : 1, Mz = az ? s.Nz - 3 : 1;
+        if (Mx <= 0 || My <= 0 || Mz <= 0) return SPDK_OK;
+#define SPDK_FBT(AY_, AZ_)                                                                                     \
+    do {                                                                                                       \
+        using TL = FbTile<AY_, AZ_>;                                                                           \
+        const int bx = (Mx + TL::TX - 1) / TL::TX, by = (My + TL::TY - 1) / TL::TY, bz = (Mz + TL::TZ - 1) / TL::TZ; \
+        const unsigned nb = (unsigned)((size_t)bx * by * bz);                                                  \
+        hipLaunchKernelGGL((k_fallback_tile<AY_, AZ_>), dim3((nb + kXcd - 1) / kXcd * kXcd), dim3(BS), 0, c->stream, \
+                           W, th, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx, FV(s), fv_stag(s, 0, ax), fv_stag(s, 1, ay),       \
+                           fv_stag(s, 2, az), geo_of(c), dt_sub, c->prm.gamma, bx, by, bz);                    \
+    } while (0)
+        if (az) SPDK_FBT(1, 1);
+        else if (ay) SPDK_FBT(1, 0);
+        else SPDK_FBT(0, 0);
+#undef SPDK_FBT
+        SPDK_LAUNCH_CHECK("muscl_hllc_tile");
+        return SPDK_OK;
+    }
     uint32_t total = (uint32_t)((size_t)(s.Nz - 4 * az + az) * (s.Ny - 4 * ay + ay) * (s.Nx - 4 * ax + ax));
     hipLaunchKernelGGL(k_fallback, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, W, th, Fx, Fy ? Fy : Fx, Fz ? Fz : Fx,
                        FV(s), fv_stag(s, 0, ax), fv_stag(s, 1, ay), fv_stag(s, 2, az), geo_of(c), dt_sub, ax, ay, az,

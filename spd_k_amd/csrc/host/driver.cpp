@@ -63,6 +63,7 @@ spdk_ctx* make_context(const RunConfig& cfg, const Operators& op, const Axis* ax
     prm.gamma = cfg.gamma;
     prm.cfl = cfg.cfl;
     prm.nu = cfg.nu;
+    prm.beta = cfg.beta;
     prm.nad_tol = cfg.nad_tol;
     prm.nad_delta = cfg.nad_delta;
     prm.nad_moore = cfg.nad_moore;
@@ -140,7 +141,6 @@ struct HydroRun {
     double t = 0, dt = 0, Dt = 0;
     bool fused = true;
     bool fused_pred = false;  // fused predictor and flux evaluation ahead of the stage FV update (fallback runs)
-    void* ws_fl = nullptr;    // mixed predictor with fallback: face traces for the fused flux evaluation
     spdk_sd_shape s_sp{}, s_ader{}, s_fp[3]{};
     spdk_fv_shape s_fv{}, s_th{}, s_fvd[3]{};
     double *U_sp = nullptr, *W_sp = nullptr, *W_cv = nullptr, *U_cv = nullptr, *T_sweep = nullptr, *U_ader = nullptr,
@@ -148,7 +148,13 @@ struct HydroRun {
     double *U_fp[3] = {}, *F_fp[3] = {};
     double *U_new = nullptr, *W_new = nullptr, *W_old = nullptr, *troubles = nullptr, *theta = nullptr,
            *theta_tmp = nullptr;
-    double *F_fv[3] = {}, *alpha[3] = {}, *T_fp[3] = {};
+    double *F_fv[3] = {}, *T_fp[3] = {};
+    double* dU[3] = {};  // viscous gradients at the solution points (nu > 0), zeroed
+    uint16_t* abits = nullptr;  // SED coefficient words of the trouble detection (zeroed)
+    // sub-cell face fluxes of every ADER node, [t * 3 + d], when the flux
+    // evaluation forms the face integrals itself (fv_direct)
+    std::vector<double*> F_fvn;
+    bool fv_direct = false;
     void* ws = nullptr;
     std::vector<double> host;  // staging for outputs
     bool timing = false;
@@ -196,11 +202,9 @@ struct HydroRun {
         // with the fallback, the fused kernel runs the predictor and the last
         // flux evaluation, the stage kernels the FV update
         fused_pred = fused && cfg.fallback;
-        size_t fl_bytes = 0;  // mixed with fallback: the flux evaluation in the fused kernel where a variant exists
-        if (mixed && cfg.fallback) {
-            int Nq[3] = {X.n_total, Y.n_total, Z.n_total};
-            fl_bytes = spdk_fused_fluxes_workspace_bytes(ctx, Nq);
-        }
+        // with the fallback, the fused FP64 sweep and the mixed predictor's
+        // flux evaluation form the face integrals of every node themselves
+        fv_direct = fused_pred || (mixed && cfg.fallback);
         spdk_timing_enable(ctx, timing ? 1 : 0);
 
         s_sp = spdk_sd_shape{1, kNVAR, Z.n_total, Y.n_total, X.n_total, Z.n_sp, Y.n_sp, X.n_sp};
@@ -243,17 +247,15 @@ struct HydroRun {
         if ((!fused && (!mixed || cfg.fallback)) || fused_pred) {  // the stage arrays (with fallback, also behind
                                                                      // the mixed or fused predictor)
             U_cv = mem.alloc(ctx, nsp);
-            if (!fused_pred) U_ader = mem.alloc(ctx, sd_count(s_ader));
+            if (!fv_direct) U_ader = mem.alloc(ctx, sd_count(s_ader));
+            if (cfg.nu > 0.0)  // every axis: an inactive axis' gradient stays zero
+                for (int d = 0; d < 3; d++) dU[d] = mem.alloc(ctx, sd_count(s_ader));
             if (cfg.integrator == INT_RK) U0 = mem.alloc(ctx, nsp);
-            const bool stage_fluxes = !fused_pred && fl_bytes == 0;
+            const bool stage_fluxes = !fv_direct;
             for (int d = 0; d < 3; d++) {
                 if (!cfg.act[d]) continue;
                 if (stage_fluxes) U_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
-                if (d) F_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
-            }
-            if (fl_bytes) {
-                CK(spdk_malloc(&ws_fl, fl_bytes));
-                mem.ptrs.push_back(ws_fl);
+                if (d && !fv_direct) F_fp[d] = mem.alloc(ctx, sd_count(s_fp[d]));
             }
         }
         if (fused) {
@@ -276,12 +278,17 @@ struct HydroRun {
             troubles = mem.alloc(ctx, nfv);
             theta = mem.alloc(ctx, fv_count(s_th));
             theta_tmp = mem.alloc(ctx, fv_count(s_th));
+            if (fv_direct) F_fvn.assign((size_t)T * 3, nullptr);
+            abits = (uint16_t*)mem.alloc(ctx, (nfv / kNVAR * sizeof(uint16_t) + sizeof(double) - 1) / sizeof(double));
             for (int d = 0; d < 3; d++) {
                 s_fvd[d] = s_fv;
                 if (cfg.act[d]) (d == 0 ? s_fvd[d].Nx : d == 1 ? s_fvd[d].Ny : s_fvd[d].Nz) += 1;
                 if (!cfg.act[d]) continue;
+                if (fv_direct) {  // one face array per node (zeroed: variable 5 and the ghost faces stay 0)
+                    for (int a = 0; a < T; a++) F_fvn[(size_t)a * 3 + d] = mem.alloc(ctx, fv_count(s_fvd[d]));
+                    continue;
+                }
                 F_fv[d] = mem.alloc(ctx, fv_count(s_fvd[d]));
-                alpha[d] = mem.alloc(ctx, nfv);
                 spdk_sd_shape st = s_fp[d];
                 st.nt = 1;
                 T_fp[d] = mem.alloc(ctx, sd_count(st));
@@ -333,7 +340,8 @@ struct HydroRun {
         n_output++;
     }
 
-    void solve_fluxes() {  // src/hydro_ader.hpp:278-289 (inviscid)
+    void solve_fluxes() {  // src/hydro_ader.hpp:278-289, with Viscosity (:380-405) when nu > 0
+        const bool visc = cfg.nu > 0.0;
         for (int d = 0; d < 3; d++)
             if (cfg.act[d]) CK(spdk_transform_1d(ctx, U_ader, s_ader, U_fp[d], s_fp[d], SPDK_M_SP_TO_FP, d, -1));
         for (int d = 0; d < 3; d++)
@@ -341,7 +349,17 @@ struct HydroRun {
         for (int d = 0; d < 3; d++)
             if (cfg.act[d]) CK(spdk_fill_sd_traces(ctx, U_fp[d], s_fp[d], d));
         for (int d = 0; d < 3; d++)
-            if (cfg.act[d]) CK(spdk_riemann_llf(ctx, U_fp[d], F_fp[d], s_fp[d], d));
+            if (cfg.act[d]) CK(spdk_sd_riemann(ctx, U_fp[d], F_fp[d], s_fp[d], d, visc ? 1 : 0));
+        if (!visc) return;
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_sd_gradient(ctx, U_fp[d], s_fp[d], dU[d], s_ader, d));
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d])
+                CK(spdk_viscous_flux(ctx, U_fp[d], s_fp[d], dU[d], dU[(d + 1) % 3], dU[(d + 2) % 3], s_ader, d));
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_fill_sd_traces(ctx, U_fp[d], s_fp[d], d));
+        for (int d = 0; d < 3; d++)
+            if (cfg.act[d]) CK(spdk_sd_rusanov(ctx, U_fp[d], F_fp[d], s_fp[d], d));
     }
     void correct() {
         if (cfg.fallback) fv_update();
@@ -368,18 +386,23 @@ struct HydroRun {
         const int mask = (1 << 0) | (1 << 4);
         CK(spdk_transform(ctx, U_sp, U_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
         for (int a = 0; a < T; a++) {
-            for (int d = 0; d < 3; d++)
-                if (cfg.act[d]) CK(spdk_fv_face_integral(ctx, F_fp[d], s_fp[d], F_fv[d], s_fvd[d], T_fp[d], a, d));
+            double* Fv[3] = {F_fv[0], F_fv[1], F_fv[2]};
+            if (fv_direct) {  // the flux evaluation formed this node's face integrals
+                for (int d = 0; d < 3; d++) Fv[d] = F_fvn[(size_t)a * 3 + d];
+            } else {
+                for (int d = 0; d < 3; d++)
+                    if (cfg.act[d]) CK(spdk_fv_face_integral(ctx, F_fp[d], s_fp[d], F_fv[d], s_fvd[d], T_fp[d], a, d));
+            }
             // update = 0 and the primitives of U_new and U_old in one pass, ghosts filled on W
-            CK(spdk_fv_update_prims(ctx, U_new, W_new, W_old, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], s_fv, a, dt));
+            CK(spdk_fv_update_prims(ctx, U_new, W_new, W_old, U_cv, s_sp, Fv[0], Fv[1], Fv[2], s_fv, a, dt));
             CK(spdk_fill_fv_ghosts(ctx, W_old, s_fv));
             CK(spdk_fill_fv_ghosts(ctx, W_new, s_fv));
-            CK(spdk_detect_troubles(ctx, W_new, W_old, troubles, alpha[0], alpha[1], alpha[2], s_fv, 1, mask));
+            CK(spdk_detect_troubles_bits(ctx, W_new, W_old, troubles, abits, s_fv, 1, mask));
             CK(spdk_fill_fv_ghosts(ctx, troubles, s_fv));
             CK(spdk_blend_theta(ctx, troubles, theta_tmp, theta, s_fv));
             CK(spdk_fill_fv_ghosts(ctx, theta, s_th));
-            CK(spdk_muscl_hllc_blend(ctx, W_old, theta, F_fv[0], F_fv[1], F_fv[2], s_fv, op.wt[a] * dt));
-            CK(spdk_fv_update_blended(ctx, U_new, U_cv, s_sp, F_fv[0], F_fv[1], F_fv[2], theta, s_fv, a, dt));
+            CK(spdk_muscl_hllc_blend(ctx, W_old, theta, Fv[0], Fv[1], Fv[2], s_fv, op.wt[a] * dt));
+            CK(spdk_fv_update_blended(ctx, U_new, U_cv, s_sp, Fv[0], Fv[1], Fv[2], theta, s_fv, a, dt));
         }
         CK(spdk_transform(ctx, U_cv, U_sp, T_sweep, s_sp, SPDK_M_CV_TO_SP));
     }
@@ -394,16 +417,16 @@ struct HydroRun {
                                      ": the state has left the physical range (t = " + std::to_string(t) + ")");
         return next;
     }
+    // the x flux-point fluxes are formed only for a step whose output shows them
+    double* fx_out(bool output_due) { return cfg.out_flux && (output_due || cfg.final_output) ? F_fp[0] : nullptr; }
     double step_raw(bool output_due) {
         if (mixed && cfg.fallback) {
             // the predictor in mixed precision (fixed count or residual
             // test, zero-order or learned guess), then the last flux
             // evaluation and the FV update with trouble detection and the
             // MUSCL fallback in FP64 (src/hydro_ader.hpp:291-305, :500-586)
-            CK(spdk_mixed_predict(ctx, U_sp, s_sp, ws, dt, &pc, &pst));
-            CK(spdk_mixed_export_predictor(ctx, ws, U_ader, s_ader));
-            if (ws_fl) CK(spdk_fused_fluxes(ctx, U_ader, s_ader, ws_fl, F_fp[0], F_fp[1], F_fp[2]));
-            else solve_fluxes();
+            CK(spdk_mixed_predict_fv_fluxes(ctx, U_sp, s_sp, ws, dt, &pc, &pst, F_fvn.data(), s_fv, fx_out(output_due),
+                                            cfg.fv_valu ? 1 : 0));
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
             CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
@@ -412,7 +435,9 @@ struct HydroRun {
             return next;
         }
         if (fused_pred) {
-            CK(spdk_fused_predict_fluxes(ctx, U_sp, s_sp, ws, dt, F_fp[0], F_fp[1], F_fp[2], s_ader));
+            // the x flux-point fluxes only when an output will show them
+            CK(spdk_fused_predict_fv_fluxes(ctx, U_sp, s_sp, ws, dt, F_fvn.data(), s_fv,
+                                            fx_out(output_due), s_ader));
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
             CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
@@ -422,14 +447,13 @@ struct HydroRun {
         }
         if (mixed) {
             double next = 0;
-            CK(spdk_hydro_step_mixed(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr,
-                                     output_due ? F_fp[0] : nullptr, &pc, &pst, &next));
+            CK(spdk_hydro_step_mixed(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr, fx_out(output_due), &pc,
+                                     &pst, &next));
             return next;
         }
         if (fused) {
             double next = 0;
-            CK(spdk_hydro_step_fused(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr, output_due ? F_fp[0] : nullptr,
-                                     &next));
+            CK(spdk_hydro_step_fused(ctx, U_sp, s_sp, ws, dt, output_due ? W_cv : nullptr, fx_out(output_due), &next));
             return next;
         }
         if (cfg.integrator == INT_RK) rk_step();

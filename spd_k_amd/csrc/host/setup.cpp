@@ -137,6 +137,11 @@ std::string configure(Params& pin, RunConfig& c) {
     c.max_steps = pin.has("time/nlim") ? pin.get<int>("time/nlim") : -1;
     c.path = pin.has("job/path") ? pin.get<std::string>("job/path") : std::string("fused");
     if (c.path != "fused" && c.path != "stages") return "job/path must be fused or stages";
+    {
+        const std::string fc = pin.get_or<std::string>("job/fv_contraction", "mfma");
+        if (fc != "mfma" && fc != "valu") return "job/fv_contraction must be mfma or valu";
+        c.fv_valu = fc == "valu";
+    }
     c.timing = pin.has("job/timing") && pin.get<bool>("job/timing");
     if (pin.has("job/precision")) c.precision = pin.get<std::string>("job/precision");
     if (c.precision != "fp64" && c.precision != "mixed") return "job/precision must be fp64 or mixed";

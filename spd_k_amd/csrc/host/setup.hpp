@@ -52,6 +52,7 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     double loop_c[2] = {0.5, 0.5}, loop_R = 0.3, loop_A0 = 0.001;
     int max_steps = -1;      // time/nlim: stop after this many steps
     std::string path = "fused";  // job/path: fused | stages
+    bool fv_valu = false;        // job/fv_contraction=valu: mixed face integrals on the VALU instead of MFMA
     bool timing = false;         // job/timing: HIP-event timing of the dominant kernel
     std::string precision = "fp64";  // job/precision: fp64 | mixed (FP32 state and contractions)
     std::string picard = "fixed";    // job/picard: fixed (p predictor sweeps) | residual

@@ -143,6 +143,14 @@ struct MArgs {
     int path;       // 0: every slice is d_old (zero-order guess); 1: slices from D
     int tin_slices; // slices present in the readable trace buffers (1 or T)
     int out_pred;   // predictor (else corrector)
+    // FV face-flux sweep: sub-cell face fluxes of slice t along axis d (FV
+    // layout, FP64), the FV cell counts (ghost layers included) and ghost
+    // layers per axis, S x S of sp_to_cv in FP32 (row o1*N + o2, column
+    // ll*N + nn), and a switch to the VALU contraction
+    double* Ffv[kMaxTm][3];
+    int fvn[3], fgh[3];
+    const float* SS;
+    int fv_valu;
 };
 
 // ---- FP64 physics in the frame of the face normal: slot 1 is the normal
@@ -315,7 +323,7 @@ __device__ __forceinline__ void face_sources(const MArgs& a, int d, int ei, int 
 template <int N, int DIM>
 __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* Fslot, const double* X, int d, int q,
                                        int ts, int ei, int ej, int ek, size_t e, bool want_bal, double (&bal)[NVP],
-                                       double bw) {
+                                       double bw, bool lower_face_only = false) {
     using G = MT<N, DIM>;
     constexpr int M = G::M, NT = G::NT;
     const size_t E = (size_t)a.E;
@@ -398,6 +406,7 @@ __device__ __forceinline__ void pencil(const MArgs& a, const float* Dsl, float* 
     // physical flux exactly
 #pragma unroll
     for (int side = 0; side < 2; side++) {
+        if (side == 1 && lower_face_only) break;
         const int f = side ? M - 1 : 0;
         const bool raw = side ? (ed == Nd - 1) : (ed == 0);
         double own[NVP], fl[NVP], uL[NVP], uR[NVP];
@@ -543,12 +552,143 @@ __device__ __forceinline__ void cons_shift(double* du, const double* bal, double
     for (int v = 0; v < NVP; v++) du[v] += X_shift[v];
 }
 
-// CONS = false: a predictor sweep (out_pred set); CONS = true: the
-// corrector (face balance, conservative update, cv state and dt). Compiled
-// apart, the predictor needs fewer VGPRs (3 waves per SIMD at 3-D p=3
-// instead of 2).
-template <int N, int DIM, bool CONS>
+// face_integral (src/transforms.cpp:378-481) of the round's axis d for every
+// slice, from the flux deviations F - F(mu) in the flux slots (slot t =
+// slice t): each element writes the sub-cell faces of its cells' lower sides
+// along d (the right ghost along d the domain's last face), so every FV face
+// is written once and the FV update built on them conserves exactly. The
+// transverse contraction runs in FP32 on the deviations and F(mu) is added
+// in FP64. 3-D p=3: on MFMA, one wave per element (v_mfma_f32_16x16x4_f32:
+// A = S x S, 16 output pairs (o1, o2) by 16 transverse points (ll, nn);
+// B = 16 panels (slice, variable, face) per tile, 5 tiles); otherwise (or
+// with fv_valu) on the VALU, one output position per thread.
+template <int N, int DIM>
+__device__ __forceinline__ void mfv_faces(const MArgs& a, const float* Fs, const double* X, int d, int s, bool live,
+                                          int ei, int ej, int ek) {
+    using G = MT<N, DIM>;
+    constexpr int M = G::M, FSZ = G::FSZ, T = G::T, TPE = G::TPE;
+    const int ec[3] = {ei, ej, ek};
+    const int Nd = d == 0 ? a.Ne[0] : (d == 1 ? a.Ne[1] : a.Ne[2]);
+    // Along a periodic axis the domain's first and last faces are one face:
+    // the first active element writes both copies, so they hold the same
+    // bits (each element adds its own FP32-rounded deviations to its own mean
+    // flux) and the FV update's face sums telescope exactly.
+    const bool per = (d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2])) == SPDK_BC_PERIODIC;
+    bool own = live;
+#pragma unroll
+    for (int x = 0; x < DIM; x++)
+        own = own && ec[x] >= 1 && ec[x] <= (x == d ? (per ? a.Ne[x] - 2 : a.Ne[x] - 1) : a.Ne[x] - 2);
+    const bool last = ec[d] == Nd - 1;
+    const bool wrap = per && ec[d] == 1;  // also writes the last face
+    const int dim1 = d == 0 ? 1 : (d == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
+    const int nx = a.fvn[0] + (d == 0), ny = a.fvn[1] + (d == 1), nz = a.fvn[2] + (d == 2);
+    const size_t vstride = (size_t)nz * ny * nx;
+    // offset of output position (ox, oy, oz) of this element, or (wrapped) of
+    // the element at the end of the axis
+    auto fv_off = [&](int ox, int oy, int oz, bool wrapped = false) -> size_t {
+        const int ex = wrapped && d == 0 ? a.Ne[0] - 1 : ei, ey = wrapped && d == 1 ? a.Ne[1] - 1 : ej;
+        const int ez = wrapped && d == 2 ? a.Ne[2] - 1 : ek;
+        const int I = ox + a.fgh[0] + (ex - 1) * N;
+        const int J = DIM >= 2 ? oy + a.fgh[1] + (ey - 1) * N : 0;
+        const int K = DIM == 3 ? oz + a.fgh[2] + (ez - 1) * N : 0;
+        return ((size_t)K * ny + J) * nx + I;
+    };
+    // pencil of axis d through transverse points ll (along dim1) and nn (along dim2)
+    auto pq = [&](int ll, int nn) -> int { return DIM == 3 ? (d == 1 ? ll * N + nn : nn * N + ll) : (DIM == 2 ? (d == 0 ? ll : nn) : 0); };
+    if (N == 4 && DIM == 3 && TPE == 64 && !a.fv_valu) {
+        if (!own) return;  // one wave per element: the wave leaves together
+        const int lane = s;  // = the lane in the element's wave
+        const int li = lane & 15, lk = lane >> 4;
+        float av[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) av[ks] = a.SS[li * 16 + 4 * ks + lk];  // A[row li][k = 4 ks + lk]
+        double* base[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) base[t] = d == 0 ? a.Ffv[t][0] : (d == 1 ? a.Ffv[t][1] : a.Ffv[t][2]);
+        constexpr int NCOL = T * NVP * N;  // panels (t, v, f), f fastest
+#pragma unroll
+        for (int ct = 0; ct < NCOL / 16; ct++) {
+            const int col = ct * 16 + li;
+            const int t = col / (NVP * N), v = (col / N) % NVP, f = col % N;
+            const float* Fp = Fs + (t * NVP + v) * FSZ;
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                const int k = 4 * ks + lk, ll = k / N, nn = k % N;  // B[k][col]
+                const float b = Fp[fidx<N, DIM>(d, f, pq(ll, nn))];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks], b, acc, 0, 0, 0);
+            }
+            if (!last || f == 0) {
+                const double fm = X[NVP + d * NVP + v];
+                double* ob = (t == 0 ? base[0] : (t == 1 ? base[1] : (t == 2 ? base[2] : base[3]))) + (size_t)v * vstride;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int row = 4 * lk + i, o1 = row / N, o2 = row % N;  // D[row][col]
+                    int o[3];
+                    o[d] = f;
+                    o[dim1] = o1;
+                    o[dim2] = o2;
+                    const double val = fm + (double)acc[i];
+                    ob[fv_off(o[0], o[1], o[2])] = val;
+                    if (wrap && f == 0) ob[fv_off(o[0], o[1], o[2], true)] = val;
+                }
+            }
+        }
+        return;
+    }
+    // VALU: thread s -> output position (x fastest) of every slice and variable
+    if (!own || s >= TPE) return;
+    const int o[3] = {s % N, DIM >= 2 ? (s / N) % N : 0, DIM == 3 ? s / (N * N) : 0};
+    const int f = o[d];
+    if (last && f != 0) return;
+    const int o1 = o[dim1], o2 = o[dim2];
+    float w[N][N];  // S(o1, ll) S(o2, nn) of the active transverse axes
+#pragma unroll
+    for (int ll = 0; ll < N; ll++)
+#pragma unroll
+        for (int nn = 0; nn < N; nn++) {
+            const float s1 = a.S2C ? (float)a.S2C[o1 * N + ll] : 0.f, s2 = (float)a.S2C[o2 * N + nn];
+            w[ll][nn] = DIM == 3 ? s1 * s2 : 0.f;
+        }
+    const size_t off = fv_off(o[0], o[1], o[2]);
+    const bool also_last = wrap && f == 0;
+    const size_t off_last = also_last ? fv_off(o[0], o[1], o[2], true) : 0;
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        double* ob = d == 0 ? a.Ffv[t][0] : (d == 1 ? a.Ffv[t][1] : a.Ffv[t][2]);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            const float* Fp = Fs + (t * NVP + v) * FSZ;
+            float acc = 0.f;
+            if (DIM == 3) {
+#pragma unroll
+                for (int ll = 0; ll < N; ll++)
+#pragma unroll
+                    for (int nn = 0; nn < N; nn++) acc += Fp[fidx<N, DIM>(d, f, pq(ll, nn))] * w[ll][nn];
+            } else if (DIM == 2) {
+                const int oa = d == 0 ? o1 : o2;  // the one active transverse axis (y for x faces, x for y faces)
+#pragma unroll
+                for (int l = 0; l < N; l++) acc += Fp[fidx<N, DIM>(d, f, l)] * (float)a.S2C[oa * N + l];
+            } else {
+                acc = Fp[fidx<N, DIM>(d, f, 0)];
+            }
+            const double val = X[NVP + d * NVP + v] + (double)acc;
+            ob[(size_t)v * vstride + off] = val;
+            if (also_last) ob[(size_t)v * vstride + off_last] = val;
+        }
+    }
+    (void)M;
+}
+
+// MODE 0: a predictor sweep (out_pred set); MODE 1: the corrector (face
+// balance, conservative update, cv state and dt); MODE 2: the flux
+// evaluation of the predictor's slices with their face integrals written
+// for the FV update (mfv_faces), nothing else. Compiled apart, the predictor
+// needs fewer VGPRs (3 waves per SIMD at 3-D p=3 instead of 2).
+template <int N, int DIM, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_MSWEEP_WAVES))) k_msweep(MArgs a) {
+    constexpr bool CONS = MODE == 1;
     using G = MT<N, DIM>;
     constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T, FSZ = G::FSZ;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -627,7 +767,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
     }
     float dold[NVP];
 #pragma unroll
-    for (int v = 0; v < NVP; v++) dold[v] = act_e ? a.d_old[(size_t)v * slabE + eo] : 0.f;
+    for (int v = 0; v < NVP; v++) dold[v] = act_e && MODE != 2 ? a.d_old[(size_t)v * slabE + eo] : 0.f;
     if (act_e) {
         if (a.path) {
 #pragma unroll
@@ -667,12 +807,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
             const int slot = a.path ? t : d;
             const int ts = a.tin_slices > 1 ? t : 0;
             const double bw = (a.path ? pick(a.wdt, t) : wsum) * pencil_weight<N, DIM>(a, q) * pick(a.ih, d);
+            // the face integrals (MODE 2) read faces 0..M-2 only, unless the
+            // x flux points are written as well
             pencil<N, DIM>(a, Ds + t * NVP * PSZ, Fs + slot * NVP * FSZ, X, d, q, ts, ei, ej, ek, (size_t)e, cons,
-                           bal, bw);
+                           bal, bw, MODE == 2 && !a.F_x);
         }
         __syncthreads();
+        if (MODE == 2) mfv_faces<N, DIM>(a, Fs, X, r, s, act_e, ei, ej, ek);
         if (act_e) {
-            const int d0 = a.path ? r : 0, d1 = a.path ? r + 1 : DIM;
+            const int d0 = a.path ? r : 0, d1 = MODE == 2 ? d0 : (a.path ? r + 1 : DIM);
 #pragma unroll 1
             for (int d = d0; d < d1; d++) {
                 const int qdd = d == 0 ? qx : (d == 1 ? qy : qz), idd = d == 0 ? ii : (d == 1 ? jj : kk);
@@ -707,6 +850,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
         }
         __syncthreads();
     }
+    if (MODE == 2) return;
     if (!a.path)  // the zero-order guess has identical slices
 #pragma unroll
         for (int t = 1; t < T; t++)
@@ -1120,12 +1264,12 @@ __global__ void k_gscale(const double* __restrict__ mu, const float* __restrict_
 
 template <int N, int DIM>
 __global__ void k_gfeat(const float* __restrict__ d_old, const float* __restrict__ S, int Eint, int e0, int nrows,
-                        float* __restrict__ X, long ldx) {
+                        float* __restrict__ X, long ldx, const int* __restrict__ list) {
     constexpr int TPE = MT<N, DIM>::TPE, NF = (DIM + 2) * TPE;
     const long total = (long)nrows * (NF + kGExtra);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         const int row = (int)(i / (NF + kGExtra)), j = (int)(i % (NF + kGExtra));
-        const size_t e = (size_t)e0 + row;
+        const size_t e = list ? (size_t)list[e0 + row] : (size_t)e0 + row;
         const float* sc = S + (size_t)kGS * e;
         float x;
         if (j < NF) {
@@ -1161,7 +1305,8 @@ __global__ void k_gtarget(const float* __restrict__ D, const float* __restrict__
 // guess from network outputs: D(t, v, e, sp) for the elements [e0, e0 + nrows)
 template <int N, int DIM>
 __global__ void k_gdecode(const float* __restrict__ Y, long ldy, const float* __restrict__ d_old,
-                          const float* __restrict__ S, int Eint, int e0, int nrows, float* __restrict__ D) {
+                          const float* __restrict__ S, int Eint, int e0, int nrows, float* __restrict__ D,
+                          const int* __restrict__ list) {
     constexpr int TPE = MT<N, DIM>::TPE, NV = DIM + 2, T = MT<N, DIM>::T;
     const size_t E = (size_t)Eint, slabE = E * TPE;
     const long total = (long)T * NVP * nrows * TPE;
@@ -1170,7 +1315,7 @@ __global__ void k_gdecode(const float* __restrict__ Y, long ldy, const float* __
         const long r2 = i / TPE;
         const int row = (int)(r2 % nrows);
         const int tv = (int)(r2 / nrows), t = tv / NVP, v = tv % NVP;
-        const size_t e = (size_t)e0 + row;
+        const size_t e = list ? (size_t)list[e0 + row] : (size_t)e0 + row;
         const float d0 = d_old[(size_t)v * slabE + e * TPE + s];
         float dn = d0;
         const int vi = v == 4 ? DIM + 1 : v;
@@ -1181,6 +1326,22 @@ __global__ void k_gdecode(const float* __restrict__ Y, long ldy, const float* __
         if ((v == 4 || v <= DIM) && sc[6] > kAgate)
             dn = d0 + Y[(long)row * ldy + (t * NV + vi) * TPE + s] * (sc[v] * sc[7]);
         D[((size_t)t * NVP + v) * slabE + e * TPE + s] = dn;
+    }
+}
+
+// Elements whose amplitude is above the gate (the others keep the
+// zero-order guess): appended to list in any order; *counter must be zero.
+__global__ void k_glist(const float* __restrict__ S, int Eint, int* __restrict__ list, int* counter) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < Eint && S[(size_t)kGS * e + 6] > kAgate) list[atomicAdd(counter, 1)] = e;
+}
+__global__ void k_izero(int* p) { *p = 0; }
+// The zero-order guess in every slice of the predictor buffer.
+__global__ void k_gbroadcast(const float* __restrict__ d_old, float* __restrict__ D, int T, size_t slab) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slab; i += (size_t)gridDim.x * blockDim.x) {
+        const float x = d_old[i];  // slab = NVP * E * TPE
+        const size_t v = i / (slab / NVP), r = i % (slab / NVP);
+        for (int t = 0; t < T; t++) D[((size_t)t * NVP + v) * (slab / NVP) + r] = x;
     }
 }
 
@@ -1252,12 +1413,20 @@ void sp_weights(int n, double* w) {
 int ensure_mixed_mats(spdk_ctx* c) {
     if (c->mixed_mats) return SPDK_OK;
     const int n = c->n, m = c->m;
-    std::vector<float> h((size_t)m * n + 3 * (size_t)n * m, 0.f);
+    // sp_to_fp, dfp_to_sp / h_d per axis, then S x S of sp_to_cv (row o1 n + o2, column ll n + nn)
+    std::vector<float> h((size_t)m * n + 3 * (size_t)n * m + (size_t)n * n * n * n, 0.f);
     const std::vector<double>& s2f = c->mat_host[SPDK_M_SP_TO_FP];
     const std::vector<double>& dm = c->mat_host[SPDK_M_DFP_TO_SP];
+    const std::vector<double>& s2c = c->mat_host[SPDK_M_SP_TO_CV];
     for (int i = 0; i < m * n; i++) h[i] = (float)s2f[i];
     for (int d = 0; d < 3; d++)
         for (int i = 0; i < n * m; i++) h[(size_t)m * n + d * n * m + i] = (float)(dm[i] / c->prm.h[d]);
+    float* ss = h.data() + (size_t)m * n + 3 * (size_t)n * m;
+    for (int o1 = 0; o1 < n; o1++)
+        for (int o2 = 0; o2 < n; o2++)
+            for (int ll = 0; ll < n; ll++)
+                for (int nn = 0; nn < n; nn++)
+                    ss[(o1 * n + o2) * n * n + ll * n + nn] = (float)(s2c[o1 * n + ll] * s2c[o2 * n + nn]);
     void* p = nullptr;
     SPDK_CHECK(hipMalloc(&p, h.size() * sizeof(float)));
     c->owned.push_back(p);
@@ -1290,7 +1459,7 @@ int guess_data(spdk_ctx* c, void* ws, size_t E, double dt, int e0, int nrows, fl
     constexpr int TPE = MT<N, DIM>::TPE;
     if (X) {
         hipLaunchKernelGGL((k_gfeat<N, DIM>), dim3(gblocks((long)nrows * ((DIM + 2) * TPE + kGExtra))), dim3(256), 0,
-                           c->stream, d_old, (const float*)S, (int)E, e0, nrows, X, ldx);
+                           c->stream, d_old, (const float*)S, (int)E, e0, nrows, X, ldx, (const int*)nullptr);
         SPDK_LAUNCH_CHECK("k_gfeat");
     }
     if (Y) {
@@ -1321,31 +1490,64 @@ int mlp_guess(spdk_ctx* c, void* ws, size_t E, double dt) {
         c->guess_x = (float*)x;
         c->guess_y = (float*)y;
     }
+    if (c->guess_list_cap < E + 1) {
+        void* l = nullptr;
+        SPDK_CHECK(hipMalloc(&l, (E + 1) * sizeof(int)));
+        c->owned.push_back(l);
+        c->guess_list = (int*)l;
+        c->guess_list_cap = E + 1;
+    }
     MLayout L = mlayout<N, DIM>(E);
     char* W = (char*)ws;
     hipLaunchKernelGGL((k_gscale<N, DIM>), dim3((unsigned)((E + 255) / 256)), dim3(256), 0, c->stream,
                        (const double*)(W + L.mu), (const float*)(W + L.dold), (int)E, 0, (int)E, dt, hmin_of(c),
                        c->prm.gamma, (float*)(W + L.gsc));
     SPDK_LAUNCH_CHECK("k_gscale");
-    for (size_t e0 = 0; e0 < E; e0 += c->guess_cap) {
-        const int nrows = (int)std::min((size_t)c->guess_cap, E - e0);
+    // the network runs only for elements above the amplitude gate; every
+    // element first gets the zero-order guess
+    int* list = c->guess_list;
+    int* counter = c->guess_list + E;
+    hipLaunchKernelGGL(k_izero, dim3(1), dim3(1), 0, c->stream, counter);
+    SPDK_LAUNCH_CHECK("k_izero");
+    hipLaunchKernelGGL(k_glist, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, c->stream, (const float*)(W + L.gsc),
+                       (int)E, list, counter);
+    SPDK_LAUNCH_CHECK("k_glist");
+    const size_t slab = (size_t)NVP * E * TPE;
+    hipLaunchKernelGGL(k_gbroadcast, dim3(gblocks((long)slab)), dim3(256), 0, c->stream, (const float*)(W + L.dold),
+                       (float*)(W + L.D), MT<N, DIM>::T, slab);
+    SPDK_LAUNCH_CHECK("k_gbroadcast");
+    SPDK_CHECK(hipMemcpyAsync(c->red_host, counter, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    int nlist = 0;
+    std::memcpy(&nlist, c->red_host, sizeof(int));
+    for (int e0 = 0; e0 < nlist; e0 += c->guess_cap) {
+        const int nrows = std::min(c->guess_cap, nlist - e0);
         hipLaunchKernelGGL((k_gfeat<N, DIM>), dim3(gblocks((long)nrows * din)), dim3(256), 0, c->stream,
-                           (const float*)(W + L.dold), (const float*)(W + L.gsc), (int)E, (int)e0, nrows, c->guess_x,
-                           (long)din);
+                           (const float*)(W + L.dold), (const float*)(W + L.gsc), (int)E, e0, nrows, c->guess_x,
+                           (long)din, (const int*)list);
         SPDK_LAUNCH_CHECK("k_gfeat");
         int rc = spdk_mlp_forward(net, c->guess_x, din, nrows, c->guess_y, dout);
         if (rc) return rc;
         hipLaunchKernelGGL((k_gdecode<N, DIM>), dim3(gblocks((long)MT<N, DIM>::T * NVP * nrows * TPE)), dim3(256), 0,
                            c->stream, (const float*)c->guess_y, (long)dout, (const float*)(W + L.dold),
-                           (const float*)(W + L.gsc), (int)E, (int)e0, nrows, (float*)(W + L.D));
+                           (const float*)(W + L.gsc), (int)E, e0, nrows, (float*)(W + L.D), (const int*)list);
         SPDK_LAUNCH_CHECK("k_gdecode");
     }
     return SPDK_OK;
 }
 
+// Where the flux evaluation of the predictor writes its face integrals (the
+// FV update follows; spdk_mixed_predict_fv_fluxes).
+struct FvOut {
+    double* const* Ffv;  // [t * 3 + d]
+    int fvn[3], fgh[3];
+    int valu;
+};
+
 template <int N, int DIM>
 int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, double* W_cv, double* F_x,
-              const spdk_picard* pc, spdk_picard_stats* st, double* dt_next, bool predict_only) {
+              const spdk_picard* pc, spdk_picard_stats* st, double* dt_next, bool predict_only,
+              const FvOut* fvo = nullptr) {
     using G = MT<N, DIM>;
     constexpr int T = G::T;
     if (c->T != T) { set_error("mixed path: ADER with T = p+1 time nodes only"); return SPDK_E_CONFIG; }
@@ -1402,13 +1604,17 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     const size_t lds = G::LDS_BYTES;
 
     // nb blocks: every block, or the length of args.blist
+    // timing covers the Picard sweeps (predictor and corrector), not the
+    // face-integral flux evaluation
     auto sweep = [&](const MArgs& args, unsigned nb) -> int {
-        bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool;
+        bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool && !args.Ffv[0][0];
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
-        if (!args.out_pred || args.U_new)
-            hipLaunchKernelGGL((k_msweep<N, DIM, true>), dim3(nb), block, lds, c->stream, args);
+        if (args.Ffv[0][0])
+            hipLaunchKernelGGL((k_msweep<N, DIM, 2>), dim3(nb), block, lds, c->stream, args);
+        else if (!args.out_pred || args.U_new)
+            hipLaunchKernelGGL((k_msweep<N, DIM, 1>), dim3(nb), block, lds, c->stream, args);
         else
-            hipLaunchKernelGGL((k_msweep<N, DIM, false>), dim3(nb), block, lds, c->stream, args);
+            hipLaunchKernelGGL((k_msweep<N, DIM, 0>), dim3(nb), block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_msweep");
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used++][1], c->stream));
         return SPDK_OK;
@@ -1445,6 +1651,7 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     };
     // fixed count: p predictor sweeps from the zero-order guess, then the
     // corrector (left to the caller when predict_only)
+    int cur = 0;  // newest trace version
     auto fixed_count = [&]() -> int {
         for (int k = 0; k < (predict_only ? T - 1 : T); k++) {
             MArgs s = a;
@@ -1457,6 +1664,7 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
             s.W_cv = last ? W_cv : nullptr;
             s.F_x = last ? F_x : nullptr;
             s.sweep = k;
+            cur = k + 1;
             int r;
             if ((r = sweep(s, L.blocks))) return r;
         }
@@ -1478,7 +1686,6 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
             SPDK_LAUNCH_CHECK("k_mtrace");
         }
         bool guess_phase = external;
-        int cur = 0;                           // newest trace version
         int ksw = 0;                           // sweeps this step (histogram index)
         int kph = 0;                           // sweeps in this phase
         const unsigned char* flags = nullptr;  // unconverged after the last sweep (null: all)
@@ -1556,6 +1763,20 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         if ((rc = fixed_count())) return rc;
         sweeps += T;
     }
+    if (fvo) {  // the flux evaluation of the final slices with their face integrals, every element
+        MArgs sa = a;
+        sa.cur = ++cur;
+        sa.path = 1;
+        sa.tin_slices = T;
+        sa.out_pred = 0;
+        sa.F_x = F_x;
+        for (int t = 0; t < T; t++)
+            for (int d = 0; d < 3; d++) sa.Ffv[t][d] = fvo->Ffv[t * 3 + d];
+        for (int d = 0; d < 3; d++) { sa.fvn[d] = fvo->fvn[d]; sa.fgh[d] = fvo->fgh[d]; }
+        sa.SS = c->mixed_mats + (size_t)c->m * c->n + 3 * (size_t)c->n * c->m;
+        sa.fv_valu = fvo->valu;
+        if ((rc = sweep(sa, L.blocks))) return rc;
+    }
     if (!predict_only) {
         if ((rc = reduce(a.part, nullptr, nullptr, L.blocks, nullptr))) return rc;
         if (dt_next) *dt_next = c->red_host[0];
@@ -1564,13 +1785,16 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         st->steps++;
         st->sweeps += sweeps;
     }
-    for (int i = 0; i < c->ev_used; i++) {
-        float ms = 0;
-        SPDK_CHECK(hipEventElapsedTime(&ms, c->evs[i][0], c->evs[i][1]));
-        c->kernel_ms += ms;
-        c->kernel_launches++;
+    if (c->ev_used > 0) {  // harvest the sweep timings (a predictor-only call has not synchronised yet)
+        SPDK_CHECK(hipEventSynchronize(c->evs[c->ev_used - 1][1]));
+        for (int i = 0; i < c->ev_used; i++) {
+            float ms = 0;
+            SPDK_CHECK(hipEventElapsedTime(&ms, c->evs[i][0], c->evs[i][1]));
+            c->kernel_ms += ms;
+            c->kernel_launches++;
+        }
+        c->ev_used = 0;
     }
-    c->ev_used = 0;
     return SPDK_OK;
 }
 
@@ -1644,14 +1868,14 @@ int run_iterate(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt,
         s.tin_slices = k == 0 ? 1 : G::T;
         s.out_pred = 1;
         s.sweep = k;
-        hipLaunchKernelGGL((k_msweep<N, DIM, false>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, s);
+        hipLaunchKernelGGL((k_msweep<N, DIM, 0>), dim3(L.blocks), dim3(G::BLOCK), G::LDS_BYTES, c->stream, s);
         SPDK_LAUNCH_CHECK("k_msweep");
     }
     return SPDK_OK;
 }
 
 typedef int (*mstep_fn)(spdk_ctx*, double*, const int*, void*, double, double*, double*, const spdk_picard*,
-                        spdk_picard_stats*, double*, bool);
+                        spdk_picard_stats*, double*, bool, const FvOut*);
 typedef MLayout (*mlayout_fn)(size_t);
 typedef int (*mprep_fn)(spdk_ctx*, const double*, const int*, void*);
 typedef int (*miter_fn)(spdk_ctx*, double*, const int*, void*, double, int);
@@ -1842,7 +2066,34 @@ int spdk_hydro_step_mixed(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
     if (rc0 || (rc0 = check_picard(c, pc))) return rc0;
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);
-    return v->step(c, U_sp, Ne, ws, dt, W_cv, F_x, pc, stats, dt_next, false);
+    return v->step(c, U_sp, Ne, ws, dt, W_cv, F_x, pc, stats, dt_next, false, nullptr);
+}
+
+int spdk_mixed_predict_fv_fluxes(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt,
+                                 const spdk_picard* pc, spdk_picard_stats* stats, double* const* F_fv,
+                                 spdk_fv_shape sc, double* F_x, int valu) {
+    const MVariant* v = nullptr;
+    int rc0 = check_mixed(c, U_sp, s, ws, &v);
+    if (rc0 || (rc0 = check_picard(c, pc))) return rc0;
+    if (!F_fv) { set_error("mixed fv fluxes: null face arrays"); return SPDK_E_ARG; }
+    const int Ne[3] = {s.Nx, s.Ny, s.Nz}, nc[3] = {sc.Nx, sc.Ny, sc.Nz};
+    FvOut fo;
+    fo.Ffv = F_fv;
+    fo.valu = valu;
+    for (int d = 0; d < 3; d++) {
+        const bool act = c->prm.active[d] != 0;
+        const bool ok = act ? (c->ngh[d] == 1 && nc[d] == (Ne[d] - 2) * c->n + 2 * c->fgh[d]) : nc[d] == 1;
+        if (!ok || sc.nv != NV) {
+            set_error("mixed fv fluxes: FV cell shape disagrees with U_sp along axis " + std::to_string(d));
+            return SPDK_E_SHAPE;
+        }
+        for (int t = 0; t < c->T; t++)
+            if (act && !F_fv[t * 3 + d]) { set_error("mixed fv fluxes: null face array"); return SPDK_E_ARG; }
+        fo.fvn[d] = nc[d];
+        fo.fgh[d] = c->fgh[d];
+    }
+    std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);
+    return v->step(c, U_sp, Ne, ws, dt, nullptr, F_x, pc, stats, nullptr, true, &fo);
 }
 
 int spdk_mixed_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, const spdk_picard* pc,
@@ -1852,7 +2103,7 @@ int spdk_mixed_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, dou
     if (rc0 || (rc0 = check_picard(c, pc))) return rc0;
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_msweep", sizeof(c->kernel_name) - 1);
-    return v->step(c, U_sp, Ne, ws, dt, nullptr, nullptr, pc, stats, nullptr, true);
+    return v->step(c, U_sp, Ne, ws, dt, nullptr, nullptr, pc, stats, nullptr, true, nullptr);
 }
 
 }  // extern "C"

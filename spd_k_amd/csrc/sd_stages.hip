@@ -224,8 +224,8 @@ __global__ void __launch_bounds__(BS) k_sd_bc(double* __restrict__ U, SD s, int 
 // ---------------------------------------------------------------------------
 // sd_riemann_solver_t<D,...>: src/hydro.cpp:368-418
 template <int D>
-__global__ void __launch_bounds__(BS) k_riemann(const double* __restrict__ U, double* __restrict__ F, SD s, double gm,
-                                                uint32_t total) {
+__global__ void __launch_bounds__(BS) k_riemann(double* __restrict__ U, double* __restrict__ F, SD s, double gm,
+                                                int viscous, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
     if (idx >= total) return;
     int Nx = s.Nx - (D == 0), Ny = s.Ny - (D == 1), Nz = s.Nz - (D == 2);
@@ -256,6 +256,112 @@ __global__ void __launch_bounds__(BS) k_riemann(const double* __restrict__ U, do
             F[tb + NVP * slab + oL] = s5;
             F[tb + NVP * slab + oR] = s5;
         }
+        if (viscous) {  // riemann_wind(u_L, u_R, 0.5) into both face states (src/hydro.cpp:402-409)
+            for (int v = 0; v < s.nv; v++) {
+                const double a = U[tb + v * slab + oL], b = U[tb + v * slab + oR];
+                const double w = 0.5 * (b + a) + 0.5 * (b - a);
+                U[tb + v * slab + oL] = w;
+                U[tb + v * slab + oR] = w;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Viscous terms (src/hydro.cpp:420-555), active when nu > 0.
+// sd_rusanov_solver: riemann_wind(-0.5) of (right element's face value, left
+// element's) added to F on both sides of every element face along D.
+template <int D>
+__global__ void __launch_bounds__(BS) k_rusanov(const double* __restrict__ U, double* __restrict__ F, SD s,
+                                                uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int Nx = s.Nx - (D == 0), Ny = s.Ny - (D == 1), Nz = s.Nz - (D == 2);
+    int px = D == 0 ? 1 : s.nx, py = D == 1 ? 1 : s.ny, pz = D == 2 ? 1 : s.nz;
+    int k, j, i, kk, jj, ii;
+    split6(idx, Nz, Ny, Nx, pz, py, px, k, j, i, kk, jj, ii);
+    int n = D == 0 ? s.nx : (D == 1 ? s.ny : s.nz);
+    size_t oL, oR;
+    if (D == 0) { oL = s.off(0, 0, k, j, i, kk, jj, n - 1); oR = s.off(0, 0, k, j, i + 1, kk, jj, 0); }
+    else if (D == 1) { oL = s.off(0, 0, k, j, i, kk, n - 1, ii); oR = s.off(0, 0, k, j + 1, i, kk, 0, ii); }
+    else { oL = s.off(0, 0, k, j, i, n - 1, jj, ii); oR = s.off(0, 0, k + 1, j, i, 0, jj, ii); }
+    const size_t slab = s.slab();
+    for (int t = 0; t < s.nt; t++)
+        for (int v = 0; v < s.nv; v++) {
+            const size_t o = ((size_t)t * s.nv + v) * slab;
+            const double uL = U[o + oR], uR = U[o + oL];
+            const double f = 0.5 * (uR + uL) + (-0.5) * (uR - uL);
+            F[o + oL] += f;
+            F[o + oR] += f;
+        }
+}
+// compute_gradient: du = sum_ll U(ll) D(id, ll) / h over the flux points along
+// D, at the solution points of the active elements.
+template <int D>
+__global__ void __launch_bounds__(BS) k_gradient(const double* __restrict__ U, SD su, double* __restrict__ dU, SD sd,
+                                                 const double* __restrict__ Dm, int ldm, double h, int gx, int gy,
+                                                 int gz, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i, kk, jj, ii;
+    split6(idx, sd.Nz - 2 * gz, sd.Ny - 2 * gy, sd.Nx - 2 * gx, sd.nz, sd.ny, sd.nx, k, j, i, kk, jj, ii);
+    k += gz; j += gy; i += gx;
+    const int q = D == 0 ? sd.nx : (D == 1 ? sd.ny : sd.nz);
+    const int id = D == 0 ? ii : (D == 1 ? jj : kk);
+    for (int t = 0; t < sd.nt; t++)
+        for (int v = 0; v < sd.nv; v++) {
+            double du = 0;
+            for (int ll = 0; ll < q + 1; ll++) {
+                const double u = D == 0 ? U[su.off(t, v, k, j, i, kk, jj, ll)]
+                                        : (D == 1 ? U[su.off(t, v, k, j, i, kk, ll, ii)] : U[su.off(t, v, k, j, i, ll, jj, ii)]);
+                du += u * Dm[id * ldm + ll] / h;
+            }
+            dU[sd.off(t, v, k, j, i, kk, jj, ii)] = du;
+        }
+}
+// compute_viscous_flux with viscous_fluxes, at every flux point along D.
+template <int D>
+__global__ void __launch_bounds__(BS) k_viscous_flux(double* __restrict__ U, SD su, const double* __restrict__ dU1,
+                                                     const double* __restrict__ dU2, const double* __restrict__ dU3,
+                                                     SD sd, const double* __restrict__ S, int lds, double gm, double nu,
+                                                     double beta, uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    int k, j, i, kk, jj, ii;
+    split6(idx, su.Nz, su.Ny, su.Nx, su.nz, su.ny, su.nx, k, j, i, kk, jj, ii);
+    constexpr int v1 = 1 + D, v2 = 1 + (D + 1) % 3, v3 = 1 + (D + 2) % 3;
+    const int q = D == 0 ? su.nx : (D == 1 ? su.ny : su.nz);
+    const int id = D == 0 ? ii : (D == 1 ? jj : kk);
+    for (int t = 0; t < su.nt; t++) {
+        double u[NV] = {0, 0, 0, 0, 0, 0}, du1[NV], du2[NV], du3[NV];
+        for (int v = 0; v < su.nv && v < NV; v++) {
+            du1[v] = 0;
+            du2[v] = 0;
+            du3[v] = 0;
+            u[v] = U[su.off(t, v, k, j, i, kk, jj, ii)];
+            for (int ll = 0; ll < q - 1; ll++) {
+                const int a = D == 0 ? ll : ii, b = D == 1 ? ll : jj, c = D == 2 ? ll : kk;
+                const size_t o = sd.off(t, v, k, j, i, c, b, a);
+                const double sm = S[id * lds + ll];
+                du1[v] += dU1[o] * sm;
+                du2[v] += dU2[o] * sm;
+                du3[v] += dU3[o] * sm;
+            }
+        }
+        double w[NVP];
+        prim(u, w, gm);
+        double f[NV] = {0, 0, 0, 0, 0, 0};
+        f[0] = 0;
+        f[4] = 0;
+        f[v1] = 2 * nu * du1[v1] + beta * du1[v1];
+        f[v1] += beta * du2[v2];
+        f[v2] = nu * (du1[v2] + du2[v1]);
+        f[4] += w[v2] * f[v2];
+        f[v1] += beta * du3[v3];
+        f[v3] = nu * (du1[v3] + du3[v1]);
+        f[4] += w[v3] * f[v3];
+        f[4] += w[v1] * f[v1];
+        for (int v = 0; v < su.nv && v < NV; v++) U[su.off(t, v, k, j, i, kk, jj, ii)] = f[v];
     }
 }
 
@@ -582,7 +688,10 @@ int spdk_transform_1d(spdk_ctx* c, const double* A, spdk_sd_shape sa, double* B,
         case 5: hipLaunchKernelGGL(k_transform_1d<5>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
         case 6: hipLaunchKernelGGL(k_transform_1d<6>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
         case 7: hipLaunchKernelGGL(k_transform_1d<7>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
-        default: set_error("transform: more than 7 points along the axis"); return SPDK_E_SHAPE;
+        case 8: hipLaunchKernelGGL(k_transform_1d<8>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 9: hipLaunchKernelGGL(k_transform_1d<9>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        case 10: hipLaunchKernelGGL(k_transform_1d<10>, g, dim3(BS), 0, c->stream, A, SD(sa), B, SD(sb), c->mat[mat], c->mcols[mat], dim, t_src, total); break;
+        default: set_error("transform: more than 10 points along the axis"); return SPDK_E_SHAPE;
     }
     SPDK_LAUNCH_CHECK("transform_1d");
     return SPDK_OK;
@@ -657,18 +766,94 @@ int spdk_fill_sd_traces(spdk_ctx* c, double* U, spdk_sd_shape s, int dim) {
     return SPDK_OK;
 }
 
-int spdk_riemann_llf(spdk_ctx* c, const double* U, double* F, spdk_sd_shape s, int dim) {
+int spdk_sd_riemann(spdk_ctx* c, double* U, double* F, spdk_sd_shape s, int dim, int viscous) {
     REQUIRE_SHAPE(s);
-    if (s.nv < NVP || dim < 0 || dim > 2) return SPDK_E_ARG;
+    if (s.nv < NVP || dim < 0 || dim > 2 || !U || !F) return SPDK_E_ARG;
     int Nx = s.Nx - (dim == 0), Ny = s.Ny - (dim == 1), Nz = s.Nz - (dim == 2);
     int px = dim == 0 ? 1 : s.nx, py = dim == 1 ? 1 : s.ny, pz = dim == 2 ? 1 : s.nz;
     uint32_t total = (uint32_t)((size_t)Nz * Ny * Nx * pz * py * px);
     if (total == 0) return SPDK_OK;
     dim3 g(nblocks(total, BS));
-    if (dim == 0) hipLaunchKernelGGL(k_riemann<0>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
-    if (dim == 1) hipLaunchKernelGGL(k_riemann<1>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
-    if (dim == 2) hipLaunchKernelGGL(k_riemann<2>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, total);
+    const int vis = viscous ? 1 : 0;
+    if (dim == 0) hipLaunchKernelGGL(k_riemann<0>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, vis, total);
+    if (dim == 1) hipLaunchKernelGGL(k_riemann<1>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, vis, total);
+    if (dim == 2) hipLaunchKernelGGL(k_riemann<2>, g, dim3(BS), 0, c->stream, U, F, SD(s), c->prm.gamma, vis, total);
     SPDK_LAUNCH_CHECK("riemann_llf");
+    return SPDK_OK;
+}
+
+int spdk_riemann_llf(spdk_ctx* c, const double* U, double* F, spdk_sd_shape s, int dim) {
+    // the inviscid solver never writes U
+    return spdk_sd_riemann(c, const_cast<double*>(U), F, s, dim, 0);
+}
+
+int spdk_sd_rusanov(spdk_ctx* c, const double* U, double* F, spdk_sd_shape s, int dim) {
+    REQUIRE_SHAPE(s);
+    if (dim < 0 || dim > 2 || !U || !F) return SPDK_E_ARG;
+    int Nx = s.Nx - (dim == 0), Ny = s.Ny - (dim == 1), Nz = s.Nz - (dim == 2);
+    int px = dim == 0 ? 1 : s.nx, py = dim == 1 ? 1 : s.ny, pz = dim == 2 ? 1 : s.nz;
+    uint32_t total = (uint32_t)((size_t)Nz * Ny * Nx * pz * py * px);
+    if (total == 0) return SPDK_OK;
+    dim3 g(nblocks(total, BS));
+    if (dim == 0) hipLaunchKernelGGL(k_rusanov<0>, g, dim3(BS), 0, c->stream, U, F, SD(s), total);
+    if (dim == 1) hipLaunchKernelGGL(k_rusanov<1>, g, dim3(BS), 0, c->stream, U, F, SD(s), total);
+    if (dim == 2) hipLaunchKernelGGL(k_rusanov<2>, g, dim3(BS), 0, c->stream, U, F, SD(s), total);
+    SPDK_LAUNCH_CHECK("sd_rusanov");
+    return SPDK_OK;
+}
+
+// The flux-point array of axis dim and a solution-point array of the same
+// mesh and slices: the same element counts, one more point along dim.
+static bool fp_sp_pair(const spdk_sd_shape& f, const spdk_sd_shape& s, int dim) {
+    const int nf[3] = {f.nx, f.ny, f.nz}, ns[3] = {s.nx, s.ny, s.nz};
+    if (f.nt != s.nt || f.nv != s.nv || f.Nx != s.Nx || f.Ny != s.Ny || f.Nz != s.Nz) return false;
+    for (int d = 0; d < 3; d++)
+        if (nf[d] != (d == dim ? ns[d] + 1 : ns[d])) return false;
+    return true;
+}
+
+int spdk_sd_gradient(spdk_ctx* c, const double* U, spdk_sd_shape s_fp, double* dU, spdk_sd_shape s_sp, int dim) {
+    REQUIRE_SHAPE(s_fp);
+    REQUIRE_SHAPE(s_sp);
+    if (dim < 0 || dim > 2 || !U || !dU || !c->prm.active[dim]) return SPDK_E_ARG;
+    if (!fp_sp_pair(s_fp, s_sp, dim) || c->mcols[SPDK_M_DFP_TO_SP] != (dim == 0 ? s_fp.nx : (dim == 1 ? s_fp.ny : s_fp.nz))) {
+        set_error("sd_gradient: flux-point / solution-point shapes");
+        return SPDK_E_SHAPE;
+    }
+    int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    uint32_t total = (uint32_t)((size_t)(s_sp.Nz - 2 * gz) * (s_sp.Ny - 2 * gy) * (s_sp.Nx - 2 * gx) * s_sp.nz * s_sp.ny *
+                                s_sp.nx);
+    if (total == 0) return SPDK_OK;
+    dim3 g(nblocks(total, BS));
+    const double h = c->prm.h[dim];
+    const double* Dm = c->mat[SPDK_M_DFP_TO_SP];
+    const int ld = c->mcols[SPDK_M_DFP_TO_SP];
+    if (dim == 0) hipLaunchKernelGGL(k_gradient<0>, g, dim3(BS), 0, c->stream, U, SD(s_fp), dU, SD(s_sp), Dm, ld, h, gx, gy, gz, total);
+    if (dim == 1) hipLaunchKernelGGL(k_gradient<1>, g, dim3(BS), 0, c->stream, U, SD(s_fp), dU, SD(s_sp), Dm, ld, h, gx, gy, gz, total);
+    if (dim == 2) hipLaunchKernelGGL(k_gradient<2>, g, dim3(BS), 0, c->stream, U, SD(s_fp), dU, SD(s_sp), Dm, ld, h, gx, gy, gz, total);
+    SPDK_LAUNCH_CHECK("sd_gradient");
+    return SPDK_OK;
+}
+
+int spdk_viscous_flux(spdk_ctx* c, double* U, spdk_sd_shape s_fp, const double* dU1, const double* dU2, const double* dU3,
+                      spdk_sd_shape s_sp, int dim) {
+    REQUIRE_SHAPE(s_fp);
+    REQUIRE_SHAPE(s_sp);
+    if (dim < 0 || dim > 2 || !U || !dU1 || !dU2 || !dU3 || s_fp.nv < NVP || s_fp.nv > NV) return SPDK_E_ARG;
+    if (!fp_sp_pair(s_fp, s_sp, dim) || c->mcols[SPDK_M_SP_TO_FP] + 1 != (dim == 0 ? s_fp.nx : (dim == 1 ? s_fp.ny : s_fp.nz))) {
+        set_error("viscous_flux: flux-point / solution-point shapes");
+        return SPDK_E_SHAPE;
+    }
+    uint32_t total = (uint32_t)SD(s_fp).total_cells();
+    if (total == 0) return SPDK_OK;
+    dim3 g(nblocks(total, BS));
+    const double* S = c->mat[SPDK_M_SP_TO_FP];
+    const int ld = c->mcols[SPDK_M_SP_TO_FP];
+    const double gm = c->prm.gamma, nu = c->prm.nu, beta = c->prm.beta;
+    if (dim == 0) hipLaunchKernelGGL(k_viscous_flux<0>, g, dim3(BS), 0, c->stream, U, SD(s_fp), dU1, dU2, dU3, SD(s_sp), S, ld, gm, nu, beta, total);
+    if (dim == 1) hipLaunchKernelGGL(k_viscous_flux<1>, g, dim3(BS), 0, c->stream, U, SD(s_fp), dU1, dU2, dU3, SD(s_sp), S, ld, gm, nu, beta, total);
+    if (dim == 2) hipLaunchKernelGGL(k_viscous_flux<2>, g, dim3(BS), 0, c->stream, U, SD(s_fp), dU1, dU2, dU3, SD(s_sp), S, ld, gm, nu, beta, total);
+    SPDK_LAUNCH_CHECK("viscous_flux");
     return SPDK_OK;
 }
 

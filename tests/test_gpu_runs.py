@@ -79,6 +79,22 @@ CASES = {
                             "fused"),
     "rectangular2d": (["-i", S, "mesh/nx1=8", "mesh/nx2=4", "mesh/nx3=1", "mesh/x1len=2.0", "mesh/x2len=0.5",
                        "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    # p = 7 in 3-D, the zone-cycles sweep's high degree (stage kernels: no tile variant at 8 points per axis)
+    "sine3d_p7_rk3": (["-i", S, "mesh/p=7", "mesh/nx1=3", "mesh/nx2=3", "mesh/nx3=2", "time/integrator=rk3",
+                       "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    "sine3d_p7_rk3_fallback": (["-i", S, "mesh/p=7", "mesh/nx1=3", "mesh/nx2=2", "mesh/nx3=3", "job/fallback=true",
+                                "time/integrator=rk3", "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    "sine3d_p7_ader": (["-i", S, "mesh/p=7", "mesh/nx1=2", "mesh/nx2=3", "mesh/nx3=2", "time/tlim=0.004",
+                        "output/dt=0.002"], "fused"),
+    # viscous terms (hydro/nu > 0: the stage sequence with compute_gradient, compute_viscous_flux, Rusanov)
+    "user1d_viscous": (["-i", os.path.join(op.INPUTS, "user.athinput"), "mesh/nx1=16", "mesh/nx2=1", "mesh/nx3=1",
+                        "job/fallback=false", "hydro/nu=0.005", "problem/amp=0.5", "problem/sigma=0.1",
+                        "problem/radius=0.5", "problem/v1=1.0", "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    "user2d_viscous_rk3_fallback": (["-i", os.path.join(op.INPUTS, "user.athinput"), "mesh/nx1=8", "mesh/nx2=6",
+                                     "hydro/nu=0.002", "hydro/beta=-0.001", "time/integrator=rk3",
+                                     "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    "kh3d_viscous_p2": (["-i", KH, "mesh/p=2", "mesh/nx1=4", "mesh/nx2=4", "mesh/nx3=3", "job/fallback=false",
+                         "hydro/nu=0.001", "time/tlim=0.006", "output/dt=0.003"], "fused"),
     # p = 1 in 3-D: more pencils (3 * 4) than points per element (8)
     "sine3d_p1": (["-i", S, "mesh/p=1", "mesh/nx1=6", "mesh/nx2=5", "mesh/nx3=4", "time/tlim=0.02", "output/dt=0.01"],
                   "fused"),
@@ -426,8 +442,12 @@ def test_mixed_predictor_with_fallback(tmp_path, name, form):
     residual test, or residual test from the learned guess) feeding the FP64
     flux evaluation and FV update with NAD + MUSCL/HLLC blending: mass
     conserved to round-off, every variable within 1e-4 of the largest value of
-    the FP64 run, troubled-cell masks differing in under 1% of the cells (a
-    cell at the NAD threshold may flip)."""
+    the FP64 run with the fixed count (2e-3 with the residual test: its
+    predictor is iterated to the tolerance, not for p sweeps, so it differs
+    from the fixed-count FP64 run by the fixed count's own iteration error,
+    largest relative to the 0.01-amplitude transverse velocity of the
+    per-point noise), troubled-cell masks differing in under 1% of the cells
+    (a cell at the NAD threshold may flip)."""
     if form == "learned" and name != "blast3d":
         pytest.skip("the shipped network is the 3-D p=3 one")
     a, b = str(tmp_path / "fp64"), str(tmp_path / "mixed")
@@ -451,7 +471,7 @@ def test_mixed_predictor_with_fallback(tmp_path, name, form):
     mb = np.fromfile(os.path.join(b, f"troubles_N{n}_{i}_0.dat"))
     flips = float(((ma != 0) != (mb != 0)).mean())
     print(f"{name}: max rel diff {worst:.3e}, mask flips {flips:.2e}, flagged {float((ma != 0).mean()):.3e}")
-    assert worst < 1e-4
+    assert worst < (1e-4 if form == "fixed" else 2e-3)
     assert flips < 1e-2
 
 
@@ -497,3 +517,81 @@ def test_output_fields_and_progress(tmp_path):
     with pytest.raises(RuntimeError):
         op.run_driver(args + ["output/fields=W_cv,bogus"], str(tmp_path / "bad"))
 
+
+
+# ---- the configured forms at their full sizes
+@pytest.mark.parametrize("form", ["fixed", "residual"])
+def test_kh3d_96_mixed_with_fallback(tmp_path, form):
+    """3-D KH, 96^3 elements, p=3, seeded perturbation, as configured:
+    transforms and predictor contractions in FP32, Riemann, NAD and the
+    conservative update in FP64 (the FV update with NAD + MUSCL/HLLC
+    blending behind the mixed predictor), fixed count and residual Picard.
+    Three steps against the FP64 run: the same total mass to round-off, the state
+    finite, every primitive variable within FP32-level distance of the FP64
+    run on average (mean relative difference < 1e-6; 1e-5 for the residual
+    test, whose predictor differs from the fixed count's by that count's
+    iteration error, largest on the noise) and at most 2e-3 of its
+    largest value at any point (a cell at the NAD threshold may flip and take
+    the other reconstruction; the transverse velocities' scale is the
+    0.01 noise amplitude), and the troubled cells of the last step differing
+    in under 1e-3 of the cells."""
+    base = ["-i", KH, "job/fallback=true", "mesh/nx1=96", "mesh/nx2=96", "mesh/nx3=96", "problem/perturb=random",
+            "problem/amp=0.01", "time/tlim=1", "time/nlim=3", "output/dt=-1", "job/final_output=true",
+            "output/fields=W_cv,troubles"]
+    a, b = str(tmp_path / "fp64"), str(tmp_path / "mixed")
+    try:
+        op.run_driver(base, a, timeout=900)
+        out = op.run_driver(base + ["job/precision=mixed"] + PICARD_FORMS[form], b, timeout=900)
+        print(re.findall(r"evolution: .*|picard: .*", out))
+        g = op.grid(a)
+        ia, ib = op.output_indices(a), op.output_indices(b)
+        assert ia == ib == [0], (ia, ib)  # the final state only
+        i = 0
+        # both runs conserve mass, so their totals agree to round-off
+        m0, m1 = op.total_mass(a, i), op.total_mass(b, i)
+        assert abs(m1 - m0) / abs(m0) < 1e-12
+        for v in range(5):
+            x, y = op.load_sd(a, g, i, v), op.load_sd(b, g, i, v)
+            assert np.isfinite(y).all()
+            scale_mean = max(float(np.abs(x).mean()), 1e-300)
+            scale_max = max(float(np.abs(x).max()), 1e-300)
+            dm = float(np.abs(x - y).mean()) / scale_mean
+            dx = float(np.abs(x - y).max()) / scale_max
+            print(f"var {v}: mean rel {dm:.3e} max rel {dx:.3e}")
+            assert dm < (1e-6 if form == "fixed" else 1e-5) and dx < 2e-3, (v, dm, dx)
+        n = 4 * g["N"][0] + 4
+        ma = np.fromfile(os.path.join(a, f"troubles_N{n}_{i}_0.dat"))
+        mb = np.fromfile(os.path.join(b, f"troubles_N{n}_{i}_0.dat"))
+        flips = float(((ma != 0) != (mb != 0)).mean())
+        print(f"flagged {float((ma != 0).mean()):.3e}, flips {flips:.3e}")
+        assert flips < 1e-3
+    finally:  # 2.9 GB per field file
+        shutil.rmtree(a, ignore_errors=True)
+        shutil.rmtree(b, ignore_errors=True)
+
+
+def test_induction_2d_full_size_mixed(tmp_path):
+    """The 2-D field-loop configuration in its FP32 form at its full size
+    (1024^2 elements, p=3, v = (1, 0.5)): six ADER steps of the mixed form
+    against six of the FP64 form. div B stays at round-off (< 1e-11) and the
+    magnetic energy stays within 2e-5 relative of the FP64 run's."""
+    args = LOOP2D + ["mesh/nx1=1024", "mesh/nx2=1024", "induction/nu=0", "time/tlim=1", "time/nlim=6", "output/dt=-1",
+                     "job/final_output=true"]
+    a, b = str(tmp_path / "fp64"), str(tmp_path / "mixed")
+    ta = op.run_driver(args, a, timeout=600)
+    tb = op.run_driver(args + ["job/precision=mixed"], b, timeout=600)
+    assert "induction path: fused mixed" in tb
+    da, db = _diag(ta), _diag(tb)
+    print(da, db)
+    assert len(da) == len(db) == 2
+    for (t1, m1, e1), (t2, m2, e2) in zip(da, db):
+        assert t1 == t2
+        assert m2 < 1e-11, m2
+        assert abs(e1 - e2) <= 2e-5 * e1, (e1, e2)
+    assert re.search(r"evolution: 6 steps", tb)
+    i = op.output_indices(a, "B2_cv")[-1]
+    x, y = _b2(a, i), _b2(b, i)
+    assert np.isfinite(y).all()
+    rel = float(np.abs(x - y).max() / np.abs(x).max())
+    print(f"max |dB| / max |B| = {rel:.3e}")
+    assert rel < 2e-5

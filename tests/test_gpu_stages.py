@@ -249,6 +249,13 @@ def test_fv_fallback_stages(act, N, opts):
     c.detect_troubles(dev(Wn), dev(Wo), tr_g, al[0], al[1], al[2], 1, 0b10001)
     assert np.array_equal(host(tr_g), tr_o)
     assert tr_o[5].sum() > 0
+    # the same troubles with the SED coefficients as bit words, twice in a
+    # row on one word array (the driver's use: words persist across calls)
+    tr_b = torch.zeros(s_fv, dtype=torch.float64, device="cuda")
+    abits = torch.zeros(s_fv[1:], dtype=torch.int16, device="cuda")
+    for _ in range(2):
+        c.detect_troubles_bits(dev(Wn), dev(Wo), tr_b, abits, 1, 0b10001)
+        assert np.array_equal(host(tr_b), tr_o)
     # FV ghosts of the flags, then theta
     op.orc().orc_fv_bc(ctypes.byref(cfg), op.dp(tr_o), op.ip(s_fv))
     c.fv_boundaries(tr_g)
@@ -337,3 +344,58 @@ def test_fv_update_prims(act, N, bc):
         assert np.array_equal(np.signbit(a), np.signbit(b)), name
     c.close()
 
+
+
+@pytest.mark.parametrize("act,N", [((1, 0, 0), (5, 1, 1)), ((1, 1, 0), (4, 3, 1)), ((1, 1, 1), (3, 3, 2))])
+@pytest.mark.parametrize("p", [1, 3])
+def test_viscous_stages(act, N, p):
+    """The viscous stage entries against the oracle on the same inputs, bit
+    for bit: sd_riemann_solver with the viscous flag (F and the replaced
+    face states), compute_gradient, compute_viscous_flux (nu, beta) and
+    sd_rusanov_solver, every axis (src/hydro.cpp:368-555)."""
+    rng = np.random.default_rng(700 + 10 * p + sum(act))
+    nu, beta = 0.003, -0.0017
+    c, m, _, _ = ctx_for(p, act, N=N, nu=nu, beta=beta)
+    n, T = p + 1, p + 1
+    cfg = op.OrcCfg.make(act=act)
+    L = op.orc()
+    L.orc_gradient.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_viscous_flux.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    h = [1.0 / N[d] if act[d] else 1.0 for d in range(3)]
+    s_sp = sd_shape(T, 6, N, n, act)
+    dU_o = [np.zeros(s_sp) for _ in range(3)]
+    dU_g = [torch.zeros(s_sp, dtype=torch.float64, device="cuda") for _ in range(3)]
+    Ufp = {}
+    for d in range(3):
+        if not act[d]:
+            continue
+        s_fp = sd_shape(T, 6, N, n, act, stag=d)
+        U = state(s_fp, rng)
+        F = rng.normal(size=s_fp)
+        # Riemann with the viscous flag
+        Uo, Fo = U.copy(), F.copy()
+        L.orc_riemann_viscous(ctypes.byref(cfg), op.dp(Uo), op.dp(Fo), op.ip(s_fp), d)
+        Ug, Fg = dev(U), dev(F)
+        c.sd_riemann_solver(Ug, Fg, d, viscous=True)
+        assert np.array_equal(host(Fg), Fo), f"riemann F axis {d}"
+        assert np.array_equal(host(Ug), Uo), f"riemann U axis {d}"
+        # gradient of those face-updated values
+        L.orc_gradient(ctypes.byref(cfg), op.dp(Uo), op.ip(s_fp), op.dp(dU_o[d]), op.ip(s_sp), h[d],
+                       op.dp(m["dfp_to_sp"]), n, n + 1, d)
+        c.compute_gradient(Ug, dU_g[d], d)
+        assert np.array_equal(host(dU_g[d]), dU_o[d]), f"gradient axis {d}"
+        Ufp[d] = (Uo, Ug, s_fp, Fo, Fg)
+    for d, (Uo, Ug, s_fp, Fo, Fg) in Ufp.items():
+        o = [dU_o[d], dU_o[(d + 1) % 3], dU_o[(d + 2) % 3]]
+        g = [dU_g[d], dU_g[(d + 1) % 3], dU_g[(d + 2) % 3]]
+        L.orc_viscous_flux(ctypes.byref(cfg), op.dp(Uo), op.ip(s_fp), op.dp(o[0]), op.dp(o[1]), op.dp(o[2]),
+                           op.ip(s_sp), op.dp(m["sp_to_fp"]), n + 1, n, nu, beta, d)
+        c.compute_viscous_flux(Ug, g[0], g[1], g[2], d)
+        assert np.array_equal(host(Ug), Uo), f"viscous flux axis {d}"
+        L.orc_rusanov(op.dp(Uo), op.dp(Fo), op.ip(s_fp), d)
+        c.sd_rusanov_solver(Ug, Fg, d)
+        assert np.array_equal(host(Fg), Fo), f"rusanov axis {d}"
+    c.close()

@@ -316,6 +316,18 @@ size_t spdk_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
  * Bit-identical to the stage sequence. */
 int spdk_hydro_step_fused(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, void* workspace, double dt, double* W_cv,
                           double* F_x, double* dt_next);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): spdk_hydro_step_fused with its workspace taken from a
+ * static per-GPU array of spdk_fused_noalloc_max_bytes() bytes (64 MiB; each
+ * GPU the library runs on holds it until the process ends). The call
+ * allocates nothing; meshes whose workspace (spdk_fused_workspace_bytes)
+ * exceeds the array are rejected with SPDK_E_SHAPE. Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other threads,
+ * other streams) race. Same kernels and results as spdk_hydro_step_fused. */
+size_t spdk_fused_noalloc_max_bytes(void);
+int spdk_hydro_step_fused_noalloc(spdk_ctx* ctx, double* U_sp, spdk_sd_shape s, double dt, double* W_cv, double* F_x,
+                                  double* dt_next);
 /* The predictor part of ADER_step only (src/hydro_ader.hpp:291-303: copy_ader
  * and the p Solve_fluxes + update_prediction pairs) with the fused tile
  * kernel, bit-identical to the stage sequence, its T slices written to

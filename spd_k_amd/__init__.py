@@ -77,6 +77,7 @@ def lib():
         L.spdk_last_error.restype = ctypes.c_char_p
         L.spdk_fused_workspace_bytes.restype = ctypes.c_size_t
         L.spdk_fused_fluxes_workspace_bytes.restype = ctypes.c_size_t
+        L.spdk_fused_noalloc_max_bytes.restype = ctypes.c_size_t
         L.spdk_ctx_stream.restype = ctypes.c_void_p
         L.spdk_mlp_param_count.restype = ctypes.c_size_t
         for f in ("spdk_mlp_set_params", "spdk_mlp_get_params", "spdk_mlp_destroy", "spdk_mlp_dims"):
@@ -322,4 +323,13 @@ class Context:
         out = ctypes.c_double()
         check(L.spdk_hydro_step_fused(self.ctx, _ptr(U_sp), SDShape.of(U_sp.shape), _ptr(ws), ctypes.c_double(dt),
                                       _ptr(W_cv), None, ctypes.byref(out)), "hydro_step_fused")
+        return out.value
+
+    def hydro_step_fused_noalloc(self, U_sp, dt, W_cv=None):
+        """The same step with the library's static per-GPU workspace (no
+        allocation in the call; at most spdk_fused_noalloc_max_bytes() of
+        workspace; calls overlapping on one GPU race)."""
+        out = ctypes.c_double()
+        check(lib().spdk_hydro_step_fused_noalloc(self.ctx, _ptr(U_sp), SDShape.of(U_sp.shape), ctypes.c_double(dt),
+                                                  _ptr(W_cv), None, ctypes.byref(out)), "hydro_step_fused_noalloc")
         return out.value

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include "ctx.hpp"
 #include "physics.hpp"
@@ -787,6 +788,35 @@ const Variant* find_variant(spdk_ctx* c) {
     return nullptr;
 }
 
+// Workspace of the allocation-free step entry: one array per GPU, resident
+// until the process ends. A kernel takes its address as seen from the
+// device, read per device with hipGetSymbolAddress (the array's name in host
+// code is a host-side stand-in, not a device address).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per device).
+int scratch_for_current_device(void** out) {
+    int dev = -1;
+    SPDK_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) { set_error("fused noalloc: device index out of range"); return SPDK_E_ARG; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("fused noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return SPDK_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SPDK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -832,6 +862,27 @@ int spdk_hydro_step_fused(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, 
     if (!c || !U_sp || !ws) { set_error("fused: null argument"); return SPDK_E_ARG; }
     const Variant* v = nullptr;
     if (int rc = check_step_fused(c, s, &v)) return rc;
+    int Ne[3] = {s.Nx, s.Ny, s.Nz};
+    std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
+    if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr);
+    return v->rk(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr);
+}
+
+size_t spdk_fused_noalloc_max_bytes(void) { return kScratchBytes; }
+
+int spdk_hydro_step_fused_noalloc(spdk_ctx* c, double* U_sp, spdk_sd_shape s, double dt, double* W_cv, double* F_x,
+                                  double* dt_next) {
+    if (!c || !U_sp) { set_error("fused noalloc: null argument"); return SPDK_E_ARG; }
+    const Variant* v = nullptr;
+    if (int rc = check_step_fused(c, s, &v)) return rc;
+    const size_t need = v->layout((size_t)s.Nx * s.Ny * s.Nz, c->T).total * sizeof(double);
+    if (need > kScratchBytes) {
+        set_error("fused noalloc: the step needs " + std::to_string(need) + " bytes of workspace, more than the " +
+                  std::to_string(kScratchBytes) + " of the static scratch; use spdk_hydro_step_fused");
+        return SPDK_E_SHAPE;
+    }
+    void* ws = nullptr;
+    if (int rc = scratch_for_current_device(&ws)) return rc;
     int Ne[3] = {s.Nx, s.Ny, s.Nz};
     std::strncpy(c->kernel_name, "k_sweep", sizeof(c->kernel_name) - 1);
     if (c->T > 1) return v->ader(c, U_sp, Ne, ws, dt, W_cv, F_x, dt_next, nullptr);

@@ -33,7 +33,7 @@ struct spdk_ctx {
     static constexpr int kEvPool = 64;
     hipEvent_t evs[kEvPool][2] = {};
     int ev_used = 0;  // event pairs recorded since the last harvest
-    bool fb_tile = false;  // MUSCL-HLLC fallback on cell tiles (else one thread per cell)
+    bool fb_tile = true;  // MUSCL-HLLC fallback on cell tiles (else one thread per cell)
     // device buffers created on first use by a path and freed with the context
     float* mixed_mats = nullptr;  // FP32 sp_to_fp and dfp_to_sp / h_d
     spdk_mlp* guess_net = nullptr;  // learned Picard guess (not owned)

@@ -104,6 +104,8 @@ __device__ __forceinline__ int spidx(int d, int ll, int q) {
 // after the pencils) and the dim2 sum over nn of u S(o2, nn) is written;
 // otherwise the single contraction of face_integral_ref. Sums in the
 // reference's order, so the faces equal the stage kernels' bit for bit.
+__device__ __forceinline__ int el_of(int tid, int tpe) { return tid / tpe; }
+
 template <int N, int DIM>
 __device__ __forceinline__ void fv_faces(const SweepArgs& a, double* scr, const double* Fs, int t, int s, bool live,
                                          int ei, int ej, int ek) {
@@ -142,15 +144,47 @@ __device__ __forceinline__ void fv_faces(const SweepArgs& a, double* scr, const 
                     }
             }
             __syncthreads();
-            if (put) {
+            double accv[NVP];
+            if (own) {
                 const int o1 = o[dim1], o2 = o[dim2];
 #pragma unroll
                 for (int v = 0; v < NVP; v++) {
                     double acc = 0;
 #pragma unroll
                     for (int nn = 0; nn < N; nn++) acc += scr[v * TPE + (f * N + o1) * N + nn] * S[o2 * N + nn];
-                    out[(size_t)v * vstride] = acc;
+                    accv[v] = acc;
                 }
+            }
+            if (G::EPB == 4 && N == 4) {
+                // staged: the block's four elements (consecutive along x) write
+                // 16 consecutive faces per row, whole 128-byte rows
+                __syncthreads();
+                if (own)
+#pragma unroll
+                    for (int v = 0; v < NVP; v++) scr[v * TPE + s] = accv[v];
+                __syncthreads();
+                const int tid = threadIdx.x, xx = tid % 16, row = tid / 16;
+                const int e2l = xx / N, ox = xx % N, oy = row % N, oz = row / N;
+                const int e2 = blockIdx.x * G::EPB + e2l;
+                if (e2 < a.E) {
+                    const int ec2[3] = {e2 % a.Ne[0], (e2 / a.Ne[0]) % a.Ne[1], e2 / (a.Ne[0] * a.Ne[1])};
+                    bool own2 = true;
+#pragma unroll
+                    for (int x = 0; x < DIM; x++)
+                        own2 = own2 && ec2[x] >= 1 && ec2[x] <= (x == d ? a.Ne[x] - 1 : a.Ne[x] - 2);
+                    const int o2v[3] = {ox, oy, oz};
+                    if (own2 && (ec2[d] < a.Ne[d] - 1 || o2v[d] == 0)) {
+                        const int I2 = ox + a.fgh[0] + (ec2[0] - 1) * N, J2 = oy + a.fgh[1] + (ec2[1] - 1) * N;
+                        const int K2 = oz + a.fgh[2] + (ec2[2] - 1) * N;
+                        double* out2 = a.Ffv[t][d] + ((size_t)K2 * ny + J2) * nx + I2;
+                        const double* sc2 = scr + (long)(e2l - el_of(threadIdx.x, TPE)) * G::LDS_ELEM;
+#pragma unroll
+                        for (int v = 0; v < NVP; v++) out2[(size_t)v * vstride] = sc2[v * TPE + ox + N * (oy + N * oz)];
+                    }
+                }
+            } else if (put) {
+#pragma unroll
+                for (int v = 0; v < NVP; v++) out[(size_t)v * vstride] = accv[v];
             }
             __syncthreads();
         } else if (put) {  // at most one transverse axis active: face_integral_ref

@@ -681,6 +681,101 @@ __device__ __forceinline__ void mfv_faces(const MArgs& a, const float* Fs, const
     (void)M;
 }
 
+// 3-D p=3 MFMA face integrals of the round's axis d with the writes staged
+// through LDS: each wave (element) forms its 5 tiles of 16 x 16 outputs, puts
+// them over its own flux slots (read for the last time by its MFMAs), and after
+// a block barrier every thread writes one value per (slice, variable) such
+// that 16 consecutive threads write 16 consecutive faces along x (the
+// block's elements are consecutive along x): whole 128-byte rows instead of
+// one element's runs of 4. Same values and ownership rules as mfv_faces.
+// Called by every thread of the block.
+template <int N, int DIM>
+__device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* lds_base, int blk, int d, int s,
+                                                 int el, bool live) {
+    using G = MT<N, DIM>;
+    constexpr int FSZ = G::FSZ, T = G::T, TPE = G::TPE, EPB = G::EPB;
+    static_assert(N == 4 && DIM == 3 && TPE == 64 && EPB == 4, "3-D p=3 tiles");
+    constexpr int NCOL = T * NVP * N;  // panels (t, v, f), f fastest
+    const int dim1 = d == 0 ? 1 : (d == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
+    float* Fs = (float*)((double*)(lds_base + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES) + G::LDS_X) + G::LDS_D;
+    // pencil of axis d through transverse points ll (along dim1) and nn (along dim2)
+    auto pq = [&](int ll, int nn) -> int { return d == 1 ? ll * N + nn : nn * N + ll; };
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 acc[NCOL / 16];
+    if (live) {  // one wave per element: uniform
+        const int lane = s, li = lane & 15, lk = lane >> 4;
+        float av[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) av[ks] = a.SS[li * 16 + 4 * ks + lk];
+#pragma unroll
+        for (int ct = 0; ct < NCOL / 16; ct++) {
+            const int col = ct * 16 + li;
+            const int t = col / (NVP * N), v = (col / N) % NVP, f = col % N;
+            const float* Fp = Fs + (t * NVP + v) * FSZ;
+            acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                const int k = 4 * ks + lk, ll = k / N, nn = k % N;
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks], Fp[fidx<N, DIM>(d, f, pq(ll, nn))], acc[ct], 0, 0, 0);
+            }
+        }
+        // results over the element's flux slots: res[(t, v)][position, x fastest]
+#pragma unroll
+        for (int ct = 0; ct < NCOL / 16; ct++) {
+            const int col = ct * 16 + li;
+            const int t = col / (NVP * N), v = (col / N) % NVP, f = col % N;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 4 * lk + i, o1 = row / N, o2 = row % N;
+                int o[3];
+                o[d] = f;
+                o[dim1] = o1;
+                o[dim2] = o2;
+                Fs[(t * NVP + v) * TPE + o[0] + N * (o[1] + N * o[2])] = acc[ct][i];
+            }
+        }
+    }
+    __syncthreads();
+    // thread -> (element e2 of the block, position): 16 faces along x per row
+    const int tid = threadIdx.x, xx = tid % (EPB * N), row = tid / (EPB * N);
+    const int e2l = xx / N, ox = xx % N, oy = row % N, oz = row / N;
+    const int e2 = blk * EPB + e2l;
+    if (e2 >= a.E) return;
+    const int ei2 = e2 % a.Ne[0], ej2 = (e2 / a.Ne[0]) % a.Ne[1], ek2 = e2 / (a.Ne[0] * a.Ne[1]);
+    const int ec[3] = {ei2, ej2, ek2};
+    const int Nd = d == 0 ? a.Ne[0] : (d == 1 ? a.Ne[1] : a.Ne[2]);
+    const bool per = (d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2])) == SPDK_BC_PERIODIC;
+    bool own = true;
+#pragma unroll
+    for (int x = 0; x < DIM; x++)
+        own = own && ec[x] >= 1 && ec[x] <= (x == d ? (per ? a.Ne[x] - 2 : a.Ne[x] - 1) : a.Ne[x] - 2);
+    const int o[3] = {ox, oy, oz};
+    const int f = o[d];
+    if (!own || (ec[d] == Nd - 1 && f != 0)) return;
+    const bool wrap = per && ec[d] == 1 && f == 0;
+    const int nx = a.fvn[0] + (d == 0), ny = a.fvn[1] + (d == 1), nz = a.fvn[2] + (d == 2);
+    const size_t vstride = (size_t)nz * ny * nx;
+    auto off = [&](int ex, int ey, int ez) -> size_t {
+        const int I = ox + a.fgh[0] + (ex - 1) * N, J = oy + a.fgh[1] + (ey - 1) * N, K = oz + a.fgh[2] + (ez - 1) * N;
+        return ((size_t)K * ny + J) * nx + I;
+    };
+    const size_t o0 = off(ei2, ej2, ek2);
+    const size_t ow = wrap ? off(d == 0 ? a.Ne[0] - 1 : ei2, d == 1 ? a.Ne[1] - 1 : ej2, d == 2 ? a.Ne[2] - 1 : ek2) : 0;
+    const double* X2 = (const double*)(lds_base + (size_t)e2l * G::ELEM_BYTES);
+    const float* res = (const float*)(X2 + G::LDS_X) + G::LDS_D;
+    const int pos = ox + N * (oy + N * oz);
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        double* ob = d == 0 ? a.Ffv[t][0] : (d == 1 ? a.Ffv[t][1] : a.Ffv[t][2]);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            const double val = X2[NVP + d * NVP + v] + (double)res[(t * NVP + v) * TPE + pos];
+            ob[(size_t)v * vstride + o0] = val;
+            if (wrap) ob[(size_t)v * vstride + ow] = val;
+        }
+    }
+}
+
 // MODE 0: a predictor sweep (out_pred set); MODE 1: the corrector (face
 // balance, conservative update, cv state and dt); MODE 2: the flux
 // evaluation of the predictor's slices with their face integrals written
@@ -813,7 +908,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                            bal, bw, MODE == 2 && !a.F_x);
         }
         __syncthreads();
-        if (MODE == 2) mfv_faces<N, DIM>(a, Fs, X, r, s, act_e, ei, ej, ek);
+        if (MODE == 2) {
+            if constexpr (N == 4 && DIM == 3) {
+                if (!a.F_x && !a.fv_valu) mfv_faces_staged<N, DIM>(a, lds_raw, blk, r, s, el, act_e);
+                else mfv_faces<N, DIM>(a, Fs, X, r, s, act_e, ei, ej, ek);
+            } else {
+                mfv_faces<N, DIM>(a, Fs, X, r, s, act_e, ei, ej, ek);
+            }
+        }
         if (act_e) {
             const int d0 = a.path ? r : 0, d1 = MODE == 2 ? d0 : (a.path ? r + 1 : DIM);
 #pragma unroll 1

@@ -214,6 +214,31 @@ int spdk_fv_update_prims(spdk_ctx* ctx, double* U_new, double* W_new, double* W_
 int spdk_fv_update_blended(spdk_ctx* ctx, double* U_new, double* U_cv, spdk_sd_shape s_cv, const double* Fx,
                            const double* Fy, const double* Fz, const double* theta, spdk_fv_shape sc, int t_id,
                            double dt);
+/* FV_Update_solution (src/hydro_ader.hpp:500-575) with the cell values kept
+ * in the FV cell layout (shape sc = the context's FV cells, 6 variables;
+ * variable 5 is never touched) across the ADER nodes, in two buffers that
+ * take turns. Same bits as the U_cv sequence above.
+ * from_sd: U_fv = transform_a_to_b(U_sp, sp -> cv) of the active elements
+ *   (src/hydro_ader.hpp:501), written at their FV cell positions.
+ * update_prims: the update = 0 call reading U_fv: U_next, W_new =
+ *   prim(U_next), W_old = prim(U_fv) (spdk_fv_update_prims).
+ * update_blended: the update = 1 call: rewrites U_next only in cells whose
+ *   theta or a face neighbour's theta is nonzero (every cell with gravity);
+ *   the caller then swaps U_fv and U_next (U_cv = U_new without a copy).
+ * to_sd: U_sp = transform_a_to_b(cv -> sp) (src/hydro_ader.hpp:570) — ghost
+ *   elements by the sp -> cv -> sp round trip — and, when W_cv / dt_next are
+ *   given, W_cv = cv(prim(U_sp)) and the next dt (compute_primitives,
+ *   transform_a_to_b, compute_dt: src/hydro_ader.hpp:235-243).
+ * The tile transforms need leading active axes (x, x-y, x-y-z), one slice and
+ * at most 6 points per axis (SPDK_E_SHAPE otherwise). */
+int spdk_fv_state_from_sd(spdk_ctx* ctx, const double* U_sp, spdk_sd_shape s, double* U_fv, spdk_fv_shape sc);
+int spdk_fv_state_update_prims(spdk_ctx* ctx, double* U_next, double* W_new, double* W_old, const double* U_fv,
+                               const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id,
+                               double dt);
+int spdk_fv_state_update_blended(spdk_ctx* ctx, double* U_next, const double* U_fv, const double* Fx, const double* Fy,
+                                 const double* Fz, const double* theta, spdk_fv_shape sc, int t_id, double dt);
+int spdk_fv_state_to_sd(spdk_ctx* ctx, const double* U_fv, spdk_fv_shape sc, double* U_sp, spdk_sd_shape s, double* W_cv,
+                        double* dt_next);
 /* boundaries, FV cell-centred fields (src/boundary.cpp:123-190), all active axes x->y->z */
 int spdk_fill_fv_ghosts(spdk_ctx* ctx, double* U, spdk_fv_shape s);
 /* compute_primitives, FV (src/hydro.cpp:134-153) */

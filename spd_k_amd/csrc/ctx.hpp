@@ -23,7 +23,7 @@ struct spdk_ctx {
     int fvnf[3] = {}, fvn[3] = {};
     double* red = nullptr;       // reduction partials (device)
     double* red_host = nullptr;  // pinned host landing slot
-    int red_blocks = 1024;
+    int red_blocks = 2048;
     // optional timing of the dominant kernel
     bool timing = false;
     double kernel_ms = 0;

@@ -37,7 +37,9 @@ struct Tile {
     // LDS per element (doubles): state + fluxes of every axis
     static constexpr int LDS_U = NVP * TPE;
     static constexpr int LDS_F = DIM * NVP * FSZ;
-    static constexpr int LDS_ELEM = LDS_U + LDS_F;
+    // + 4 doubles: the element areas of a block start 8 banks apart (reads
+    // across elements at one in-element offset do not share a bank)
+    static constexpr int LDS_ELEM = LDS_U + LDS_F + 4;
     static_assert(BLOCK == 256, "tile workgroups are 256 threads");
     static_assert(LDS_F >= LDS_U, "cv ping-pong reuses the flux area");
 };

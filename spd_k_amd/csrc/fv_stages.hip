@@ -3,6 +3,7 @@
 // blending and the blended MUSCL-Hancock + HLLC face fluxes. Each kernel
 // restates the reference function cited at its launcher.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "ctx.hpp"
 #include "physics.hpp"
 
@@ -285,6 +286,294 @@ __global__ void __launch_bounds__(BS) k_fv_update_blended(double* __restrict__ U
     }
     const size_t slab = s.slab();
     for (int v = 0; v < s.nv; v++) Ucv[v * slab + c.o] = Un[sc.off(v, c.K, c.J, c.I)];
+}
+
+// ---------------------------------------------------------------------------
+// The FV state across the ADER nodes, in the FV cell layout
+// (src/hydro_ader.hpp:500-575). The cell values of the step's start are
+// formed once from U_sp and written at their FV cell positions; every node
+// then reads and writes contiguous rows, and two state buffers take turns:
+// the node's update = 0 result goes to the other buffer, the blended update
+// rewrites only the cells whose face fluxes the blend changed, and the
+// buffers swap. At the end the cell values go back to the solution points.
+// The arithmetic of every value is that of the SD-layout sequence
+// (transform_a_to_b, fv_update_solution with update = 0 and 1, U_cv = U_new).
+
+// Elements of one workgroup: EPB consecutive elements of one row along x.
+template <int Q, int D>
+struct ElTile {
+    static constexpr int TPE = D == 1 ? Q : (D == 2 ? Q * Q : Q * Q * Q);
+    static constexpr int EPB = TPE >= BS ? 1 : BS / TPE;
+};
+
+// transform_a_to_b of one element (src/transforms.cpp:81-118 per axis, x then
+// y then z): point s of the element enters as `in`, its transformed value is
+// returned. b0/b1 are the element's two LDS planes; every thread of the block
+// calls it (it holds barriers).
+template <int Q, int D>
+__device__ __forceinline__ double el_transform(double in, double* b0, double* b1, const double* sM, int s, bool live) {
+    const int ii = s % Q, jj = (s / Q) % Q, kk = s / (Q * Q);
+    __syncthreads();
+    if (live) b0[s] = in;
+    __syncthreads();
+    double u = 0;
+    if (live) {
+#pragma unroll
+        for (int l = 0; l < Q; l++) u += b0[s - ii + l] * sM[ii * Q + l];
+    }
+    if (D >= 2) {
+        if (live) b1[s] = u;
+        __syncthreads();
+        u = 0;
+        if (live) {
+#pragma unroll
+            for (int l = 0; l < Q; l++) u += b1[s + (l - jj) * Q] * sM[jj * Q + l];
+        }
+        if (D == 3) {
+            if (live) b0[s] = u;
+            __syncthreads();
+            u = 0;
+            if (live) {
+#pragma unroll
+                for (int l = 0; l < Q; l++) u += b0[s + (l - kk) * Q * Q] * sM[kk * Q + l];
+            }
+        }
+    }
+    return u;
+}
+
+// Slot q of a workgroup's FV rows: the row (jj, kk) of the EPB elements, x
+// fastest, so the elements' cells of one row are one contiguous run.
+template <int Q, int D>
+__device__ __forceinline__ void fv_slot(int q, int& e, int& pt, int& ii, int& jj, int& kk) {
+    constexpr int EPB = ElTile<Q, D>::EPB;
+    ii = q % Q;
+    e = (q / Q) % EPB;
+    const int r = q / (Q * EPB);
+    jj = D >= 2 ? r % Q : 0;
+    kk = D == 3 ? r / Q : 0;
+    pt = (kk * Q + jj) * Q + ii;
+}
+
+// U_sp (SD layout, active elements) -> cell values at the FV cell positions
+// of U_fv. Workgroup g: row g / gpr of active elements, elements
+// (g % gpr) * EPB ... of that row.
+template <int Q, int D>
+__global__ void __launch_bounds__(BS) k_fv_state_in(const double* __restrict__ Usp, SD s, double* __restrict__ Ufv, FV sc,
+                                                    const double* __restrict__ M, int ldm, int gx, int gy, int gz,
+                                                    int fx, int fy, int fz, int ex, int ey, int gpr, uint32_t groups) {
+    using TL = ElTile<Q, D>;
+    constexpr int TPE = TL::TPE, EPB = TL::EPB;
+    __shared__ double b0[EPB * TPE], b1[EPB * TPE], sM[Q * Q];
+    const uint32_t g = blockIdx.x;
+    if (g >= groups) return;
+    const int tid = threadIdx.x, el = tid / TPE, sp = tid % TPE;
+    const int row = (int)(g / gpr), i0 = (int)(g % gpr) * EPB;
+    const int j = row % ey, k = row / ey;
+    const int i = i0 + el;
+    const bool live = el < EPB && i < ex;
+    if (tid < Q * Q) sM[tid] = M[(tid / Q) * ldm + tid % Q];
+    const size_t slab = s.slab();
+    const size_t eo = ((((size_t)(k + gz) * s.Ny + (j + gy)) * s.Nx) + (i + gx)) * TPE + sp;
+    double* e0 = b0 + (live ? el : 0) * TPE;
+    double* e1 = b1 + (live ? el : 0) * TPE;
+    for (int v = 0; v < NVP; v++) {
+        const double u = el_transform<Q, D>(live ? Usp[v * slab + eo] : 0.0, e0, e1, sM, sp, live);
+        __syncthreads();
+        if (live) b1[el * TPE + sp] = u;
+        __syncthreads();
+        if (tid < EPB * TPE) {
+            int e, pt, ii, jj, kk;
+            fv_slot<Q, D>(tid, e, pt, ii, jj, kk);
+            if (i0 + e < ex) {
+                const int I = fx + (i0 + e) * Q + ii;
+                const int J = D >= 2 ? fy + j * Q + jj : fy + j;
+                const int K = D == 3 ? fz + k * Q + kk : fz + k;
+                Ufv[sc.off(v, K, J, I)] = b1[e * TPE + pt];
+            }
+        }
+    }
+}
+
+// The update = 0 call on the FV state (k_fv_update_prims with U_cv read in
+// the FV cell layout): U_next, W_new = prim(U_next), W_old = prim(U_fv).
+// Variable 5 (zero throughout) is neither read nor written.
+__global__ void __launch_bounds__(BS) k_fv_state_update_prims(double* __restrict__ Un, double* __restrict__ Wn,
+                                                              double* __restrict__ Wo, const double* __restrict__ Ucv,
+                                                              const double* __restrict__ Fx,
+                                                              const double* __restrict__ Fy,
+                                                              const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz,
+                                                              Geo geo, FvUpd a, double gm, int fx, int fy, int fz, int Mx,
+                                                              int My, uint32_t total) {
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
+    if (idx >= total) return;
+    const int I = fx + (int)(idx % Mx), J = fy + (int)((idx / Mx) % My), K = fz + (int)(idx / ((uint32_t)Mx * My));
+    const size_t o = sc.off(0, K, J, I), pl = sc.plane();
+    double uo[NVP], un[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        double h, F = 0;
+        uo[v] = Ucv[v * pl + o];
+        if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
+        if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
+        if (a.act[2]) { h = geo.f[2][K + 1] - geo.f[2][K]; F += (Fz[sz.off(v, K + 1, J, I)] - Fz[sz.off(v, K, J, I)]) / h; }
+        if (a.grav) {
+            double S = 0.0;
+            if (v == 1) S = uo[0] * a.g[0];
+            else if (v == 2) S = uo[0] * a.g[1];
+            else if (v == 3) S = uo[0] * a.g[2];
+            else if (v == 4) S = uo[1] * a.g[0] + uo[2] * a.g[1] + uo[3] * a.g[2];
+            F -= S;
+        }
+        un[v] = uo[v] - a.wdt * F;
+        Un[v * pl + o] = un[v];
+    }
+    double wn[NVP], wo[NVP];
+    prim(un, wn, gm);
+    prim(uo, wo, gm);
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        Wn[v * pl + o] = wn[v];
+        Wo[v * pl + o] = wo[v];
+    }
+}
+
+// The update = 1 call on the FV state: a cell whose theta and face
+// neighbours' theta are all zero keeps the update = 0 result already in
+// U_next (its face fluxes did not change); the other cells are recomputed
+// from U_fv with the blended fluxes. With gravity every cell is recomputed,
+// the momentum and energy sources taken from the values just formed (the
+// update = 1 form writes U_cv variable by variable and reads it back).
+__global__ void __launch_bounds__(BS) k_fv_state_update_blended(double* __restrict__ Un, const double* __restrict__ Ucv,
+                                                                const double* __restrict__ Fx,
+                                                                const double* __restrict__ Fy,
+                                                                const double* __restrict__ Fz,
+                                                                const double* __restrict__ th, FV sc, FV sx, FV sy,
+                                                                FV sz, Geo geo, FvUpd a, int fx, int fy, int fz, int Mx,
+                                                                int My, uint32_t total) {
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
+    if (idx >= total) return;
+    const int I = fx + (int)(idx % Mx), J = fy + (int)((idx / Mx) % My), K = fz + (int)(idx / ((uint32_t)Mx * My));
+    const size_t sy1 = (size_t)sc.Nx, sz1 = (size_t)sc.Nx * sc.Ny;
+    const size_t o = ((size_t)K * sc.Ny + J) * sc.Nx + I, pl = sc.plane();
+    bool touched = a.grav || th[o] != 0.0;
+    if (a.act[0]) touched = touched || th[o - 1] != 0.0 || th[o + 1] != 0.0;
+    if (a.act[1]) touched = touched || th[o - sy1] != 0.0 || th[o + sy1] != 0.0;
+    if (a.act[2]) touched = touched || th[o - sz1] != 0.0 || th[o + sz1] != 0.0;
+    if (!touched) return;
+    double un[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) {
+        double h, F = 0;
+        const double uo = Ucv[v * pl + o];
+        if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
+        if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
+        if (a.act[2]) { h = geo.f[2][K + 1] - geo.f[2][K]; F += (Fz[sz.off(v, K + 1, J, I)] - Fz[sz.off(v, K, J, I)]) / h; }
+        if (a.grav) {
+            double S = 0.0;
+            if (v == 1) S = un[0] * a.g[0];
+            else if (v == 2) S = un[0] * a.g[1];
+            else if (v == 3) S = un[0] * a.g[2];
+            else if (v == 4) S = un[1] * a.g[0] + un[2] * a.g[1] + un[3] * a.g[2];
+            F -= S;
+        }
+        un[v] = uo - a.wdt * F;
+        Un[v * pl + o] = un[v];
+    }
+}
+
+// FV state -> U_sp of every element (active elements from their FV cells,
+// ghost elements by the sp -> cv -> sp round trip the SD-layout sequence
+// gives them), and optionally W_cv = cv(prim(U_sp)) and the per-workgroup
+// minimum of the next dt over the active points (src/hydro.cpp:108-132,
+// src/hydro_ader.hpp:235-243, the k_dt_partial formula). Workgroups stride
+// over the element groups (row g / gpr of all elements, x fastest).
+struct DtArgs {
+    double gm, cfl, h[3], nu;
+    int act[3];
+};
+template <int Q, int D>
+__global__ void __launch_bounds__(BS) k_fv_state_out(const double* __restrict__ Ufv, FV sc, double* __restrict__ Usp,
+                                                     double* __restrict__ Wcv, SD s, const double* __restrict__ Mcs,
+                                                     const double* __restrict__ Msc, int ldm, int gx, int gy, int gz,
+                                                     int fx, int fy, int fz, int gpr, uint32_t groups, DtArgs dta,
+                                                     double* __restrict__ part) {
+    using TL = ElTile<Q, D>;
+    constexpr int TPE = TL::TPE, EPB = TL::EPB;
+    __shared__ double b0[EPB * TPE], b1[EPB * TPE], sI[EPB * TPE], sMc[Q * Q], sMs[Q * Q];
+    __shared__ double red[BS];
+    const int tid = threadIdx.x, el = tid / TPE, sp = tid % TPE;
+    if (tid < Q * Q) {
+        sMc[tid] = Mcs[(tid / Q) * ldm + tid % Q];
+        sMs[tid] = Msc[(tid / Q) * ldm + tid % Q];
+    }
+    const bool want_w = Wcv != nullptr, want_dt = part != nullptr;
+    const size_t slab = s.slab();
+    double r = __builtin_inf();
+    for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int row = (int)(g / gpr), i0 = (int)(g % gpr) * EPB;
+        const int j = row % s.Ny, k = row / s.Ny, i = i0 + el;
+        const bool live = el < EPB && i < s.Nx;
+        const bool act_row = j >= gy && j < s.Ny - gy && k >= gz && k < s.Nz - gz;
+        const bool active = live && act_row && i >= gx && i < s.Nx - gx;
+        const size_t eo = (((size_t)k * s.Ny + j) * s.Nx + i) * TPE + sp;
+        double* e0 = b0 + (live ? el : 0) * TPE;
+        double* e1 = b1 + (live ? el : 0) * TPE;
+        double u[NVP];
+        for (int v = 0; v < NVP; v++) {
+            // the cell values: FV rows for active elements, the forward
+            // transform of U_sp for ghost elements
+            __syncthreads();
+            if (act_row && tid < EPB * TPE) {
+                int e, pt, ii, jj, kk;
+                fv_slot<Q, D>(tid, e, pt, ii, jj, kk);
+                const int ie = i0 + e;
+                if (ie >= gx && ie < s.Nx - gx) {
+                    const int I = fx + (ie - gx) * Q + ii;
+                    const int J = D >= 2 ? fy + (j - gy) * Q + jj : fy + (j - gy);
+                    const int K = D == 3 ? fz + (k - gz) * Q + kk : fz + (k - gz);
+                    sI[e * TPE + pt] = Ufv[sc.off(v, K, J, I)];
+                }
+            }
+            const double cg = el_transform<Q, D>(live && !active ? Usp[v * slab + eo] : 0.0, e0, e1, sMs, sp,
+                                                 live && !active);
+            const double cv = active ? sI[el * TPE + sp] : cg;
+            u[v] = el_transform<Q, D>(cv, e0, e1, sMc, sp, live);
+            if (live) Usp[v * slab + eo] = u[v];
+        }
+        if (!want_w && !want_dt) continue;
+        double w[NVP];
+        if (live) prim(u, w, dta.gm);
+        else
+            for (int v = 0; v < NVP; v++) w[v] = 0.0;
+        double wc[NVP];
+        for (int v = 0; v < NVP; v++) {
+            wc[v] = el_transform<Q, D>(w[v], e0, e1, sMs, sp, live);
+            if (live && want_w) Wcv[v * slab + eo] = wc[v];
+        }
+        if (active && want_dt) {
+            const double c_s = sound_speed(wc[0], wc[4], dta.gm);
+            double c_max = 0, dx_min = 1, dt_min;
+            if (dta.act[0]) { c_max += (fabs(wc[1]) + c_s); dx_min = smin(dx_min, dta.h[0]); }
+            if (dta.act[1]) { c_max += (fabs(wc[2]) + c_s); dx_min = smin(dx_min, dta.h[1]); }
+            if (dta.act[2]) { c_max += (fabs(wc[3]) + c_s); dx_min = smin(dx_min, dta.h[2]); }
+            dt_min = dta.cfl * dx_min / c_max / Q;
+            if (dta.nu > 0.0) {
+                const double dx_sub = dx_min / Q;
+                const double dt_visc = 0.25 * dta.cfl * dx_sub * dx_sub / dta.nu;
+                dt_min = nan_min(dt_min, dt_visc);
+            }
+            r = nan_min(r, dt_min);
+        }
+    }
+    if (!want_dt) return;
+    red[tid] = r;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = nan_min(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -978,6 +1267,169 @@ int spdk_fv_update_blended(spdk_ctx* c, double* Un, double* Ucv, spdk_sd_shape s
                        Fy ? Fy : Fx, Fz ? Fz : Fx, theta, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
                        fv_stag(sc, 2, a.act[2]), geo_of(c), a, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], total);
     SPDK_LAUNCH_CHECK("fv_update_blended");
+    return SPDK_OK;
+}
+
+}  // extern "C"
+
+// FV-state arrays have the context's FV cell shape (6 variables, its cell
+// counts with ghost layers; one cell along an inactive axis).
+static int fv_state_shape_ok(const spdk_ctx* c, const spdk_fv_shape& sc, const char* who) {
+    const int Nf[3] = {sc.Nx, sc.Ny, sc.Nz};
+    bool ok = sc.nv == NV;
+    for (int d = 0; d < 3; d++)
+        ok = ok && (c->prm.active[d] ? (Nf[d] == c->fvn[d] && Nf[d] > 2 * c->fgh[d]) : Nf[d] == 1);
+    if (!ok) set_error(std::string(who) + ": FV state shape differs from the context's FV cells");
+    return ok ? SPDK_OK : SPDK_E_SHAPE;
+}
+// The element-tile transforms: active axes x, x-y or x-y-z, Q points per
+// axis (Q <= 6), square Q x Q cv/sp matrices. D = number of active axes.
+static int fv_state_tile_ok(const spdk_ctx* c, const spdk_sd_shape& s, const spdk_fv_shape& sc, const char* who, int& D) {
+    if (int rc = fv_shapes_ok(c, s, sc, who)) return rc;
+    if (int rc = fv_state_shape_ok(c, sc, who)) return rc;
+    const int* a = c->prm.active;
+    D = (a[0] != 0) + (a[1] != 0) + (a[2] != 0);
+    const int q = s.nx;
+    bool ok = s.nt == 1 && a[0] && (D < 2 || a[1]) && (D < 3 || a[2]) && q >= 1 && q <= 6;
+    for (int m : {SPDK_M_SP_TO_CV, SPDK_M_CV_TO_SP}) ok = ok && c->mrows[m] == q && c->mcols[m] == q;
+    if (!ok) {
+        set_error(std::string(who) + ": needs one slice, leading active axes and at most 6 points per axis");
+        return SPDK_E_SHAPE;
+    }
+    return SPDK_OK;
+}
+static FvUpd fv_upd(const spdk_ctx* c, int t_id, double dt, int update) {
+    FvUpd a;
+    for (int d = 0; d < 3; d++) { a.act[d] = c->prm.active[d]; a.g[d] = c->prm.g[d]; }
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
+    a.wdt = c->wt_host[t_id] * dt;
+    a.update = update;
+    return a;
+}
+
+__global__ void __launch_bounds__(BS) k_state_min_final(double* __restrict__ part, int n) {
+    __shared__ double red[BS];
+    double r = __builtin_inf();
+    for (int i = threadIdx.x; i < n; i += BS) r = nan_min(r, part[i]);
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = nan_min(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[0] = red[0];
+}
+
+extern "C" {
+
+int spdk_fv_state_from_sd(spdk_ctx* c, const double* U_sp, spdk_sd_shape s, double* U_fv, spdk_fv_shape sc) {
+    if (!c || !U_sp || !U_fv) { set_error("fv_state_from_sd: null argument"); return SPDK_E_ARG; }
+    int D = 0;
+    if (int rc = fv_state_tile_ok(c, s, sc, "fv_state_from_sd", D)) return rc;
+    const int gx = c->ngh[0], gy = c->ngh[1], gz = c->ngh[2];
+    const int ex = s.Nx - 2 * gx, ey = s.Ny - 2 * gy, ez = s.Nz - 2 * gz;
+    const double* M = c->mat[SPDK_M_SP_TO_CV];
+    const int ldm = c->mcols[SPDK_M_SP_TO_CV];
+#define SPDK_FVS_IN(QQ, DD)                                                                                           \
+    case QQ: {                                                                                                        \
+        constexpr int EPB = ElTile<QQ, DD>::EPB;                                                                      \
+        const int gpr = (ex + EPB - 1) / EPB;                                                                         \
+        const uint32_t groups = (uint32_t)((size_t)gpr * ey * ez);                                                    \
+        if (groups) hipLaunchKernelGGL((k_fv_state_in<QQ, DD>), dim3(groups), dim3(BS), 0, c->stream, U_sp, SD(s), U_fv, \
+                                       FV(sc), M, ldm, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], ex, ey, gpr,     \
+                                       groups);                                                                      \
+        break;                                                                                                        \
+    }
+#define SPDK_FVS_Q(DD) \
+    switch (s.nx) { SPDK_FVS_IN(1, DD) SPDK_FVS_IN(2, DD) SPDK_FVS_IN(3, DD) SPDK_FVS_IN(4, DD) SPDK_FVS_IN(5, DD) SPDK_FVS_IN(6, DD) }
+    if (D == 1) { SPDK_FVS_Q(1) }
+    else if (D == 2) { SPDK_FVS_Q(2) }
+    else { SPDK_FVS_Q(3) }
+#undef SPDK_FVS_Q
+#undef SPDK_FVS_IN
+    SPDK_LAUNCH_CHECK("fv_state_from_sd");
+    return SPDK_OK;
+}
+
+int spdk_fv_state_update_prims(spdk_ctx* c, double* U_next, double* W_new, double* W_old, const double* U_fv,
+                               const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id,
+                               double dt) {
+    if (!c || !U_next || !W_new || !W_old || !U_fv || !Fx) {
+        set_error("fv_state_update_prims: null argument");
+        return SPDK_E_ARG;
+    }
+    if (U_next == U_fv) { set_error("fv_state_update_prims: U_next and U_fv must differ"); return SPDK_E_ARG; }
+    if (int rc = fv_state_shape_ok(c, sc, "fv_state_update_prims")) return rc;
+    if (t_id < 0 || t_id >= c->T) { set_error("fv_state_update_prims: slice"); return SPDK_E_ARG; }
+    const FvUpd a = fv_upd(c, t_id, dt, 0);
+    const int Mx = sc.Nx - 2 * c->fgh[0], My = sc.Ny - 2 * c->fgh[1], Mz = sc.Nz - 2 * c->fgh[2];
+    const uint32_t total = (uint32_t)((size_t)Mx * My * Mz);
+    hipLaunchKernelGGL(k_fv_state_update_prims, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, U_next, W_new, W_old, U_fv,
+                       Fx, Fy ? Fy : Fx, Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
+                       fv_stag(sc, 2, a.act[2]), geo_of(c), a, c->prm.gamma, c->fgh[0], c->fgh[1], c->fgh[2], Mx, My,
+                       total);
+    SPDK_LAUNCH_CHECK("fv_state_update_prims");
+    return SPDK_OK;
+}
+
+int spdk_fv_state_update_blended(spdk_ctx* c, double* U_next, const double* U_fv, const double* Fx, const double* Fy,
+                                 const double* Fz, const double* theta, spdk_fv_shape sc, int t_id, double dt) {
+    if (!c || !U_next || !U_fv || !Fx || !theta) { set_error("fv_state_update_blended: null argument"); return SPDK_E_ARG; }
+    if (U_next == U_fv) { set_error("fv_state_update_blended: U_next and U_fv must differ"); return SPDK_E_ARG; }
+    if (int rc = fv_state_shape_ok(c, sc, "fv_state_update_blended")) return rc;
+    if (t_id < 0 || t_id >= c->T) { set_error("fv_state_update_blended: slice"); return SPDK_E_ARG; }
+    const FvUpd a = fv_upd(c, t_id, dt, 1);
+    const int Mx = sc.Nx - 2 * c->fgh[0], My = sc.Ny - 2 * c->fgh[1], Mz = sc.Nz - 2 * c->fgh[2];
+    const uint32_t total = (uint32_t)((size_t)Mx * My * Mz);
+    hipLaunchKernelGGL(k_fv_state_update_blended, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, U_next, U_fv, Fx,
+                       Fy ? Fy : Fx, Fz ? Fz : Fx, theta, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
+                       fv_stag(sc, 2, a.act[2]), geo_of(c), a, c->fgh[0], c->fgh[1], c->fgh[2], Mx, My, total);
+    SPDK_LAUNCH_CHECK("fv_state_update_blended");
+    return SPDK_OK;
+}
+
+int spdk_fv_state_to_sd(spdk_ctx* c, const double* U_fv, spdk_fv_shape sc, double* U_sp, spdk_sd_shape s, double* W_cv,
+                        double* dt_next) {
+    if (!c || !U_sp || !U_fv) { set_error("fv_state_to_sd: null argument"); return SPDK_E_ARG; }
+    int D = 0;
+    if (int rc = fv_state_tile_ok(c, s, sc, "fv_state_to_sd", D)) return rc;
+    const spdk_params& P = c->prm;
+    DtArgs dta;
+    dta.gm = P.gamma;
+    dta.cfl = P.cfl;
+    dta.nu = P.nu;
+    for (int d = 0; d < 3; d++) { dta.h[d] = P.h[d]; dta.act[d] = P.active[d]; }
+    double* part = dt_next ? c->red : nullptr;
+    const int ldm = c->mcols[SPDK_M_SP_TO_CV];
+    int nb = 0;
+#define SPDK_FVS_OUT(QQ, DD)                                                                                          \
+    case QQ: {                                                                                                        \
+        constexpr int EPB = ElTile<QQ, DD>::EPB;                                                                      \
+        const int gpr = (s.Nx + EPB - 1) / EPB;                                                                       \
+        const uint32_t groups = (uint32_t)((size_t)gpr * s.Ny * s.Nz);                                               \
+        nb = (int)std::min<size_t>(groups, (size_t)c->red_blocks);                                                    \
+        if (nb) hipLaunchKernelGGL((k_fv_state_out<QQ, DD>), dim3(nb), dim3(BS), 0, c->stream, U_fv, FV(sc), U_sp, W_cv, \
+                                   SD(s), c->mat[SPDK_M_CV_TO_SP], c->mat[SPDK_M_SP_TO_CV], ldm, c->ngh[0], c->ngh[1], \
+                                   c->ngh[2], c->fgh[0], c->fgh[1], c->fgh[2], gpr, groups, dta, part);             \
+        break;                                                                                                        \
+    }
+#define SPDK_FVS_Q(DD)                                                                                                \
+    switch (s.nx) {                                                                                                   \
+        SPDK_FVS_OUT(1, DD) SPDK_FVS_OUT(2, DD) SPDK_FVS_OUT(3, DD) SPDK_FVS_OUT(4, DD) SPDK_FVS_OUT(5, DD)           \
+        SPDK_FVS_OUT(6, DD)                                                                                           \
+    }
+    if (D == 1) { SPDK_FVS_Q(1) }
+    else if (D == 2) { SPDK_FVS_Q(2) }
+    else { SPDK_FVS_Q(3) }
+#undef SPDK_FVS_Q
+#undef SPDK_FVS_OUT
+    SPDK_LAUNCH_CHECK("fv_state_to_sd");
+    if (!dt_next) return SPDK_OK;
+    hipLaunchKernelGGL(k_state_min_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb);
+    SPDK_LAUNCH_CHECK("fv_state_min_final");
+    SPDK_CHECK(hipMemcpyAsync(c->red_host, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPDK_CHECK(hipStreamSynchronize(c->stream));
+    *dt_next = c->red_host[0];
     return SPDK_OK;
 }
 

@@ -155,6 +155,11 @@ struct HydroRun {
     // evaluation forms the face integrals itself (fv_direct)
     std::vector<double*> F_fvn;
     bool fv_direct = false;
+    // the FV update keeps the cell values in the FV cell layout across the
+    // ADER nodes (U_fva and U_new take turns), when the element-tile
+    // transforms apply (leading active axes, at most 6 points per axis)
+    bool fv_state = false;
+    double* U_fva = nullptr;
     void* ws = nullptr;
     std::vector<double> host;  // staging for outputs
     bool timing = false;
@@ -278,6 +283,12 @@ struct HydroRun {
             troubles = mem.alloc(ctx, nfv);
             theta = mem.alloc(ctx, fv_count(s_th));
             theta_tmp = mem.alloc(ctx, fv_count(s_th));
+            {
+                const int na = cfg.act[0] + cfg.act[1] + cfg.act[2];
+                fv_state = !cfg.fv_layout_sd && op.p + 1 <= 6 && cfg.act[0] && (na < 2 || cfg.act[1]) &&
+                           (na < 3 || cfg.act[2]);
+            }
+            if (fv_state) U_fva = mem.alloc(ctx, nfv);
             if (fv_direct) F_fvn.assign((size_t)T * 3, nullptr);
             abits = (uint16_t*)mem.alloc(ctx, (nfv / kNVAR * sizeof(uint16_t) + sizeof(double) - 1) / sizeof(double));
             for (int d = 0; d < 3; d++) {
@@ -382,7 +393,40 @@ struct HydroRun {
             if (rk_a[s] > 0) CK(spdk_rk_combine(ctx, U_sp, U0, s_sp, rk_a[s]));
         }
     }
+    // The FV update with the cell values in the FV cell layout: U_fva and
+    // U_new take turns as the state (the blended update rewrites only the
+    // cells whose fluxes the blend changed); at the end U_sp, and when given
+    // W_cv and the next dt, in the same pass.
+    void fv_update_state(double* W_out, double* dt_next) {  // src/hydro_ader.hpp:500-575
+        const int mask = (1 << 0) | (1 << 4);
+        CK(spdk_fv_state_from_sd(ctx, U_sp, s_sp, U_fva, s_fv));
+        double *A = U_fva, *B = U_new;
+        for (int a = 0; a < T; a++) {
+            double* Fv[3] = {F_fv[0], F_fv[1], F_fv[2]};
+            if (fv_direct) {
+                for (int d = 0; d < 3; d++) Fv[d] = F_fvn[(size_t)a * 3 + d];
+            } else {
+                for (int d = 0; d < 3; d++)
+                    if (cfg.act[d]) CK(spdk_fv_face_integral(ctx, F_fp[d], s_fp[d], F_fv[d], s_fvd[d], T_fp[d], a, d));
+            }
+            CK(spdk_fv_state_update_prims(ctx, B, W_new, W_old, A, Fv[0], Fv[1], Fv[2], s_fv, a, dt));
+            CK(spdk_fill_fv_ghosts(ctx, W_old, s_fv));
+            CK(spdk_fill_fv_ghosts(ctx, W_new, s_fv));
+            CK(spdk_detect_troubles_bits(ctx, W_new, W_old, troubles, abits, s_fv, 1, mask));
+            CK(spdk_fill_fv_ghosts(ctx, troubles, s_fv));
+            CK(spdk_blend_theta(ctx, troubles, theta_tmp, theta, s_fv));
+            CK(spdk_fill_fv_ghosts(ctx, theta, s_th));
+            CK(spdk_muscl_hllc_blend(ctx, W_old, theta, Fv[0], Fv[1], Fv[2], s_fv, op.wt[a] * dt));
+            CK(spdk_fv_state_update_blended(ctx, B, A, Fv[0], Fv[1], Fv[2], theta, s_fv, a, dt));
+            std::swap(A, B);
+        }
+        CK(spdk_fv_state_to_sd(ctx, A, s_fv, U_sp, s_sp, W_out, dt_next));
+    }
     void fv_update() {  // src/hydro_ader.hpp:500-575
+        if (fv_state) {
+            fv_update_state(nullptr, nullptr);
+            return;
+        }
         const int mask = (1 << 0) | (1 << 4);
         CK(spdk_transform(ctx, U_sp, U_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
         for (int a = 0; a < T; a++) {
@@ -417,6 +461,13 @@ struct HydroRun {
                                      ": the state has left the physical range (t = " + std::to_string(t) + ")");
         return next;
     }
+    // the FV update, then W_cv (for a step whose output shows it) and the next
+    // dt from the same pass
+    double fv_update_finish(bool output_due) {
+        double next = 0;
+        fv_update_state(output_due ? W_cv : nullptr, &next);
+        return next;
+    }
     // the x flux-point fluxes are formed only for a step whose output shows them
     double* fx_out(bool output_due) { return cfg.out_flux && (output_due || cfg.final_output) ? F_fp[0] : nullptr; }
     double step_raw(bool output_due) {
@@ -427,6 +478,7 @@ struct HydroRun {
             // MUSCL fallback in FP64 (src/hydro_ader.hpp:291-305, :500-586)
             CK(spdk_mixed_predict_fv_fluxes(ctx, U_sp, s_sp, ws, dt, &pc, &pst, F_fvn.data(), s_fv, fx_out(output_due),
                                             cfg.fv_valu ? 1 : 0));
+            if (fv_state) return fv_update_finish(output_due);
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
             CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));
@@ -438,6 +490,7 @@ struct HydroRun {
             // the x flux-point fluxes only when an output will show them
             CK(spdk_fused_predict_fv_fluxes(ctx, U_sp, s_sp, ws, dt, F_fvn.data(), s_fv,
                                             fx_out(output_due), s_ader));
+            if (fv_state) return fv_update_finish(output_due);
             fv_update();
             CK(spdk_primitives(ctx, U_sp, W_sp, s_sp));
             CK(spdk_transform(ctx, W_sp, W_cv, T_sweep, s_sp, SPDK_M_SP_TO_CV));

@@ -141,6 +141,9 @@ std::string configure(Params& pin, RunConfig& c) {
         const std::string fc = pin.get_or<std::string>("job/fv_contraction", "mfma");
         if (fc != "mfma" && fc != "valu") return "job/fv_contraction must be mfma or valu";
         c.fv_valu = fc == "valu";
+        const std::string fl = pin.get_or<std::string>("job/fv_layout", "fv");
+        if (fl != "fv" && fl != "sd") return "job/fv_layout must be fv or sd";
+        c.fv_layout_sd = fl == "sd";
     }
     c.timing = pin.has("job/timing") && pin.get<bool>("job/timing");
     if (pin.has("job/precision")) c.precision = pin.get<std::string>("job/precision");

@@ -53,6 +53,7 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     int max_steps = -1;      // time/nlim: stop after this many steps
     std::string path = "fused";  // job/path: fused | stages
     bool fv_valu = false;        // job/fv_contraction=valu: mixed face integrals on the VALU instead of MFMA
+    bool fv_layout_sd = false;   // job/fv_layout=sd: FV update state in the SD layout (U_cv) instead of the FV cell layout
     bool timing = false;         // job/timing: HIP-event timing of the dominant kernel
     std::string precision = "fp64";  // job/precision: fp64 | mixed (FP32 state and contractions)
     std::string picard = "fixed";    // job/picard: fixed (p predictor sweeps) | residual

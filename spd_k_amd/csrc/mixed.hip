@@ -75,7 +75,8 @@ constexpr int kGExtra = 6;       // extra features per element
 constexpr float kAmin = 1e-6f;   // floor of the element amplitude
 constexpr float kAgate = 1e-5f;  // amplitude at or below which the guess is the zero-order one
 
-// Minimum waves per SIMD the sweep kernel is compiled for (bounds its VGPRs).
+// Minimum waves per SIMD the sweep kernel is compiled for (bounds its VGPRs);
+// the 3-D p=3 predictor is held at 3 (168 VGPRs), where it fits without spills.
 #ifndef SPDK_MSWEEP_WAVES
 #define SPDK_MSWEEP_WAVES 1
 #endif
@@ -93,7 +94,9 @@ struct MT {
     static constexpr int LDS_D = T * NVP * PSZ;      // floats: state slices
     static constexpr int LDS_F = SLOTS * NVP * FSZ;  // floats: flux slots
     static constexpr int LDS_X = 6 * NVP + 2;        // doubles: mu, F(mu) per axis, residual, shift, mean state
-    static constexpr int ELEM_BYTES = ((LDS_X * 8 + (LDS_D + LDS_F) * 4) + 15) / 16 * 16;
+    // + 32 bytes: the element areas of a block start 8 banks apart, so reads
+    // that cross elements at one in-element offset do not share a bank
+    static constexpr int ELEM_BYTES = ((LDS_X * 8 + (LDS_D + LDS_F) * 4) + 15) / 16 * 16 + 32;
     static constexpr int EPB0 = TPE >= 256 ? 1 : 256 / TPE;
     static constexpr int EPB_LDS = 61440 / ELEM_BYTES;
     static constexpr int EPB = EPB0 < EPB_LDS ? EPB0 : EPB_LDS;
@@ -696,6 +699,10 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
     constexpr int FSZ = G::FSZ, T = G::T, TPE = G::TPE, EPB = G::EPB;
     static_assert(N == 4 && DIM == 3 && TPE == 64 && EPB == 4, "3-D p=3 tiles");
     constexpr int NCOL = T * NVP * N;  // panels (t, v, f), f fastest
+    // staged results: one row of RS floats per (slice, variable); RS = 8 mod 32
+    // banks puts the four variables a 16-lane group writes in distinct banks
+    constexpr int RS = 72;
+    static_assert(T * NVP * RS <= G::LDS_F, "staged results fit the flux slots");
     const int dim1 = d == 0 ? 1 : (d == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
     float* Fs = (float*)((double*)(lds_base + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES) + G::LDS_X) + G::LDS_D;
     // pencil of axis d through transverse points ll (along dim1) and nn (along dim2)
@@ -731,7 +738,7 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
                 o[d] = f;
                 o[dim1] = o1;
                 o[dim2] = o2;
-                Fs[(t * NVP + v) * TPE + o[0] + N * (o[1] + N * o[2])] = acc[ct][i];
+                Fs[(t * NVP + v) * RS + o[0] + N * (o[1] + N * o[2])] = acc[ct][i];
             }
         }
     }
@@ -769,7 +776,7 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
         double* ob = d == 0 ? a.Ffv[t][0] : (d == 1 ? a.Ffv[t][1] : a.Ffv[t][2]);
 #pragma unroll
         for (int v = 0; v < NVP; v++) {
-            const double val = X2[NVP + d * NVP + v] + (double)res[(t * NVP + v) * TPE + pos];
+            const double val = X2[NVP + d * NVP + v] + (double)res[(t * NVP + v) * RS + pos];
             ob[(size_t)v * vstride + o0] = val;
             if (wrap) ob[(size_t)v * vstride + ow] = val;
         }
@@ -782,7 +789,7 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
 // for the FV update (mfv_faces), nothing else. Compiled apart, the predictor
 // needs fewer VGPRs (3 waves per SIMD at 3-D p=3 instead of 2).
 template <int N, int DIM, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_MSWEEP_WAVES))) k_msweep(MArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 0 && N == 4 && DIM == 3 ? 3 : SPDK_MSWEEP_WAVES))) k_msweep(MArgs a) {
     constexpr bool CONS = MODE == 1;
     using G = MT<N, DIM>;
     constexpr int M = G::M, NT = G::NT, TPE = G::TPE, EPB = G::EPB, T = G::T, FSZ = G::FSZ;
@@ -980,11 +987,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                 }
         }
         __syncthreads();
-        // traces of the new state into this element's older buffer: task = (t, d, q)
+        // traces of the new state into this element's older buffer: task =
+        // (d, t, q), q fastest: at 3-D p=3 each 64-lane pass is one axis
         const int wb = act_e ? (a.ver[(size_t)a.E + e] < a.ver[e] ? 1 : 0) : 0;
         if (act_e) {
             for (int task = s; task < T * DIM * NT; task += TPE) {
-                const int t = task / (DIM * NT), r = task % (DIM * NT), d = r / NT, q = r % NT;
+                const int d = task / (T * NT), t = (task / NT) % T, q = task % NT;
                 const Geo gt = geo<N, DIM>(d, q);
 #pragma unroll
                 for (int v = 0; v < NVP; v++) {
@@ -1003,12 +1011,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPDK_M
                 }
             }
         }
-        __syncthreads();
-        if (act_e) Fs[s] = res;
-        __syncthreads();
+        // element maximum of the point residuals (all >= +0; a NaN stays)
+        float re = 0.f;
+        if constexpr (TPE == 64) {  // one element per wave: across the lanes
+            re = res;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) re = nan_maxf(re, __shfl_xor(re, off, 64));
+        } else {
+            __syncthreads();
+            if (act_e) Fs[s] = res;
+            __syncthreads();
+            if (act_e && s == 0)
+                for (int j = 0; j < TPE; j++) re = nan_maxf(re, Fs[j]);
+        }
         if (act_e && s == 0) {
-            float re = 0.f;
-            for (int j = 0; j < TPE; j++) re = nan_maxf(re, Fs[j]);
             double sc = 0.0;
 #pragma unroll
             for (int v = 0; v < NVP; v++) sc = fmax(sc, fabs(X[v]));

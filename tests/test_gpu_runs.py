@@ -108,6 +108,11 @@ CASES = {
 for _k in ("kh2d_p4_fallback", "blast2d_fallback", "blast3d_fallback", "sod1d_gradfree", "shu_osher1d",
            "implosion2d_reflective", "square2d_knobs", "sedov_cells2d", "user2d"):
     CASES[_k + "_fusedpred"] = (CASES[_k][0], "fused")
+# The FV update keeps its state in the FV cell layout by default (p <= 5);
+# job/fv_layout=sd runs the SD-layout U_cv sequence: both against the oracle.
+for _k in ("kh2d_p4_fallback", "blast2d_fallback_rk3", "blast3d_fallback", "rti2d_gravity_rk3", "sod1d_gradfree",
+           "implosion2d_reflective"):
+    CASES[_k + "_sdlayout"] = (CASES[_k][0] + ["job/fv_layout=sd"], CASES[_k][1])
 
 
 @pytest.mark.parametrize("name", sorted(CASES))

@@ -115,6 +115,38 @@ def mixed_sweep_bytes(p, dim, E, corrector=False):
     return float(first + (T - 2) * middle) * E, T - 1
 
 
+def face_sweep_bytes(p, dim, E, mixed):
+    """Compulsory HBM bytes of the flux-evaluation sweep that forms the FV face
+    integrals of every ADER node (the fallback step's last sweep): the
+    predictor slices in (T V TPE; FP32 deviations in the mixed form, FP64
+    otherwise), their face traces (2 dim NT T V, same type), the element means
+    (mixed: V 8), and the face arrays out: per element and axis its TPE lower
+    faces, V variables, T nodes, FP64 (T dim V TPE 8)."""
+    n = T = p + 1
+    V = 5
+    TPE, NT = n ** dim, n ** (dim - 1)
+    w = 4 if mixed else 8
+    return float(T * V * TPE * w + 2 * dim * NT * T * V * w + (V * 8 if mixed else 0) + T * dim * V * TPE * 8) * E
+
+
+def fp64_sweep_bytes(p, dim, E):
+    """Compulsory HBM bytes of the p predictor sweeps of the FP64 element-tile
+    kernel in one ADER step (each array once per sweep, FP64):
+      first sweep  : U_sp (V TPE) + neighbour traces (2 dim NT V) + slices
+                     out (T V TPE) + traces out (2 dim NT T V)
+      later sweeps : slices in + U_sp + traces in (T slices) + slices out +
+                     traces out.
+    Returns (bytes, sweeps)."""
+    n = T = p + 1
+    V = 5
+    TPE, NT = n ** dim, n ** (dim - 1)
+    U, S = V * TPE * 8, T * V * TPE * 8
+    tr1, trT = 2 * dim * NT * V * 8, 2 * dim * NT * T * V * 8
+    first = U + tr1 + S + trT
+    later = S + U + trT + S + trT
+    return float(first + (T - 2) * later) * E, T - 1
+
+
 def hbm_traffic(n, steps=1):
     """HBM bytes per predictor-sweep launch of the fixed-count mixed step, from two
     rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch). Each

@@ -108,6 +108,14 @@ __device__ __forceinline__ int spidx(int d, int ll, int q) {
 // reference's order, so the faces equal the stage kernels' bit for bit.
 __device__ __forceinline__ int el_of(int tid, int tpe) { return tid / tpe; }
 
+// Element block of a face-integral sweep workgroup: workgroups go to the 8
+// XCDs in turn and XCD x takes the x-th contiguous eighth of the blocks, so
+// the face rows that neighbouring blocks share (rows are not line-aligned)
+// are written through one L2. The grid is padded to a multiple of 8.
+__device__ __forceinline__ int face_sweep_block() {
+    return (int)((blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8);
+}
+
 template <int N, int DIM>
 __device__ __forceinline__ void fv_faces(const SweepArgs& a, double* scr, const double* Fs, int t, int s, bool live,
                                          int ei, int ej, int ek) {
@@ -167,7 +175,7 @@ __device__ __forceinline__ void fv_faces(const SweepArgs& a, double* scr, const 
                 __syncthreads();
                 const int tid = threadIdx.x, xx = tid % 16, row = tid / 16;
                 const int e2l = xx / N, ox = xx % N, oy = row % N, oz = row / N;
-                const int e2 = blockIdx.x * G::EPB + e2l;
+                const int e2 = face_sweep_block() * G::EPB + e2l;
                 if (e2 < a.E) {
                     const int ec2[3] = {e2 % a.Ne[0], (e2 / a.Ne[0]) % a.Ne[1], e2 / (a.Ne[0] * a.Ne[1])};
                     bool own2 = true;
@@ -227,7 +235,9 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     const int tid = threadIdx.x;
     const int el = tid / TPE;      // element slot in the tile
     const int s = tid % TPE;       // solution point within the element
-    const int e = blockIdx.x * EPB + el;
+    const int bx = FVF ? face_sweep_block() : (int)blockIdx.x;
+    if (FVF && bx * EPB >= a.E) return;  // padding of the XCD-ordered grid (uniform per block)
+    const int e = bx * EPB + el;
     const bool live = el < EPB && e < a.E;
     double* us = lds + (el < EPB ? el : 0) * G::LDS_ELEM;
     double* Fs = us + G::LDS_U;
@@ -689,7 +699,7 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
         bool rec = c->timing && !x && c->ev_used < spdk_ctx::kEvPool;
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
         if (args.Ffv[0][0])
-            hipLaunchKernelGGL((k_sweep<N, DIM, T, true>), grid, block, lds, c->stream, args);
+            hipLaunchKernelGGL((k_sweep<N, DIM, T, true>), dim3((L.blocks + 7) / 8 * 8), block, lds, c->stream, args);
         else
             hipLaunchKernelGGL((k_sweep<N, DIM, T, false>), grid, block, lds, c->stream, args);
         SPDK_LAUNCH_CHECK("k_sweep");

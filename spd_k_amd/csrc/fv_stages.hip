@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(BS) k_fv_state_in(const double* __restrict__ U
     using TL = ElTile<Q, D>;
     constexpr int TPE = TL::TPE, EPB = TL::EPB;
     __shared__ double b0[EPB * TPE], b1[EPB * TPE], sM[Q * Q];
-    const uint32_t g = blockIdx.x;
+    const uint32_t g = xcd_block();  // neighbouring groups share FV row lines: one L2
     if (g >= groups) return;
     const int tid = threadIdx.x, el = tid / TPE, sp = tid % TPE;
     const int row = (int)(g / gpr), i0 = (int)(g % gpr) * EPB;
@@ -1335,7 +1335,8 @@ int spdk_fv_state_from_sd(spdk_ctx* c, const double* U_sp, spdk_sd_shape s, doub
         constexpr int EPB = ElTile<QQ, DD>::EPB;                                                                      \
         const int gpr = (ex + EPB - 1) / EPB;                                                                         \
         const uint32_t groups = (uint32_t)((size_t)gpr * ey * ez);                                                    \
-        if (groups) hipLaunchKernelGGL((k_fv_state_in<QQ, DD>), dim3(groups), dim3(BS), 0, c->stream, U_sp, SD(s), U_fv, \
+        if (groups) hipLaunchKernelGGL((k_fv_state_in<QQ, DD>), dim3((groups + kXcd - 1) / kXcd * kXcd), dim3(BS), 0,  \
+                                       c->stream, U_sp, SD(s), U_fv,                                                 \
                                        FV(sc), M, ldm, gx, gy, gz, c->fgh[0], c->fgh[1], c->fgh[2], ex, ey, gpr,     \
                                        groups);                                                                      \
         break;                                                                                                        \

@@ -796,7 +796,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int tid = threadIdx.x;
     const int el = tid / TPE, s = tid % TPE;
-    const int blk = a.blist ? a.blist[blockIdx.x] : (int)blockIdx.x;
+    // MODE 2: workgroups go to the 8 XCDs in turn; XCD x takes the x-th
+    // contiguous eighth of the element blocks, so the face rows that
+    // neighbouring blocks share (rows are not line-aligned) are written
+    // through one L2. The grid is a multiple of 8; blocks past the end exit.
+    int blk;
+    if (a.blist) {
+        blk = a.blist[blockIdx.x];
+    } else if (MODE == 2) {
+        blk = (int)((blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8);
+        if (blk * EPB >= a.E) return;  // uniform per block
+    } else {
+        blk = (int)blockIdx.x;
+    }
     const int e = blk * EPB + el;
     const bool live = el < EPB && e < a.E;
     unsigned char* base = lds_raw + (size_t)(el < EPB ? el : 0) * G::ELEM_BYTES;
@@ -1728,7 +1740,8 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         bool rec = c->timing && c->ev_used < spdk_ctx::kEvPool && !args.Ffv[0][0];
         if (rec) SPDK_CHECK(hipEventRecord(c->evs[c->ev_used][0], c->stream));
         if (args.Ffv[0][0])
-            hipLaunchKernelGGL((k_msweep<N, DIM, 2>), dim3(nb), block, lds, c->stream, args);
+            hipLaunchKernelGGL((k_msweep<N, DIM, 2>), dim3(args.blist ? nb : (nb + 7) / 8 * 8), block, lds, c->stream,
+                               args);
         else if (!args.out_pred || args.U_new)
             hipLaunchKernelGGL((k_msweep<N, DIM, 1>), dim3(nb), block, lds, c->stream, args);
         else
@@ -1893,6 +1906,13 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
         for (int d = 0; d < 3; d++) { sa.fvn[d] = fvo->fvn[d]; sa.fgh[d] = fvo->fgh[d]; }
         sa.SS = c->mixed_mats + (size_t)c->m * c->n + 3 * (size_t)c->n * c->m;
         sa.fv_valu = fvo->valu;
+        // no residual or activity partials: the grid is padded to a multiple
+        // of 8 workgroups (XCD order), beyond the partial arrays' length
+        sa.res_part = nullptr;
+        sa.cnt_part = nullptr;
+        sa.big_in = nullptr;
+        sa.big_out = nullptr;
+        sa.blist = nullptr;
         if ((rc = sweep(sa, L.blocks))) return rc;
     }
     if (!predict_only) {

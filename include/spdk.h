@@ -221,7 +221,9 @@ int spdk_fv_update_blended(spdk_ctx* ctx, double* U_new, double* U_cv, spdk_sd_s
  * from_sd: U_fv = transform_a_to_b(U_sp, sp -> cv) of the active elements
  *   (src/hydro_ader.hpp:501), written at their FV cell positions.
  * update_prims: the update = 0 call reading U_fv: U_next, W_new =
- *   prim(U_next), W_old = prim(U_fv) (spdk_fv_update_prims).
+ *   prim(U_next), W_old = prim(U_fv) (spdk_fv_update_prims); of W_new only
+ *   the variables in the bit mask w_new_vars are written (bit v: variable v;
+ *   trouble detection and PAD read density and pressure, 0x11).
  * update_blended: the update = 1 call: rewrites U_next only in cells whose
  *   theta or a face neighbour's theta is nonzero (every cell with gravity);
  *   the caller then swaps U_fv and U_next (U_cv = U_new without a copy).
@@ -234,7 +236,7 @@ int spdk_fv_update_blended(spdk_ctx* ctx, double* U_new, double* U_cv, spdk_sd_s
 int spdk_fv_state_from_sd(spdk_ctx* ctx, const double* U_sp, spdk_sd_shape s, double* U_fv, spdk_fv_shape sc);
 int spdk_fv_state_update_prims(spdk_ctx* ctx, double* U_next, double* W_new, double* W_old, const double* U_fv,
                                const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id,
-                               double dt);
+                               double dt, int w_new_vars);
 int spdk_fv_state_update_blended(spdk_ctx* ctx, double* U_next, const double* U_fv, const double* Fx, const double* Fy,
                                  const double* Fz, const double* theta, spdk_fv_shape sc, int t_id, double dt);
 int spdk_fv_state_to_sd(spdk_ctx* ctx, const double* U_fv, spdk_fv_shape sc, double* U_sp, spdk_sd_shape s, double* W_cv,

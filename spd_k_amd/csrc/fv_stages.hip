@@ -397,14 +397,16 @@ __global__ void __launch_bounds__(BS) k_fv_state_in(const double* __restrict__ U
 
 // The update = 0 call on the FV state (k_fv_update_prims with U_cv read in
 // the FV cell layout): U_next, W_new = prim(U_next), W_old = prim(U_fv).
-// Variable 5 (zero throughout) is neither read nor written.
+// Variable 5 (zero throughout) is neither read nor written; of W_new only
+// the variables in wn_vars are written (trouble detection reads density and
+// pressure).
 __global__ void __launch_bounds__(BS) k_fv_state_update_prims(double* __restrict__ Un, double* __restrict__ Wn,
                                                               double* __restrict__ Wo, const double* __restrict__ Ucv,
                                                               const double* __restrict__ Fx,
                                                               const double* __restrict__ Fy,
                                                               const double* __restrict__ Fz, FV sc, FV sx, FV sy, FV sz,
                                                               Geo geo, FvUpd a, double gm, int fx, int fy, int fz, int Mx,
-                                                              int My, uint32_t total) {
+                                                              int My, unsigned wn_vars, uint32_t total) {
     uint32_t idx = xcd_block() * BS + threadIdx.x;
     if (idx >= total) return;
     const int I = fx + (int)(idx % Mx), J = fy + (int)((idx / Mx) % My), K = fz + (int)(idx / ((uint32_t)Mx * My));
@@ -433,7 +435,7 @@ __global__ void __launch_bounds__(BS) k_fv_state_update_prims(double* __restrict
     prim(uo, wo, gm);
 #pragma unroll
     for (int v = 0; v < NVP; v++) {
-        Wn[v * pl + o] = wn[v];
+        if ((wn_vars >> v) & 1u) Wn[v * pl + o] = wn[v];
         Wo[v * pl + o] = wo[v];
     }
 }
@@ -1354,7 +1356,7 @@ int spdk_fv_state_from_sd(spdk_ctx* c, const double* U_sp, spdk_sd_shape s, doub
 
 int spdk_fv_state_update_prims(spdk_ctx* c, double* U_next, double* W_new, double* W_old, const double* U_fv,
                                const double* Fx, const double* Fy, const double* Fz, spdk_fv_shape sc, int t_id,
-                               double dt) {
+                               double dt, int w_new_vars) {
     if (!c || !U_next || !W_new || !W_old || !U_fv || !Fx) {
         set_error("fv_state_update_prims: null argument");
         return SPDK_E_ARG;
@@ -1368,7 +1370,7 @@ int spdk_fv_state_update_prims(spdk_ctx* c, double* U_next, double* W_new, doubl
     hipLaunchKernelGGL(k_fv_state_update_prims, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, U_next, W_new, W_old, U_fv,
                        Fx, Fy ? Fy : Fx, Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
                        fv_stag(sc, 2, a.act[2]), geo_of(c), a, c->prm.gamma, c->fgh[0], c->fgh[1], c->fgh[2], Mx, My,
-                       total);
+                       (unsigned)w_new_vars & 0x1fu, total);
     SPDK_LAUNCH_CHECK("fv_state_update_prims");
     return SPDK_OK;
 }

@@ -409,7 +409,8 @@ struct HydroRun {
                 for (int d = 0; d < 3; d++)
                     if (cfg.act[d]) CK(spdk_fv_face_integral(ctx, F_fp[d], s_fp[d], F_fv[d], s_fvd[d], T_fp[d], a, d));
             }
-            CK(spdk_fv_state_update_prims(ctx, B, W_new, W_old, A, Fv[0], Fv[1], Fv[2], s_fv, a, dt));
+            // W_new: the detected variables and density and pressure (PAD)
+            CK(spdk_fv_state_update_prims(ctx, B, W_new, W_old, A, Fv[0], Fv[1], Fv[2], s_fv, a, dt, mask | 0x11));
             CK(spdk_fill_fv_ghosts(ctx, W_old, s_fv));
             CK(spdk_fill_fv_ghosts(ctx, W_new, s_fv));
             CK(spdk_detect_troubles_bits(ctx, W_new, W_old, troubles, abits, s_fv, 1, mask));

@@ -187,9 +187,10 @@ def hbm_traffic(n, steps=1):
             "fetch_scale": vals["FETCH_SIZE"][1], "write_scale": vals["WRITE_SIZE"][1]}
 
 
-def cpu_baseline(sample_n=24, steps=3):
+def cpu_baseline(sample_n=24, steps=12):
     """The CPU oracle (OpenMP, every host thread OMP_NUM_THREADS allows) on a
-    sample of the same workload: sample_n^3 elements, p=3, `steps` steps."""
+    sample of the same workload: sample_n^3 elements, p=3, `steps` steps (about
+    10 s of CPU work on 16 threads)."""
     exe = os.path.join(ROOT, "oracle", "_build", "spdk_oracle_omp")
     if not os.path.exists(exe):
         return None

@@ -418,7 +418,12 @@ typedef struct spdk_picard {
                        (from the zero-order guess and the cap p, that is the
                        fixed-count predictor); from an external or learned
                        guess it restarts from the zero-order guess with the
-                       fixed count p                                           */
+                       fixed count p, unless guess_keep                        */
+    int guess_keep; /* external or learned guess: 1 = an element reaching the
+                       cap keeps its state, as from the zero-order guess (no
+                       restart). Loses accuracy where the guess is poor: the
+                       KH-trained network on the 3-D sine wave gives 130x the
+                       FP64 error with it, 1.001x with the restart            */
 } spdk_picard;
 typedef struct spdk_picard_stats {
     long long steps;     /* accumulated by spdk_hydro_step_mixed                 */

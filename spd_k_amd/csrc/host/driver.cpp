@@ -238,6 +238,7 @@ struct HydroRun {
             pc.guess = SPDK_GUESS_ZERO;
             pc.tol = cfg.picard_tol;
             pc.max_sweeps = cfg.picard_max;
+            pc.guess_keep = cfg.guess_keep ? 1 : 0;
             if (cfg.guess == "mlp") {
                 MlpFile f = load_mlp(cfg.mlp_weights);
                 if (f.p != cfg.p || f.dim != cfg.ndim)

@@ -152,6 +152,11 @@ std::string configure(Params& pin, RunConfig& c) {
     if (c.picard != "fixed" && c.picard != "residual") return "job/picard must be fixed or residual";
     if (pin.has("job/picard_tol")) c.picard_tol = pin.get<double>("job/picard_tol");
     if (pin.has("job/picard_max")) c.picard_max = pin.get<int>("job/picard_max");
+    {
+        const std::string gc = pin.get_or<std::string>("job/guess_cap", "restart");
+        if (gc != "restart" && gc != "keep") return "job/guess_cap must be restart or keep";
+        c.guess_keep = gc == "keep";
+    }
     if (pin.has("job/guess")) c.guess = pin.get<std::string>("job/guess");
     if (c.guess != "zero" && c.guess != "mlp") return "job/guess must be zero or mlp";
     if (pin.has("job/mlp_weights")) c.mlp_weights = pin.get<std::string>("job/mlp_weights");

@@ -59,6 +59,7 @@ struct RunConfig {  // src/global.hpp:41-58, src/main.cpp:126-189
     std::string picard = "fixed";    // job/picard: fixed (p predictor sweeps) | residual
     double picard_tol = 1e-6;        // job/picard_tol: element residual tolerance
     int picard_max = 0;              // job/picard_max: per-element cap of predictor sweeps (0: the fixed count p)
+    bool guess_keep = false;         // job/guess_cap=keep: from the learned guess, elements at the cap keep their state (else restart)
     std::string guess = "zero";      // job/guess: zero | mlp
     std::string mlp_weights;         // job/mlp_weights: trained network file (job/guess=mlp)
     std::string mode = "run";        // job/mode: run | train_guess

@@ -49,7 +49,8 @@
 // every element ends the step (its face fluxes come from one set of traces,
 // so conservation holds). From an external or learned guess, the elements
 // still unconverged at the cap restart from the zero-order guess and get
-// the fixed count again (k_mrestart), the fallback of that form.
+// the fixed count again (k_mrestart), the fallback of that form; or, with
+// guess_keep, they keep the state of the cap as from the zero-order guess.
 //
 // Trace buffers. Each element writes its traces to the older of two buffers
 // and records the sweep number (ver); readers take, per neighbour, the
@@ -1851,7 +1852,7 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
                 nb = nlist;
                 continue;
             }
-            if (guess_phase) {
+            if (guess_phase && !pc->guess_keep) {
                 // unconverged from the guess: restart those elements from the
                 // zero-order guess with the fixed count
                 hipLaunchKernelGGL((k_mrestart<N, DIM>), grid, block, lds, c->stream, (const float*)a.d_old, a.D,

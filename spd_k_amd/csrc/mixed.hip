@@ -1467,11 +1467,15 @@ __global__ void k_glist(const float* __restrict__ S, int Eint, int* __restrict__
     if (e < Eint && S[(size_t)kGS * e + 6] > kAgate) list[atomicAdd(counter, 1)] = e;
 }
 __global__ void k_izero(int* p) { *p = 0; }
-// The zero-order guess in every slice of the predictor buffer.
-__global__ void k_gbroadcast(const float* __restrict__ d_old, float* __restrict__ D, int T, size_t slab) {
+// The zero-order guess in every slice of the predictor buffer, for the
+// elements at or below the amplitude gate (the others get the network's
+// guess from k_gdecode: the same predicate as k_glist).
+__global__ void k_gbroadcast(const float* __restrict__ d_old, float* __restrict__ D, int T, size_t slab,
+                             const float* __restrict__ S, int tpe) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slab; i += (size_t)gridDim.x * blockDim.x) {
-        const float x = d_old[i];  // slab = NVP * E * TPE
-        const size_t v = i / (slab / NVP), r = i % (slab / NVP);
+        const size_t v = i / (slab / NVP), r = i % (slab / NVP);  // slab = NVP * E * TPE
+        if (S[(size_t)kGS * (r / tpe) + 6] > kAgate) continue;
+        const float x = d_old[i];
         for (int t = 0; t < T; t++) D[((size_t)t * NVP + v) * (slab / NVP) + r] = x;
     }
 }
@@ -1645,7 +1649,7 @@ int mlp_guess(spdk_ctx* c, void* ws, size_t E, double dt) {
     SPDK_LAUNCH_CHECK("k_glist");
     const size_t slab = (size_t)NVP * E * TPE;
     hipLaunchKernelGGL(k_gbroadcast, dim3(gblocks((long)slab)), dim3(256), 0, c->stream, (const float*)(W + L.dold),
-                       (float*)(W + L.D), MT<N, DIM>::T, slab);
+                       (float*)(W + L.D), MT<N, DIM>::T, slab, (const float*)(W + L.gsc), TPE);
     SPDK_LAUNCH_CHECK("k_gbroadcast");
     SPDK_CHECK(hipMemcpyAsync(c->red_host, counter, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPDK_CHECK(hipStreamSynchronize(c->stream));

@@ -64,6 +64,7 @@ P = 3
 TOL = 1e-6        # element residual tolerance of the residual forms
 MATCH = 1.1       # matched L1 error: within this factor of the FP64 form's
 WEIGHTS = os.path.join(ROOT, "spd_k_amd", "weights", "kh3d_p3.mlp")
+WEIGHTS_2D_P4 = os.path.join(ROOT, "spd_k_amd", "weights", "kh2d_p4.mlp")  # cfg4's network (106 -> 256 -> 256 -> 500)
 DRIVER = os.path.join(ROOT, "spd_k_amd", "_build", "spdk")
 
 
@@ -288,7 +289,9 @@ def all_configs(steps=6, warmup=2):
            stage sequence and the default (fused predictor and flux sweep,
            stage FV update), `steps` steps;
       cfg4 2-D KH 512^2 p=4 (seeded perturbation, CFL 0.5 without the
-           fallback), FP64 / mixed / residual (1e-6 and 1e-10) forms.
+           fallback), FP64 / mixed / residual (1e-6 and 1e-10) forms, and the
+           residual forms from the learned guess (spd_k_amd/weights/kh2d_p4.mlp)
+           with their sweep histograms.
     cfg3 and cfg5 are the default run's workloads."""
     import spd_k_amd
     from spd_k_amd.outputs import l1_vs_exact_sine, output_indices
@@ -302,6 +305,9 @@ def all_configs(steps=6, warmup=2):
                "ms_per_step": r["evolution_seconds"] / r["steps"] * 1e3, "steps": r["steps"], "dofs": r["dofs"]}
         if r.get("picard_steps"):
             rec["sweeps_per_step"] = r["picard_sweeps"] / r["picard_steps"]
+            h = r.get("picard_hist") or []
+            if any(h):  # elements by predictor sweeps to the tolerance (bin 31: capped)
+                rec["histogram"] = {("capped" if b == 31 else str(b)): int(n) for b, n in enumerate(h) if n}
         return rec
 
     with tempfile.TemporaryDirectory() as d:
@@ -328,6 +334,12 @@ def all_configs(steps=6, warmup=2):
                             ("residual_fp32_tol1e-10", ["job/precision=mixed", "job/picard=residual",
                                                         "job/picard_tol=1e-10"])):
             c4[name] = timed(kh2 + extra, os.path.join(d, "c4_" + name))
+        if os.path.exists(WEIGHTS_2D_P4):
+            for tol in ("1e-6", "1e-10"):
+                c4[f"learned_fp32_tol{tol}"] = timed(kh2 + ["job/precision=mixed", "job/picard=residual",
+                                                            f"job/picard_tol={tol}", "job/guess=mlp",
+                                                            f"job/mlp_weights={WEIGHTS_2D_P4}"],
+                                                     os.path.join(d, "c4_learned" + tol))
         out["cfg4_kh2d_512_p4"] = c4
     return out
 

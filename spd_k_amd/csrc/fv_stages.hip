@@ -440,6 +440,64 @@ __global__ void __launch_bounds__(BS) k_fv_state_update_prims(double* __restrict
     }
 }
 
+// The same update with z active, one thread per (x, y) column walking KC
+// cells up in z: the z face shared by two cells is read once and carried in
+// registers (in the one-thread-per-cell form the two readers are a plane
+// apart in dispatch order, and the second read misses L2). Same arithmetic
+// per cell.
+template <int KC>
+__global__ void __launch_bounds__(BS) k_fv_state_update_prims_z(double* __restrict__ Un, double* __restrict__ Wn,
+                                                                double* __restrict__ Wo, const double* __restrict__ Ucv,
+                                                                const double* __restrict__ Fx,
+                                                                const double* __restrict__ Fy,
+                                                                const double* __restrict__ Fz, FV sc, FV sx, FV sy,
+                                                                FV sz, Geo geo, FvUpd a, double gm, int fx, int fy,
+                                                                int fz, int Mx, int My, int Mz, unsigned wn_vars,
+                                                                uint32_t total) {
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
+    if (idx >= total) return;  // total = Mx * My * ceil(Mz / KC)
+    const int I = fx + (int)(idx % Mx), J = fy + (int)((idx / Mx) % My);
+    const int k0 = (int)(idx / ((uint32_t)Mx * My)) * KC;
+    const size_t pl = sc.plane();
+    double flo[NVP];  // the lower z face of the current cell
+#pragma unroll
+    for (int v = 0; v < NVP; v++) flo[v] = Fz[sz.off(v, fz + k0, J, I)];
+    for (int kk = 0; kk < KC && k0 + kk < Mz; kk++) {
+        const int K = fz + k0 + kk;
+        const size_t o = sc.off(0, K, J, I);
+        double uo[NVP], un[NVP];
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            double h, F = 0;
+            uo[v] = Ucv[v * pl + o];
+            if (a.act[0]) { h = geo.f[0][I + 1] - geo.f[0][I]; F += (Fx[sx.off(v, K, J, I + 1)] - Fx[sx.off(v, K, J, I)]) / h; }
+            if (a.act[1]) { h = geo.f[1][J + 1] - geo.f[1][J]; F += (Fy[sy.off(v, K, J + 1, I)] - Fy[sy.off(v, K, J, I)]) / h; }
+            const double fup = Fz[sz.off(v, K + 1, J, I)];
+            h = geo.f[2][K + 1] - geo.f[2][K];
+            F += (fup - flo[v]) / h;
+            flo[v] = fup;
+            if (a.grav) {
+                double S = 0.0;
+                if (v == 1) S = uo[0] * a.g[0];
+                else if (v == 2) S = uo[0] * a.g[1];
+                else if (v == 3) S = uo[0] * a.g[2];
+                else if (v == 4) S = uo[1] * a.g[0] + uo[2] * a.g[1] + uo[3] * a.g[2];
+                F -= S;
+            }
+            un[v] = uo[v] - a.wdt * F;
+            Un[v * pl + o] = un[v];
+        }
+        double wn[NVP], wo[NVP];
+        prim(un, wn, gm);
+        prim(uo, wo, gm);
+#pragma unroll
+        for (int v = 0; v < NVP; v++) {
+            if ((wn_vars >> v) & 1u) Wn[v * pl + o] = wn[v];
+            Wo[v * pl + o] = wo[v];
+        }
+    }
+}
+
 // The update = 1 call on the FV state: a cell whose theta and face
 // neighbours' theta are all zero keeps the update = 0 result already in
 // U_next (its face fluxes did not change); the other cells are recomputed
@@ -1366,6 +1424,16 @@ int spdk_fv_state_update_prims(spdk_ctx* c, double* U_next, double* W_new, doubl
     if (t_id < 0 || t_id >= c->T) { set_error("fv_state_update_prims: slice"); return SPDK_E_ARG; }
     const FvUpd a = fv_upd(c, t_id, dt, 0);
     const int Mx = sc.Nx - 2 * c->fgh[0], My = sc.Ny - 2 * c->fgh[1], Mz = sc.Nz - 2 * c->fgh[2];
+    if (a.act[2]) {  // z columns of 8 cells per thread
+        constexpr int KC = 8;
+        const uint32_t tz = (uint32_t)((size_t)Mx * My * ((Mz + KC - 1) / KC));
+        hipLaunchKernelGGL(k_fv_state_update_prims_z<KC>, dim3(xcd_grid(tz)), dim3(BS), 0, c->stream, U_next, W_new,
+                           W_old, U_fv, Fx, Fy ? Fy : Fx, Fz, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
+                           fv_stag(sc, 2, a.act[2]), geo_of(c), a, c->prm.gamma, c->fgh[0], c->fgh[1], c->fgh[2], Mx,
+                           My, Mz, (unsigned)w_new_vars & 0x1fu, tz);
+        SPDK_LAUNCH_CHECK("fv_state_update_prims");
+        return SPDK_OK;
+    }
     const uint32_t total = (uint32_t)((size_t)Mx * My * Mz);
     hipLaunchKernelGGL(k_fv_state_update_prims, dim3(xcd_grid(total)), dim3(BS), 0, c->stream, U_next, W_new, W_old, U_fv,
                        Fx, Fy ? Fy : Fx, Fz ? Fz : Fx, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),

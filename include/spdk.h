@@ -258,7 +258,8 @@ int spdk_detect_troubles(spdk_ctx* ctx, const double* W_new, const double* W_old
 int spdk_detect_troubles_bits(spdk_ctx* ctx, const double* W_new, const double* W_old, double* troubles,
                               uint16_t* abits, spdk_fv_shape s, int pad, int mask);
 /* apply_blending + blending_ring, or theta_from_troubles when blending is off
- * (src/trouble_detection.cpp:191-242) */
+ * (src/trouble_detection.cpp:191-242). theta_tmp is scratch: with all three
+ * axes active both passes run in one tiled kernel that does not write it. */
 int spdk_blend_theta(spdk_ctx* ctx, const double* troubles, double* theta_tmp, double* theta, spdk_fv_shape s_tr);
 /* fallback_fluxes (src/hydro.cpp:753-788): MUSCL-Hancock + HLLC blended into
  * the FV face fluxes; dt_sub = w_t * dt */

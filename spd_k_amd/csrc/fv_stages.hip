@@ -892,6 +892,71 @@ __global__ void __launch_bounds__(BS) k_ring(const double* __restrict__ ti, doub
                 if (tp[dk * sz + dj * sy + di] > 0) ring = 0.25;
     to[c0] = smax(t, ring);
 }
+// apply_blending then blending_ring (3-D) in one pass on 16 x 4 x 4 cell
+// tiles: the aggregate troubles of the tile and a two-cell halo are staged in
+// LDS, theta_tmp is formed on the tile and a one-cell halo (0 outside cells
+// 1..N-2, where apply_blending does not write it), then the ring. The
+// troubles are 0 or 1 and every theta_tmp is >= +0, so the maxima do not
+// depend on the visiting order; k_blend's order is kept anyway. theta_tmp is
+// not written.
+struct ThTile {
+    static constexpr int TX = 16, TY = 4, TZ = 4;
+    static constexpr int RX = TX + 4, RY = TY + 4, RZ = TZ + 4;  // troubles with the two-cell halo
+    static constexpr int PX = TX + 2, PY = TY + 2, PZ = TZ + 2;  // theta_tmp with the one-cell halo
+    static_assert(TX * TY * TZ == BS, "one cell per thread");
+};
+__global__ void __launch_bounds__(BS) k_blend_ring_tile(const double* __restrict__ tr, double* __restrict__ th, FV s,
+                                                        int bx, int by, int bz) {
+    using TT = ThTile;
+    constexpr int NR = TT::RX * TT::RY * TT::RZ, NP = TT::PX * TT::PY * TT::PZ;
+    __shared__ double st[NR];
+    __shared__ double sp[NP];
+    const uint32_t b = xcd_block();
+    if (b >= (uint32_t)bx * by * bz) return;  // padding blocks of the XCD grid (uniform per block)
+    const int bi = (int)(b % bx), bj = (int)((b / bx) % by), bk = (int)(b / ((uint32_t)bx * by));
+    const int i0 = 1 + bi * TT::TX, j0 = 1 + bj * TT::TY, k0 = 1 + bk * TT::TZ;  // first output cell
+    const int tid = threadIdx.x;
+    const double* tv = tr + (size_t)(s.nv - 1) * s.plane();
+    for (int q = tid; q < NR; q += BS) {
+        const int gi = i0 - 2 + q % TT::RX, gj = j0 - 2 + (q / TT::RX) % TT::RY, gk = k0 - 2 + q / (TT::RX * TT::RY);
+        if (gi >= 0 && gi < s.Nx && gj >= 0 && gj < s.Ny && gk >= 0 && gk < s.Nz)
+            st[q] = tv[((size_t)gk * s.Ny + gj) * s.Nx + gi];
+    }
+    __syncthreads();
+    const double w[4] = {1.0, 0.75, 0.5, 0.375};
+    for (int q = tid; q < NP; q += BS) {
+        const int x = q % TT::PX, y = (q / TT::PX) % TT::PY, z = q / (TT::PX * TT::PY);
+        const int gi = i0 - 1 + x, gj = j0 - 1 + y, gk = k0 - 1 + z;
+        double t = 0;
+        if (gi >= 1 && gi <= s.Nx - 2 && gj >= 1 && gj <= s.Ny - 2 && gk >= 1 && gk <= s.Nz - 2) {
+            const double* c = st + ((z + 1) * TT::RY + (y + 1)) * TT::RX + (x + 1);
+#pragma unroll
+            for (int dk = -1; dk <= 1; dk++)
+#pragma unroll
+                for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+                    for (int di = -1; di <= 1; di++) {
+                        const int n0 = (di != 0) + (dj != 0) + (dk != 0);
+                        t = smax(t, w[n0] * c[(dk * TT::RY + dj) * TT::RX + di]);
+                    }
+        }
+        sp[q] = t;
+    }
+    __syncthreads();
+    const int tx = tid % TT::TX, ty = (tid / TT::TX) % TT::TY, tz = tid / (TT::TX * TT::TY);
+    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+    if (i > s.Nx - 2 || j > s.Ny - 2 || k > s.Nz - 2) return;
+    const double* p = sp + ((tz + 1) * TT::PY + (ty + 1)) * TT::PX + (tx + 1);
+    double ring = 0;
+#pragma unroll
+    for (int dk = -1; dk <= 1; dk++)
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+            for (int di = -1; di <= 1; di++)
+                if (p[(dk * TT::PY + dj) * TT::PX + di] > 0) ring = 0.25;
+    th[((size_t)k * s.Ny + j) * s.Nx + i] = smax(p[0], ring);
+}
 // theta_from_troubles (src/trouble_detection.cpp:213-221)
 __global__ void __launch_bounds__(BS) k_theta_raw(const double* __restrict__ tr, double* __restrict__ th, FV s, uint32_t total) {
     uint32_t idx = blockIdx.x * BS + threadIdx.x;
@@ -1607,7 +1672,15 @@ int spdk_detect_troubles_bits(spdk_ctx* c, const double* Wn, const double* Wo, d
 
 int spdk_blend_theta(spdk_ctx* c, const double* tr, double* th_tmp, double* th, spdk_fv_shape s) {
     int ax = c->prm.active[0], ay = c->prm.active[1], az = c->prm.active[2];
-    if (c->prm.blending) {
+    if (c->prm.blending && ax && ay && az) {  // 3-D: both passes on cell tiles, theta_tmp not written
+        const int bx = (s.Nx - 2 + ThTile::TX - 1) / ThTile::TX, by = (s.Ny - 2 + ThTile::TY - 1) / ThTile::TY;
+        const int bz = (s.Nz - 2 + ThTile::TZ - 1) / ThTile::TZ;
+        const size_t nb = (size_t)bx * by * bz;
+        hipLaunchKernelGGL(k_blend_ring_tile, dim3((unsigned)((nb + kXcd - 1) / kXcd * kXcd)), dim3(BS), 0, c->stream,
+                           tr, th, FV(s), bx, by, bz);
+        SPDK_LAUNCH_CHECK("blend_ring");
+        (void)th_tmp;
+    } else if (c->prm.blending) {
         uint32_t t2 = (uint32_t)((size_t)(s.Nz - 2 * az) * (s.Ny - 2 * ay) * (s.Nx - 2 * ax));
         const int a3[3] = {ax, ay, az};
         SPDK_AXES(k_blend, dim3(xcd_grid(t2)), dim3(BS), 0, c->stream, tr, th_tmp, FV(s), t2)

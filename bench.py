@@ -148,22 +148,28 @@ def fp64_sweep_bytes(p, dim, E):
     return float(first + (T - 2) * later) * E, T - 1
 
 
+READ_REQ = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
+
+
 def hbm_traffic(n, steps=1):
     """HBM bytes per predictor-sweep launch of the fixed-count mixed step, from two
-    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch). Each
-    counter is scaled by its reading on the warm-up's device copy of U_sp, a
-    plain copy of known size. None when rocprofv3 is absent or fails."""
+    rocprofv3 --pmc passes: the L2's memory read requests by size (32, 64 and
+    128 bytes, and their total), and WRITE_SIZE (KiB per dispatch; write
+    requests are at most 64 bytes, which it counts by size). FETCH_SIZE is not
+    used: it counts a 128-byte read request as 64 bytes. The warm-up's device
+    copy of U_sp, a plain copy of known size, checks both (copy_check: measured
+    / copied bytes, reads and writes). None when rocprofv3 is absent or fails."""
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
         return None
     E = (n + 2) ** 3
     copy_bytes = 6 * E * (P + 1) ** 3 * 8
-    vals = {}
+    per = {}  # counter -> (sweep average, largest copy reading)
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-            out = os.path.join(d, ctr)
-            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "p", "--", DRIVER] + kh_args(n) + \
-                  [f"time/nlim={steps}", "job/warmup=1", "job/precision=mixed"]
+        for name, ctrs in (("reads", READ_REQ), ("writes", ("WRITE_SIZE",))):
+            out = os.path.join(d, name)
+            cmd = [prof, "--pmc"] + list(ctrs) + ["--output-format", "csv", "-d", out, "-o", "p", "--", DRIVER] + \
+                kh_args(n) + [f"time/nlim={steps}", "job/warmup=1", "job/precision=mixed"]
             try:
                 r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
                                    text=True, timeout=180)
@@ -172,20 +178,29 @@ def hbm_traffic(n, steps=1):
             files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
             if r.returncode != 0 or not files:
                 return None
-            copy, sweeps = [], []
+            acc = {c: ({}, {}) for c in ctrs}  # counter -> (copy dispatches, sweep dispatches)
             for row in csv.DictReader(open(files[0])):
-                v = float(row["Counter_Value"])
-                if "copyBuffer" in row["Kernel_Name"]:
-                    copy.append(v)
-                elif "k_msweep<4, 3, 0>" in row["Kernel_Name"]:  # the predictor sweeps
-                    sweeps.append(v)
-            if not copy or not sweeps:
-                return None
-            scale = copy_bytes / (max(copy) * 1024.0)
-            vals[ctr] = (sum(sweeps) / len(sweeps) * 1024.0 * scale, scale)
-    return {"bytes_per_launch": vals["FETCH_SIZE"][0] + vals["WRITE_SIZE"][0],
-            "fetch_bytes": vals["FETCH_SIZE"][0], "write_bytes": vals["WRITE_SIZE"][0],
-            "fetch_scale": vals["FETCH_SIZE"][1], "write_scale": vals["WRITE_SIZE"][1]}
+                c = row["Counter_Name"]
+                if c not in acc:
+                    continue
+                k = row["Kernel_Name"]
+                slot = 0 if "copyBuffer" in k else (1 if "k_msweep<4, 3, 0>" in k else None)
+                if slot is not None:
+                    dd = acc[c][slot]
+                    dd[row["Dispatch_Id"]] = dd.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+            for c, (cp, sw) in acc.items():
+                if not cp or not sw:
+                    return None
+                per[c] = (sum(sw.values()) / len(sw), max(cp.values()))
+    size = {"TCC_EA0_RDREQ_32B_sum": 32.0, "TCC_EA0_RDREQ_64B_sum": 64.0, "TCC_EA0_RDREQ_128B_sum": 128.0}
+    reads = sum(per[c][0] * w for c, w in size.items())
+    reads_copy = sum(per[c][1] * w for c, w in size.items())
+    sized = sum(per[c][0] for c in size)  # requests of a known size, against the total
+    writes, writes_copy = per["WRITE_SIZE"][0] * 1024.0, per["WRITE_SIZE"][1] * 1024.0
+    return {"bytes_per_launch": reads + writes, "fetch_bytes": reads, "write_bytes": writes,
+            "read_requests": {c.split("_")[3]: per[c][0] for c in size},
+            "read_requests_sized_fraction": sized / per["TCC_EA0_RDREQ_sum"][0] if per["TCC_EA0_RDREQ_sum"][0] else None,
+            "copy_check": {"reads": reads_copy / copy_bytes, "writes": writes_copy / copy_bytes}}
 
 
 def cpu_baseline(sample_n=24, steps=12):

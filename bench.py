@@ -120,14 +120,15 @@ def face_sweep_bytes(p, dim, E, mixed):
     """Compulsory HBM bytes of the flux-evaluation sweep that forms the FV face
     integrals of every ADER node (the fallback step's last sweep): the
     predictor slices in (T V TPE; FP32 deviations in the mixed form, FP64
-    otherwise), their face traces (2 dim NT T V, same type), the element means
-    (mixed: V 8), and the face arrays out: per element and axis its TPE lower
-    faces, V variables, T nodes, FP64 (T dim V TPE 8)."""
+    otherwise), the lower neighbour's face trace per axis (each element solves
+    the Riemann problem at its lower faces only: dim NT T V, same type), the
+    element means (mixed: V 8), and the face arrays out: per element and axis
+    its TPE lower faces, V variables, T nodes, FP64 (T dim V TPE 8)."""
     n = T = p + 1
     V = 5
     TPE, NT = n ** dim, n ** (dim - 1)
     w = 4 if mixed else 8
-    return float(T * V * TPE * w + 2 * dim * NT * T * V * w + (V * 8 if mixed else 0) + T * dim * V * TPE * 8) * E
+    return float(T * V * TPE * w + dim * NT * T * V * w + (V * 8 if mixed else 0) + T * dim * V * TPE * 8) * E
 
 
 def fp64_sweep_bytes(p, dim, E):

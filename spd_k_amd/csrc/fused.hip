@@ -77,6 +77,8 @@ struct SweepArgs {
     int out_pred;         // write predictor U_out + tr_out, else corrector
     int rk_first;         // RK: write U0 = U_sp
     int want_dt;          // corrector: compute cv primitives and dt partials
+    double g[3];          // constant gravity (hydro/g1..g3)
+    int grav;             // any component nonzero
 };
 
 __device__ __forceinline__ int fidx(int d, int ll, int q, int N, int NT, int M) {
@@ -397,6 +399,15 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
                 }
                 dudt[t][v] = acc;
             }
+            if (a.grav && !a.out_pred) {
+                // update_solution's gravity source (src/transforms.cpp:17-27,
+                // :364-367) from the ADER slice staged in LDS: rho g and m . g
+                const double rho = us[s], m1 = us[TPE + s], m2 = us[2 * TPE + s], m3 = us[3 * TPE + s];
+                dudt[t][1] -= rho * a.g[0];
+                dudt[t][2] -= rho * a.g[1];
+                dudt[t][3] -= rho * a.g[2];
+                dudt[t][4] -= m1 * a.g[0] + m2 * a.g[1] + m3 * a.g[2];
+            }
             if (a.F_x && !a.out_pred) {  // x flux-point fluxes of the final sweep (outputs)
                 for (int idx = s; idx < G::FSZ; idx += TPE) {
                     size_t o = (((size_t)t * NV) * E + e) * G::FSZ + idx;
@@ -416,6 +427,36 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
             for (int v = 0; v < NVP; v++) dudt[t][v] = dudt[0][v];
     }
 
+    if (a.out_pred && a.grav && live) {
+        // update_prediction's gravity source (src/transforms.cpp:17-27,
+        // :309-313). That kernel writes U_ader variable by variable, and the
+        // source reads U_ader back: the momenta's source takes the density
+        // just written, the energy's the momenta just written. Those values
+        // are formed here exactly as the predictor below forms them.
+        double rn[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            double du = 0;
+#pragma unroll
+            for (int l = 0; l < T; l++) du += dudt[l][0] * a.inv[t * T + l] * a.w[l] * a.dt;
+            rn[t] = u_old[0] - du;
+        }
+        double mn[T][3];
+#pragma unroll
+        for (int v = 1; v <= 3; v++) {
+#pragma unroll
+            for (int t = 0; t < T; t++) dudt[t][v] -= rn[t] * a.g[v - 1];
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                double du = 0;
+#pragma unroll
+                for (int l = 0; l < T; l++) du += dudt[l][v] * a.inv[t * T + l] * a.w[l] * a.dt;
+                mn[t][v - 1] = u_old[v] - du;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < T; t++) dudt[t][4] -= mn[t][0] * a.g[0] + mn[t][1] * a.g[1] + mn[t][2] * a.g[2];
+    }
     if (a.out_pred) {
         // ---- predictor (src/transforms.cpp:316-322) and traces of the new slices
 #pragma unroll
@@ -691,6 +732,8 @@ int run_step(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, do
     a.dt = dt;
     a.gm = c->prm.gamma;
     a.cfl = c->prm.cfl;
+    for (int d = 0; d < 3; d++) a.g[d] = c->prm.g[d];
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
     size_t lds = sizeof(double) * (size_t)G::EPB * G::LDS_ELEM;
     if (lds < sizeof(double) * G::BLOCK) lds = sizeof(double) * G::BLOCK;
     dim3 grid(L.blocks), block(G::BLOCK);
@@ -873,8 +916,8 @@ size_t spdk_fused_workspace_bytes(spdk_ctx* c, const int N[3]) {
         set_error("fused path: no tile variant for this degree/dimensionality");
         return 0;
     }
-    if (c->T > kMaxT || c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
-        set_error("fused path covers the inviscid, source-free update");
+    if (c->T > kMaxT || c->prm.nu != 0.0) {
+        set_error("fused path covers the inviscid update");
         return 0;
     }
     Layout L = v->layout((size_t)N[0] * N[1] * N[2], c->T);
@@ -883,8 +926,8 @@ size_t spdk_fused_workspace_bytes(spdk_ctx* c, const int N[3]) {
 
 // Checks of a whole fused step on U_sp of shape s.
 static int check_step_fused(spdk_ctx* c, const spdk_sd_shape& s, const Variant** vout) {
-    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
-        set_error("fused path covers the inviscid, source-free update");
+    if (c->prm.nu != 0.0) {
+        set_error("fused path covers the inviscid update");
         return SPDK_E_CONFIG;
     }
     const Variant* v = find_variant(c);
@@ -938,8 +981,8 @@ int spdk_hydro_step_fused_noalloc(spdk_ctx* c, double* U_sp, spdk_sd_shape s, do
 int spdk_fused_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, double dt, double* U_ader,
                        spdk_sd_shape s_ader) {
     if (!c || !U_sp || !ws || !U_ader) { set_error("fused predict: null argument"); return SPDK_E_ARG; }
-    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0 || c->T < 2) {
-        set_error("fused predict covers the inviscid, source-free ADER predictor");
+    if (c->prm.nu != 0.0 || c->T < 2) {
+        set_error("fused predict covers the inviscid ADER predictor");
         return SPDK_E_CONFIG;
     }
     const Variant* v = find_variant(c);
@@ -967,8 +1010,8 @@ int spdk_fused_predict(spdk_ctx* c, double* U_sp, spdk_sd_shape s, void* ws, dou
 // flux-point arrays of the active axes.
 static int check_flux_args(spdk_ctx* c, spdk_sd_shape s_ader, double* const F[3], const Variant** vout,
                            bool need_F = true) {
-    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0 || c->T < 2) {
-        set_error("fused flux evaluation covers the inviscid, source-free ADER step");
+    if (c->prm.nu != 0.0 || c->T < 2) {
+        set_error("fused flux evaluation covers the inviscid ADER step");
         return SPDK_E_CONFIG;
     }
     const Variant* v = find_variant(c);

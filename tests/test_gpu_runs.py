@@ -68,6 +68,16 @@ CASES = {
                                 "time/tlim=0.03", "output/dt=0.015"], "stages"),
     "rti2d_gravity_rk3": (["-i", os.path.join(op.INPUTS, "rti.athinput"), "mesh/nx2=16", "time/tlim=0.02",
                            "output/dt=0.01"], "stages"),
+    # gravity in the fused sweeps (predictor, corrector, RK stage) and the FV-state update
+    "rti2d_gravity_ader_fused": (["-i", os.path.join(op.INPUTS, "rti.athinput"), "mesh/nx2=16", "time/integrator=ader",
+                                  "time/tlim=0.02", "output/dt=0.01"], "fused"),
+    "rti2d_gravity_rk3_nofallback": (["-i", os.path.join(op.INPUTS, "rti.athinput"), "mesh/nx2=16",
+                                      "job/fallback=false", "time/tlim=0.02", "output/dt=0.01"], "fused"),
+    "kh3d_gravity_ader": (["-i", KH, "job/fallback=false", "mesh/nx1=6", "mesh/nx2=6", "mesh/nx3=4", "hydro/g1=0.3",
+                           "hydro/g2=-0.5", "hydro/g3=0.2", "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    "kh3d_gravity_ader_fallback": (["-i", KH, "mesh/nx1=6", "mesh/nx2=6", "mesh/nx3=4", "hydro/g1=0.3",
+                                    "hydro/g2=-0.5", "hydro/g3=0.2", "problem/perturb=random", "problem/amp=0.01",
+                                    "time/tlim=0.01", "output/dt=0.005"], "fused"),
     "square2d_knobs": (["-i", os.path.join(op.INPUTS, "square.athinput"), "fallback/NAD=delta",
                         "fallback/NAD_neighbors=1st", "fallback/SED=false", "fallback/blending=false",
                         "time/tlim=0.02", "output/dt=0.01"], "stages"),
@@ -600,3 +610,13 @@ def test_induction_2d_full_size_mixed(tmp_path):
     rel = float(np.abs(x - y).max() / np.abs(x).max())
     print(f"max |dB| / max |B| = {rel:.3e}")
     assert rel < 2e-5
+
+
+@pytest.mark.parametrize("name", ["kh3d_gravity_ader", "rti2d_gravity_rk3_nofallback", "kh3d_gravity_ader_fallback"])
+def test_gravity_takes_the_fused_sweeps(tmp_path, name):
+    """The gravity parity cases above run the element-tile sweeps, not the
+    stage sequence: the timed kernel of the run is k_sweep."""
+    import spd_k_amd
+    args, path = CASES[name]
+    r = spd_k_amd.run(args + [f"job/path={path}", "job/timing=true"], str(tmp_path / "run"))
+    assert r["kernel_name"] == "k_sweep" and r["kernel_launches"] > 0, r["kernel_name"]

@@ -334,7 +334,8 @@ int spdk_fill_fv_ghosts_aligned(spdk_ctx* ctx, double* U, spdk_fv_shape s, int a
 
 /* ---- fused element-tile path (hydro) -------------------------------------- */
 /* Device workspace for the fused path of a mesh with ghost-inclusive element
- * counts N[3]; all scratch lives in caller memory. */
+ * counts N[3]; all scratch lives in caller memory. The fused path covers the
+ * inviscid update, gravity included (0 and an error for nu > 0). */
 size_t spdk_fused_workspace_bytes(spdk_ctx* ctx, const int N[3]);
 /* One full SD step on U_sp (1,6,Nz,Ny,Nx,n,n,n) with the plain FP64 arithmetic
  * of ADER_step/RK_step + update_solution (src/hydro_ader.hpp:291-325),

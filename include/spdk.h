@@ -400,10 +400,12 @@ int spdk_fused_fluxes(spdk_ctx* ctx, const double* U_ader, spdk_sd_shape s_ader,
 /* The ADER step of src/hydro_ader.hpp:291-305 with the space-time state, its
  * face traces and the transform / divergence / predictor contractions in
  * FP32 (as deviations from each element's FP64 mean), and fluxes, Riemann
- * solves, the corrector accumulation, cv transform and dt in FP64. Not
- * bit-identical to the reference; its error against the FP64 path is
- * measured per configuration. No reference counterpart beyond the step it
- * replaces. */
+ * solves, the corrector accumulation, cv transform and dt in FP64. Constant
+ * gravity is a source in both: the predictor's in the FP32 deviations (the
+ * mean's part rounded once from FP64), the corrector's summed in FP64 from
+ * the final slices; nu > 0 is refused. Not bit-identical to the reference;
+ * its error against the FP64 path is measured per configuration. No
+ * reference counterpart beyond the step it replaces. */
 #define SPDK_GUESS_ZERO 0     /* U_ader = U at every node (copy_ader)             */
 #define SPDK_GUESS_EXTERNAL 1 /* caller-filled predictor buffer                   */
 #define SPDK_GUESS_MLP 2      /* the network attached with spdk_mixed_attach_mlp  */

@@ -144,6 +144,8 @@ struct MArgs {
     double gm, cfl;
     double tol;
     int sweep;
+    double g[3];    // constant gravity (hydro/g1..g3)
+    int grav;       // any component nonzero
     int path;       // 0: every slice is d_old (zero-order guess); 1: slices from D
     int tin_slices; // slices present in the readable trace buffers (1 or T)
     int out_pred;   // predictor (else corrector)
@@ -584,6 +586,13 @@ __device__ __forceinline__ void mfv_faces(const MArgs& a, const float* Fs, const
         own = own && ec[x] >= 1 && ec[x] <= (x == d ? (per ? a.Ne[x] - 2 : a.Ne[x] - 1) : a.Ne[x] - 2);
     const bool last = ec[d] == Nd - 1;
     const bool wrap = per && ec[d] == 1;  // also writes the last face
+    // A reflective wall's face (face 0 of the first interior element or of
+    // the upper ghost): the FP64 LLF of the mirrored pair is exactly zero in
+    // every component but the normal momentum, and is written so (the mean
+    // flux plus its FP32-rounded deviation would let mass and energy through
+    // at that rounding).
+    const bool refl = (d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2])) == SPDK_BC_REFLECTIVE;
+    const bool wall_el = refl && (ec[d] == 1 || last);
     const int dim1 = d == 0 ? 1 : (d == 1 ? 2 : 0), dim2 = dim1 == 0 ? 1 : (dim1 == 1 ? 2 : 0);
     const int nx = a.fvn[0] + (d == 0), ny = a.fvn[1] + (d == 1), nz = a.fvn[2] + (d == 2);
     const size_t vstride = (size_t)nz * ny * nx;
@@ -633,7 +642,7 @@ __device__ __forceinline__ void mfv_faces(const MArgs& a, const float* Fs, const
                     o[d] = f;
                     o[dim1] = o1;
                     o[dim2] = o2;
-                    const double val = fm + (double)acc[i];
+                    const double val = wall_el && f == 0 && v != 1 + d ? 0.0 : fm + (double)acc[i];
                     ob[fv_off(o[0], o[1], o[2])] = val;
                     if (wrap && f == 0) ob[fv_off(o[0], o[1], o[2], true)] = val;
                 }
@@ -677,7 +686,7 @@ __device__ __forceinline__ void mfv_faces(const MArgs& a, const float* Fs, const
             } else {
                 acc = Fp[fidx<N, DIM>(d, f, 0)];
             }
-            const double val = X[NVP + d * NVP + v] + (double)acc;
+            const double val = wall_el && f == 0 && v != 1 + d ? 0.0 : X[NVP + d * NVP + v] + (double)acc;
             ob[(size_t)v * vstride + off] = val;
             if (also_last) ob[(size_t)v * vstride + off_last] = val;
         }
@@ -761,6 +770,9 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
     const int f = o[d];
     if (!own || (ec[d] == Nd - 1 && f != 0)) return;
     const bool wrap = per && ec[d] == 1 && f == 0;
+    // a reflective wall's face: zero in every component but the normal momentum (as mfv_faces)
+    const bool wall = f == 0 && (ec[d] == 1 || ec[d] == Nd - 1) &&
+                      (d == 0 ? a.bc[0] : (d == 1 ? a.bc[1] : a.bc[2])) == SPDK_BC_REFLECTIVE;
     const int nx = a.fvn[0] + (d == 0), ny = a.fvn[1] + (d == 1), nz = a.fvn[2] + (d == 2);
     const size_t vstride = (size_t)nz * ny * nx;
     auto off = [&](int ex, int ey, int ez) -> size_t {
@@ -777,7 +789,7 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
         double* ob = d == 0 ? a.Ffv[t][0] : (d == 1 ? a.Ffv[t][1] : a.Ffv[t][2]);
 #pragma unroll
         for (int v = 0; v < NVP; v++) {
-            const double val = X2[NVP + d * NVP + v] + (double)res[(t * NVP + v) * RS + pos];
+            const double val = wall && v != 1 + d ? 0.0 : X2[NVP + d * NVP + v] + (double)res[(t * NVP + v) * RS + pos];
             ob[(size_t)v * vstride + o0] = val;
             if (wrap) ob[(size_t)v * vstride + ow] = val;
         }
@@ -979,6 +991,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
 #pragma unroll
             for (int v = 0; v < NVP; v++) dudt[t][v] = dudt[0][v];
 
+    if (!CONS && a.grav && act_e) {
+        // update_prediction's gravity source (src/transforms.cpp:17-27,
+        // :309-313), dudt -= S, in the reference's in-place order: the
+        // momenta's source takes the density just written, the energy's the
+        // momenta just written (formed here as the update below forms them).
+        // S = S(mean) + S(deviation): the mean part in FP64, rounded once.
+        float rn[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int l = 0; l < T; l++) acc += a.cdt[t * kMaxTm + l] * dudt[l][0];
+            rn[t] = dold[0] - acc;
+        }
+        float mn[T][3];
+#pragma unroll
+        for (int v = 1; v <= 3; v++) {
+            const float gv = (float)a.g[v - 1], smu = (float)(X[0] * a.g[v - 1]);
+#pragma unroll
+            for (int t = 0; t < T; t++) dudt[t][v] -= smu + rn[t] * gv;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                float acc = 0.f;
+#pragma unroll
+                for (int l = 0; l < T; l++) acc += a.cdt[t * kMaxTm + l] * dudt[l][v];
+                mn[t][v - 1] = dold[v] - acc;
+            }
+        }
+        const float smu4 = (float)(X[1] * a.g[0] + X[2] * a.g[1] + X[3] * a.g[2]);
+        const float g0 = (float)a.g[0], g1 = (float)a.g[1], g2 = (float)a.g[2];
+#pragma unroll
+        for (int t = 0; t < T; t++) dudt[t][4] -= smu4 + (mn[t][0] * g0 + mn[t][1] * g1 + mn[t][2] * g2);
+    }
     if (!CONS) {
         // ---- predictor: d_new(t) = d_old - sum_l cdt(t,l) dudt(l); residual vs the input state
         float res = 0.f;
@@ -1080,8 +1125,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
 #pragma unroll
         for (int t = 0; t < T; t++) du[v] += a.wdt[t] * (double)dudt[t][v];
     }
+    // update_solution's gravity source (src/transforms.cpp:364-367) from the
+    // final predictor slices, in FP64: sum_t w_t dt S_t, added after the
+    // conservative shift (which balances the fluxes only)
+    double sg[NVP] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (a.grav && act_e) {
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const float* Dt = Ds + (a.path ? t : 0) * NVP * PSZ + so;  // path 0: one slice, d_old
+            const double rho = X[0] + (double)Dt[0];
+            const double m1 = X[1] + (double)Dt[PSZ], m2 = X[2] + (double)Dt[2 * PSZ], m3 = X[3] + (double)Dt[3 * PSZ];
+            sg[1] += a.wdt[t] * (rho * a.g[0]);
+            sg[2] += a.wdt[t] * (rho * a.g[1]);
+            sg[3] += a.wdt[t] * (rho * a.g[2]);
+            sg[4] += a.wdt[t] * (m1 * a.g[0] + m2 * a.g[1] + m3 * a.g[2]);
+        }
+    }
     __syncthreads();
     cons_shift<N, DIM>(du, bal, wsp, a.wtot, (double*)Ds, X + G::XSHIFT, act_e, s);
+#pragma unroll
+    for (int v = 1; v < NVP; v++) du[v] -= sg[v];
 #pragma unroll
     for (int v = 0; v < NVP; v++) un[v] = act_e ? a.U_sp[(size_t)v * slabE + eo] - du[v] : 1.0;
     if (act_e)
@@ -1728,6 +1791,8 @@ int run_mixed(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt, d
     a.E = (int)E;
     a.gm = c->prm.gamma;
     a.cfl = c->prm.cfl;
+    for (int d = 0; d < 3; d++) a.g[d] = c->prm.g[d];
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
     a.tol = pc ? pc->tol : 0.0;
     unsigned char* conv = (unsigned char*)(W + L.conv);
     unsigned char* big[2] = {(unsigned char*)(W + L.big), (unsigned char*)(W + L.big) + E};
@@ -2004,6 +2069,8 @@ int run_iterate(spdk_ctx* c, double* U_sp, const int Ne[3], void* ws, double dt,
     a.E = (int)E;
     a.gm = c->prm.gamma;
     a.cfl = c->prm.cfl;
+    for (int d = 0; d < 3; d++) a.g[d] = c->prm.g[d];
+    a.grav = (a.g[0] != 0.0 || a.g[1] != 0.0 || a.g[2] != 0.0);
     for (int k = 0; k < nsweeps; k++) {
         MArgs s = a;
         s.cur = k + 1;
@@ -2056,8 +2123,8 @@ size_t spdk_mixed_workspace_bytes(spdk_ctx* c, const int N[3]) {
         set_error("mixed path: ADER hydro with a tile variant for this degree/dimensionality only");
         return 0;
     }
-    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
-        set_error("mixed path covers the inviscid, source-free update");
+    if (c->prm.nu != 0.0) {
+        set_error("mixed path covers the inviscid update (gravity included)");
         return 0;
     }
     return v->layout((size_t)N[0] * N[1] * N[2]).total;
@@ -2157,8 +2224,8 @@ int spdk_mixed_histogram(spdk_ctx* c, void* ws, const int N[3], unsigned long lo
 
 static int check_mixed(spdk_ctx* c, const void* U_sp, const spdk_sd_shape& s, const void* ws, const MVariant** out) {
     if (!c || !U_sp || !ws) { set_error("mixed: null argument"); return SPDK_E_ARG; }
-    if (c->prm.g[0] != 0.0 || c->prm.g[1] != 0.0 || c->prm.g[2] != 0.0 || c->prm.nu != 0.0) {
-        set_error("mixed path covers the inviscid, source-free update");
+    if (c->prm.nu != 0.0) {
+        set_error("mixed path covers the inviscid update (gravity included)");
         return SPDK_E_CONFIG;
     }
     const MVariant* v = find_mvariant(c);

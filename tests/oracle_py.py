@@ -152,7 +152,7 @@ def compare_dirs(a, b):
 
 
 # ---- readers of the output format (shared with the package) ----------------
-from spd_k_amd.outputs import grid, load_sd, output_indices  # noqa: E402
+from spd_k_amd.outputs import element_means, grid, load_sd, output_indices  # noqa: E402,F401
 
 
 def total_mass(outdir, i, prefix="W_cv"):

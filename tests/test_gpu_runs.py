@@ -314,6 +314,37 @@ def test_mixed_kh_close_to_fp64(tmp_path, dims):
     assert op.mass_drift(b) < 1e-13
 
 
+GRAV = ["hydro/g1=0.3", "hydro/g2=-0.5", "hydro/g3=0.2"]
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8, 3), (10, 12, 1, 4)])
+def test_mixed_gravity_close_to_fp64(tmp_path, dims):
+    """KH with random perturbation under constant gravity (3-D p=3, 2-D p=4,
+    no fallback): with the mixed path's gravity sources (FP32 deviations in
+    the predictor, an FP64 sum in the corrector) every primitive variable
+    stays within FP32-level relative distance of the FP64 path, the velocity
+    distance is under 1e-3 of what gravity itself changes (the FP64 run
+    without gravity), and mass is conserved to round-off."""
+    nx, ny, nz, p = dims
+    args = ["-i", KH, "job/fallback=false", f"mesh/nx1={nx}", f"mesh/nx2={ny}", f"mesh/nx3={nz}", f"mesh/p={p}",
+            "problem/perturb=random", "problem/amp=0.01", "time/tlim=0.05", "output/dt=0.025"]
+    grav = GRAV if nz > 1 else GRAV[:2]
+    a, b = _mixed_vs_fp64(tmp_path, args + grav)
+    c = str(tmp_path / "nograv")
+    op.run_driver(args, c)
+    g = op.grid(a)
+    i = op.output_indices(a)[-1]
+    for v in range(5) if nz > 1 else (0, 1, 2, 4):  # v_z is zero in 2-D
+        mean, mx = _rel(a, b, i, v)
+        x, y, z = op.load_sd(a, g, i, v), op.load_sd(b, g, i, v), op.load_sd(c, g, i, v)
+        dm, dg = float(np.abs(x - y).max()), float(np.abs(x - z).max())
+        print(f"var {v}: rel mean {mean:.2e} max {mx:.2e}; |mixed - fp64| {dm:.2e}, gravity's effect {dg:.2e}")
+        assert mean < 1e-6 and mx < 1e-5, (v, mean, mx)
+        if 1 <= v <= len(grav):
+            assert dm < 1e-3 * dg, (v, dm, dg)
+    assert op.mass_drift(b) < 1e-13
+
+
 def _picard(text):
     m = re.search(r"picard: (\d+) steps, ([0-9.]+) sweeps/step, (\d+) steps with capped elements", text)
     assert m, text[-500:]
@@ -436,6 +467,12 @@ FB_CASES = {
                 "hydro/gamma=1.6666666666667", "time/tlim=0.02", "output/dt=0.01"],
     "kh2d_p3": ["-i", KH, "mesh/nx1=24", "mesh/nx2=24", "problem/perturb=random", "problem/amp=0.01",
                 "time/tlim=0.1", "output/dt=0.05"],
+    "rti2d_gravity": ["-i", os.path.join(op.INPUTS, "rti.athinput"), "mesh/nx2=16", "time/integrator=ader",
+                      "time/tlim=0.05", "output/dt=0.025"],
+    "blast3d_reflective": ["-i", S, "job/fallback=true", "mesh/nx1=8", "mesh/nx2=8", "mesh/nx3=8",
+                           "mesh/x1_bc=reflective", "mesh/x2_bc=reflective", "mesh/x3_bc=reflective",
+                           "problem/problem=spherical_blast", "problem/radius=0.2", "hydro/gamma=1.6666666666667",
+                           "time/tlim=0.004", "output/dt=0.002"],
     "blast3d": ["-i", S, "job/fallback=true", "mesh/nx1=8", "mesh/nx2=8", "mesh/nx3=8",
                 "problem/problem=spherical_blast", "problem/radius=0.2", "hydro/gamma=1.6666666666667",
                 "time/tlim=0.004", "output/dt=0.002"],
@@ -488,6 +525,35 @@ def test_mixed_predictor_with_fallback(tmp_path, name, form):
     print(f"{name}: max rel diff {worst:.3e}, mask flips {flips:.2e}, flagged {float((ma != 0).mean()):.3e}")
     assert worst < (1e-4 if form == "fixed" else 2e-3)
     assert flips < 1e-2
+
+
+def test_mixed_blast3d_matched_accuracy(tmp_path):
+    """The 3-D blast at p=3 with the fallback to t=0.02 (the shock well into
+    the domain): the mixed form's element means of density and pressure are
+    within 1.1x the FP64 form's L1 distance from a run at twice the
+    resolution (FP64, 16^3 elements averaged onto the 8^3 ones), mass
+    conserved. Pointwise the two forms part after t ~ 0.013: FP32-level
+    differences are amplified at the shock (the FP64 run's own rounding
+    breaks the blast's symmetry at 1e-8..1e-6 by t=0.02, the mixed run's at
+    1e-2), so the check is on accuracy, not on distance (printed)."""
+    def args(n, extra=()):
+        return ["-i", S, "job/fallback=true", f"mesh/nx1={n}", f"mesh/nx2={n}", f"mesh/nx3={n}",
+                "problem/problem=spherical_blast", "problem/radius=0.2", "hydro/gamma=1.6666666666667",
+                "time/tlim=0.02", "output/dt=0.02", "output/fields=W_cv"] + list(extra)
+    ref, a, b = (str(tmp_path / x) for x in ("ref", "fp64", "mixed"))
+    op.run_driver(args(16), ref)
+    op.run_driver(args(8), a)
+    op.run_driver(args(8, ["job/precision=mixed"]), b)
+    for v in (0, 4):
+        R = op.element_means(ref, v).reshape(8, 2, 8, 2, 8, 2).mean(axis=(1, 3, 5))  # uniform elements
+        l1 = {}
+        for name, d in (("fp64", a), ("mixed", b)):
+            m = op.element_means(d, v)
+            l1[name] = float(np.abs(m - R).mean() / np.abs(R).mean())
+            asym = float(np.abs(m - m.transpose(1, 0, 2)).max())
+            print(f"var {v} {name}: L1 vs 2x resolution {l1[name]:.4e}, x-y asymmetry {asym:.1e}")
+        assert l1["mixed"] <= 1.1 * l1["fp64"], (v, l1)
+    assert op.mass_drift(b) < 1e-12
 
 
 def test_kh3d_to_t1_mixed_with_fallback(tmp_path):
@@ -615,8 +681,12 @@ def test_induction_2d_full_size_mixed(tmp_path):
 @pytest.mark.parametrize("name", ["kh3d_gravity_ader", "rti2d_gravity_rk3_nofallback", "kh3d_gravity_ader_fallback"])
 def test_gravity_takes_the_fused_sweeps(tmp_path, name):
     """The gravity parity cases above run the element-tile sweeps, not the
-    stage sequence: the timed kernel of the run is k_sweep."""
+    stage sequence: the run's kernel is k_sweep, with timed launches (the
+    predictor entry of a fallback run names its kernel but does not time it:
+    it returns without waiting for its launches)."""
     import spd_k_amd
     args, path = CASES[name]
     r = spd_k_amd.run(args + [f"job/path={path}", "job/timing=true"], str(tmp_path / "run"))
-    assert r["kernel_name"] == "k_sweep" and r["kernel_launches"] > 0, r["kernel_name"]
+    assert r["kernel_name"] == "k_sweep", r
+    if "fallback" not in name:
+        assert r["kernel_launches"] > 0, r

@@ -791,6 +791,121 @@ __global__ void __launch_bounds__(BS) k_nad_sed(const double* __restrict__ Wn, c
     if (BITS && a.use_sed && in2) abits[c0] = (uint16_t)bits;
 }
 
+// The same detection in 3-D (Moore box, alpha bits), one thread per (x, y)
+// column walking KC cells up in z: the box's three W_old planes and the SED
+// z stencil of W_new slide along the column in registers, so each plane is
+// loaded once per column instead of three (box) or five (SED) times, and
+// the one-thread-per-cell form's plane-apart re-reads (L2 misses) go away.
+// Cells 1..N-2 of every axis; every value is formed by the same expression
+// as in k_nad_sed, and the box's max and min by plane (each plane reduced
+// once, in (y, x) order, then the three planes): for finite values max and
+// min are exact in any grouping, so the flags are the same; a non-finite
+// state stops the run at the next dt anyway.
+template <int KC>
+__global__ void __launch_bounds__(BS) k_nad_sed_z(const double* __restrict__ Wn, const double* __restrict__ Wo,
+                                                  double* __restrict__ tr, uint16_t* __restrict__ abits, FV s, Det a,
+                                                  Geo geo, uint32_t total) {
+    uint32_t idx = xcd_block() * BS + threadIdx.x;
+    if (idx >= total) return;  // total = (Nx - 2) * (Ny - 2) * ceil((Nz - 2) / KC)
+    const int mx1 = s.Nx - 2, my1 = s.Ny - 2;
+    const int i = 1 + (int)(idx % mx1), j = 1 + (int)((idx / mx1) % my1);
+    const int k0 = 1 + (int)(idx / ((uint32_t)mx1 * my1)) * KC;
+    const int kn = min(KC, s.Nz - 1 - k0);  // cells k0 .. k0 + kn - 1 (<= Nz - 2)
+    const size_t pl = s.plane();
+    const long sy = s.Nx, sz = (long)s.Nx * s.Ny;
+    const bool in2xy = i >= 2 && i <= s.Nx - 3 && j >= 2 && j <= s.Ny - 3;
+    const bool sed = a.use_sed && in2xy;
+    static_assert(KC <= 8, "alpha words of the column packed in two 64-bit registers");
+    uint64_t bits[2] = {0, 0};  // 16-bit word of cell kk at bits 16 * (kk % 4) of bits[kk / 4]
+    for (int v = 0; v < s.nv - 1; v++) {
+        if (!((a.mask >> v) & 1)) continue;
+        const double* wo = Wo + v * pl + ((size_t)k0 * s.Ny + j) * s.Nx + i;  // cell (k0, j, i)
+        const double* wn = Wn + v * pl + ((size_t)k0 * s.Ny + j) * s.Nx + i;
+        // max and min of the 3 x 3 W_old patch of planes k-1, k, k+1 (index dk + 1)
+        double pmx[3], pmn[3];
+        auto patch = [&](const double* q, double& hi, double& lo) {
+            hi = lo = q[-sy - 1];
+#pragma unroll
+            for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+                for (int di = -1; di <= 1; di++) {
+                    const double u = q[dj * sy + di];
+                    hi = smax(hi, u);
+                    lo = smin(lo, u);
+                }
+        };
+        patch(wo - sz, pmx[1], pmn[1]);
+        patch(wo, pmx[2], pmn[2]);
+        double Z[5];  // W_new of the column at k-2 .. k+2 (planes below 0 are never used)
+#pragma unroll
+        for (int q = 0; q < 4; q++) Z[q + 1] = (k0 - 2 + q >= 0) ? wn[(q - 2) * sz] : 0.0;
+#pragma unroll 1
+        for (int kk = 0; kk < kn; kk++) {
+            const int k = k0 + kk;
+            const double* o = wo + (long)kk * sz;
+            const double* n = wn + (long)kk * sz;
+            // slide: the box gains plane k+1, the column gains k+2 (k + 2 <= Nz - 1 whenever SED is formed;
+            // at k = Nz - 2 the plane k + 2 is outside the array and not loaded)
+            pmx[0] = pmx[1]; pmn[0] = pmn[1];
+            pmx[1] = pmx[2]; pmn[1] = pmn[2];
+            patch(o + sz, pmx[2], pmn[2]);
+#pragma unroll
+            for (int q = 0; q < 4; q++) Z[q] = Z[q + 1];
+            Z[4] = (k + 2 <= s.Nz - 1) ? n[2 * sz] : 0.0;
+            // NAD (k_nad_sed, Moore box)
+            {
+                double un = Z[2];
+                double mx = smax(smax(pmx[0], pmx[1]), pmx[2]), mn = smin(smin(pmn[0], pmn[1]), pmn[2]);
+                if (a.delta) {
+                    double eps = a.tol * (mx - mn);
+                    mn -= eps;
+                    mx += eps;
+                } else {
+                    mn -= fabs(mn) * a.tol;
+                    mx += fabs(mx) * a.tol;
+                }
+                tr[v * pl + ((size_t)k * s.Ny + j) * s.Nx + i] = (un > mx || un < mn) ? 1.0 : 0.0;
+            }
+            if (sed && k >= 2 && k <= s.Nz - 3) {
+                double u = Z[2];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double* c = geo.c[d];
+                    const double* f = geo.f[d];
+                    const int l = d == 0 ? i : (d == 1 ? j : k);
+                    double wm2, wm1, wp1, wp2;
+                    if (d == 2) {
+                        wm2 = Z[0]; wm1 = Z[1]; wp1 = Z[3]; wp2 = Z[4];
+                    } else {
+                        const long st = d == 0 ? 1 : sy;
+                        wm2 = n[-2 * st]; wm1 = n[-st]; wp1 = n[st]; wp2 = n[2 * st];
+                    }
+                    double h = (c[l + 1] - c[l - 1]);
+                    double hp = (c[l + 2] - c[l]);
+                    double hm = (c[l] - c[l - 2]);
+                    double du = (wp1 - wm1) / h;
+                    double dup = (wp2 - u) / hp;
+                    double dum = (u - wm2) / hm;
+                    double d2u = (dup - dum) / (c[l + 1] - c[l - 1]);
+                    double dv = 0.5 * d2u * (f[l + 1] - f[l]);
+                    if (alpha_fn(dv, dum, du, dup) >= 1.0) {
+                        const uint64_t bit = 1ull << (16 * (kk & 3) + v * 3 + d);
+                        if (kk < 4) bits[0] |= bit;
+                        else bits[1] |= bit;
+                    }
+                }
+            }
+        }
+    }
+    if (sed)
+#pragma unroll 1
+        for (int kk = 0; kk < kn; kk++) {
+            const int k = k0 + kk;
+            const uint64_t w = (kk < 4 ? bits[0] : bits[1]) >> (16 * (kk & 3));
+            if (k >= 2 && k <= s.Nz - 3) abits[((size_t)k * s.Ny + j) * s.Nx + i] = (uint16_t)(w & 0xFFFFu);
+        }
+}
+
 // relax_NAD (src/trouble_detection.cpp:152-183) then PAD_criteria (:244-259), cells 1..N-2
 template <int AX, int AY, int AZ, bool BITS = false>
 __global__ void __launch_bounds__(BS) k_relax_pad(const double* __restrict__ Wn, double* __restrict__ tr,
@@ -1598,6 +1713,14 @@ static int detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, doub
     a.mask = mask;
     a.tol = c->prm.nad_tol;
     uint32_t total = (uint32_t)FV(s).plane();
+    if (bits && a.act[0] && a.act[1] && a.act[2] && a.moore && s.Nx >= 3 && s.Ny >= 3 && s.Nz >= 3) {
+        constexpr int KC = 8;
+        const uint32_t tz = (uint32_t)(s.Nx - 2) * (s.Ny - 2) * ((s.Nz - 2 + KC - 1) / KC);
+        hipLaunchKernelGGL((k_nad_sed_z<KC>), dim3(xcd_grid(tz)), dim3(BS), 0, c->stream, Wn, Wo, tr, abits, FV(s),
+                           a, geo_of(c), tz);
+        SPDK_LAUNCH_CHECK("nad_sed_z");
+        total = 0;
+    }
 #define SPDK_NAD(X, Y, Z)                                                                                       \
     do {                                                                                                        \
         if (bits)                                                                                               \
@@ -1607,7 +1730,7 @@ static int detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, doub
             hipLaunchKernelGGL((k_nad_sed<X, Y, Z, false>), dim3(xcd_grid(total)), dim3(BS), 0, c->stream, Wn, Wo, tr, \
                                alx, aly ? aly : alx, alz ? alz : alx, nullptr, FV(s), a, geo_of(c), total);       \
     } while (0)
-    switch ((a.act[0] ? 4 : 0) | (a.act[1] ? 2 : 0) | (a.act[2] ? 1 : 0)) {
+    if (total) switch ((a.act[0] ? 4 : 0) | (a.act[1] ? 2 : 0) | (a.act[2] ? 1 : 0)) {
         case 0: SPDK_NAD(0, 0, 0); break;
         case 1: SPDK_NAD(0, 0, 1); break;
         case 2: SPDK_NAD(0, 1, 0); break;

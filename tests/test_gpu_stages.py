@@ -221,7 +221,7 @@ def fv_shape(nv, N, n, act, stag=None):
 
 
 @pytest.mark.parametrize("act,N", GEOMS)
-@pytest.mark.parametrize("opts", [dict(), dict(nad_delta=1, nad_moore=0), dict(sed=0, blending=0)])
+@pytest.mark.parametrize("opts", [dict(), dict(nad_delta=1, nad_moore=0), dict(nad_delta=1), dict(sed=0, blending=0)])
 def test_fv_fallback_stages(act, N, opts):
     p = 3
     n = p + 1

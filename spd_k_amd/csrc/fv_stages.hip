@@ -7,6 +7,22 @@
 #include "ctx.hpp"
 #include "physics.hpp"
 
+// Launch-shape constants of the 3-D FV kernels (measured at 96^3 elements):
+// cells per z column of the update and trouble-detection kernels, and the
+// x and z extents of the fallback's cell tile (y makes up the 256 threads).
+#ifndef SPDK_UPD_KC
+#define SPDK_UPD_KC 8
+#endif
+#ifndef SPDK_NAD_KC
+#define SPDK_NAD_KC 8
+#endif
+#ifndef SPDK_FB_TX
+#define SPDK_FB_TX 8
+#endif
+#ifndef SPDK_FB_TZ
+#define SPDK_FB_TZ 4
+#endif
+
 using namespace spdk;
 static constexpr int BS = 256;
 
@@ -1228,7 +1244,8 @@ __global__ void __launch_bounds__(BS) k_fallback(const double* __restrict__ W, c
 // faces are the same bits. A tile with no nonzero theta in reach exits.
 template <int AY, int AZ>
 struct FbTile {
-    static constexpr int TX = AZ ? 16 : (AY ? 16 : 256), TY = AZ ? 4 : (AY ? 16 : 1), TZ = AZ ? 4 : 1;
+    static constexpr int TX = AZ ? SPDK_FB_TX : (AY ? 16 : 256), TY = AZ ? BS / (SPDK_FB_TX * SPDK_FB_TZ) : (AY ? 16 : 1),
+                         TZ = AZ ? SPDK_FB_TZ : 1;
     static constexpr int NC = TX * TY * TZ;          // = BS
     static constexpr int HX = TY * TZ, HY = AY ? TX * TZ : 0, HZ = AZ ? TX * TY : 0;  // halo cells per layer
     static_assert(NC == BS, "one cell per thread");
@@ -1605,7 +1622,7 @@ int spdk_fv_state_update_prims(spdk_ctx* c, double* U_next, double* W_new, doubl
     const FvUpd a = fv_upd(c, t_id, dt, 0);
     const int Mx = sc.Nx - 2 * c->fgh[0], My = sc.Ny - 2 * c->fgh[1], Mz = sc.Nz - 2 * c->fgh[2];
     if (a.act[2]) {  // z columns of 8 cells per thread
-        constexpr int KC = 8;
+        constexpr int KC = SPDK_UPD_KC;
         const uint32_t tz = (uint32_t)((size_t)Mx * My * ((Mz + KC - 1) / KC));
         hipLaunchKernelGGL(k_fv_state_update_prims_z<KC>, dim3(xcd_grid(tz)), dim3(BS), 0, c->stream, U_next, W_new,
                            W_old, U_fv, Fx, Fy ? Fy : Fx, Fz, FV(sc), fv_stag(sc, 0, a.act[0]), fv_stag(sc, 1, a.act[1]),
@@ -1714,7 +1731,7 @@ static int detect_troubles(spdk_ctx* c, const double* Wn, const double* Wo, doub
     a.tol = c->prm.nad_tol;
     uint32_t total = (uint32_t)FV(s).plane();
     if (bits && a.act[0] && a.act[1] && a.act[2] && a.moore && s.Nx >= 3 && s.Ny >= 3 && s.Nz >= 3) {
-        constexpr int KC = 8;
+        constexpr int KC = SPDK_NAD_KC;
         const uint32_t tz = (uint32_t)(s.Nx - 2) * (s.Ny - 2) * ((s.Nz - 2 + KC - 1) / KC);
         hipLaunchKernelGGL((k_nad_sed_z<KC>), dim3(xcd_grid(tz)), dim3(BS), 0, c->stream, Wn, Wo, tr, abits, FV(s),
                            a, geo_of(c), tz);

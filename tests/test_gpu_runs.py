@@ -556,6 +556,27 @@ def test_mixed_blast3d_matched_accuracy(tmp_path):
     assert op.mass_drift(b) < 1e-12
 
 
+def test_mixed_rti_full_run(tmp_path):
+    """The reference's RTI input (gravity, reflective walls, fallback on) to
+    t = 1.8 with the ADER step, 2,373 steps: the mixed form's density ends as
+    far from the FP64 run as an FP64 run whose initial perturbation amplitude
+    changed by 1e-7 relative (within 3x of that control's L1 distance;
+    measured 2.4e-4 against 1.8e-4), mass conserved to round-off in both."""
+    args = ["-i", os.path.join(op.INPUTS, "rti.athinput"), "time/integrator=ader", "output/dt=0.9",
+            "output/fields=W_cv"]
+    a, b, c = (str(tmp_path / x) for x in ("fp64", "mixed", "control"))
+    op.run_driver(args, a, timeout=600)
+    op.run_driver(args + ["job/precision=mixed"], b, timeout=600)
+    op.run_driver(args + ["problem/amp=0.0250000025"], c, timeout=600)
+    g = op.grid(a)
+    i = op.output_indices(a)[-1]
+    rho = {k: op.load_sd(d, g, i, 0) for k, d in (("fp64", a), ("mixed", b), ("control", c))}
+    dist = {k: float(np.abs(rho[k] - rho["fp64"]).mean() / np.abs(rho["fp64"]).mean()) for k in ("mixed", "control")}
+    print(dist)
+    assert 0 < dist["control"] and dist["mixed"] < 3 * dist["control"], dist
+    assert op.mass_drift(a) < 1e-12 and op.mass_drift(b) < 1e-12
+
+
 def test_kh3d_to_t1_mixed_with_fallback(tmp_path):
     """The 3-D KH recipe (seeded perturbation) to t = 1 with the fallback, at
     24^3: the mixed-predictor form against the FP64 form, L1 and Linf of the

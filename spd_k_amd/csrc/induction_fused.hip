@@ -39,6 +39,11 @@
 #include <type_traits>
 #include "ctx.hpp"
 
+// Element tiles of the induction sweep in XCD-contiguous order (0: launch order).
+#ifndef SPDK_ISWEEP_XCD
+#define SPDK_ISWEEP_XCD 1
+#endif
+
 using namespace spdk;
 
 namespace {
@@ -306,6 +311,12 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
     __shared__ float sP[PLAIN ? 1 : T * T];                  // invader * w * dt
     __shared__ int sEl[EPB][3];
     __shared__ int sPerm[G::EP];
+    // workgroups go to the 8 XCDs in turn; XCD x takes the x-th contiguous
+    // eighth of the element tiles, so a tile's y (and z) neighbours, whose
+    // face traces it reads, were mostly written through the same L2. The grid
+    // is a multiple of 8; tiles past the end exit (uniform per block).
+    const int blk = SPDK_ISWEEP_XCD ? (int)((blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8) : (int)blockIdx.x;
+    if (blk * EPB >= a.nel) return;
     for (int i = threadIdx.x; i < G::EP; i += BS) sPerm[i] = kPerm<N, D>.p[i];
     for (int i = threadIdx.x; i < N * M; i += BS) {
         sDm[i] = a.Dm[i];
@@ -318,7 +329,7 @@ __global__ void __launch_bounds__(BS) k_isweep(IArgs a) {
         if (!PLAIN) sP[threadIdx.x] = (float)(a.inv[threadIdx.x] * a.w[threadIdx.x % T] * a.dt);
     }
     if (threadIdx.x < T) sW[threadIdx.x] = a.w[threadIdx.x];
-    const int e0 = blockIdx.x * EPB;
+    const int e0 = blk * EPB;
     const int ne = min(EPB, a.nel - e0);
     if ((int)threadIdx.x < ne) {
         int lin = e0 + threadIdx.x;
@@ -479,7 +490,8 @@ int run_step(spdk_ctx* c, IArgs a, void* ws) {
     static_assert(perm_is_permutation<N, D>(), "edge-point order must be a permutation");
     const size_t half = (size_t)T * a.nel * IG<N, D>::EP;
     R* buf[2] = {(R*)ws, (R*)ws + half};
-    const unsigned nb = (unsigned)((a.nel + TL::EPB - 1) / TL::EPB);
+    unsigned nb = (unsigned)((a.nel + TL::EPB - 1) / TL::EPB);
+    if (SPDK_ISWEEP_XCD) nb = (nb + 7) / 8 * 8;
     const unsigned nstart = (unsigned)std::min<size_t>(((size_t)a.nel * IG<N, D>::EP + BS - 1) / BS, 65536);
     a.Eout = buf[0];
     a.Ein = nullptr;

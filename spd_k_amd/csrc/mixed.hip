@@ -796,6 +796,24 @@ __device__ __forceinline__ void mfv_faces_staged(const MArgs& a, unsigned char* 
     }
 }
 
+// Barrier between sweep phases that touch only the element's own LDS area.
+// With one element per wave (TPE = 64) the wave's own order is enough: a
+// wave's LDS operations execute in order, and the wavefront-scope fences keep
+// the compiler from moving its LDS accesses across the point. The waves of a
+// block then run their phases independently (a block barrier makes each wait
+// for the slowest). Otherwise, and in the face sweep (MODE 2, whose staged
+// writes cross elements), the block barrier.
+template <int TPE, int MODE>
+__device__ __forceinline__ void elem_sync() {
+    if constexpr (TPE == 64 && MODE != 2) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // MODE 0: a predictor sweep (out_pred set); MODE 1: the corrector (face
 // balance, conservative update, cv state and dt); MODE 2: the flux
 // evaluation of the predictor's slices with their face integrals written
@@ -860,7 +878,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
         }
         X[G::XACT] = act ? 1.0 : 0.0;
     }
-    __syncthreads();
+    elem_sync<TPE, MODE>();
     const bool act_e = live && X[G::XACT] != 0.0;
     if (!__syncthreads_or(act_e)) {  // nothing to do in this block
         if (live && s == 0 && a.big_out) a.big_out[e] = 0;
@@ -906,7 +924,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
             for (int v = 0; v < NVP; v++) Ds[v * PSZ + so] = dold[v];
         }
     }
-    __syncthreads();
+    elem_sync<TPE, MODE>();
 
     float dudt[T][NVP];
 #pragma unroll
@@ -939,7 +957,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
             pencil<N, DIM>(a, Ds + t * NVP * PSZ, Fs + slot * NVP * FSZ, X, d, q, ts, ei, ej, ek, (size_t)e, cons,
                            bal, bw, MODE == 2 && !a.F_x);
         }
-        __syncthreads();
+        elem_sync<TPE, MODE>();
         if (MODE == 2) {
             if constexpr (N == 4 && DIM == 3) {
                 if (!a.F_x && !a.fv_valu) mfv_faces_staged<N, DIM>(a, lds_raw, blk, r, s, el, act_e);
@@ -982,7 +1000,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
                     }
             }
         }
-        __syncthreads();
+        elem_sync<TPE, MODE>();
     }
     if (MODE == 2) return;
     if (!a.path)  // the zero-order guess has identical slices
@@ -1044,7 +1062,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE =
                     Ds[(t * NVP + v) * PSZ + so] = dn;
                 }
         }
-        __syncthreads();
+        elem_sync<TPE, MODE>();
         // traces of the new state into this element's older buffer: task =
         // (d, t, q), q fastest: at 3-D p=3 each 64-lane pass is one axis
         const int wb = act_e ? (a.ver[(size_t)a.E + e] < a.ver[e] ? 1 : 0) : 0;

@@ -682,6 +682,63 @@ __global__ void __launch_bounds__(BS) k_fv_bc(double* __restrict__ U, FV s, int 
     }
 }
 
+// The same ghost layers of every active axis in one launch: each ghost cell
+// (enumerated once: the z layers, then the y layers of the interior z
+// planes, then the x layers of the interior rows) takes the interior cell
+// that the x, then y, then z fills of k_fv_bc would leave in it, times the
+// reflective sign of each axis it is a ghost of. Same bits: the values are
+// copies with exact sign flips.
+struct BcAxes {
+    int type[3], act[3];
+};
+__device__ __forceinline__ int bc_src(int c, int n, int type, bool& flip) {
+    flip = false;
+    if (c >= 2 && c <= n - 3) return c;
+    const bool lo = c < 2;
+    const int l = lo ? c : c - (n - 2);  // layer 0 or 1 of its side
+    if (type == SPDK_BC_PERIODIC) return lo ? n - 4 + l : 2 + l;
+    if (type == SPDK_BC_GRADFREE) return lo ? 2 + l : n - 4 + l;
+    flip = true;
+    return lo ? 3 - l : n - 3 - l;
+}
+__global__ void __launch_bounds__(BS) k_fv_bc_all(double* __restrict__ U, FV s, BcAxes b, uint32_t nzl, uint32_t nyl,
+                                                  uint32_t total) {
+    uint32_t idx = blockIdx.x * BS + threadIdx.x;
+    if (idx >= total) return;
+    const int kz0 = b.act[2] ? 2 : 0, kz1 = b.act[2] ? s.Nz - 3 : 0;   // interior z planes
+    const int jy0 = b.act[1] ? 2 : 0, jy1 = b.act[1] ? s.Ny - 3 : 0;   // interior y rows
+    int k, j, i;
+    if (idx < nzl) {  // z layers: 4 planes of Ny x Nx
+        i = idx % s.Nx;
+        j = (idx / s.Nx) % s.Ny;
+        const int L = idx / (s.Nx * s.Ny);
+        k = L < 2 ? L : s.Nz - 4 + L;
+    } else if ((idx -= nzl) < nyl) {  // y layers of the interior planes: 4 rows of Nx
+        i = idx % s.Nx;
+        const int L = (idx / s.Nx) % 4;
+        j = L < 2 ? L : s.Ny - 4 + L;
+        k = kz0 + (int)(idx / (s.Nx * 4));
+    } else {  // x layers of the interior rows
+        idx -= nyl;
+        const int L = idx % 4;
+        i = L < 2 ? L : s.Nx - 4 + L;
+        j = jy0 + (int)((idx / 4) % (jy1 - jy0 + 1));
+        k = kz0 + (int)(idx / (4 * (jy1 - jy0 + 1)));
+    }
+    (void)kz1;
+    bool fx = false, fy = false, fz = false;
+    const int si = b.act[0] ? bc_src(i, s.Nx, b.type[0], fx) : i;
+    const int sj = b.act[1] ? bc_src(j, s.Ny, b.type[1], fy) : j;
+    const int sk = b.act[2] ? bc_src(k, s.Nz, b.type[2], fz) : k;
+    for (int v = 0; v < s.nv; v++) {
+        double u = U[s.off(v, sk, sj, si)];
+        if (fx && v == 1) u = -1.0 * u;
+        if (fy && v == 2) u = -1.0 * u;
+        if (fz && v == 3) u = -1.0 * u;
+        U[s.off(v, k, j, i)] = u;
+    }
+}
+
 // compute_primitives, FV: src/hydro.cpp:134-153
 __global__ void __launch_bounds__(BS) k_fv_prim(const double* __restrict__ U, double* __restrict__ W, FV s, double gm,
                                                 uint32_t total) {
@@ -1702,6 +1759,19 @@ int spdk_fv_state_to_sd(spdk_ctx* c, const double* U_fv, spdk_fv_shape sc, doubl
 }
 
 int spdk_fill_fv_ghosts(spdk_ctx* c, double* U, spdk_fv_shape s) {
+    if (c->prm.active[0] && s.Nx >= 5 && (!c->prm.active[1] || s.Ny >= 5) && (!c->prm.active[2] || s.Nz >= 5)) {
+        BcAxes b;
+        for (int d = 0; d < 3; d++) { b.type[d] = c->prm.bc[d]; b.act[d] = c->prm.active[d]; }
+        const uint32_t nzl = b.act[2] ? 4u * s.Ny * s.Nx : 0u;
+        const uint32_t zin = b.act[2] ? (uint32_t)(s.Nz - 4) : (uint32_t)s.Nz;
+        const uint32_t nyl = b.act[1] ? zin * 4u * s.Nx : 0u;
+        const uint32_t yin = b.act[1] ? (uint32_t)(s.Ny - 4) : (uint32_t)s.Ny;
+        const uint32_t nxl = zin * yin * 4u;
+        const uint32_t total = nzl + nyl + nxl;
+        hipLaunchKernelGGL(k_fv_bc_all, dim3(nblocks(total, BS)), dim3(BS), 0, c->stream, U, FV(s), b, nzl, nyl, total);
+        SPDK_LAUNCH_CHECK("fill_fv_ghosts");
+        return SPDK_OK;
+    }
     for (int d = 0; d < 3; d++) {
         if (!c->prm.active[d]) continue;
         uint32_t total = (uint32_t)((size_t)(d == 2 ? 2 : s.Nz) * (d == 1 ? 2 : s.Ny) * (d == 0 ? 2 : s.Nx));

@@ -22,6 +22,11 @@
 #ifndef SPDK_FB_TZ
 #define SPDK_FB_TZ 4
 #endif
+// waves per SIMD the fallback tile kernel is compiled for (4: 126 VGPRs, no
+// spills; 3.75 -> 3.45 ms per call at 96^3 against the unbounded 160)
+#ifndef SPDK_FB_WAVES
+#define SPDK_FB_WAVES 4
+#endif
 
 using namespace spdk;
 static constexpr int BS = 256;
@@ -1309,7 +1314,7 @@ struct FbTile {
     static_assert(HX + HY + HZ <= BS, "one halo cell per thread");
 };
 template <int AY, int AZ>
-__global__ void __launch_bounds__(BS) k_fallback_tile(const double* __restrict__ W, const double* __restrict__ th,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SPDK_FB_WAVES))) k_fallback_tile(const double* __restrict__ W, const double* __restrict__ th,
                                                       double* __restrict__ Fx, double* __restrict__ Fy,
                                                       double* __restrict__ Fz, FV s, FV fx, FV fy, FV fz, Geo g,
                                                       double dt, double gm, int bx, int by, int bz) {

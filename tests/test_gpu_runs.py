@@ -75,6 +75,9 @@ CASES = {
                                       "job/fallback=false", "time/tlim=0.02", "output/dt=0.01"], "fused"),
     "kh3d_gravity_ader": (["-i", KH, "job/fallback=false", "mesh/nx1=6", "mesh/nx2=6", "mesh/nx3=4", "hydro/g1=0.3",
                            "hydro/g2=-0.5", "hydro/g3=0.2", "time/tlim=0.01", "output/dt=0.005"], "fused"),
+    # reflective walls on all three axes with the fallback (ghost cells of every edge and corner)
+    "implosion3d_reflective": (["-i", os.path.join(op.INPUTS, "implosion.athinput"), "mesh/nx1=6", "mesh/nx2=6",
+                                "mesh/nx3=4", "mesh/x3_bc=reflective", "time/tlim=0.02", "output/dt=0.01"], "fused"),
     "kh3d_gravity_ader_fallback": (["-i", KH, "mesh/nx1=6", "mesh/nx2=6", "mesh/nx3=4", "hydro/g1=0.3",
                                     "hydro/g2=-0.5", "hydro/g3=0.2", "problem/perturb=random", "problem/amp=0.01",
                                     "time/tlim=0.01", "output/dt=0.005"], "fused"),
